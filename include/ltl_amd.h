@@ -1,0 +1,194 @@
+/*
+ * ltl_amd.h -- C ABI of the MI355X LTL learner (drop-in boundary).
+ *
+ * The reference learner (MojtabaValizadeh/ltl-learning-on-gpus) exposes its
+ * hot path through one in-process C++ function,
+ *
+ *   std::string LTLI(const unsigned short *costFun, unsigned short maxCost,
+ *                    unsigned RlxUnqChkTyp, unsigned NegType,
+ *                    std::set<char> alphabet, int &LTLcost,
+ *                    std::uint64_t &allLTLs, pos, neg)
+ *                                                  (code/ltli6463.cu:1043-1052)
+ *
+ * and through the CLI that wraps it (code/ltli6463.cu:1575-1681).  This
+ * header replaces that function with plain-C entry points: plain pointers and
+ * sizes, no C++ or torch types.  All entry points are synchronous on the
+ * calling host thread.  One context per host thread; a context is not
+ * re-entrant.  Nothing is printed by the library.
+ */
+#ifndef LTL_AMD_H
+#define LTL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  The reference returns strings ("not_found",
+ * "see_the_error", code/ltli6463.cu:1082, 1461) and always exits 0; here
+ * every entry point returns one of these. */
+#define LTL_FOUND        1   /* a formula was found */
+#define LTL_NOT_FOUND    0   /* none up to max_cost ("not_found") */
+#define LTL_E_ARG       (-1) /* invalid argument */
+#define LTL_E_LIMIT     (-2) /* instance beyond limits ("see_the_error") */
+#define LTL_E_OOM       (-3) /* device memory exhausted */
+#define LTL_E_HIP       (-4) /* HIP runtime error */
+#define LTL_E_COLLISION (-5) /* dedup fingerprint collision detected */
+#define LTL_E_NODEVICE  (-6) /* no GPU visible */
+
+/* Operator codes, in the cost-vector order of the reference
+ * (costFun = p, ~, &, |, X, F, G, U; code/ltli6463.cu:1142-1150). */
+enum { LTL_ATOM = 0, LTL_NOT = 1, LTL_AND = 2, LTL_OR = 3,
+       LTL_NEXT = 4, LTL_FIN = 5, LTL_GLOB = 6, LTL_UNTIL = 7 };
+
+/* Dedup modes.  EXACT compares full characteristic sets; REF1..REF3 are the
+ * reference's relaxed uniqueness keys (RlxUnqChkTyp, code/ltli6463.cu:57-141). */
+enum { LTL_DEDUP_EXACT = 0, LTL_DEDUP_REF1 = 1, LTL_DEDUP_REF2 = 2, LTL_DEDUP_REF3 = 3 };
+
+/* Candidate order within the last cost level. */
+enum { LTL_ORDER_PLAIN = 0, LTL_ORDER_LEARNED = 1 };
+
+/* A learning instance.  Replaces the reference's pos/neg token vectors and
+ * alphabet set (code/ltli6463.cu:1043-1052): the atom characteristic sets
+ * are given already encoded, one uint64 per trace, bit k set iff the atom
+ * holds at position k (the encoding of code/ltli6463.cu:1100-1133).
+ * Traces 0..num_pos-1 are positive, the rest negative. */
+typedef struct {
+    int32_t num_pos;
+    int32_t num_neg;
+    int32_t num_atoms;
+    int32_t reserved;
+    const uint8_t  *lens;   /* [num_pos+num_neg], each 1..64 */
+    const uint64_t *atoms;  /* [num_atoms * (num_pos+num_neg)], atom-major */
+    const char     *names;  /* num_atoms single-letter names, or NULL */
+} ltl_problem;
+
+/* Options.  cost/max_cost/neg_type/dedup_mode carry the reference's
+ * costFun, maxCost, NegType and RlxUnqChkTyp (code/ltli6463.cu:1581-1620). */
+typedef struct {
+    uint16_t cost[8];          /* p, ~, &, |, X, F, G, U; each >= 1 */
+    uint16_t max_cost;
+    uint8_t  neg_type;         /* 1: negate any formula, 2: atoms only */
+    uint8_t  dedup_mode;       /* LTL_DEDUP_* */
+    int32_t  hamming_k;        /* accept iff misclassified traces <= k */
+    int32_t  order;            /* LTL_ORDER_* */
+    int32_t  hash_slots_log2;  /* dedup table size; 0 = sized from budget */
+    int32_t  precheck;         /* 1 = test a level on per-trace summaries
+                                  before generating it (default) */
+    int32_t  group_lanes;      /* lanes per characteristic set; 0 = tuned */
+    int64_t  capacity;         /* max stored sets; 0 = sized from budget */
+    uint64_t hbm_budget_bytes; /* 0 = 75% of free device memory */
+    int32_t  device;           /* HIP device ordinal */
+    int32_t  profile;          /* 1 = time every kernel with HIP events */
+    int32_t  reserved[6];
+} ltl_options;
+
+/* One node of the returned formula, rebuilt on the host from parent indices
+ * (cf. the device BFS and host recursion of code/ltli6463.cu:29-55, 877-928).
+ * Node 0 is the root; left/right index into the node array (-1 if none). */
+typedef struct {
+    int32_t op;      /* LTL_* operator code */
+    int32_t atom;    /* atom index for LTL_ATOM, else -1 */
+    int32_t left;
+    int32_t right;
+} ltl_node;
+
+typedef struct {
+    int32_t  status;      /* LTL_FOUND / LTL_NOT_FOUND / LTL_E_* */
+    int32_t  cost;        /* the reference's LTLcost out-parameter */
+    uint64_t all_ltls;    /* the reference's allLTLs out-parameter */
+    int64_t  found_id;    /* canonical candidate id of the result, -1 */
+    int64_t  n_stored;    /* characteristic sets stored (incl. atoms) */
+    int32_t  on_the_fly;  /* store overflowed into check-only mode */
+    int32_t  n_nodes;
+    ltl_node *nodes;      /* owned by the library */
+    char     *formula;    /* reference grammar, or "not_found"; owned */
+    double   seconds;     /* wall time of the call */
+    double   kernel_seconds; /* summed device time of the enumeration kernels */
+    uint64_t hbm_bytes;   /* algorithmic bytes of the enumeration kernels */
+    uint64_t candidates;  /* candidates generated (full sets computed) */
+    uint64_t checked;     /* candidates tested on summaries only */
+} ltl_result;
+
+typedef struct ltl_ctx ltl_ctx;
+
+/* Fill *opt with the defaults: unit costs, max_cost 10, neg_type 1, exact
+ * dedup, plain order, precheck on. */
+void ltl_default_options(ltl_options *opt);
+
+/* Create a context on opt->device.  Device memory is reserved lazily and
+ * reused across calls (the reference allocates and frees about 5 GB per
+ * call, code/ltli6463.cu:1153-1187, 1486-1493).  Returns NULL on error;
+ * *status (if not NULL) receives the LTL_E_* code. */
+ltl_ctx *ltl_create(const ltl_options *opt, int32_t *status);
+
+/* Replace the options of an existing context. */
+int32_t ltl_set_options(ltl_ctx *ctx, const ltl_options *opt);
+
+/* Learn the minimal-cost formula separating the instance.  Replaces
+ * LTLI() (code/ltli6463.cu:1043).  Returns res->status. */
+int32_t ltl_learn(ltl_ctx *ctx, const ltl_problem *pb, ltl_result *res);
+
+/* Learn n independent instances in one persistent launch (no reference
+ * counterpart; the reference runs one process per instance through
+ * code/dc.py:23-43).  res[i] receives instance i.  Returns the number of
+ * instances solved, or a negative LTL_E_* code. */
+int32_t ltl_learn_batch(ltl_ctx *ctx, const ltl_problem *pbs, int32_t n, ltl_result *res);
+
+/* Free what a result owns (formula, nodes). */
+void ltl_result_free(ltl_result *res);
+
+/* Release the context and its device memory. */
+void ltl_destroy(ltl_ctx *ctx);
+
+/* ---- Introspection (no reference counterpart; the reference keeps a
+ * commented-out dump of its cache, code/ltli6463.cu:1463-1484) ---------- */
+
+/* Per-kernel statistics of the last call (profile = 1). */
+typedef struct {
+    char     name[24];
+    uint64_t launches;
+    double   seconds;      /* summed device time (HIP events) */
+    uint64_t alg_bytes;    /* algorithmic bytes: operand reads per candidate,
+                              table slot, temporaries, survivor writes */
+    uint64_t min_bytes;    /* compulsory bytes: each operand set read once */
+    uint64_t items;        /* candidates (gen) or survivors (emit) */
+} ltl_kernel_stat;
+
+/* Copy up to max entries; returns the number of kernel kinds. */
+int32_t ltl_get_kernel_stats(const ltl_ctx *ctx, ltl_kernel_stat *out, int32_t max);
+
+/* Read stored sets [first, first+count) of the last call, unpacked to one
+ * uint64 per trace (cs[k*W + i]); op/left/right may be NULL. */
+int32_t ltl_store_read(ltl_ctx *ctx, int64_t first, int64_t count, uint64_t *cs,
+                       uint8_t *op, int32_t *left, int32_t *right);
+
+/* Level boundaries of the last call: start[c], end[c] for c < max. */
+int32_t ltl_levels(const ltl_ctx *ctx, int64_t *start, int64_t *end, int32_t max);
+
+/* ---- Host helpers around the boundary (no GPU needed) ------------------ */
+
+/* Read a trace file in the reference format (code/ltli6463.cu:1500-1573):
+ * positive lines, "---", negative lines, "---", operators, "---", alphabet.
+ * Fills lens[max_traces], atoms[26*max_traces] (atom-major, stride W after
+ * the call), names[27].  Returns 0 or LTL_E_ARG / LTL_E_LIMIT. */
+int32_t ltl_read_trace_file(const char *path, int32_t max_traces,
+                            int32_t *num_pos, int32_t *num_neg, int32_t *num_atoms,
+                            uint8_t *lens, uint64_t *atoms, char *names);
+
+/* Evaluate a formula in the reference output grammar on every trace
+ * (bit-parallel; the product counterpart of code/dc.py:136-186).  Fills
+ * out_cs[W]; returns 0 or LTL_E_ARG on a parse error. */
+int32_t ltl_eval_formula(const char *formula, const ltl_problem *pb, uint64_t *out_cs);
+
+/* Misclassified traces of a characteristic set (0 = separates). */
+int32_t ltl_mismatches(const uint64_t *cs, int32_t num_pos, int32_t num_neg);
+
+/* Library version string. */
+const char *ltl_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,336 @@
+"""MI355X LTL learner: Python binding of the C ABI in include/ltl_amd.h.
+
+The library (libltl_amd.so, built by build.py) holds the HIP kernels and the
+C++ host scheduler; this module only marshals arguments.  It mirrors the
+reference's in-process entry point
+
+    std::string LTLI(costFun, maxCost, RlxUnqChkTyp, NegType, alphabet,
+                     int &LTLcost, uint64 &allLTLs, pos, neg)
+                                            (code/ltli6463.cu:1043-1052)
+
+as ``LTLI(...)`` returning ``(formula, LTLcost, allLTLs)``, and exposes the
+context API (``Learner``) for repeated calls.  There is no CPU fallback: if
+the library is missing or no GPU is visible, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libltl_amd.so")
+
+FOUND, NOT_FOUND = 1, 0
+E_ARG, E_LIMIT, E_OOM, E_HIP, E_COLLISION, E_NODEVICE = -1, -2, -3, -4, -5, -6
+ATOM, NOT, AND, OR, NEXT, FIN, GLOB, UNTIL = range(8)
+ORDER_PLAIN, ORDER_LEARNED = 0, 1
+
+
+class LtlError(RuntimeError):
+    pass
+
+
+class Problem(ctypes.Structure):
+    _fields_ = [("num_pos", ctypes.c_int32), ("num_neg", ctypes.c_int32),
+                ("num_atoms", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("lens", ctypes.POINTER(ctypes.c_uint8)),
+                ("atoms", ctypes.POINTER(ctypes.c_uint64)),
+                ("names", ctypes.c_char_p)]
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("cost", ctypes.c_uint16 * 8), ("max_cost", ctypes.c_uint16),
+                ("neg_type", ctypes.c_uint8), ("dedup_mode", ctypes.c_uint8),
+                ("hamming_k", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("hash_slots_log2", ctypes.c_int32), ("precheck", ctypes.c_int32),
+                ("group_lanes", ctypes.c_int32), ("capacity", ctypes.c_int64),
+                ("hbm_budget_bytes", ctypes.c_uint64), ("device", ctypes.c_int32),
+                ("profile", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+
+
+class Node(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int32), ("atom", ctypes.c_int32),
+                ("left", ctypes.c_int32), ("right", ctypes.c_int32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("cost", ctypes.c_int32),
+                ("all_ltls", ctypes.c_uint64), ("found_id", ctypes.c_int64),
+                ("n_stored", ctypes.c_int64), ("on_the_fly", ctypes.c_int32),
+                ("n_nodes", ctypes.c_int32), ("nodes", ctypes.POINTER(Node)),
+                ("formula", ctypes.c_char_p), ("seconds", ctypes.c_double),
+                ("kernel_seconds", ctypes.c_double), ("hbm_bytes", ctypes.c_uint64),
+                ("candidates", ctypes.c_uint64), ("checked", ctypes.c_uint64)]
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char * 24), ("launches", ctypes.c_uint64),
+                ("seconds", ctypes.c_double), ("alg_bytes", ctypes.c_uint64),
+                ("min_bytes", ctypes.c_uint64), ("items", ctypes.c_uint64)]
+
+
+class GenInstance(ctypes.Structure):
+    _fields_ = [("num_pos", ctypes.c_int32), ("num_neg", ctypes.c_int32),
+                ("num_atoms", ctypes.c_int32), ("max_cost", ctypes.c_int32),
+                ("hamming_k", ctypes.c_int32), ("neg_type", ctypes.c_int32),
+                ("lens", ctypes.POINTER(ctypes.c_uint8)),
+                ("atoms", ctypes.POINTER(ctypes.c_uint64)),
+                ("names", ctypes.c_char * 32), ("planted", ctypes.c_char * 512)]
+
+
+EXPORTS = ["ltl_default_options", "ltl_create", "ltl_set_options", "ltl_learn", "ltl_learn_batch",
+           "ltl_result_free", "ltl_destroy", "ltl_get_kernel_stats", "ltl_store_read", "ltl_levels",
+           "ltl_read_trace_file", "ltl_eval_formula", "ltl_mismatches", "ltl_version",
+           "ltl_gen_config", "ltl_gen_custom", "ltl_gen_free"]
+
+_lib = None
+
+
+def lib():
+    """Load libltl_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise LtlError(f"{LIB_PATH} is missing: run build.py (no CPU fallback exists)")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.POINTER
+    L.ltl_default_options.argtypes = [P(Options)]
+    L.ltl_create.argtypes = [P(Options), P(ctypes.c_int32)]
+    L.ltl_create.restype = ctypes.c_void_p
+    L.ltl_set_options.argtypes = [ctypes.c_void_p, P(Options)]
+    L.ltl_learn.argtypes = [ctypes.c_void_p, P(Problem), P(Result)]
+    L.ltl_learn_batch.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int32, P(Result)]
+    L.ltl_result_free.argtypes = [P(Result)]
+    L.ltl_destroy.argtypes = [ctypes.c_void_p]
+    L.ltl_get_kernel_stats.argtypes = [ctypes.c_void_p, P(KernelStat), ctypes.c_int32]
+    L.ltl_store_read.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, P(ctypes.c_uint64),
+                                 P(ctypes.c_uint8), P(ctypes.c_int32), P(ctypes.c_int32)]
+    L.ltl_levels.argtypes = [ctypes.c_void_p, P(ctypes.c_int64), P(ctypes.c_int64), ctypes.c_int32]
+    L.ltl_read_trace_file.argtypes = [ctypes.c_char_p, ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32),
+                                      P(ctypes.c_int32), P(ctypes.c_uint8), P(ctypes.c_uint64), ctypes.c_char_p]
+    L.ltl_eval_formula.argtypes = [ctypes.c_char_p, P(Problem), P(ctypes.c_uint64)]
+    L.ltl_mismatches.argtypes = [P(ctypes.c_uint64), ctypes.c_int32, ctypes.c_int32]
+    L.ltl_version.restype = ctypes.c_char_p
+    L.ltl_gen_config.argtypes = [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int32, P(GenInstance)]
+    L.ltl_gen_custom.argtypes = [ctypes.c_uint64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_char_p,
+                                 ctypes.c_int32, P(GenInstance)]
+    L.ltl_gen_free.argtypes = [P(GenInstance)]
+    _lib = L
+    return L
+
+
+def _u64p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+
+
+class Instance:
+    """Atom characteristic sets (atom-major, one uint64 per trace), lengths,
+    class sizes (positives first) and atom letters."""
+
+    def __init__(self, num_pos, num_neg, lens, atoms, names=None, planted=None,
+                 max_cost=None, hamming_k=0, neg_type=1):
+        self.num_pos, self.num_neg = int(num_pos), int(num_neg)
+        self.lens = np.ascontiguousarray(lens, dtype=np.uint8)
+        self.atoms = np.ascontiguousarray(atoms, dtype=np.uint64).reshape(-1, self.W)
+        self.names = names or "".join(chr(ord("a") + i) for i in range(self.atoms.shape[0]))
+        self.planted, self.max_cost, self.hamming_k, self.neg_type = planted, max_cost, hamming_k, neg_type
+
+    @property
+    def W(self):
+        return self.num_pos + self.num_neg
+
+    def problem(self):
+        self._names_b = self.names.encode()
+        return Problem(self.num_pos, self.num_neg, self.atoms.shape[0], 0,
+                       self.lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                       _u64p(self.atoms), self._names_b)
+
+
+def _from_gen(g):
+    W = g.num_pos + g.num_neg
+    lens = np.ctypeslib.as_array(g.lens, (W,)).copy()
+    atoms = np.ctypeslib.as_array(g.atoms, (g.num_atoms * W,)).reshape(g.num_atoms, W).copy()
+    inst = Instance(g.num_pos, g.num_neg, lens, atoms, g.names.decode(), g.planted.decode(),
+                    g.max_cost, g.hamming_k, g.neg_type)
+    lib().ltl_gen_free(ctypes.byref(g))
+    return inst
+
+
+def generate(config, seed=1, index=0):
+    """Workload `config` (1..5) from the seeded generator (include/ltl_gen.h)."""
+    g = GenInstance()
+    rc = lib().ltl_gen_config(int(config), ctypes.c_uint64(seed), int(index), ctypes.byref(g))
+    if rc != 0:
+        raise LtlError(f"ltl_gen_config({config}) -> {rc}")
+    return _from_gen(g)
+
+
+def generate_custom(seed, num_pos, num_neg, num_atoms, len_min, len_max, rate=0.5,
+                    planted=None, planted_size=5):
+    g = GenInstance()
+    rc = lib().ltl_gen_custom(ctypes.c_uint64(seed), num_pos, num_neg, num_atoms, len_min, len_max,
+                              float(rate), planted.encode() if planted else None, int(planted_size),
+                              ctypes.byref(g))
+    if rc != 0:
+        raise LtlError(f"ltl_gen_custom -> {rc}")
+    return _from_gen(g)
+
+
+def read_trace_file(path, max_traces=4096):
+    """Reference trace-file format (code/ltli6463.cu:1500-1573)."""
+    lens = np.zeros(max_traces, dtype=np.uint8)
+    atoms = np.zeros(26 * max_traces, dtype=np.uint64)
+    names = ctypes.create_string_buffer(32)
+    P, N, A = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    rc = lib().ltl_read_trace_file(path.encode(), max_traces, ctypes.byref(P), ctypes.byref(N),
+                                   ctypes.byref(A), lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                   _u64p(atoms), names)
+    if rc != 0:
+        raise LtlError(f"{path}: error {rc}")
+    W = P.value + N.value
+    return Instance(P.value, N.value, lens[:W], atoms[:A.value * W].reshape(A.value, W), names.value.decode())
+
+
+def eval_formula(inst, formula):
+    out = np.zeros(inst.W, dtype=np.uint64)
+    pb = inst.problem()
+    rc = lib().ltl_eval_formula(formula.encode(), ctypes.byref(pb), _u64p(out))
+    if rc != 0:
+        raise LtlError(f"cannot parse {formula!r}")
+    return out
+
+
+def mismatches(inst, cs):
+    cs = np.ascontiguousarray(cs, dtype=np.uint64)
+    return lib().ltl_mismatches(_u64p(cs), inst.num_pos, inst.num_neg)
+
+
+def make_options(cost=(1,) * 8, max_cost=10, neg_type=1, hamming_k=0, order=ORDER_PLAIN,
+                 hash_slots_log2=0, precheck=1, capacity=0, budget=0, device=0, profile=0):
+    o = Options()
+    lib().ltl_default_options(ctypes.byref(o))
+    for i, c in enumerate(cost):
+        o.cost[i] = int(c)
+    o.max_cost, o.neg_type, o.hamming_k, o.order = int(max_cost), int(neg_type), int(hamming_k), int(order)
+    o.hash_slots_log2, o.precheck, o.capacity = int(hash_slots_log2), int(precheck), int(capacity)
+    o.hbm_budget_bytes, o.device, o.profile = int(budget), int(device), int(profile)
+    return o
+
+
+class Learner:
+    """A context (ltl_create ... ltl_destroy) with reusable device memory."""
+
+    def __init__(self, budget=1 << 30, **opts):
+        self._opts = dict(budget=budget, **opts)
+        st = ctypes.c_int32(0)
+        o = make_options(**self._opts)
+        self._ctx = lib().ltl_create(ctypes.byref(o), ctypes.byref(st))
+        if not self._ctx:
+            raise LtlError(f"ltl_create failed: {st.value}")
+
+    def close(self):
+        if self._ctx:
+            lib().ltl_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_options(self, **opts):
+        """Apply the constructor's options overridden by `opts` (not sticky)."""
+        o = make_options(**dict(self._opts, **opts))
+        rc = lib().ltl_set_options(self._ctx, ctypes.byref(o))
+        if rc != 0:
+            raise LtlError(f"ltl_set_options -> {rc}")
+
+    def learn(self, inst, **opts):
+        """ltl_learn with the constructor's options overridden by `opts`."""
+        self.set_options(**opts)
+        pb = inst.problem()
+        res = Result()
+        rc = lib().ltl_learn(self._ctx, ctypes.byref(pb), ctypes.byref(res))
+        out = dict(status=res.status, cost=res.cost, all_ltls=res.all_ltls, found_id=res.found_id,
+                   n_stored=res.n_stored, on_the_fly=res.on_the_fly,
+                   formula=(res.formula or b"").decode(), seconds=res.seconds,
+                   kernel_seconds=res.kernel_seconds, hbm_bytes=res.hbm_bytes,
+                   candidates=res.candidates, checked=res.checked,
+                   nodes=[(res.nodes[i].op, res.nodes[i].atom, res.nodes[i].left, res.nodes[i].right)
+                          for i in range(res.n_nodes)])
+        lib().ltl_result_free(ctypes.byref(res))
+        if rc < 0:
+            raise LtlError(f"ltl_learn -> {rc}")
+        return out
+
+    def kernel_stats(self):
+        arr = (KernelStat * 16)()
+        n = lib().ltl_get_kernel_stats(self._ctx, arr, 16)
+        return {arr[i].name.decode(): dict(launches=arr[i].launches, seconds=arr[i].seconds,
+                                           alg_bytes=arr[i].alg_bytes, min_bytes=arr[i].min_bytes,
+                                           items=arr[i].items) for i in range(min(n, 16))}
+
+    def levels(self, max_levels=64):
+        s = np.zeros(max_levels, dtype=np.int64)
+        e = np.zeros(max_levels, dtype=np.int64)
+        n = lib().ltl_levels(self._ctx, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                             e.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), max_levels)
+        return s[:n], e[:n]
+
+    def store(self, first, count, W):
+        cs = np.zeros(count * W, dtype=np.uint64)
+        op = np.zeros(count, dtype=np.uint8)
+        left = np.zeros(count, dtype=np.int32)
+        right = np.zeros(count, dtype=np.int32)
+        rc = lib().ltl_store_read(self._ctx, first, count, _u64p(cs),
+                                  op.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
+                                  left.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                  right.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        if rc != 0:
+            raise LtlError(f"ltl_store_read -> {rc}")
+        return cs.reshape(count, W), op, left, right
+
+
+def _instance_from_tokens(pos, neg, alphabet):
+    """Encode the reference's token traces (code/ltli6463.cu:1100-1133)."""
+    names = "".join(sorted(set(alphabet)))
+    traces = list(pos) + list(neg)
+    W = len(traces)
+    lens = np.array([len(t) for t in traces], dtype=np.uint8)
+    atoms = np.zeros((len(names), W), dtype=np.uint64)
+    for a, ch in enumerate(names):
+        for i, tr in enumerate(traces):
+            w = 0
+            for k, tok in enumerate(tr):
+                if ch in tok:
+                    w |= 1 << k
+            atoms[a, i] = w
+    return Instance(len(pos), len(neg), lens, atoms, names)
+
+
+def LTLI(costFun, maxCost, RlxUnqChkTyp, NegType, alphabet, pos, neg, learner=None):
+    """Drop-in for the reference's LTLI() (code/ltli6463.cu:1043-1052).
+
+    pos/neg: lists of traces, a trace a list of tokens, a token a string of
+    the letters holding at that position.  Returns (formula, LTLcost,
+    allLTLs) where formula is "not_found" or "see_the_error" on failure, as
+    in the reference.  RlxUnqChkTyp is accepted for signature parity; the
+    device path deduplicates exactly (the reference's relaxed keys only
+    matter above 126 total trace positions, code/ltli6463.cu:66-121).
+    """
+    inst = _instance_from_tokens(pos, neg, alphabet)
+    if inst.W == 0 or int(inst.lens.max(initial=0)) > 64:
+        return "see_the_error", int(costFun[0]), 0
+    own = learner is None
+    L = learner or Learner(budget=1 << 30)
+    try:
+        r = L.learn(inst, cost=tuple(costFun), max_cost=maxCost, neg_type=NegType)
+    except LtlError:
+        return "see_the_error", int(costFun[0]), 0
+    finally:
+        if own:
+            L.close()
+    return r["formula"], r["cost"], r["all_ltls"]

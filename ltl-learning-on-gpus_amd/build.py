@@ -1,0 +1,65 @@
+"""Build the HIP library (gfx950) and the CLI in-tree.
+
+    python ltl-learning-on-gpus_amd/build.py
+
+Outputs (git-ignored, shipped with the tree):
+    ltl-learning-on-gpus_amd/libltl_amd.so   C ABI of include/ltl_amd.h + ltl_gen.h
+    ltl-learning-on-gpus_amd/ltli            reference-compatible CLI
+"""
+import os
+import shutil
+import subprocess
+import sys
+import concurrent.futures as cf
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libltl_amd.so")
+CLI = os.path.join(HERE, "ltli")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed FP32 instructions are disabled for every HIP translation unit.
+COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+          "-Wno-unused-command-line-argument", "-I", os.path.join(ROOT, "include")]
+SOURCES = ["kernels.hip", "learn.cpp", "host_util.cpp"]
+
+
+def _compile(src):
+    out = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + COMMON + ["-c", os.path.join(CSRC, src), "-o", out + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile {src} failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build(verbose=False):
+    os.makedirs(OBJ, exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=4) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(LIB + ".tmp", LIB)
+    cli_src = os.path.join(CSRC, "ltli_main.cpp")
+    if os.path.exists(cli_src):
+        cmd = [HIPCC, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), cli_src,
+               "-o", CLI + ".tmp", "-L", HERE, "-lltl_amd", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"cli failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(CLI + ".tmp", CLI)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,493 @@
+// kernels.hip -- characteristic-set enumeration kernels for gfx950 (wave64).
+//
+// Per chunk of candidates the host enqueues, on one stream:
+//   gen     compute every candidate set word by word, fold it into two NH
+//           fingerprints and the target check, test it (atomicMin of the
+//           canonical id of accepted candidates), and intern fingerprint 1 in
+//           the open-addressing table (64-bit atomicCAS on the key, atomicMin
+//           of the candidate id on the value);
+//   decide  a candidate survives iff the table value still holds its own id;
+//           duplicates are verified against fingerprint 2 of the winner;
+//   scan    survivor counts per 64-candidate tile -> store offsets;
+//   finish  advance the store count, detect overflow;
+//   emit    recompute the survivors and append them, in canonical order,
+//           to the store; commit their table values to store indices.
+// This replaces the reference's per-thread operator kernels + Thrust
+// compaction (code/ltli6463.cu:176-873, 994-1035).  Operators follow
+// code/ltli6463.cu:203, 295, 385, 468, 551-556, 638-645, 739-771, applied
+// to B-bit trace slots: every shift is masked so that no bit crosses a slot.
+#include <hipcub/hipcub.hpp>
+
+#include "../../include/ltl_amd.h"
+#include "kernels.hpp"
+
+namespace ltl {
+namespace {
+
+template <int B>
+__host__ __device__ constexpr uint64_t rep(uint64_t per) {
+    uint64_t m = 0;
+    for (int j = 0; j < 64 / B; ++j) m |= per << (j * B);
+    return m;
+}
+
+// All bits of each B-bit slot except its top s bits (0 < s < B).
+template <int B>
+__host__ __device__ constexpr uint64_t keep(int s) {
+    return B == 64 ? (~0ull >> s) : rep<B>((1ull << (B - s)) - 1ull);
+}
+
+template <int B>
+__device__ __forceinline__ uint64_t f_fin(uint64_t x) {
+#pragma unroll
+    for (int s = 1; s < B; s <<= 1) x |= (x >> s) & keep<B>(s);
+    return x;
+}
+
+template <int B>
+__device__ __forceinline__ uint64_t f_until(uint64_t l, uint64_t r) {
+    uint64_t run = l, y = r;
+#pragma unroll
+    for (int s = 1; s < B; s <<= 1) {
+        y |= run & ((y >> s) & keep<B>(s));
+        run &= (run >> s) & keep<B>(s);
+    }
+    return y;
+}
+
+template <int OP, int B>
+__device__ __forceinline__ uint64_t apply1(uint64_t x, uint64_t lm) {
+    if constexpr (OP == LTL_NOT) return ~x & lm;
+    else if constexpr (OP == LTL_NEXT) return (x >> 1) & keep<B>(1);
+    else if constexpr (OP == LTL_FIN) return f_fin<B>(x);
+    else return x & ~f_fin<B>(~x & lm);
+}
+
+// NH compression step (Black et al., UMAC, CRYPTO 1999): the two 32-bit
+// halves plus key halves, multiplied to 64 bits, summed mod 2^64.
+__device__ __forceinline__ uint64_t nh(uint64_t c, uint64_t k) {
+    const uint32_t lo = (uint32_t)c + (uint32_t)k;
+    const uint32_t hi = (uint32_t)(c >> 32) + (uint32_t)(k >> 32);
+    return (uint64_t)lo * (uint64_t)hi;
+}
+
+struct Acc {
+    uint64_t f1 = 0, f2 = 0;
+    uint32_t mism = 0;
+};
+
+__device__ __forceinline__ void fold(Acc &a, uint64_t c, const WordAux &x) {
+    a.f1 += nh(c, x.k1);
+    a.f2 += nh(c, x.k2);
+    a.mism += (uint32_t)__popcll((c ^ x.tgt) & x.chk);
+}
+
+// Table key from fingerprint 1: a 64-bit finalizer spreads the NH sum over
+// all bits (slot index = low bits); the empty key is never produced.
+__device__ __forceinline__ uint64_t table_key(uint64_t f) {
+    f ^= f >> 33; f *= 0xff51afd7ed558ccdull;
+    f ^= f >> 33; f *= 0xc4ceb9fe1a85ec53ull;
+    f ^= f >> 33;
+    return f == kEmpty ? kEmpty - 2 : f;
+}
+
+// Linear probing; key CAS from empty, then atomicMin of the value.
+__device__ uint32_t tab_insert(uint64_t *tab, uint32_t tmask, uint64_t key, uint64_t val, DevState *st) {
+    uint32_t p = (uint32_t)key & tmask;
+    for (uint32_t n = 0; n <= tmask; ++n) {
+        unsigned long long *kp = reinterpret_cast<unsigned long long *>(tab + 2 * (size_t)p);
+        unsigned long long k = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == kEmpty) {
+            k = atomicCAS(kp, (unsigned long long)kEmpty, (unsigned long long)key);
+            if (k == kEmpty) k = key;
+        }
+        if (k == key) {
+            atomicMin(kp + 1, (unsigned long long)val);
+            return p;
+        }
+        p = (p + 1) & tmask;
+    }
+    st->probe_fail = 1;
+    return 0;
+}
+
+__device__ __forceinline__ size_t row0(int64_t s, int nw) {
+    return (size_t)(s >> 6) * ((size_t)nw << 6) + (size_t)(s & 63);
+}
+
+// Workgroups of a binary chunk are laid out so that consecutive workgroups
+// dispatched to one XCD (dispatch is round-robin over the 8 XCDs) walk the
+// left rows of the same right stripe, which then stays in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+    const int q = nb / 8, r = nb % 8, x = bid % 8, j = bid / 8;
+    return x < r ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+}
+
+__device__ __forceinline__ bool run_stopped(const ChunkArgs &A) {
+    return __hip_atomic_load(&A.st->found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kEmpty;
+}
+__device__ __forceinline__ bool chunk_otf(const ChunkArgs &A) {
+    return __hip_atomic_load(&A.st->overflow_chunk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < A.chunk_idx;
+}
+
+struct Tile {
+    int64_t a, b;      // binary: left row, right column (block-local); unary: operand
+    int64_t ti;        // tile index in the chunk
+    bool valid;
+};
+
+__device__ __forceinline__ bool tile_binary(const ChunkArgs &A, Tile &t) {
+    const int nA = (int)(A.a1 - A.a0), nAq = (nA + 3) >> 2;
+    const int lw = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int bs = lw / nAq, aq = lw % nAq;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    t.a = A.a0 + 4 * aq + wave;
+    if (t.a >= A.a1) return false;
+    t.b = A.b0 + 64 * (int64_t)bs + (threadIdx.x & 63);
+    t.valid = t.b < A.b1;
+    t.ti = (t.a - A.a0) * A.nbs + bs;
+    return true;
+}
+
+__device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t) {
+    t.ti = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t.ti >= A.n_tiles) return false;
+    t.a = A.a0 + 64 * t.ti + (threadIdx.x & 63);
+    t.b = 0;
+    t.valid = t.a < A.a1;
+    return true;
+}
+
+// ---------------------------------------------------------------------------
+// gen
+// ---------------------------------------------------------------------------
+template <int OP, int B>
+__global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_binary(A, t)) return;
+    const bool otf = chunk_otf(A);
+    const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
+    const uint64_t *pl = A.store + row0(sl, A.nw);
+    const uint64_t *pr = A.store + row0(sr, A.nw);
+    Acc c0, c1;
+    for (int w = 0; w < A.nw; ++w) {
+        const WordAux x = A.aux[w];
+        const uint64_t l = pl[(size_t)w << 6], r = pr[(size_t)w << 6];
+        if constexpr (OP == LTL_AND) fold(c0, l & r, x);
+        else if constexpr (OP == LTL_OR) fold(c0, l | r, x);
+        else {
+            fold(c0, f_until<B>(l, r), x);
+            fold(c1, f_until<B>(r, l), x);
+        }
+    }
+    if (!t.valid) return;
+    const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
+    const uint64_t id0 = A.id_base + (OP == LTL_UNTIL ? 2 * pair : pair);
+    if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
+    if (OP == LTL_UNTIL && c1.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
+    if (otf) return;
+    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
+    if constexpr (OP == LTL_UNTIL) {
+        A.t_slot[2 * ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, A.st);
+        A.t_fp2[2 * ix] = c0.f2;
+        A.t_slot[2 * ix + 1] = tab_insert(A.tab, A.tmask, table_key(c1.f1), kPending | (id0 + 1), A.st);
+        A.t_fp2[2 * ix + 1] = c1.f2;
+    } else {
+        A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, A.st);
+        A.t_fp2[ix] = c0.f2;
+    }
+}
+
+template <int OP, int B>
+__global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_unary(A, t)) return;
+    const bool otf = chunk_otf(A);
+    const int64_t s = A.ls + (t.valid ? t.a : A.a0);
+    const uint64_t *p = A.store + row0(s, A.nw);
+    Acc c0;
+    for (int w = 0; w < A.nw; ++w) {
+        const WordAux x = A.aux[w];
+        fold(c0, apply1<OP, B>(p[(size_t)w << 6], x.lm), x);
+    }
+    if (!t.valid) return;
+    const uint64_t id0 = A.id_base + (uint64_t)t.a;
+    if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
+    if (otf) return;
+    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
+    A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, A.st);
+    A.t_fp2[ix] = c0.f2;
+}
+
+// Atoms: interned with committed values (store index = atom index), so that
+// the first of equal atoms wins (code/ltli6463.cu:144-173).
+template <int B>
+__global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms) {
+    const int a = (int)threadIdx.x;
+    if (a >= n_atoms) return;
+    const uint64_t *p = A.store + row0(a, A.nw);
+    Acc c0;
+    for (int w = 0; w < A.nw; ++w) fold(c0, p[(size_t)w << 6], A.aux[w]);
+    tab_insert(A.tab, A.tmask, table_key(c0.f1), (uint64_t)a, A.st);
+    A.m_fp2[a] = c0.f2;
+    A.m_op[a] = LTL_ATOM;
+    A.m_left[a] = -1;
+    A.m_right[a] = -1;
+}
+
+// ---------------------------------------------------------------------------
+// decide
+// ---------------------------------------------------------------------------
+// Temp index of a pending candidate id of this chunk (its tile and lane).
+__device__ __forceinline__ int64_t temp_index_of(const ChunkArgs &A, int op, uint64_t id) {
+    const uint64_t rel = id - A.id_base;
+    if (op == LTL_AND || op == LTL_OR || op == LTL_UNTIL) {
+        const uint64_t pair = op == LTL_UNTIL ? rel >> 1 : rel;
+        const int64_t a = (int64_t)(pair / (uint64_t)A.nR), b = (int64_t)(pair % (uint64_t)A.nR);
+        if (a < A.a0 || a >= A.a1 || b < A.b0 || b >= A.b1) return -1;
+        const int64_t ti = (a - A.a0) * A.nbs + (b - A.b0) / 64;
+        const int64_t ix = ti * 64 + (b - A.b0) % 64;
+        return op == LTL_UNTIL ? 2 * ix + (int64_t)(rel & 1) : ix;
+    }
+    const int64_t a = (int64_t)rel;
+    if (a < A.a0 || a >= A.a1) return -1;
+    return a - A.a0;
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_decide(ChunkArgs A) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    const int64_t ti = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ti >= A.n_tiles) return;
+    const int lane = threadIdx.x & 63;
+    constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
+    constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
+    bool valid;
+    uint64_t id0;
+    if (bin) {
+        const int64_t a = A.a0 + ti / A.nbs, b = A.b0 + (ti % A.nbs) * 64 + lane;
+        valid = b < A.b1;
+        const uint64_t pair = (uint64_t)a * (uint64_t)A.nR + (uint64_t)b;
+        id0 = A.id_base + ND * pair;
+    } else {
+        const int64_t a = A.a0 + ti * 64 + lane;
+        valid = a < A.a1;
+        id0 = A.id_base + (uint64_t)a;
+    }
+    const size_t ix = (size_t)ti * 64 + lane;
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        bool surv = false;
+        if (valid) {
+            const size_t k = ND * ix + d;
+            const uint32_t slot = A.t_slot[k];
+            const uint64_t f2 = A.t_fp2[k];
+            const uint64_t v = A.tab[2 * (size_t)slot + 1];
+            const uint64_t id = id0 + d;
+            if (v == (kPending | id)) {
+                surv = true;
+            } else if (v < kPending) {
+                if (A.m_fp2[v] != f2) A.st->collision = 1;
+            } else {
+                const int64_t o = temp_index_of(A, OP, v & ~kPending);
+                if (o < 0 || A.t_fp2[o] != f2) A.st->collision = 1;
+            }
+        }
+        const uint64_t m = __ballot(surv);
+        cnt += (uint32_t)__popcll(m);
+        if (lane == 0) A.t_mask[ND * ti + d] = m;
+    }
+    if (lane == 0) A.t_cnt[ti] = cnt;
+}
+
+__global__ void k_finish(ChunkArgs A) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    const int64_t n = A.n_tiles;
+    const long long total = (long long)A.t_off[n - 1] + (long long)A.t_cnt[n - 1];
+    const long long base = A.st->n_stored;
+    A.st->chunk_base = base;
+    A.st->chunk_total = total;
+    const long long room = A.capacity - base;
+    if (total > room) {
+        if (A.st->overflow_chunk > A.chunk_idx) A.st->overflow_chunk = A.chunk_idx;
+        A.st->dropped += (unsigned long long)(total - room);
+        A.st->n_stored = A.capacity;
+    } else {
+        A.st->n_stored = base + total;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// emit
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t lanes_below() {
+    const int lane = threadIdx.x & 63;
+    return lane ? (~0ull >> (64 - lane)) : 0ull;
+}
+
+__device__ __forceinline__ void commit(const ChunkArgs &A, int64_t pos, size_t k, int op, int32_t l, int32_t r) {
+    A.m_op[pos] = (uint8_t)op;
+    A.m_left[pos] = l;
+    A.m_right[pos] = r;
+    A.m_fp2[pos] = A.t_fp2[k];
+    A.tab[2 * (size_t)A.t_slot[k] + 1] = (uint64_t)pos;
+}
+
+template <int OP, int B>
+__global__ __launch_bounds__(256) void k_emit2(ChunkArgs A) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    Tile t;
+    if (!tile_binary(A, t)) return;
+    if (A.t_cnt[t.ti] == 0) return;
+    constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
+    const uint64_t below = lanes_below();
+    const uint64_t bit = 1ull << (threadIdx.x & 63);
+    const uint64_t m0 = A.t_mask[ND * t.ti], m1 = ND == 2 ? A.t_mask[ND * t.ti + 1] : 0;
+    const int64_t base = A.st->chunk_base + (int64_t)A.t_off[t.ti];
+    const int64_t pos0 = base + __popcll(m0 & below) + __popcll(m1 & below);
+    const int64_t pos1 = pos0 + ((m0 & bit) ? 1 : 0);
+    const bool w0 = (m0 & bit) && pos0 < A.capacity;
+    const bool w1 = (m1 & bit) && pos1 < A.capacity;
+    const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
+    const uint64_t *pl = A.store + row0(sl, A.nw);
+    const uint64_t *pr = A.store + row0(sr, A.nw);
+    uint64_t *q0 = A.store + row0(w0 ? pos0 : 0, A.nw);
+    uint64_t *q1 = A.store + row0(w1 ? pos1 : 0, A.nw);
+    for (int w = 0; w < A.nw; ++w) {
+        const uint64_t l = pl[(size_t)w << 6], r = pr[(size_t)w << 6];
+        if constexpr (OP == LTL_AND) { if (w0) q0[(size_t)w << 6] = l & r; }
+        else if constexpr (OP == LTL_OR) { if (w0) q0[(size_t)w << 6] = l | r; }
+        else {
+            if (w0) q0[(size_t)w << 6] = f_until<B>(l, r);
+            if (w1) q1[(size_t)w << 6] = f_until<B>(r, l);
+        }
+    }
+    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
+    if (w0) commit(A, pos0, ND * ix, OP, (int32_t)sl, (int32_t)sr);
+    if (w1) commit(A, pos1, ND * ix + 1, OP, (int32_t)sr, (int32_t)sl);
+}
+
+template <int OP, int B>
+__global__ __launch_bounds__(256) void k_emit1(ChunkArgs A) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    Tile t;
+    if (!tile_unary(A, t)) return;
+    if (A.t_cnt[t.ti] == 0) return;
+    const uint64_t m0 = A.t_mask[t.ti];
+    const uint64_t bit = 1ull << (threadIdx.x & 63);
+    const int64_t pos = A.st->chunk_base + (int64_t)A.t_off[t.ti] + __popcll(m0 & lanes_below());
+    const bool w0 = (m0 & bit) && pos < A.capacity;
+    const int64_t s = A.ls + (t.valid ? t.a : A.a0);
+    const uint64_t *p = A.store + row0(s, A.nw);
+    uint64_t *q = A.store + row0(w0 ? pos : 0, A.nw);
+    for (int w = 0; w < A.nw; ++w) {
+        const uint64_t c = apply1<OP, B>(p[(size_t)w << 6], A.aux[w].lm);
+        if (w0) q[(size_t)w << 6] = c;
+    }
+    if (w0) commit(A, pos, (size_t)t.ti * 64 + (threadIdx.x & 63), OP, (int32_t)s, -1);
+}
+
+__global__ void k_fill(uint64_t *p, uint64_t v, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+template <template <int, int> class K>
+struct Nothing {};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static dim3 grid_binary(const ChunkArgs &a) {
+    const int64_t nAq = (a.a1 - a.a0 + 3) / 4;
+    return dim3((unsigned)(nAq * a.nbs));
+}
+static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles + 3) / 4)); }
+
+#define LTL_DISPATCH_B(KER, OP, B, GRID, BLOCK, S, ...)                                    \
+    switch (B) {                                                                            \
+    case 8: hipLaunchKernelGGL((KER<OP, 8>), GRID, BLOCK, 0, S, __VA_ARGS__); break;        \
+    case 16: hipLaunchKernelGGL((KER<OP, 16>), GRID, BLOCK, 0, S, __VA_ARGS__); break;      \
+    case 32: hipLaunchKernelGGL((KER<OP, 32>), GRID, BLOCK, 0, S, __VA_ARGS__); break;      \
+    default: hipLaunchKernelGGL((KER<OP, 64>), GRID, BLOCK, 0, S, __VA_ARGS__); break;      \
+    }
+
+hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s) {
+    switch (op) {
+    case LTL_NOT: LTL_DISPATCH_B(k_gen1, LTL_NOT, B, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_NEXT: LTL_DISPATCH_B(k_gen1, LTL_NEXT, B, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_FIN: LTL_DISPATCH_B(k_gen1, LTL_FIN, B, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_GLOB: LTL_DISPATCH_B(k_gen1, LTL_GLOB, B, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_AND: LTL_DISPATCH_B(k_gen2, LTL_AND, B, grid_binary(a), dim3(256), s, a); break;
+    case LTL_OR: LTL_DISPATCH_B(k_gen2, LTL_OR, B, grid_binary(a), dim3(256), s, a); break;
+    case LTL_UNTIL: LTL_DISPATCH_B(k_gen2, LTL_UNTIL, B, grid_binary(a), dim3(256), s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
+    switch (op) {
+    case LTL_NOT: LTL_DISPATCH_B(k_emit1, LTL_NOT, B, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_NEXT: LTL_DISPATCH_B(k_emit1, LTL_NEXT, B, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_FIN: LTL_DISPATCH_B(k_emit1, LTL_FIN, B, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_GLOB: LTL_DISPATCH_B(k_emit1, LTL_GLOB, B, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_AND: LTL_DISPATCH_B(k_emit2, LTL_AND, B, grid_binary(a), dim3(256), s, a); break;
+    case LTL_OR: LTL_DISPATCH_B(k_emit2, LTL_OR, B, grid_binary(a), dim3(256), s, a); break;
+    case LTL_UNTIL: LTL_DISPATCH_B(k_emit2, LTL_UNTIL, B, grid_binary(a), dim3(256), s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_decide(int op, const ChunkArgs &a, hipStream_t s) {
+    const dim3 g = grid_tiles(a), b(256);
+    switch (op) {
+    case LTL_NOT: hipLaunchKernelGGL(k_decide<LTL_NOT>, g, b, 0, s, a); break;
+    case LTL_NEXT: hipLaunchKernelGGL(k_decide<LTL_NEXT>, g, b, 0, s, a); break;
+    case LTL_FIN: hipLaunchKernelGGL(k_decide<LTL_FIN>, g, b, 0, s, a); break;
+    case LTL_GLOB: hipLaunchKernelGGL(k_decide<LTL_GLOB>, g, b, 0, s, a); break;
+    case LTL_AND: hipLaunchKernelGGL(k_decide<LTL_AND>, g, b, 0, s, a); break;
+    case LTL_OR: hipLaunchKernelGGL(k_decide<LTL_OR>, g, b, 0, s, a); break;
+    case LTL_UNTIL: hipLaunchKernelGGL(k_decide<LTL_UNTIL>, g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finish(const ChunkArgs &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s) {
+    switch (B) {
+    case 8: hipLaunchKernelGGL(k_atoms<8>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
+    case 16: hipLaunchKernelGGL(k_atoms<16>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
+    case 32: hipLaunchKernelGGL(k_atoms<32>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
+    default: hipLaunchKernelGGL(k_atoms<64>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s) {
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill, dim3(blocks ? blocks : 1), dim3(256), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+size_t scan_temp_bytes(int n_max) {
+    size_t bytes = 0;
+    hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, (const uint32_t *)nullptr, (uint32_t *)nullptr, n_max);
+    return bytes;
+}
+
+hipError_t launch_scan(const uint32_t *in, uint32_t *out, int n, void *temp, size_t temp_bytes, hipStream_t s) {
+    return hipcub::DeviceScan::ExclusiveSum(temp, temp_bytes, in, out, n, s);
+}
+
+}  // namespace ltl

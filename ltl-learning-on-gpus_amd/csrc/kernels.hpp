@@ -1,0 +1,85 @@
+// kernels.hpp -- device data layout and launch interface of the enumeration
+// kernels (gfx950, wave64).
+//
+// Store layout ("stripes").  A characteristic set (CS) over W traces is held
+// as nw packed words: trace i sits in slot i % TPW of word i / TPW, where a
+// slot is B bits (B = 8, 16, 32 or 64, the smallest power of two >= the
+// longest trace) and TPW = 64 / B.  Sets are grouped 64 to a stripe; inside
+// a stripe the layout is word-major, so word w of the 64 sets of a stripe is
+// one contiguous 512-byte row:  addr(s, w) = (s/64)*nw*64 + w*64 + s%64.
+// One lane handles one candidate; a wave reads or writes a full row.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ltl {
+
+constexpr uint64_t kEmpty = ~0ull;          // empty table key / value, no result
+constexpr uint64_t kPending = 1ull << 62;   // table value tag: candidate id
+
+// Per packed word: dedup hash keys and check masks (wave-uniform).
+struct WordAux {
+    uint64_t k1, k2;  // NH keys of the two fingerprints
+    uint64_t lm;      // bits inside the traces (length mask)
+    uint64_t chk;     // position 0 of every valid slot
+    uint64_t tgt;     // position 0 of every positive slot
+    uint64_t pad[3];
+};
+
+// Device-resident run state (one per context).
+struct DevState {
+    unsigned long long found;     // minimal accepted candidate id, kEmpty = none
+    long long n_stored;           // sets in the store
+    long long chunk_base;         // store index of the current chunk's first survivor
+    long long chunk_total;        // survivors of the current chunk
+    long long overflow_chunk;     // first chunk that overflowed the store
+    unsigned long long dropped;   // survivors not stored (overflow)
+    int collision;                // fingerprint collision detected
+    int probe_fail;               // table full
+    unsigned long long pad[2];
+};
+
+// One chunk of one (operator, split) block, in canonical order.  Binary:
+// rows a in [a0, a1) of the left level, columns b in [b0, b1) of the right
+// level (block-local); unary: operands a in [a0, a1), b0 = b1 = 0.
+struct ChunkArgs {
+    uint64_t *store;
+    const WordAux *aux;
+    uint64_t *tab;            // 2 words per slot: key, value
+    DevState *st;
+    uint32_t *t_slot;         // per candidate: table slot
+    uint64_t *t_fp2;          // per candidate: verification fingerprint
+    uint64_t *t_mask;         // per tile: survivor masks (2 for U)
+    uint32_t *t_cnt;          // per tile: survivors
+    uint32_t *t_off;          // per tile: exclusive prefix of t_cnt
+    uint8_t *m_op;
+    int32_t *m_left, *m_right;
+    uint64_t *m_fp2;
+    int64_t capacity;
+    int64_t chunk_idx;
+    uint64_t id_base;         // canonical id of the block's first candidate
+    int64_t ls, rs;           // store index of the left / right level start
+    int64_t a0, a1, b0, b1;
+    int64_t nR;               // right level size (binary), 1 (unary)
+    int32_t nw;
+    int32_t hamming_k;
+    uint32_t tmask;           // table slots - 1
+    int32_t nbs;              // 64-wide column stripes per row in the chunk
+    int32_t n_tiles;          // tiles in the chunk
+    int32_t pad_;
+};
+
+// Launchers (stream-ordered, asynchronous).
+hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s);
+hipError_t launch_decide(int op, const ChunkArgs &a, hipStream_t s);
+hipError_t launch_finish(const ChunkArgs &a, hipStream_t s);
+hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s);
+hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s);
+// Exclusive scan of t_cnt into t_off (n elements); temp sized by scan_temp_bytes.
+size_t scan_temp_bytes(int n_max);
+hipError_t launch_scan(const uint32_t *in, uint32_t *out, int n, void *temp, size_t temp_bytes,
+                       hipStream_t s);
+
+}  // namespace ltl

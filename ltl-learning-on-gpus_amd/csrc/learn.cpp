@@ -1,0 +1,698 @@
+// learn.cpp -- host scheduler behind the C ABI (include/ltl_amd.h).
+//
+// Mirrors the reference's LTLI() (code/ltli6463.cu:1043-1497): atoms first,
+// then cost levels c1+1..maxCost, within a level the operators in the order
+// ~, &, |, X, F, G, U, binary operators over the splits i = c1..(c-cop)/2
+// (left cost i, right cost c-cop-i), overflow into check-only mode and the
+// "last round" rule.  Differences by design:
+//   - device memory is one pool per context, reused across calls;
+//   - kernels of a level are enqueued back to back; the host synchronises
+//     once per level (the reference copies the result index back after
+//     every launch, code/ltli6463.cu:1232 etc.);
+//   - the result is the accepted candidate with the smallest canonical id
+//     (deterministic; the reference keeps the first atomicCAS winner,
+//     code/ltli6463.cu:243).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "../../include/ltl_amd.h"
+#include "formula.hpp"
+#include "kernels.hpp"
+
+using namespace ltl;
+
+namespace {
+
+enum { K_GEN1, K_GEN2, K_GENU, K_DECIDE, K_SCAN, K_EMIT1, K_EMIT2, K_EMITU, K_NKIND };
+const char *kKindName[K_NKIND] = {"gen_unary", "gen_and_or", "gen_until", "decide",
+                                  "scan_finish", "emit_unary", "emit_and_or", "emit_until"};
+
+struct Block {
+    uint64_t id_base, size;
+    int op;
+    int64_t ls, nL, rs, nR;
+};
+
+struct PendingEvent {
+    int kind;
+    hipEvent_t a, b;
+    uint64_t alg, min, items;
+};
+
+uint64_t splitmix(uint64_t x) {
+    uint64_t z = x + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct ltl_ctx {
+    ltl_options opt{};
+    hipStream_t stream = nullptr;
+    char *pool = nullptr;
+    size_t pool_bytes = 0;
+    DevState *h_state = nullptr;  // pinned
+
+    // layout of the last call
+    int W = 0, P = 0, A = 0, nw = 0, B = 64, TPW = 1;
+    int64_t capacity = 0;
+    int64_t CH = 0;  // candidate slots per chunk
+    uint32_t tmask = 0;
+    WordAux *d_aux = nullptr;
+    DevState *d_state = nullptr;
+    uint64_t *d_store = nullptr, *d_tab = nullptr, *d_fp2 = nullptr;
+    uint8_t *d_op = nullptr;
+    int32_t *d_left = nullptr, *d_right = nullptr;
+    uint32_t *t_slot = nullptr, *t_cnt = nullptr, *t_off = nullptr;
+    uint64_t *t_fp2 = nullptr, *t_mask = nullptr;
+    void *d_scan = nullptr;
+    size_t scan_bytes = 0;
+    std::vector<int64_t> lvl_s, lvl_e;
+
+    // profiling
+    ltl_kernel_stat stats[K_NKIND]{};
+    std::vector<PendingEvent> pending;
+    std::vector<hipEvent_t> free_events;
+    int64_t chunk_counter = 0;
+};
+
+static int32_t hip_fail(hipError_t e) { return e == hipErrorOutOfMemory ? LTL_E_OOM : LTL_E_HIP; }
+#define HIP_TRY(x)                                   \
+    do {                                             \
+        hipError_t e_ = (x);                         \
+        if (e_ != hipSuccess) return hip_fail(e_);   \
+    } while (0)
+
+extern "C" void ltl_default_options(ltl_options *o) {
+    memset(o, 0, sizeof(*o));
+    for (auto &c : o->cost) c = 1;
+    o->max_cost = 10;
+    o->neg_type = 1;
+    o->dedup_mode = LTL_DEDUP_EXACT;
+    o->order = LTL_ORDER_PLAIN;
+    o->precheck = 1;
+}
+
+static int32_t check_options(const ltl_options *o) {
+    for (int i = 0; i < 8; ++i)
+        if (o->cost[i] < 1 || o->cost[i] > SHRT_MAX) return LTL_E_ARG;
+    if (o->max_cost < 1 || o->max_cost > SHRT_MAX) return LTL_E_ARG;
+    if (o->neg_type < 1 || o->neg_type > 2) return LTL_E_ARG;
+    if (o->dedup_mode != LTL_DEDUP_EXACT) return LTL_E_ARG;  // reference key modes: not on the device path
+    if (o->hamming_k < 0) return LTL_E_ARG;
+    if (o->hash_slots_log2 < 0 || o->hash_slots_log2 > 31) return LTL_E_ARG;
+    return 0;
+}
+
+extern "C" ltl_ctx *ltl_create(const ltl_options *opt, int32_t *status) {
+    auto fail = [&](int32_t s) -> ltl_ctx * { if (status) *status = s; return nullptr; };
+    ltl_options o;
+    if (opt) o = *opt; else ltl_default_options(&o);
+    if (check_options(&o)) return fail(LTL_E_ARG);
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(LTL_E_NODEVICE);
+    if (o.device < 0 || o.device >= n) return fail(LTL_E_ARG);
+    if (hipSetDevice(o.device) != hipSuccess) return fail(LTL_E_HIP);
+    auto *c = new ltl_ctx;
+    c->opt = o;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_state, sizeof(DevState), hipHostMallocDefault) != hipSuccess) {
+        ltl_destroy(c);
+        return fail(LTL_E_HIP);
+    }
+    for (int k = 0; k < K_NKIND; ++k) snprintf(c->stats[k].name, sizeof(c->stats[k].name), "%s", kKindName[k]);
+    if (status) *status = 0;
+    return c;
+}
+
+extern "C" int32_t ltl_set_options(ltl_ctx *c, const ltl_options *o) {
+    if (!c || !o || check_options(o)) return LTL_E_ARG;
+    if (o->device != c->opt.device) return LTL_E_ARG;
+    c->opt = *o;
+    return 0;
+}
+
+extern "C" void ltl_destroy(ltl_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->opt.device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
+    for (auto e : c->free_events) hipEventDestroy(e);
+    if (c->pool) hipFree(c->pool);
+    if (c->h_state) hipHostFree(c->h_state);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" void ltl_result_free(ltl_result *r) {
+    if (!r) return;
+    free(r->formula);
+    free(r->nodes);
+    r->formula = nullptr;
+    r->nodes = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Pool: reserved once per context at the budget, carved per call.
+// ---------------------------------------------------------------------------
+static int32_t ensure_pool(ltl_ctx *c) {
+    size_t want = c->opt.hbm_budget_bytes;
+    if (!want) {
+        size_t fr = 0, tot = 0;
+        HIP_TRY(hipMemGetInfo(&fr, &tot));
+        want = fr / 4 * 3;
+    }
+    want = align_up(want, 1 << 21);
+    if (c->pool && c->pool_bytes >= want) return 0;
+    if (c->pool) { hipFree(c->pool); c->pool = nullptr; c->pool_bytes = 0; }
+    HIP_TRY(hipMalloc((void **)&c->pool, want));
+    c->pool_bytes = want;
+    return 0;
+}
+
+static int32_t carve(ltl_ctx *c) {
+    const size_t budget = c->pool_bytes;
+    const size_t per_cs = (size_t)c->nw * 8 + 17;
+    // chunk: per-candidate temporaries (14 B each) within 1/32 of the pool
+    int64_t ch = 1 << 26;
+    while (ch > 4096 && (size_t)ch * 14 > budget / 32) ch >>= 1;
+    c->CH = ch;
+    const size_t tiles = (size_t)ch / 64;
+    c->scan_bytes = scan_temp_bytes((int)tiles);
+    size_t fixed = align_up((size_t)c->nw * sizeof(WordAux), 256) + align_up(sizeof(DevState), 256) +
+                   align_up((size_t)ch * 4, 256) + align_up((size_t)ch * 8, 256) + align_up(tiles * 16, 256) +
+                   2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) + 8 * 256;
+    if (fixed >= budget) return LTL_E_OOM;
+    const size_t R = budget - fixed;
+    // Table: load stays <= 1/2 with a full chunk pending (cap + ch <= slots/2).
+    // Without a fixed size, start from the largest table within half the
+    // remaining pool and halve while the smaller table still holds every set
+    // the pool can store.
+    size_t slots;
+    if (c->opt.hash_slots_log2) {
+        slots = (size_t)1 << c->opt.hash_slots_log2;
+        if (slots * 16 >= R) return LTL_E_OOM;
+    } else {
+        slots = (size_t)1 << 31;
+        while (slots > 4096 && slots * 16 > R / 2) slots >>= 1;
+        while (slots > 4096) {
+            const size_t h = slots / 2;
+            const int64_t cap_h = (int64_t)((R - h * 16) / per_cs);
+            if ((int64_t)(h / 2) - ch < cap_h) break;
+            slots = h;
+        }
+    }
+    int64_t cap = std::min<int64_t>((int64_t)((R - slots * 16) / per_cs), (int64_t)(slots / 2) - ch);
+    cap = std::min<int64_t>(cap, (int64_t)INT32_MAX - 64) / 64 * 64;
+    if (cap < 64) return LTL_E_OOM;
+    if (c->opt.capacity > 0) cap = std::min<int64_t>(cap, c->opt.capacity);
+    c->capacity = cap;
+    c->tmask = (uint32_t)(slots - 1);
+
+    char *p = c->pool;
+    auto take = [&](size_t n) { char *q = p; p += align_up(n, 256); return q; };
+    c->d_aux = (WordAux *)take((size_t)c->nw * sizeof(WordAux));
+    c->d_state = (DevState *)take(sizeof(DevState));
+    c->t_slot = (uint32_t *)take((size_t)ch * 4);
+    c->t_fp2 = (uint64_t *)take((size_t)ch * 8);
+    c->t_mask = (uint64_t *)take(tiles * 16);
+    c->t_cnt = (uint32_t *)take(tiles * 4);
+    c->t_off = (uint32_t *)take(tiles * 4);
+    c->d_scan = take(c->scan_bytes);
+    c->d_tab = (uint64_t *)take(slots * 16);
+    const int64_t capr = align_up((size_t)cap, 64);
+    c->d_op = (uint8_t *)take((size_t)capr);
+    c->d_left = (int32_t *)take((size_t)capr * 4);
+    c->d_right = (int32_t *)take((size_t)capr * 4);
+    c->d_fp2 = (uint64_t *)take((size_t)capr * 8);
+    c->d_store = (uint64_t *)take((size_t)capr * c->nw * 8);
+    if ((size_t)(p - c->pool) > budget) return LTL_E_OOM;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Profiling
+// ---------------------------------------------------------------------------
+static hipEvent_t get_event(ltl_ctx *c) {
+    if (!c->free_events.empty()) {
+        hipEvent_t e = c->free_events.back();
+        c->free_events.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+
+static void drain_events(ltl_ctx *c) {
+    for (auto &p : c->pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, p.a, p.b);
+        auto &s = c->stats[p.kind];
+        s.launches++;
+        s.seconds += ms * 1e-3;
+        s.alg_bytes += p.alg;
+        s.min_bytes += p.min;
+        s.items += p.items;
+        c->free_events.push_back(p.a);
+        c->free_events.push_back(p.b);
+    }
+    c->pending.clear();
+}
+
+template <class F>
+static hipError_t timed(ltl_ctx *c, int kind, uint64_t alg, uint64_t mn, uint64_t items, F &&launch) {
+    if (!c->opt.profile) return launch();
+    PendingEvent pe{kind, get_event(c), get_event(c), alg, mn, items};
+    hipEventRecord(pe.a, c->stream);
+    hipError_t e = launch();
+    hipEventRecord(pe.b, c->stream);
+    c->pending.push_back(pe);
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// Enumeration
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Run {
+    ltl_ctx *c;
+    const ltl_problem *pb;
+    ChunkArgs base{};
+    uint64_t all_ltls = 0;
+    uint64_t candidates = 0, alg_bytes = 0;
+    bool otf = false;  // overflow seen at a previous level end
+    std::vector<Block> blocks;
+
+    int32_t run_chunk(int op, ChunkArgs a) {
+        ltl_ctx *c = this->c;
+        a.chunk_idx = ++c->chunk_counter;
+        const bool bin = op == LTL_AND || op == LTL_OR || op == LTL_UNTIL;
+        const int nd = op == LTL_UNTIL ? 2 : 1;
+        uint64_t cand, alg, mn;
+        const uint64_t cs = (uint64_t)c->nw * 8;
+        if (bin) {
+            a.nbs = (int32_t)((a.b1 - a.b0 + 63) / 64);
+            a.n_tiles = (int32_t)((a.a1 - a.a0) * a.nbs);
+            const uint64_t pairs = (uint64_t)(a.a1 - a.a0) * (uint64_t)(a.b1 - a.b0);
+            cand = pairs * nd;
+            alg = pairs * 2 * cs + cand * 20;
+            mn = (uint64_t)(a.a1 - a.a0 + a.b1 - a.b0) * cs + cand * 20;
+        } else {
+            a.nbs = 1;
+            a.n_tiles = (int32_t)((a.a1 - a.a0 + 63) / 64);
+            cand = (uint64_t)(a.a1 - a.a0);
+            alg = cand * (cs + 20);
+            mn = alg;
+        }
+        candidates += cand;
+        alg_bytes += alg;
+        const int gk = op == LTL_UNTIL ? K_GENU : bin ? K_GEN2 : K_GEN1;
+        const int ek = op == LTL_UNTIL ? K_EMITU : bin ? K_EMIT2 : K_EMIT1;
+        HIP_TRY(timed(c, gk, alg, mn, cand, [&] { return launch_gen(op, c->B, a, c->stream); }));
+        if (otf) return 0;
+        HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_decide(op, a, c->stream); }));
+        HIP_TRY(timed(c, K_SCAN, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles, [&] {
+            hipError_t e = launch_scan(c->t_cnt, c->t_off, a.n_tiles, c->d_scan, c->scan_bytes, c->stream);
+            if (e != hipSuccess) return e;
+            return launch_finish(a, c->stream);
+        }));
+        HIP_TRY(timed(c, ek, 0, 0, 0, [&] { return launch_emit(op, c->B, a, c->stream); }));
+        return 0;
+    }
+
+    int32_t unary(int op, int64_t s, int64_t e) {
+        const int64_t n = e - s;
+        if (n <= 0) return 0;
+        blocks.push_back({all_ltls, (uint64_t)n, op, s, n, 0, 1});
+        for (int64_t a0 = 0; a0 < n; a0 += c->CH) {
+            ChunkArgs a = base;
+            a.id_base = all_ltls;
+            a.ls = s; a.rs = 0; a.nR = 1;
+            a.a0 = a0; a.a1 = std::min<int64_t>(n, a0 + c->CH);
+            a.b0 = a.b1 = 0;
+            if (int32_t r = run_chunk(op, a)) return r;
+        }
+        all_ltls += (uint64_t)n;
+        return 0;
+    }
+
+    int32_t binary(int op, int64_t ls, int64_t le, int64_t rs, int64_t re) {
+        const int64_t nL = le - ls, nR = re - rs;
+        if (nL <= 0 || nR <= 0) return 0;
+        const int nd = op == LTL_UNTIL ? 2 : 1;
+        const uint64_t size = (uint64_t)nL * (uint64_t)nR * nd;
+        blocks.push_back({all_ltls, size, op, ls, nL, rs, nR});
+        const int64_t row_slots = (nR + 63) / 64 * 64 * nd;
+        if (row_slots <= c->CH) {
+            const int64_t rows = std::max<int64_t>(1, c->CH / row_slots);
+            for (int64_t a0 = 0; a0 < nL; a0 += rows) {
+                ChunkArgs a = base;
+                a.id_base = all_ltls;
+                a.ls = ls; a.rs = rs; a.nR = nR;
+                a.a0 = a0; a.a1 = std::min<int64_t>(nL, a0 + rows);
+                a.b0 = 0; a.b1 = nR;
+                if (int32_t r = run_chunk(op, a)) return r;
+            }
+        } else {
+            const int64_t cols = std::max<int64_t>(64, c->CH / nd / 64 * 64);
+            for (int64_t a0 = 0; a0 < nL; ++a0)
+                for (int64_t b0 = 0; b0 < nR; b0 += cols) {
+                    ChunkArgs a = base;
+                    a.id_base = all_ltls;
+                    a.ls = ls; a.rs = rs; a.nR = nR;
+                    a.a0 = a0; a.a1 = a0 + 1;
+                    a.b0 = b0; a.b1 = std::min<int64_t>(nR, b0 + cols);
+                    if (int32_t r = run_chunk(op, a)) return r;
+                }
+        }
+        all_ltls += size;
+        return 0;
+    }
+};
+
+struct Meta {
+    int op;
+    int32_t l, r;
+};
+
+int32_t read_meta(ltl_ctx *c, int64_t k, Meta &m) {
+    uint8_t op;
+    HIP_TRY(hipMemcpy(&op, c->d_op + k, 1, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&m.l, c->d_left + k, 4, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(&m.r, c->d_right + k, 4, hipMemcpyDeviceToHost));
+    m.op = op;
+    return 0;
+}
+
+// Build the node array (breadth first) from the root triple and the store's
+// parent indices; store indices below the atom count are atoms.
+int32_t build_tree(ltl_ctx *c, int op, int32_t l, int32_t r, int atom, std::vector<ltl_node> &nodes,
+                   std::vector<int64_t> &src) {
+    nodes.clear();
+    src.clear();
+    nodes.push_back({op, op == LTL_ATOM ? atom : -1, -1, -1});
+    src.push_back(-1);
+    struct Q { int node; int32_t l, r; };
+    std::vector<Q> q{{0, l, r}};
+    for (size_t h = 0; h < q.size(); ++h) {
+        Q cur = q[h];
+        int32_t kids[2] = {cur.l, cur.r};
+        for (int s = 0; s < 2; ++s) {
+            if (kids[s] < 0) continue;
+            Meta m;
+            if (int32_t e = read_meta(c, kids[s], m)) return e;
+            const int idx = (int)nodes.size();
+            nodes.push_back({m.op, m.op == LTL_ATOM ? kids[s] : -1, -1, -1});
+            src.push_back(kids[s]);
+            if (s == 0) nodes[cur.node].left = idx; else nodes[cur.node].right = idx;
+            if (m.op != LTL_ATOM) q.push_back({idx, m.l, m.r});
+            if (nodes.size() > 100000) return LTL_E_LIMIT;
+        }
+    }
+    return 0;
+}
+
+std::string node_string(const std::vector<ltl_node> &n, int i, const std::string &names) {
+    const ltl_node &x = n[i];
+    switch (x.op) {
+    case LTL_ATOM: return std::string(1, names[x.atom]);
+    case LTL_NOT: return "~(" + node_string(n, x.left, names) + ")";
+    case LTL_NEXT: return "X(" + node_string(n, x.left, names) + ")";
+    case LTL_FIN: return "F(" + node_string(n, x.left, names) + ")";
+    case LTL_GLOB: return "G(" + node_string(n, x.left, names) + ")";
+    default: {
+        const char *mid = x.op == LTL_AND ? ")&(" : x.op == LTL_OR ? ")|(" : ")U(";
+        return "(" + node_string(n, x.left, names) + mid + node_string(n, x.right, names) + ")";
+    }
+    }
+}
+
+}  // namespace
+
+extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res) {
+    if (!res) return LTL_E_ARG;
+    memset(res, 0, sizeof(*res));
+    res->found_id = -1;
+    if (!c || !pb || !pb->lens || !pb->atoms) return res->status = LTL_E_ARG;
+    const double t0 = now_s();
+    const ltl_options &o = c->opt;
+    const int P = pb->num_pos, N = pb->num_neg, W = P + N, A = pb->num_atoms;
+    if (P < 0 || N < 0 || W < 1 || A < 1 || A > 64) return res->status = LTL_E_ARG;
+    int maxlen = 0;
+    for (int i = 0; i < W; ++i) {
+        if (pb->lens[i] < 1 || pb->lens[i] > 64) return res->status = LTL_E_LIMIT;
+        maxlen = std::max<int>(maxlen, pb->lens[i]);
+    }
+    std::string names = pb->names ? std::string(pb->names, A) : std::string();
+    if (names.empty()) for (int a = 0; a < A; ++a) names += (char)('a' + a);
+    const int c1 = o.cost[0], c2 = o.cost[1], c3 = o.cost[2], c4 = o.cost[3];
+    const int c5 = o.cost[4], c6 = o.cost[5], c7 = o.cost[6], c8 = o.cost[7];
+    const int maxc = o.max_cost;
+    res->cost = c1;
+    for (auto &s : c->stats) { s.launches = 0; s.seconds = 0; s.alg_bytes = s.min_bytes = s.items = 0; }
+
+    // Atoms, checked in alphabet order before anything else
+    // (code/ltli6463.cu:1100-1136).
+    uint64_t all_ltls = 0;
+    std::vector<uint64_t> cs(W);
+    for (int a = 0; a < A; ++a) {
+        ++all_ltls;
+        for (int i = 0; i < W; ++i) cs[i] = pb->atoms[(size_t)a * W + i] & len_mask(pb->lens[i]);
+        if (ltl_mismatches(cs.data(), P, N) <= o.hamming_k) {
+            res->status = LTL_FOUND;
+            res->all_ltls = all_ltls;
+            res->found_id = a;
+            res->n_nodes = 1;
+            res->nodes = (ltl_node *)malloc(sizeof(ltl_node));
+            res->nodes[0] = {LTL_ATOM, a, -1, -1};
+            res->formula = strdup(std::string(1, names[a]).c_str());
+            res->seconds = now_s() - t0;
+            c->W = 0;
+            return res->status;
+        }
+    }
+
+    // Layout: B-bit trace slots, TPW traces per word.
+    if (hipSetDevice(o.device) != hipSuccess) return res->status = LTL_E_HIP;
+    int B = 8;
+    while (B < maxlen) B <<= 1;
+    c->B = B; c->TPW = 64 / B; c->W = W; c->P = P; c->A = A;
+    c->nw = (W + c->TPW - 1) / c->TPW;
+    if (int32_t e = ensure_pool(c)) return res->status = e;
+    if (int32_t e = carve(c)) return res->status = e;
+    if (c->capacity < A) return res->status = LTL_E_OOM;
+
+    // Per-word keys and masks; atoms packed into stripe 0.
+    std::vector<WordAux> aux(c->nw);
+    const uint64_t seed = 0x5eed1e55c0ffee11ull;
+    for (int w = 0; w < c->nw; ++w) {
+        WordAux &x = aux[w];
+        memset(&x, 0, sizeof(x));
+        x.k1 = splitmix(seed ^ (2ull * w));
+        x.k2 = splitmix(~seed ^ (2ull * w + 1));
+        for (int j = 0; j < c->TPW; ++j) {
+            const int i = w * c->TPW + j;
+            if (i >= W) break;
+            x.lm |= len_mask(pb->lens[i]) << (j * B);
+            x.chk |= 1ull << (j * B);
+            if (i < P) x.tgt |= 1ull << (j * B);
+        }
+    }
+    std::vector<uint64_t> stripe0((size_t)c->nw * 64, 0);
+    for (int a = 0; a < A; ++a)
+        for (int i = 0; i < W; ++i)
+            stripe0[(size_t)(i / c->TPW) * 64 + a] |=
+                (pb->atoms[(size_t)a * W + i] & len_mask(pb->lens[i])) << ((i % c->TPW) * B);
+    DevState st0{};
+    st0.found = kEmpty;
+    st0.n_stored = A;
+    st0.overflow_chunk = LLONG_MAX;
+    hipStream_t s = c->stream;
+    HIP_TRY(hipMemcpyAsync(c->d_aux, aux.data(), aux.size() * sizeof(WordAux), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->d_store, stripe0.data(), stripe0.size() * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->d_state, &st0, sizeof(st0), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_fill_u64(c->d_tab, kEmpty, 2 * ((size_t)c->tmask + 1), s));
+
+    Run run{c, pb};
+    ChunkArgs &ba = run.base;
+    ba.store = c->d_store; ba.aux = c->d_aux; ba.tab = c->d_tab; ba.st = c->d_state;
+    ba.t_slot = c->t_slot; ba.t_fp2 = c->t_fp2; ba.t_mask = c->t_mask; ba.t_cnt = c->t_cnt; ba.t_off = c->t_off;
+    ba.m_op = c->d_op; ba.m_left = c->d_left; ba.m_right = c->d_right; ba.m_fp2 = c->d_fp2;
+    ba.capacity = c->capacity;
+    ba.nw = c->nw;
+    ba.hamming_k = o.hamming_k;
+    ba.tmask = c->tmask;
+    HIP_TRY(launch_atoms(B, ba, A, s));
+    run.all_ltls = all_ltls;
+
+    const int nlev = std::max(maxc, c1) + 2;
+    c->lvl_s.assign(nlev, 0);
+    c->lvl_e.assign(nlev, 0);
+    c->lvl_s[c1] = 0;
+    c->lvl_e[c1] = A;
+    auto &Ls = c->lvl_s;
+    auto &Le = c->lvl_e;
+
+    int cost = c1 + 1, shortage = -1;
+    bool last_round = false, found = false;
+    int32_t rc = 0;
+    DevState &h = *c->h_state;
+    auto sync_state = [&]() -> int32_t {
+        HIP_TRY(hipMemcpyAsync(&h, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        drain_events(c);
+        if (h.collision) return LTL_E_COLLISION;
+        if (h.probe_fail) return LTL_E_OOM;
+        return 0;
+    };
+    int64_t n_stored = A;
+    for (cost = c1 + 1; cost <= maxc; ++cost) {
+        if (run.otf) {
+            const int dif = cost - shortage;  // code/ltli6463.cu:1202-1206
+            if (dif == c2 || dif == c1 + c3 || dif == c1 + c4 || dif == c5 || dif == c6 || dif == c7 || dif == c1 + c8)
+                last_round = true;
+        }
+        Ls[cost] = n_stored;
+        if ((o.neg_type == 1 && cost - c2 >= c1) || (o.neg_type == 2 && cost - c2 == c1))
+            if ((rc = run.unary(LTL_NOT, Ls[cost - c2], Le[cost - c2]))) break;
+        for (int i = c1; 2 * i <= cost - c3 && !rc; ++i)
+            rc = run.binary(LTL_AND, Ls[i], Le[i], Ls[cost - i - c3], Le[cost - i - c3]);
+        for (int i = c1; 2 * i <= cost - c4 && !rc; ++i)
+            rc = run.binary(LTL_OR, Ls[i], Le[i], Ls[cost - i - c4], Le[cost - i - c4]);
+        if (!rc && cost - c5 >= c1) rc = run.unary(LTL_NEXT, Ls[cost - c5], Le[cost - c5]);
+        if (!rc && cost - c6 >= c1) rc = run.unary(LTL_FIN, Ls[cost - c6], Le[cost - c6]);
+        if (!rc && cost - c7 >= c1) rc = run.unary(LTL_GLOB, Ls[cost - c7], Le[cost - c7]);
+        for (int i = c1; 2 * i <= cost - c8 && !rc; ++i)
+            rc = run.binary(LTL_UNTIL, Ls[i], Le[i], Ls[cost - i - c8], Le[cost - i - c8]);
+        if (rc) break;
+        if ((rc = sync_state())) break;
+        n_stored = h.n_stored;
+        Le[cost] = n_stored;
+        if (h.found != kEmpty) { found = true; break; }
+        if (h.overflow_chunk != LLONG_MAX) run.otf = true;
+        if (last_round) break;
+        if (run.otf && shortage == -1) shortage = cost;
+    }
+    if (rc) {
+        hipStreamSynchronize(s);
+        drain_events(c);
+        return res->status = rc;
+    }
+    if (cost == maxc + 1) cost--;  // code/ltli6463.cu:1445
+    res->cost = cost;
+    res->n_stored = n_stored;
+    res->on_the_fly = run.otf || h.overflow_chunk != LLONG_MAX;
+    res->candidates = run.candidates;
+    res->hbm_bytes = run.alg_bytes;
+    for (auto &k : c->stats) res->kernel_seconds += k.seconds;
+
+    if (found) {
+        const uint64_t id = h.found;
+        const Block *blk = nullptr;
+        for (const auto &b : run.blocks)
+            if (id >= b.id_base && id < b.id_base + b.size) { blk = &b; break; }
+        if (!blk) return res->status = LTL_E_HIP;
+        res->all_ltls = blk->id_base + blk->size;
+        res->found_id = (int64_t)id;
+        const uint64_t rel = id - blk->id_base;
+        int32_t l, r;
+        if (blk->op == LTL_AND || blk->op == LTL_OR || blk->op == LTL_UNTIL) {
+            const uint64_t pair = blk->op == LTL_UNTIL ? rel >> 1 : rel;
+            const int32_t sl = (int32_t)(blk->ls + (int64_t)(pair / (uint64_t)blk->nR));
+            const int32_t sr = (int32_t)(blk->rs + (int64_t)(pair % (uint64_t)blk->nR));
+            const bool swap = blk->op == LTL_UNTIL && (rel & 1);
+            l = swap ? sr : sl;
+            r = swap ? sl : sr;
+        } else {
+            l = (int32_t)(blk->ls + (int64_t)rel);
+            r = -1;
+        }
+        std::vector<ltl_node> nodes;
+        std::vector<int64_t> src;
+        if (int32_t e = build_tree(c, blk->op, l, r, -1, nodes, src)) return res->status = e;
+        res->n_nodes = (int32_t)nodes.size();
+        res->nodes = (ltl_node *)malloc(nodes.size() * sizeof(ltl_node));
+        memcpy(res->nodes, nodes.data(), nodes.size() * sizeof(ltl_node));
+        res->formula = strdup(node_string(nodes, 0, names).c_str());
+        res->status = LTL_FOUND;
+    } else {
+        res->all_ltls = run.all_ltls;
+        res->formula = strdup("not_found");
+        res->status = LTL_NOT_FOUND;
+    }
+    res->seconds = now_s() - t0;
+    return res->status;
+}
+
+extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n, ltl_result *res) {
+    if (!c || !pbs || !res || n < 0) return LTL_E_ARG;
+    int32_t solved = 0;
+    for (int32_t i = 0; i < n; ++i) {
+        int32_t st = ltl_learn(c, &pbs[i], &res[i]);
+        if (st < 0) return st;
+        solved += st == LTL_FOUND;
+    }
+    return solved;
+}
+
+extern "C" int32_t ltl_get_kernel_stats(const ltl_ctx *c, ltl_kernel_stat *out, int32_t max) {
+    if (!c) return LTL_E_ARG;
+    for (int k = 0; k < K_NKIND && k < max; ++k) out[k] = c->stats[k];
+    return K_NKIND;
+}
+
+extern "C" int32_t ltl_levels(const ltl_ctx *c, int64_t *start, int64_t *end, int32_t max) {
+    if (!c) return LTL_E_ARG;
+    const int32_t n = (int32_t)c->lvl_s.size();
+    for (int32_t i = 0; i < n && i < max; ++i) {
+        if (start) start[i] = c->lvl_s[i];
+        if (end) end[i] = c->lvl_e[i];
+    }
+    return n;
+}
+
+extern "C" int32_t ltl_store_read(ltl_ctx *c, int64_t first, int64_t count, uint64_t *cs, uint8_t *op,
+                                  int32_t *left, int32_t *right) {
+    if (!c || first < 0 || count < 0 || !c->W) return LTL_E_ARG;
+    if (count == 0) return 0;
+    if (first + count > c->capacity) return LTL_E_ARG;
+    HIP_TRY(hipSetDevice(c->opt.device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int64_t s0 = first / 64, s1 = (first + count - 1) / 64;
+    const size_t stripe = (size_t)c->nw * 64;
+    std::vector<uint64_t> buf((size_t)(s1 - s0 + 1) * stripe);
+    HIP_TRY(hipMemcpy(buf.data(), c->d_store + (size_t)s0 * stripe, buf.size() * 8, hipMemcpyDeviceToHost));
+    const uint64_t slot = c->B == 64 ? ~0ull : ((1ull << c->B) - 1);
+    if (cs)
+        for (int64_t k = 0; k < count; ++k) {
+            const int64_t sidx = first + k;
+            const uint64_t *base = buf.data() + (size_t)(sidx / 64 - s0) * stripe + (sidx % 64);
+            for (int i = 0; i < c->W; ++i)
+                cs[(size_t)k * c->W + i] = (base[(size_t)(i / c->TPW) * 64] >> ((i % c->TPW) * c->B)) & slot;
+        }
+    if (op) HIP_TRY(hipMemcpy(op, c->d_op + first, (size_t)count, hipMemcpyDeviceToHost));
+    if (left) HIP_TRY(hipMemcpy(left, c->d_left + first, (size_t)count * 4, hipMemcpyDeviceToHost));
+    if (right) HIP_TRY(hipMemcpy(right, c->d_right + first, (size_t)count * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" const char *ltl_version(void) { return "ltl-amd 0.1 (gfx950)"; }

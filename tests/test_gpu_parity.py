@@ -1,0 +1,119 @@
+"""Parity of the HIP enumeration (through the C ABI) with the CPU oracle.
+
+Bit-exact: status, cost, allLTLs, canonical id of the result, the formula
+string, level boundaries and every stored characteristic set with its
+operator and parent indices.  Sizes are chosen so that the oracle finishes
+in seconds."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+
+def to_oracle(O, inst):
+    return O.Instance(inst.num_pos, inst.num_neg, inst.lens, inst.atoms, inst.names)
+
+
+def check_parity(ltl, O, learner, inst, **opts):
+    gpu = learner.learn(inst, **opts)
+    okw = {k: v for k, v in opts.items() if k in ("cost", "max_cost", "neg_type", "hamming_k", "capacity")}
+    ref = O.enumerate_(to_oracle(O, inst), keep_store=True, **okw)
+    ctx = f"{inst.num_pos}+{inst.num_neg} opts={opts}"
+    assert gpu["status"] == ref["status"], ctx
+    assert gpu["cost"] == ref["cost"], ctx
+    assert gpu["all_ltls"] == ref["all_ltls"], ctx
+    assert gpu["found_id"] == ref["found_id"], ctx
+    assert gpu["formula"] == ref["formula"], ctx
+    assert gpu["on_the_fly"] == ref["on_the_fly"], ctx
+    if gpu["status"] == 1 and len(gpu["formula"]) > 1:
+        cs = O.eval_formula(to_oracle(O, inst), gpu["formula"])
+        assert O.mismatches(to_oracle(O, inst), cs) <= opts.get("hamming_k", 0)
+        assert O.formula_cost(gpu["formula"], opts.get("cost", (1,) * 8)) == gpu["cost"]
+    # complete levels: all of them when not found, those below the result's
+    s, e = learner.levels()
+    c1 = opts.get("cost", (1,) * 8)[0]
+    last = gpu["cost"] - (1 if gpu["status"] == 1 else 0)
+    if gpu["status"] == 1 and gpu["found_id"] < inst.atoms.shape[0]:
+        return gpu, ref
+    n_cmp = 0
+    for c in range(c1, last + 1):
+        assert (s[c], e[c]) == (ref["level_start"][c], ref["level_end"][c]), (ctx, c)
+        n_cmp = max(n_cmp, int(e[c]))
+    if n_cmp:
+        words, op, left, right = learner.store(0, n_cmp, inst.W)
+        assert np.array_equal(words, ref["words"][:n_cmp]), ctx
+        assert np.array_equal(op, ref["op"][:n_cmp]), ctx
+        assert np.array_equal(left, ref["left"][:n_cmp]), ctx
+        assert np.array_equal(right, ref["right"][:n_cmp]), ctx
+    return gpu, ref
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_config1(ltl, oracle, learner, seed):
+    inst = ltl.generate(1, seed=seed)
+    gpu, ref = check_parity(ltl, oracle, learner, inst, max_cost=10)
+    assert gpu["status"] == 1 and gpu["cost"] <= 5
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLDEN, "traces", "*.trace"))),
+                         ids=os.path.basename)
+@pytest.mark.parametrize("neg_type", [1, 2])
+def test_trace_fixtures(ltl, oracle, learner, path, neg_type):
+    inst = ltl.read_trace_file(path)
+    check_parity(ltl, oracle, learner, inst, max_cost=7, neg_type=neg_type)
+
+
+# (num_pos, num_neg, atoms, len_min, len_max, max_cost): covers slot widths
+# 8, 16, 32 and 64 bits and several traces per packed word.
+SHAPES = [(7, 9, 2, 2, 8, 6), (20, 13, 3, 9, 16, 5), (33, 40, 3, 17, 32, 5),
+          (25, 31, 3, 33, 64, 5), (64, 64, 4, 5, 40, 4), (3, 2, 2, 1, 4, 6)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_random_not_found_full_store(ltl, oracle, learner, shape):
+    P, N, A, lo, hi, mc = shape
+    inst = ltl.generate_custom(11 + P, P, N, A, lo, hi, 0.5, planted_size=9)
+    # max cost below the planted size: compares every stored set
+    check_parity(ltl, oracle, learner, inst, max_cost=mc)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
+def test_random_found(ltl, oracle, learner, shape):
+    P, N, A, lo, hi, _ = shape
+    inst = ltl.generate_custom(7 + N, P, N, A, lo, hi, 0.5, planted_size=5)
+    gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=7)
+    assert gpu["status"] == 1
+
+
+@pytest.mark.parametrize("cost", [(1, 2, 1, 1, 2, 3, 1, 2), (2, 1, 3, 2, 1, 1, 2, 3)])
+def test_nonunit_costs(ltl, oracle, learner, cost):
+    inst = ltl.generate(1, seed=5)
+    check_parity(ltl, oracle, learner, inst, max_cost=14, cost=cost)
+
+
+@pytest.mark.parametrize("k", [1, 3])
+def test_hamming(ltl, oracle, learner, k):
+    inst = ltl.generate_custom(3, 30, 30, 3, 5, 20, 0.5, planted_size=9)
+    check_parity(ltl, oracle, learner, inst, max_cost=7, hamming_k=k)
+
+
+@pytest.mark.parametrize("capacity", [64, 128, 320])
+def test_overflow_check_only(ltl, oracle, learner, capacity):
+    inst = ltl.generate_custom(5, 16, 16, 3, 4, 12, 0.5, planted_size=8)
+    check_parity(ltl, oracle, learner, inst, max_cost=8, capacity=capacity)
+
+
+def test_config2_levels(ltl, oracle, learner):
+    inst = ltl.generate(2, seed=1)
+    check_parity(ltl, oracle, learner, inst, max_cost=6)
+
+
+def test_config3_found(ltl, oracle, learner):
+    inst = ltl.generate(3, seed=1)
+    gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=6, hamming_k=inst.hamming_k)
+    assert gpu["status"] == 1
