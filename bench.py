@@ -1,0 +1,142 @@
+#!/usr/bin/env python3
+"""Benchmark: unique characteristic sets per second on one MI355X.
+
+Workload (one step = one full ltl_learn call, i.e. one pass of the hot path
+over the instance until the minimal formula is found): 500 positive + 500
+negative synthetic traces, 5 atoms, lengths U{10..32}, per-atom bit rates
+U[0.3, 0.7], labels from the planted G(a -> F b) & ~(c U d); unit costs, max
+cost 14, dedup table of 2^28 slots.  The instance is generated (seed 1) and
+uploaded before the timed region; each call re-initialises its device state.
+
+Prints one JSON line.  `value` = unique sets stored per second of whole-call
+wall time; `ms_per_step` = time to the minimal formula.  `roofline` is the
+dominant kernel's algorithmic bytes per launch over its HIP-event duration
+against the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (the
+reference enumeration restated in C, OpenMP) on a bounded sample of the same
+instance (levels up to cost 8).
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+"""
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "ltl-learning-on-gpus_amd")
+HBM_PEAK = 8.0e12
+METRIC = "unique char-sets/sec; time-to-min-formula (learned vs plain order)"
+
+
+def load_package():
+    name = "ltl_learning_on_gpus_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def traffic_for(kernel):
+    """HBM bytes per launch from the committed rocprofv3 PMC summary, or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    k = t.get("kernels", {}).get(kernel)
+    if not k:
+        return None
+    return int(k["bytes_per_dispatch"])
+
+
+def cpu_baseline(inst, max_cost, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    oi = O.Instance(inst.num_pos, inst.num_neg, inst.lens, inst.atoms, inst.names)
+    t = time.perf_counter()
+    r = O.enumerate_(oi, max_cost=max_cost, threads=threads)
+    dt = time.perf_counter() - t
+    uniques = r["n_stored"] - inst.atoms.shape[0]
+    return uniques / dt, dt, uniques
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--budget-gb", type=float, default=0.0, help="device pool; 0 = 75%% of free memory")
+    ap.add_argument("--max-cost", type=int, default=14)
+    ap.add_argument("--cpu-cost", type=int, default=8, help="levels timed for the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    ltl = load_package()
+    inst = ltl.generate(2, seed=1)
+    L = ltl.Learner(budget=int(args.budget_gb * 1e9), profile=1, hash_slots_log2=28,
+                    max_cost=args.max_cost, neg_type=1)
+    for _ in range(args.warmup):
+        L.learn(inst)
+    stats = {}
+    wall, uniques, results = 0.0, 0, []
+    for _ in range(args.steps):
+        t = time.perf_counter()
+        r = L.learn(inst)
+        wall += time.perf_counter() - t
+        uniques += r["n_stored"] - inst.atoms.shape[0]
+        results.append(r)
+        for k, v in L.kernel_stats().items():
+            s = stats.setdefault(k, dict(launches=0, seconds=0.0, alg_bytes=0, min_bytes=0))
+            for f in s:
+                s[f] += v[f]
+    r0 = results[0]
+    assert all(r["formula"] == r0["formula"] and r["cost"] == r0["cost"] for r in results)
+    dom = max(stats, key=lambda k: stats[k]["seconds"])
+    d = stats[dom]
+    achieved = d["alg_bytes"] / d["seconds"] / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
+            "traffic": traffic_for(dom),
+            "alg_bytes_per_launch": int(d["alg_bytes"] / max(d["launches"], 1)),
+            "avg_launch_ms": round(d["seconds"] / max(d["launches"], 1) * 1e3, 4),
+            "compulsory_gbs": round(d["min_bytes"] / d["seconds"] / 1e9, 1)}
+    out = {
+        "metric": METRIC, "value": round(uniques / wall, 1), "unit": "unique CS/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "config2: 500+500 traces, 5 atoms, lengths U{10..32}, per-atom rate "
+                               "U[0.3,0.7], planted G(a->F b)&~(c U d), unit costs, max cost 14, "
+                               "2^28 dedup slots, seed 1",
+                   "traces": inst.W, "atoms": int(inst.atoms.shape[0]), "max_cost": args.max_cost},
+        "time_to_min_formula_s": round(wall / args.steps, 4),
+        "min_cost": r0["cost"], "formula": r0["formula"], "uniques_per_step": r0["n_stored"],
+        "candidates_per_step": r0["candidates"], "checked_per_step": r0["checked"],
+        "kernels_ms_per_step": {k: round(v["seconds"] / args.steps * 1e3, 3) for k, v in stats.items()
+                                if v["launches"]},
+        "roofline": roof,
+    }
+    L.close()
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        v, dt, u = cpu_baseline(inst, args.cpu_cost, threads)
+        out["cpu_baseline"] = {"value": round(v, 1), "unit": "unique CS/s", "cores": threads,
+                               "kind": "port", "seconds": round(dt, 3),
+                               "sample": f"same instance, levels up to cost {args.cpu_cost} "
+                                         f"({u} unique sets), exact dedup, OpenMP"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

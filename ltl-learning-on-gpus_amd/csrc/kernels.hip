@@ -82,6 +82,37 @@ __device__ __forceinline__ void fold(Acc &a, uint64_t c, const WordAux &x) {
     a.mism += (uint32_t)__popcll((c ^ x.tgt) & x.chk);
 }
 
+// Per-trace summaries of one packed word, OR-ed into the summary words at
+// bit offset sh: position 0 (b0), position 1 (b1), any position (nz) and
+// every position inside the trace (fl).  They decide the target check of
+// ~, X, F, G, & and | candidates without reading the full sets.
+template <int B>
+__device__ __forceinline__ void summarize(uint64_t c, uint64_t lm, int sh, uint64_t (&v)[4]) {
+    constexpr int TPW = 64 / B;
+    constexpr uint64_t SM = B == 64 ? ~0ull : ((1ull << B) - 1ull);
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+        const uint64_t x = (c >> (j * B)) & SM, m = (lm >> (j * B)) & SM;
+        const int at = sh + j;
+        v[0] |= (x & 1ull) << at;
+        v[1] |= ((x >> 1) & 1ull) << at;
+        v[2] |= (uint64_t)(x != 0) << at;
+        v[3] |= (uint64_t)(x == m) << at;
+    }
+}
+
+// Flush the summary words of set `pos` when word w closes summary word w/B.
+template <int B>
+__device__ __forceinline__ void flush_summary(const ChunkArgs &A, uint64_t *z, int w, uint64_t (&v)[4], bool on) {
+    if ((w % B) != B - 1 && w != A.nw - 1) return;
+    const int j = w / B;
+    if (on) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) z[(size_t)(k * A.nsw + j) << 6] = v[k];
+    }
+    v[0] = v[1] = v[2] = v[3] = 0;
+}
+
 // Table key from fingerprint 1: a 64-bit finalizer spreads the NH sum over
 // all bits (slot index = low bits); the empty key is never produced.
 __device__ __forceinline__ uint64_t table_key(uint64_t f) {
@@ -171,16 +202,27 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     const uint64_t *pl = A.store + row0(sl, A.nw);
     const uint64_t *pr = A.store + row0(sr, A.nw);
     Acc c0, c1;
-    for (int w = 0; w < A.nw; ++w) {
+    auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux x = A.aux[w];
-        const uint64_t l = pl[(size_t)w << 6], r = pr[(size_t)w << 6];
         if constexpr (OP == LTL_AND) fold(c0, l & r, x);
         else if constexpr (OP == LTL_OR) fold(c0, l | r, x);
         else {
             fold(c0, f_until<B>(l, r), x);
             fold(c1, f_until<B>(r, l), x);
         }
+    };
+    // Batches of UB words: all loads of a batch are issued before use, so
+    // that each lane keeps 2*UB loads in flight.
+    constexpr int UB = 4;
+    int w = 0;
+    for (; w + UB <= A.nw; w += UB) {
+        uint64_t l[UB], r[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) << 6]; r[u] = pr[(size_t)(w + u) << 6]; }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
     }
+    for (; w < A.nw; ++w) step(w, pl[(size_t)w << 6], pr[(size_t)w << 6]);
     if (!t.valid) return;
     const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
     const uint64_t id0 = A.id_base + (OP == LTL_UNTIL ? 2 * pair : pair);
@@ -208,10 +250,20 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     const int64_t s = A.ls + (t.valid ? t.a : A.a0);
     const uint64_t *p = A.store + row0(s, A.nw);
     Acc c0;
-    for (int w = 0; w < A.nw; ++w) {
+    auto step = [&](int w, uint64_t v) {
         const WordAux x = A.aux[w];
-        fold(c0, apply1<OP, B>(p[(size_t)w << 6], x.lm), x);
+        fold(c0, apply1<OP, B>(v, x.lm), x);
+    };
+    constexpr int UB = 8;
+    int w = 0;
+    for (; w + UB <= A.nw; w += UB) {
+        uint64_t v[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) v[u] = p[(size_t)(w + u) << 6];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) step(w + u, v[u]);
     }
+    for (; w < A.nw; ++w) step(w, p[(size_t)w << 6]);
     if (!t.valid) return;
     const uint64_t id0 = A.id_base + (uint64_t)t.a;
     if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
@@ -343,6 +395,7 @@ __global__ __launch_bounds__(256) void k_emit2(ChunkArgs A) {
     if (!tile_binary(A, t)) return;
     if (A.t_cnt[t.ti] == 0) return;
     constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
+    constexpr int TPW = 64 / B;
     const uint64_t below = lanes_below();
     const uint64_t bit = 1ull << (threadIdx.x & 63);
     const uint64_t m0 = A.t_mask[ND * t.ti], m1 = ND == 2 ? A.t_mask[ND * t.ti + 1] : 0;
@@ -356,15 +409,31 @@ __global__ __launch_bounds__(256) void k_emit2(ChunkArgs A) {
     const uint64_t *pr = A.store + row0(sr, A.nw);
     uint64_t *q0 = A.store + row0(w0 ? pos0 : 0, A.nw);
     uint64_t *q1 = A.store + row0(w1 ? pos1 : 0, A.nw);
-    for (int w = 0; w < A.nw; ++w) {
-        const uint64_t l = pl[(size_t)w << 6], r = pr[(size_t)w << 6];
-        if constexpr (OP == LTL_AND) { if (w0) q0[(size_t)w << 6] = l & r; }
-        else if constexpr (OP == LTL_OR) { if (w0) q0[(size_t)w << 6] = l | r; }
-        else {
-            if (w0) q0[(size_t)w << 6] = f_until<B>(l, r);
-            if (w1) q1[(size_t)w << 6] = f_until<B>(r, l);
-        }
+    uint64_t *z0 = A.sum + row0(w0 ? pos0 : 0, 4 * A.nsw);
+    uint64_t *z1 = A.sum + row0(w1 ? pos1 : 0, 4 * A.nsw);
+    uint64_t v0[4] = {0, 0, 0, 0}, v1[4] = {0, 0, 0, 0};
+    auto step = [&](int w, uint64_t l, uint64_t r) {
+        const uint64_t lm = A.aux[w].lm;
+        const int sh = (w % B) * TPW;
+        uint64_t c0, c1 = 0;
+        if constexpr (OP == LTL_AND) c0 = l & r;
+        else if constexpr (OP == LTL_OR) c0 = l | r;
+        else { c0 = f_until<B>(l, r); c1 = f_until<B>(r, l); }
+        if (w0) { q0[(size_t)w << 6] = c0; summarize<B>(c0, lm, sh, v0); }
+        if (ND == 2 && w1) { q1[(size_t)w << 6] = c1; summarize<B>(c1, lm, sh, v1); }
+        flush_summary<B>(A, z0, w, v0, w0);
+        if (ND == 2) flush_summary<B>(A, z1, w, v1, w1);
+    };
+    constexpr int UB = 4;
+    int w = 0;
+    for (; w + UB <= A.nw; w += UB) {
+        uint64_t l[UB], r[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) << 6]; r[u] = pr[(size_t)(w + u) << 6]; }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
     }
+    for (; w < A.nw; ++w) step(w, pl[(size_t)w << 6], pr[(size_t)w << 6]);
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
     if (w0) commit(A, pos0, ND * ix, OP, (int32_t)sl, (int32_t)sr);
     if (w1) commit(A, pos1, ND * ix + 1, OP, (int32_t)sr, (int32_t)sl);
@@ -376,6 +445,7 @@ __global__ __launch_bounds__(256) void k_emit1(ChunkArgs A) {
     Tile t;
     if (!tile_unary(A, t)) return;
     if (A.t_cnt[t.ti] == 0) return;
+    constexpr int TPW = 64 / B;
     const uint64_t m0 = A.t_mask[t.ti];
     const uint64_t bit = 1ull << (threadIdx.x & 63);
     const int64_t pos = A.st->chunk_base + (int64_t)A.t_off[t.ti] + __popcll(m0 & lanes_below());
@@ -383,19 +453,101 @@ __global__ __launch_bounds__(256) void k_emit1(ChunkArgs A) {
     const int64_t s = A.ls + (t.valid ? t.a : A.a0);
     const uint64_t *p = A.store + row0(s, A.nw);
     uint64_t *q = A.store + row0(w0 ? pos : 0, A.nw);
-    for (int w = 0; w < A.nw; ++w) {
-        const uint64_t c = apply1<OP, B>(p[(size_t)w << 6], A.aux[w].lm);
-        if (w0) q[(size_t)w << 6] = c;
+    uint64_t *z = A.sum + row0(w0 ? pos : 0, 4 * A.nsw);
+    uint64_t v[4] = {0, 0, 0, 0};
+    auto step = [&](int w, uint64_t x) {
+        const uint64_t lm = A.aux[w].lm;
+        const uint64_t c = apply1<OP, B>(x, lm);
+        if (w0) { q[(size_t)w << 6] = c; summarize<B>(c, lm, (w % B) * TPW, v); }
+        flush_summary<B>(A, z, w, v, w0);
+    };
+    constexpr int UB = 8;
+    int w = 0;
+    for (; w + UB <= A.nw; w += UB) {
+        uint64_t x[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) x[u] = p[(size_t)(w + u) << 6];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) step(w + u, x[u]);
     }
+    for (; w < A.nw; ++w) step(w, p[(size_t)w << 6]);
     if (w0) commit(A, pos, (size_t)t.ti * 64 + (threadIdx.x & 63), OP, (int32_t)s, -1);
+}
+
+// ---------------------------------------------------------------------------
+// check-only pass (no dedup, no store)
+// ---------------------------------------------------------------------------
+// ~, X, F, G: one summary vector of the operand (b0 negated, b1, nz, fl).
+template <int OP>
+__global__ __launch_bounds__(256) void k_chk1(ChunkArgs A) {
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_unary(A, t) || !t.valid) return;
+    constexpr int VEC = OP == LTL_NOT ? 0 : OP == LTL_NEXT ? 1 : OP == LTL_FIN ? 2 : 3;
+    const uint64_t *p = A.sum + row0(A.ls + t.a, 4 * A.nsw) + ((size_t)(VEC * A.nsw) << 6);
+    uint32_t m = 0;
+    for (int j = 0; j < A.nsw; ++j) {
+        uint64_t v = p[(size_t)j << 6];
+        if (OP == LTL_NOT) v = ~v;
+        m += (uint32_t)__popcll((v ^ A.saux[j].tgt) & A.saux[j].val);
+    }
+    if (m <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)(A.id_base + (uint64_t)t.a));
+}
+
+// & and |: position 0 of the result is the AND / OR of the operands' b0.
+template <int OP>
+__global__ __launch_bounds__(256) void k_chk2(ChunkArgs A) {
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_binary(A, t)) return;
+    const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
+    const uint64_t *pl = A.sum + row0(sl, 4 * A.nsw);
+    const uint64_t *pr = A.sum + row0(sr, 4 * A.nsw);
+    uint32_t m = 0;
+    for (int j = 0; j < A.nsw; ++j) {
+        const uint64_t l = pl[(size_t)j << 6], r = pr[(size_t)j << 6];
+        const uint64_t c = OP == LTL_AND ? (l & r) : (l | r);
+        m += (uint32_t)__popcll((c ^ A.saux[j].tgt) & A.saux[j].val);
+    }
+    if (!t.valid || m > (uint32_t)A.hamming_k) return;
+    const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
+    atomicMin(&A.st->found, (unsigned long long)(A.id_base + pair));
+}
+
+// U: position 0 depends on the whole trace; both orders from the full words.
+template <int B>
+__global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_binary(A, t)) return;
+    const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
+    const uint64_t *pl = A.store + row0(sl, A.nw);
+    const uint64_t *pr = A.store + row0(sr, A.nw);
+    uint32_t m0 = 0, m1 = 0;
+    auto step = [&](int w, uint64_t l, uint64_t r) {
+        const WordAux &x = A.aux[w];
+        m0 += (uint32_t)__popcll((f_until<B>(l, r) ^ x.tgt) & x.chk);
+        m1 += (uint32_t)__popcll((f_until<B>(r, l) ^ x.tgt) & x.chk);
+    };
+    constexpr int UB = 4;
+    int w = 0;
+    for (; w + UB <= A.nw; w += UB) {
+        uint64_t l[UB], r[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) << 6]; r[u] = pr[(size_t)(w + u) << 6]; }
+#pragma unroll
+        for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
+    }
+    for (; w < A.nw; ++w) step(w, pl[(size_t)w << 6], pr[(size_t)w << 6]);
+    if (!t.valid) return;
+    const uint64_t id0 = A.id_base + 2 * ((uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b);
+    if (m0 <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
+    else if (m1 <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
 }
 
 __global__ void k_fill(uint64_t *p, uint64_t v, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
 }
-
-template <template <int, int> class K>
-struct Nothing {};
 
 }  // namespace
 
@@ -470,6 +622,28 @@ hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s) {
     case 16: hipLaunchKernelGGL(k_atoms<16>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
     case 32: hipLaunchKernelGGL(k_atoms<32>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
     default: hipLaunchKernelGGL(k_atoms<64>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s) {
+    const dim3 b(256);
+    switch (op) {
+    case LTL_NOT: hipLaunchKernelGGL(k_chk1<LTL_NOT>, grid_tiles(a), b, 0, s, a); break;
+    case LTL_NEXT: hipLaunchKernelGGL(k_chk1<LTL_NEXT>, grid_tiles(a), b, 0, s, a); break;
+    case LTL_FIN: hipLaunchKernelGGL(k_chk1<LTL_FIN>, grid_tiles(a), b, 0, s, a); break;
+    case LTL_GLOB: hipLaunchKernelGGL(k_chk1<LTL_GLOB>, grid_tiles(a), b, 0, s, a); break;
+    case LTL_AND: hipLaunchKernelGGL(k_chk2<LTL_AND>, grid_binary(a), b, 0, s, a); break;
+    case LTL_OR: hipLaunchKernelGGL(k_chk2<LTL_OR>, grid_binary(a), b, 0, s, a); break;
+    case LTL_UNTIL:
+        switch (B) {
+        case 8: hipLaunchKernelGGL(k_chkU<8>, grid_binary(a), b, 0, s, a); break;
+        case 16: hipLaunchKernelGGL(k_chkU<16>, grid_binary(a), b, 0, s, a); break;
+        case 32: hipLaunchKernelGGL(k_chkU<32>, grid_binary(a), b, 0, s, a); break;
+        default: hipLaunchKernelGGL(k_chkU<64>, grid_binary(a), b, 0, s, a); break;
+        }
+        break;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -27,6 +27,11 @@ struct WordAux {
     uint64_t pad[3];
 };
 
+// Per summary word j (traces 64j..64j+63): positive traces and valid traces.
+struct SumAux {
+    uint64_t tgt, val;
+};
+
 // Device-resident run state (one per context).
 struct DevState {
     unsigned long long found;     // minimal accepted candidate id, kEmpty = none
@@ -56,6 +61,8 @@ struct ChunkArgs {
     uint8_t *m_op;
     int32_t *m_left, *m_right;
     uint64_t *m_fp2;
+    uint64_t *sum;            // summaries: 4 vectors of nsw words per set, stripe layout
+    const SumAux *saux;
     int64_t capacity;
     int64_t chunk_idx;
     uint64_t id_base;         // canonical id of the block's first candidate
@@ -67,7 +74,7 @@ struct ChunkArgs {
     uint32_t tmask;           // table slots - 1
     int32_t nbs;              // 64-wide column stripes per row in the chunk
     int32_t n_tiles;          // tiles in the chunk
-    int32_t pad_;
+    int32_t nsw;              // summary words per vector: ceil(W / 64)
 };
 
 // Launchers (stream-ordered, asynchronous).
@@ -76,6 +83,9 @@ hipError_t launch_decide(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_finish(const ChunkArgs &a, hipStream_t s);
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s);
+// Check-only pass over a whole block (no dedup, no store): atomicMin of the
+// canonical id of every accepted candidate.
+hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s);
 // Exclusive scan of t_cnt into t_off (n elements); temp sized by scan_temp_bytes.
 size_t scan_temp_bytes(int n_max);

@@ -32,9 +32,10 @@ using namespace ltl;
 
 namespace {
 
-enum { K_GEN1, K_GEN2, K_GENU, K_DECIDE, K_SCAN, K_EMIT1, K_EMIT2, K_EMITU, K_NKIND };
+enum { K_GEN1, K_GEN2, K_GENU, K_DECIDE, K_SCAN, K_EMIT1, K_EMIT2, K_EMITU, K_CHK1, K_CHK2, K_CHKU, K_NKIND };
 const char *kKindName[K_NKIND] = {"gen_unary", "gen_and_or", "gen_until", "decide",
-                                  "scan_finish", "emit_unary", "emit_and_or", "emit_until"};
+                                  "scan_finish", "emit_unary", "emit_and_or", "emit_until",
+                                  "check_unary", "check_and_or", "check_until"};
 
 struct Block {
     uint64_t id_base, size;
@@ -77,7 +78,9 @@ struct ltl_ctx {
     uint32_t tmask = 0;
     WordAux *d_aux = nullptr;
     DevState *d_state = nullptr;
-    uint64_t *d_store = nullptr, *d_tab = nullptr, *d_fp2 = nullptr;
+    uint64_t *d_store = nullptr, *d_tab = nullptr, *d_fp2 = nullptr, *d_sum = nullptr;
+    SumAux *d_saux = nullptr;
+    int nsw = 0;
     uint8_t *d_op = nullptr;
     int32_t *d_left = nullptr, *d_right = nullptr;
     uint32_t *t_slot = nullptr, *t_cnt = nullptr, *t_off = nullptr;
@@ -189,7 +192,7 @@ static int32_t ensure_pool(ltl_ctx *c) {
 
 static int32_t carve(ltl_ctx *c) {
     const size_t budget = c->pool_bytes;
-    const size_t per_cs = (size_t)c->nw * 8 + 17;
+    const size_t per_cs = (size_t)c->nw * 8 + (size_t)c->nsw * 32 + 17;
     // chunk: per-candidate temporaries (14 B each) within 1/32 of the pool
     int64_t ch = 1 << 26;
     while (ch > 4096 && (size_t)ch * 14 > budget / 32) ch >>= 1;
@@ -197,6 +200,7 @@ static int32_t carve(ltl_ctx *c) {
     const size_t tiles = (size_t)ch / 64;
     c->scan_bytes = scan_temp_bytes((int)tiles);
     size_t fixed = align_up((size_t)c->nw * sizeof(WordAux), 256) + align_up(sizeof(DevState), 256) +
+                   align_up((size_t)c->nsw * sizeof(SumAux), 256) +
                    align_up((size_t)ch * 4, 256) + align_up((size_t)ch * 8, 256) + align_up(tiles * 16, 256) +
                    2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) + 8 * 256;
     if (fixed >= budget) return LTL_E_OOM;
@@ -230,6 +234,7 @@ static int32_t carve(ltl_ctx *c) {
     auto take = [&](size_t n) { char *q = p; p += align_up(n, 256); return q; };
     c->d_aux = (WordAux *)take((size_t)c->nw * sizeof(WordAux));
     c->d_state = (DevState *)take(sizeof(DevState));
+    c->d_saux = (SumAux *)take((size_t)c->nsw * sizeof(SumAux));
     c->t_slot = (uint32_t *)take((size_t)ch * 4);
     c->t_fp2 = (uint64_t *)take((size_t)ch * 8);
     c->t_mask = (uint64_t *)take(tiles * 16);
@@ -243,6 +248,7 @@ static int32_t carve(ltl_ctx *c) {
     c->d_right = (int32_t *)take((size_t)capr * 4);
     c->d_fp2 = (uint64_t *)take((size_t)capr * 8);
     c->d_store = (uint64_t *)take((size_t)capr * c->nw * 8);
+    c->d_sum = (uint64_t *)take((size_t)capr * c->nsw * 32);
     if ((size_t)(p - c->pool) > budget) return LTL_E_OOM;
     return 0;
 }
@@ -299,8 +305,39 @@ struct Run {
     ChunkArgs base{};
     uint64_t all_ltls = 0;
     uint64_t candidates = 0, alg_bytes = 0;
-    bool otf = false;  // overflow seen at a previous level end
+    bool otf = false;         // overflow seen at a previous level end
+    bool check_only = false;  // blocks are tested only (precheck pass)
+    uint64_t checked = 0;
     std::vector<Block> blocks;
+
+    // Check-only pass over one block: no temporaries, so no chunking beyond
+    // keeping the grid within limits.
+    int32_t run_check(int op, ChunkArgs a) {
+        ltl_ctx *c = this->c;
+        const bool bin = op == LTL_AND || op == LTL_OR || op == LTL_UNTIL;
+        const uint64_t sv = (uint64_t)c->nsw * 8, cs = (uint64_t)c->nw * 8;
+        uint64_t cand, alg, mn;
+        int kind;
+        if (bin) {
+            a.nbs = (int32_t)((a.b1 - a.b0 + 63) / 64);
+            a.n_tiles = (int32_t)((a.a1 - a.a0) * a.nbs);
+            const uint64_t pairs = (uint64_t)(a.a1 - a.a0) * (uint64_t)(a.b1 - a.b0);
+            const uint64_t per = op == LTL_UNTIL ? cs : sv;
+            cand = op == LTL_UNTIL ? 2 * pairs : pairs;
+            alg = pairs * 2 * per;
+            mn = (uint64_t)(a.a1 - a.a0 + a.b1 - a.b0) * per;
+            kind = op == LTL_UNTIL ? K_CHKU : K_CHK2;
+        } else {
+            a.nbs = 1;
+            a.n_tiles = (int32_t)((a.a1 - a.a0 + 63) / 64);
+            cand = (uint64_t)(a.a1 - a.a0);
+            alg = mn = cand * sv;
+            kind = K_CHK1;
+        }
+        checked += cand;
+        HIP_TRY(timed(c, kind, alg, mn, cand, [&] { return launch_check(op, c->B, a, c->stream); }));
+        return 0;
+    }
 
     int32_t run_chunk(int op, ChunkArgs a) {
         ltl_ctx *c = this->c;
@@ -343,13 +380,14 @@ struct Run {
         const int64_t n = e - s;
         if (n <= 0) return 0;
         blocks.push_back({all_ltls, (uint64_t)n, op, s, n, 0, 1});
-        for (int64_t a0 = 0; a0 < n; a0 += c->CH) {
+        const int64_t step = check_only ? ((int64_t)1 << 36) : c->CH;
+        for (int64_t a0 = 0; a0 < n; a0 += step) {
             ChunkArgs a = base;
             a.id_base = all_ltls;
             a.ls = s; a.rs = 0; a.nR = 1;
-            a.a0 = a0; a.a1 = std::min<int64_t>(n, a0 + c->CH);
+            a.a0 = a0; a.a1 = std::min<int64_t>(n, a0 + step);
             a.b0 = a.b1 = 0;
-            if (int32_t r = run_chunk(op, a)) return r;
+            if (int32_t r = check_only ? run_check(op, a) : run_chunk(op, a)) return r;
         }
         all_ltls += (uint64_t)n;
         return 0;
@@ -361,6 +399,20 @@ struct Run {
         const int nd = op == LTL_UNTIL ? 2 : 1;
         const uint64_t size = (uint64_t)nL * (uint64_t)nR * nd;
         blocks.push_back({all_ltls, size, op, ls, nL, rs, nR});
+        if (check_only) {
+            const int64_t nbs = (nR + 63) / 64;
+            const int64_t rows = std::max<int64_t>(4, ((int64_t)1 << 30) / nbs * 4);
+            for (int64_t a0 = 0; a0 < nL; a0 += rows) {
+                ChunkArgs a = base;
+                a.id_base = all_ltls;
+                a.ls = ls; a.rs = rs; a.nR = nR;
+                a.a0 = a0; a.a1 = std::min<int64_t>(nL, a0 + rows);
+                a.b0 = 0; a.b1 = nR;
+                if (int32_t r = run_check(op, a)) return r;
+            }
+            all_ltls += size;
+            return 0;
+        }
         const int64_t row_slots = (nR + 63) / 64 * 64 * nd;
         if (row_slots <= c->CH) {
             const int64_t rows = std::max<int64_t>(1, c->CH / row_slots);
@@ -497,6 +549,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     while (B < maxlen) B <<= 1;
     c->B = B; c->TPW = 64 / B; c->W = W; c->P = P; c->A = A;
     c->nw = (W + c->TPW - 1) / c->TPW;
+    c->nsw = (W + 63) / 64;
     if (int32_t e = ensure_pool(c)) return res->status = e;
     if (int32_t e = carve(c)) return res->status = e;
     if (c->capacity < A) return res->status = LTL_E_OOM;
@@ -522,13 +575,34 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         for (int i = 0; i < W; ++i)
             stripe0[(size_t)(i / c->TPW) * 64 + a] |=
                 (pb->atoms[(size_t)a * W + i] & len_mask(pb->lens[i])) << ((i % c->TPW) * B);
+    // Summaries (b0, b1, nz, fl per trace, 64 traces per word) of the atoms,
+    // and the per-summary-word target and valid masks.
+    const int nsw = c->nsw;
+    std::vector<uint64_t> sum0((size_t)4 * nsw * 64, 0);
+    std::vector<SumAux> saux(nsw, SumAux{0, 0});
+    for (int i = 0; i < W; ++i) {
+        saux[i / 64].val |= 1ull << (i % 64);
+        if (i < P) saux[i / 64].tgt |= 1ull << (i % 64);
+    }
+    for (int a = 0; a < A; ++a)
+        for (int i = 0; i < W; ++i) {
+            const uint64_t m = len_mask(pb->lens[i]), x = pb->atoms[(size_t)a * W + i] & m;
+            const uint64_t bit = 1ull << (i % 64);
+            const int j = i / 64;
+            if (x & 1) sum0[(size_t)(0 * nsw + j) * 64 + a] |= bit;
+            if (x & 2) sum0[(size_t)(1 * nsw + j) * 64 + a] |= bit;
+            if (x) sum0[(size_t)(2 * nsw + j) * 64 + a] |= bit;
+            if (x == m) sum0[(size_t)(3 * nsw + j) * 64 + a] |= bit;
+        }
     DevState st0{};
     st0.found = kEmpty;
     st0.n_stored = A;
     st0.overflow_chunk = LLONG_MAX;
     hipStream_t s = c->stream;
     HIP_TRY(hipMemcpyAsync(c->d_aux, aux.data(), aux.size() * sizeof(WordAux), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->d_saux, saux.data(), saux.size() * sizeof(SumAux), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->d_store, stripe0.data(), stripe0.size() * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(c->d_sum, sum0.data(), sum0.size() * 8, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->d_state, &st0, sizeof(st0), hipMemcpyHostToDevice, s));
     HIP_TRY(launch_fill_u64(c->d_tab, kEmpty, 2 * ((size_t)c->tmask + 1), s));
 
@@ -537,6 +611,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     ba.store = c->d_store; ba.aux = c->d_aux; ba.tab = c->d_tab; ba.st = c->d_state;
     ba.t_slot = c->t_slot; ba.t_fp2 = c->t_fp2; ba.t_mask = c->t_mask; ba.t_cnt = c->t_cnt; ba.t_off = c->t_off;
     ba.m_op = c->d_op; ba.m_left = c->d_left; ba.m_right = c->d_right; ba.m_fp2 = c->d_fp2;
+    ba.sum = c->d_sum; ba.saux = c->d_saux; ba.nsw = nsw;
     ba.capacity = c->capacity;
     ba.nw = c->nw;
     ba.hamming_k = o.hamming_k;
@@ -564,6 +639,23 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         if (h.probe_fail) return LTL_E_OOM;
         return 0;
     };
+    // The blocks of level `cost` in the reference's order
+    // (code/ltli6463.cu:1209-1440).
+    auto level_blocks = [&](int cost) -> int32_t {
+        int32_t r = 0;
+        if ((o.neg_type == 1 && cost - c2 >= c1) || (o.neg_type == 2 && cost - c2 == c1))
+            if ((r = run.unary(LTL_NOT, Ls[cost - c2], Le[cost - c2]))) return r;
+        for (int i = c1; 2 * i <= cost - c3; ++i)
+            if ((r = run.binary(LTL_AND, Ls[i], Le[i], Ls[cost - i - c3], Le[cost - i - c3]))) return r;
+        for (int i = c1; 2 * i <= cost - c4; ++i)
+            if ((r = run.binary(LTL_OR, Ls[i], Le[i], Ls[cost - i - c4], Le[cost - i - c4]))) return r;
+        if (cost - c5 >= c1 && (r = run.unary(LTL_NEXT, Ls[cost - c5], Le[cost - c5]))) return r;
+        if (cost - c6 >= c1 && (r = run.unary(LTL_FIN, Ls[cost - c6], Le[cost - c6]))) return r;
+        if (cost - c7 >= c1 && (r = run.unary(LTL_GLOB, Ls[cost - c7], Le[cost - c7]))) return r;
+        for (int i = c1; 2 * i <= cost - c8; ++i)
+            if ((r = run.binary(LTL_UNTIL, Ls[i], Le[i], Ls[cost - i - c8], Le[cost - i - c8]))) return r;
+        return 0;
+    };
     int64_t n_stored = A;
     for (cost = c1 + 1; cost <= maxc; ++cost) {
         if (run.otf) {
@@ -572,23 +664,30 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
                 last_round = true;
         }
         Ls[cost] = n_stored;
-        if ((o.neg_type == 1 && cost - c2 >= c1) || (o.neg_type == 2 && cost - c2 == c1))
-            if ((rc = run.unary(LTL_NOT, Ls[cost - c2], Le[cost - c2]))) break;
-        for (int i = c1; 2 * i <= cost - c3 && !rc; ++i)
-            rc = run.binary(LTL_AND, Ls[i], Le[i], Ls[cost - i - c3], Le[cost - i - c3]);
-        for (int i = c1; 2 * i <= cost - c4 && !rc; ++i)
-            rc = run.binary(LTL_OR, Ls[i], Le[i], Ls[cost - i - c4], Le[cost - i - c4]);
-        if (!rc && cost - c5 >= c1) rc = run.unary(LTL_NEXT, Ls[cost - c5], Le[cost - c5]);
-        if (!rc && cost - c6 >= c1) rc = run.unary(LTL_FIN, Ls[cost - c6], Le[cost - c6]);
-        if (!rc && cost - c7 >= c1) rc = run.unary(LTL_GLOB, Ls[cost - c7], Le[cost - c7]);
-        for (int i = c1; 2 * i <= cost - c8 && !rc; ++i)
-            rc = run.binary(LTL_UNTIL, Ls[i], Le[i], Ls[cost - i - c8], Le[cost - i - c8]);
-        if (rc) break;
-        if ((rc = sync_state())) break;
-        n_stored = h.n_stored;
+        // Precheck: the first accepted candidate of a level is always the
+        // first occurrence of its set (an earlier equal set would have been
+        // accepted first), so a check-only pass finds the same result as
+        // generation with dedup, without generating the level.
+        bool generate = true;
+        if (o.precheck || run.otf) {
+            const uint64_t save = run.all_ltls;
+            const size_t nb = run.blocks.size();
+            run.check_only = true;
+            rc = level_blocks(cost);
+            run.check_only = false;
+            if (rc || (rc = sync_state())) break;
+            if (h.found != kEmpty) { found = true; Le[cost] = n_stored; break; }
+            if (run.otf) generate = false;  // check-only level (overflow)
+            else { run.all_ltls = save; run.blocks.resize(nb); }
+        }
+        if (generate) {
+            if ((rc = level_blocks(cost))) break;
+            if ((rc = sync_state())) break;
+            n_stored = h.n_stored;
+            if (h.found != kEmpty) { found = true; Le[cost] = n_stored; break; }
+            if (h.overflow_chunk != LLONG_MAX) run.otf = true;
+        }
         Le[cost] = n_stored;
-        if (h.found != kEmpty) { found = true; break; }
-        if (h.overflow_chunk != LLONG_MAX) run.otf = true;
         if (last_round) break;
         if (run.otf && shortage == -1) shortage = cost;
     }
@@ -602,6 +701,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     res->n_stored = n_stored;
     res->on_the_fly = run.otf || h.overflow_chunk != LLONG_MAX;
     res->candidates = run.candidates;
+    res->checked = run.checked;
     res->hbm_bytes = run.alg_bytes;
     for (auto &k : c->stats) res->kernel_seconds += k.seconds;
 

@@ -53,10 +53,11 @@ def check_parity(ltl, O, learner, inst, **opts):
     return gpu, ref
 
 
+@pytest.mark.parametrize("precheck", [0, 1])
 @pytest.mark.parametrize("seed", [1, 2, 3])
-def test_config1(ltl, oracle, learner, seed):
+def test_config1(ltl, oracle, learner, seed, precheck):
     inst = ltl.generate(1, seed=seed)
-    gpu, ref = check_parity(ltl, oracle, learner, inst, max_cost=10)
+    gpu, ref = check_parity(ltl, oracle, learner, inst, max_cost=10, precheck=precheck)
     assert gpu["status"] == 1 and gpu["cost"] <= 5
 
 
@@ -82,11 +83,12 @@ def test_random_not_found_full_store(ltl, oracle, learner, shape):
     check_parity(ltl, oracle, learner, inst, max_cost=mc)
 
 
+@pytest.mark.parametrize("precheck", [0, 1])
 @pytest.mark.parametrize("shape", SHAPES, ids=lambda s: "x".join(map(str, s)))
-def test_random_found(ltl, oracle, learner, shape):
+def test_random_found(ltl, oracle, learner, shape, precheck):
     P, N, A, lo, hi, _ = shape
     inst = ltl.generate_custom(7 + N, P, N, A, lo, hi, 0.5, planted_size=5)
-    gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=7)
+    gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=7, precheck=precheck)
     assert gpu["status"] == 1
 
 
@@ -96,16 +98,18 @@ def test_nonunit_costs(ltl, oracle, learner, cost):
     check_parity(ltl, oracle, learner, inst, max_cost=14, cost=cost)
 
 
+@pytest.mark.parametrize("precheck", [0, 1])
 @pytest.mark.parametrize("k", [1, 3])
-def test_hamming(ltl, oracle, learner, k):
+def test_hamming(ltl, oracle, learner, k, precheck):
     inst = ltl.generate_custom(3, 30, 30, 3, 5, 20, 0.5, planted_size=9)
-    check_parity(ltl, oracle, learner, inst, max_cost=7, hamming_k=k)
+    check_parity(ltl, oracle, learner, inst, max_cost=7, hamming_k=k, precheck=precheck)
 
 
-@pytest.mark.parametrize("capacity", [64, 128, 320])
-def test_overflow_check_only(ltl, oracle, learner, capacity):
+@pytest.mark.parametrize("precheck", [0, 1])
+@pytest.mark.parametrize("capacity", [64, 100, 320])
+def test_overflow_check_only(ltl, oracle, learner, capacity, precheck):
     inst = ltl.generate_custom(5, 16, 16, 3, 4, 12, 0.5, planted_size=8)
-    check_parity(ltl, oracle, learner, inst, max_cost=8, capacity=capacity)
+    check_parity(ltl, oracle, learner, inst, max_cost=8, capacity=capacity, precheck=precheck)
 
 
 def test_config2_levels(ltl, oracle, learner):
