@@ -10,8 +10,10 @@
 //           duplicates are verified against fingerprint 2 of the winner;
 //   scan    survivor counts per 64-candidate tile -> store offsets;
 //   finish  advance the store count, detect overflow;
-//   emit    recompute the survivors and append them, in canonical order,
-//           to the store; commit their table values to store indices.
+//   compact the survivors' candidate indices, in canonical order;
+//   emit    one lane per output position: recompute the survivor, append it
+//           to the store (full contiguous segments per wave) with its
+//           summaries and parent indices; commit its table value.
 // This replaces the reference's per-thread operator kernels + Thrust
 // compaction (code/ltli6463.cu:176-873, 994-1035).  Operators follow
 // code/ltli6463.cu:203, 295, 385, 468, 551-556, 638-645, 739-771, applied
@@ -108,7 +110,7 @@ __device__ __forceinline__ void flush_summary(const ChunkArgs &A, uint64_t *z, i
     const int j = w / B;
     if (on) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) z[(size_t)(k * A.nsw + j) << 6] = v[k];
+        for (int k = 0; k < 4; ++k) z[(size_t)(k * A.nsw + j) * A.ws] = v[k];
     }
     v[0] = v[1] = v[2] = v[3] = 0;
 }
@@ -133,17 +135,18 @@ __device__ uint32_t tab_insert(uint64_t *tab, uint32_t tmask, uint64_t key, uint
             if (k == kEmpty) k = key;
         }
         if (k == key) {
-            atomicMin(kp + 1, (unsigned long long)val);
+            // Values only decrease; a read that is already <= val makes the
+            // atomic a no-op.  Popular sets (the empty set under G, the full
+            // set under F, ...) are committed early, so their many duplicates
+            // become cached loads instead of serialized atomics on one slot.
+            unsigned long long cur = __hip_atomic_load(kp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((unsigned long long)val < cur) atomicMin(kp + 1, (unsigned long long)val);
             return p;
         }
         p = (p + 1) & tmask;
     }
     st->probe_fail = 1;
     return 0;
-}
-
-__device__ __forceinline__ size_t row0(int64_t s, int nw) {
-    return (size_t)(s >> 6) * ((size_t)nw << 6) + (size_t)(s & 63);
 }
 
 // Workgroups of a binary chunk are laid out so that consecutive workgroups
@@ -199,8 +202,8 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     if (!tile_binary(A, t)) return;
     const bool otf = chunk_otf(A);
     const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
-    const uint64_t *pl = A.store + row0(sl, A.nw);
-    const uint64_t *pr = A.store + row0(sr, A.nw);
+    const uint64_t *pl = A.store + (size_t)sl;
+    const uint64_t *pr = A.store + (size_t)sr;
     Acc c0, c1;
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux x = A.aux[w];
@@ -218,11 +221,11 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     for (; w + UB <= A.nw; w += UB) {
         uint64_t l[UB], r[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) << 6]; r[u] = pr[(size_t)(w + u) << 6]; }
+        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) * A.ws]; r[u] = pr[(size_t)(w + u) * A.ws]; }
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
     }
-    for (; w < A.nw; ++w) step(w, pl[(size_t)w << 6], pr[(size_t)w << 6]);
+    for (; w < A.nw; ++w) step(w, pl[(size_t)w * A.ws], pr[(size_t)w * A.ws]);
     if (!t.valid) return;
     const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
     const uint64_t id0 = A.id_base + (OP == LTL_UNTIL ? 2 * pair : pair);
@@ -248,7 +251,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     if (!tile_unary(A, t)) return;
     const bool otf = chunk_otf(A);
     const int64_t s = A.ls + (t.valid ? t.a : A.a0);
-    const uint64_t *p = A.store + row0(s, A.nw);
+    const uint64_t *p = A.store + (size_t)s;
     Acc c0;
     auto step = [&](int w, uint64_t v) {
         const WordAux x = A.aux[w];
@@ -259,11 +262,11 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     for (; w + UB <= A.nw; w += UB) {
         uint64_t v[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) v[u] = p[(size_t)(w + u) << 6];
+        for (int u = 0; u < UB; ++u) v[u] = p[(size_t)(w + u) * A.ws];
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, v[u]);
     }
-    for (; w < A.nw; ++w) step(w, p[(size_t)w << 6]);
+    for (; w < A.nw; ++w) step(w, p[(size_t)w * A.ws]);
     if (!t.valid) return;
     const uint64_t id0 = A.id_base + (uint64_t)t.a;
     if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
@@ -279,9 +282,9 @@ template <int B>
 __global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms) {
     const int a = (int)threadIdx.x;
     if (a >= n_atoms) return;
-    const uint64_t *p = A.store + row0(a, A.nw);
+    const uint64_t *p = A.store + (size_t)a;
     Acc c0;
-    for (int w = 0; w < A.nw; ++w) fold(c0, p[(size_t)w << 6], A.aux[w]);
+    for (int w = 0; w < A.nw; ++w) fold(c0, p[(size_t)w * A.ws], A.aux[w]);
     tab_insert(A.tab, A.tmask, table_key(c0.f1), (uint64_t)a, A.st);
     A.m_fp2[a] = c0.f2;
     A.m_op[a] = LTL_ATOM;
@@ -388,90 +391,71 @@ __device__ __forceinline__ void commit(const ChunkArgs &A, int64_t pos, size_t k
     A.tab[2 * (size_t)A.t_slot[k] + 1] = (uint64_t)pos;
 }
 
-template <int OP, int B>
-__global__ __launch_bounds__(256) void k_emit2(ChunkArgs A) {
+template <int OP>
+__global__ __launch_bounds__(256) void k_compact(ChunkArgs A) {
     if (run_stopped(A) || chunk_otf(A)) return;
-    Tile t;
-    if (!tile_binary(A, t)) return;
-    if (A.t_cnt[t.ti] == 0) return;
+    const int64_t ti = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ti >= A.n_tiles || A.t_cnt[ti] == 0) return;
     constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
+    const uint64_t below = lanes_below(), bit = 1ull << (threadIdx.x & 63);
+    const uint64_t m0 = A.t_mask[ND * ti], m1 = ND == 2 ? A.t_mask[ND * ti + 1] : 0;
+    const uint32_t p0 = A.t_off[ti] + (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below));
+    const uint32_t ix = (uint32_t)(ti * 64 + (threadIdx.x & 63));
+    if (m0 & bit) A.t_list[p0] = ND * ix;
+    if (m1 & bit) A.t_list[p0 + ((m0 & bit) ? 1 : 0)] = ND * ix + 1;
+}
+
+template <int OP, int B>
+__global__ __launch_bounds__(256) void k_emit(ChunkArgs A) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= A.st->chunk_total) return;
+    const int64_t pos = A.st->chunk_base + i;
+    if (pos >= A.capacity) return;
+    constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
     constexpr int TPW = 64 / B;
-    const uint64_t below = lanes_below();
-    const uint64_t bit = 1ull << (threadIdx.x & 63);
-    const uint64_t m0 = A.t_mask[ND * t.ti], m1 = ND == 2 ? A.t_mask[ND * t.ti + 1] : 0;
-    const int64_t base = A.st->chunk_base + (int64_t)A.t_off[t.ti];
-    const int64_t pos0 = base + __popcll(m0 & below) + __popcll(m1 & below);
-    const int64_t pos1 = pos0 + ((m0 & bit) ? 1 : 0);
-    const bool w0 = (m0 & bit) && pos0 < A.capacity;
-    const bool w1 = (m1 & bit) && pos1 < A.capacity;
-    const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
-    const uint64_t *pl = A.store + row0(sl, A.nw);
-    const uint64_t *pr = A.store + row0(sr, A.nw);
-    uint64_t *q0 = A.store + row0(w0 ? pos0 : 0, A.nw);
-    uint64_t *q1 = A.store + row0(w1 ? pos1 : 0, A.nw);
-    uint64_t *z0 = A.sum + row0(w0 ? pos0 : 0, 4 * A.nsw);
-    uint64_t *z1 = A.sum + row0(w1 ? pos1 : 0, 4 * A.nsw);
-    uint64_t v0[4] = {0, 0, 0, 0}, v1[4] = {0, 0, 0, 0};
+    const uint32_t k = A.t_list[i];           // temp index of the candidate
+    const uint32_t ix = OP == LTL_UNTIL ? k >> 1 : k;
+    const bool swap = OP == LTL_UNTIL && (k & 1);
+    int64_t sl, sr = -1;
+    if (bin) {
+        const int64_t ti = ix / 64;
+        sl = A.ls + A.a0 + ti / A.nbs;
+        sr = A.rs + A.b0 + (ti % A.nbs) * 64 + (ix % 64);
+    } else {
+        sl = A.ls + A.a0 + ix;
+    }
+    const uint64_t *pl = A.store + (size_t)sl;
+    const uint64_t *pr = A.store + (size_t)(bin ? sr : sl);
+    uint64_t *q = A.store + (size_t)pos;
+    uint64_t *z = A.sum + (size_t)pos;
+    uint64_t v[4] = {0, 0, 0, 0};
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const uint64_t lm = A.aux[w].lm;
-        const int sh = (w % B) * TPW;
-        uint64_t c0, c1 = 0;
-        if constexpr (OP == LTL_AND) c0 = l & r;
-        else if constexpr (OP == LTL_OR) c0 = l | r;
-        else { c0 = f_until<B>(l, r); c1 = f_until<B>(r, l); }
-        if (w0) { q0[(size_t)w << 6] = c0; summarize<B>(c0, lm, sh, v0); }
-        if (ND == 2 && w1) { q1[(size_t)w << 6] = c1; summarize<B>(c1, lm, sh, v1); }
-        flush_summary<B>(A, z0, w, v0, w0);
-        if (ND == 2) flush_summary<B>(A, z1, w, v1, w1);
+        uint64_t c;
+        if constexpr (OP == LTL_AND) c = l & r;
+        else if constexpr (OP == LTL_OR) c = l | r;
+        else if constexpr (OP == LTL_UNTIL) c = swap ? f_until<B>(r, l) : f_until<B>(l, r);
+        else c = apply1<OP, B>(l, lm);
+        q[(size_t)w * A.ws] = c;
+        summarize<B>(c, lm, (w % B) * TPW, v);
+        flush_summary<B>(A, z, w, v, true);
     };
-    constexpr int UB = 4;
+    constexpr int UB = bin ? 4 : 8;
     int w = 0;
     for (; w + UB <= A.nw; w += UB) {
         uint64_t l[UB], r[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) << 6]; r[u] = pr[(size_t)(w + u) << 6]; }
+        for (int u = 0; u < UB; ++u) {
+            l[u] = pl[(size_t)(w + u) * A.ws];
+            r[u] = bin ? pr[(size_t)(w + u) * A.ws] : 0;
+        }
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
     }
-    for (; w < A.nw; ++w) step(w, pl[(size_t)w << 6], pr[(size_t)w << 6]);
-    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
-    if (w0) commit(A, pos0, ND * ix, OP, (int32_t)sl, (int32_t)sr);
-    if (w1) commit(A, pos1, ND * ix + 1, OP, (int32_t)sr, (int32_t)sl);
-}
-
-template <int OP, int B>
-__global__ __launch_bounds__(256) void k_emit1(ChunkArgs A) {
-    if (run_stopped(A) || chunk_otf(A)) return;
-    Tile t;
-    if (!tile_unary(A, t)) return;
-    if (A.t_cnt[t.ti] == 0) return;
-    constexpr int TPW = 64 / B;
-    const uint64_t m0 = A.t_mask[t.ti];
-    const uint64_t bit = 1ull << (threadIdx.x & 63);
-    const int64_t pos = A.st->chunk_base + (int64_t)A.t_off[t.ti] + __popcll(m0 & lanes_below());
-    const bool w0 = (m0 & bit) && pos < A.capacity;
-    const int64_t s = A.ls + (t.valid ? t.a : A.a0);
-    const uint64_t *p = A.store + row0(s, A.nw);
-    uint64_t *q = A.store + row0(w0 ? pos : 0, A.nw);
-    uint64_t *z = A.sum + row0(w0 ? pos : 0, 4 * A.nsw);
-    uint64_t v[4] = {0, 0, 0, 0};
-    auto step = [&](int w, uint64_t x) {
-        const uint64_t lm = A.aux[w].lm;
-        const uint64_t c = apply1<OP, B>(x, lm);
-        if (w0) { q[(size_t)w << 6] = c; summarize<B>(c, lm, (w % B) * TPW, v); }
-        flush_summary<B>(A, z, w, v, w0);
-    };
-    constexpr int UB = 8;
-    int w = 0;
-    for (; w + UB <= A.nw; w += UB) {
-        uint64_t x[UB];
-#pragma unroll
-        for (int u = 0; u < UB; ++u) x[u] = p[(size_t)(w + u) << 6];
-#pragma unroll
-        for (int u = 0; u < UB; ++u) step(w + u, x[u]);
-    }
-    for (; w < A.nw; ++w) step(w, p[(size_t)w << 6]);
-    if (w0) commit(A, pos, (size_t)t.ti * 64 + (threadIdx.x & 63), OP, (int32_t)s, -1);
+    for (; w < A.nw; ++w) step(w, pl[(size_t)w * A.ws], bin ? pr[(size_t)w * A.ws] : 0);
+    if (bin) commit(A, pos, k, OP, (int32_t)(swap ? sr : sl), (int32_t)(swap ? sl : sr));
+    else commit(A, pos, k, OP, (int32_t)sl, -1);
 }
 
 // ---------------------------------------------------------------------------
@@ -484,10 +468,10 @@ __global__ __launch_bounds__(256) void k_chk1(ChunkArgs A) {
     Tile t;
     if (!tile_unary(A, t) || !t.valid) return;
     constexpr int VEC = OP == LTL_NOT ? 0 : OP == LTL_NEXT ? 1 : OP == LTL_FIN ? 2 : 3;
-    const uint64_t *p = A.sum + row0(A.ls + t.a, 4 * A.nsw) + ((size_t)(VEC * A.nsw) << 6);
+    const uint64_t *p = A.sum + (size_t)(A.ls + t.a) + (size_t)(VEC * A.nsw) * A.ws;
     uint32_t m = 0;
     for (int j = 0; j < A.nsw; ++j) {
-        uint64_t v = p[(size_t)j << 6];
+        uint64_t v = p[(size_t)j * A.ws];
         if (OP == LTL_NOT) v = ~v;
         m += (uint32_t)__popcll((v ^ A.saux[j].tgt) & A.saux[j].val);
     }
@@ -501,11 +485,11 @@ __global__ __launch_bounds__(256) void k_chk2(ChunkArgs A) {
     Tile t;
     if (!tile_binary(A, t)) return;
     const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
-    const uint64_t *pl = A.sum + row0(sl, 4 * A.nsw);
-    const uint64_t *pr = A.sum + row0(sr, 4 * A.nsw);
+    const uint64_t *pl = A.sum + (size_t)sl;
+    const uint64_t *pr = A.sum + (size_t)sr;
     uint32_t m = 0;
     for (int j = 0; j < A.nsw; ++j) {
-        const uint64_t l = pl[(size_t)j << 6], r = pr[(size_t)j << 6];
+        const uint64_t l = pl[(size_t)j * A.ws], r = pr[(size_t)j * A.ws];
         const uint64_t c = OP == LTL_AND ? (l & r) : (l | r);
         m += (uint32_t)__popcll((c ^ A.saux[j].tgt) & A.saux[j].val);
     }
@@ -514,31 +498,51 @@ __global__ __launch_bounds__(256) void k_chk2(ChunkArgs A) {
     atomicMin(&A.st->found, (unsigned long long)(A.id_base + pair));
 }
 
-// U: position 0 depends on the whole trace; both orders from the full words.
+// U: position 0 of l U r holds iff r has a set bit at or below the first
+// position where l is false: with n = ~l, n ^ (n - 1) masks the positions up
+// to and including n's lowest set bit.  Per slot (the subtraction must not
+// borrow across slots), 32-bit halves for B = 32, the whole word for B = 64.
+// Bits at or above the trace length are zero in l, so n's lowest set bit is
+// at most the length; r has no bits there.
+template <int B>
+__device__ __forceinline__ uint64_t until_at0(uint64_t l, uint64_t r) {
+    if constexpr (B == 64) {
+        const uint64_t n = ~l;
+        return (uint64_t)((r & (n ^ (n - 1))) != 0);
+    } else if constexpr (B == 32) {
+        const uint32_t n0 = ~(uint32_t)l, n1 = ~(uint32_t)(l >> 32);
+        const uint32_t h0 = ((uint32_t)r & (n0 ^ (n0 - 1))) != 0;
+        const uint32_t h1 = ((uint32_t)(r >> 32) & (n1 ^ (n1 - 1))) != 0;
+        return (uint64_t)h0 | ((uint64_t)h1 << 32);
+    } else {
+        return f_until<B>(l, r);  // narrow slots: the short prefix is cheaper
+    }
+}
+
 template <int B>
 __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
     if (!tile_binary(A, t)) return;
     const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
-    const uint64_t *pl = A.store + row0(sl, A.nw);
-    const uint64_t *pr = A.store + row0(sr, A.nw);
+    const uint64_t *pl = A.store + (size_t)sl;
+    const uint64_t *pr = A.store + (size_t)sr;
     uint32_t m0 = 0, m1 = 0;
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux &x = A.aux[w];
-        m0 += (uint32_t)__popcll((f_until<B>(l, r) ^ x.tgt) & x.chk);
-        m1 += (uint32_t)__popcll((f_until<B>(r, l) ^ x.tgt) & x.chk);
+        m0 += (uint32_t)__popcll((until_at0<B>(l, r) ^ x.tgt) & x.chk);
+        m1 += (uint32_t)__popcll((until_at0<B>(r, l) ^ x.tgt) & x.chk);
     };
     constexpr int UB = 4;
     int w = 0;
     for (; w + UB <= A.nw; w += UB) {
         uint64_t l[UB], r[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) << 6]; r[u] = pr[(size_t)(w + u) << 6]; }
+        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) * A.ws]; r[u] = pr[(size_t)(w + u) * A.ws]; }
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
     }
-    for (; w < A.nw; ++w) step(w, pl[(size_t)w << 6], pr[(size_t)w << 6]);
+    for (; w < A.nw; ++w) step(w, pl[(size_t)w * A.ws], pr[(size_t)w * A.ws]);
     if (!t.valid) return;
     const uint64_t id0 = A.id_base + 2 * ((uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b);
     if (m0 <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
@@ -583,14 +587,31 @@ hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
+    // one lane per survivor; the survivor count is on the device, so the
+    // grid covers every candidate of the chunk and idle lanes exit
+    const int nd = op == LTL_UNTIL ? 2 : 1;
+    const dim3 g((unsigned)(((int64_t)a.n_tiles * 64 * nd + 255) / 256)), b(256);
     switch (op) {
-    case LTL_NOT: LTL_DISPATCH_B(k_emit1, LTL_NOT, B, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_NEXT: LTL_DISPATCH_B(k_emit1, LTL_NEXT, B, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_FIN: LTL_DISPATCH_B(k_emit1, LTL_FIN, B, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_GLOB: LTL_DISPATCH_B(k_emit1, LTL_GLOB, B, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_AND: LTL_DISPATCH_B(k_emit2, LTL_AND, B, grid_binary(a), dim3(256), s, a); break;
-    case LTL_OR: LTL_DISPATCH_B(k_emit2, LTL_OR, B, grid_binary(a), dim3(256), s, a); break;
-    case LTL_UNTIL: LTL_DISPATCH_B(k_emit2, LTL_UNTIL, B, grid_binary(a), dim3(256), s, a); break;
+    case LTL_NOT: LTL_DISPATCH_B(k_emit, LTL_NOT, B, g, b, s, a); break;
+    case LTL_NEXT: LTL_DISPATCH_B(k_emit, LTL_NEXT, B, g, b, s, a); break;
+    case LTL_FIN: LTL_DISPATCH_B(k_emit, LTL_FIN, B, g, b, s, a); break;
+    case LTL_GLOB: LTL_DISPATCH_B(k_emit, LTL_GLOB, B, g, b, s, a); break;
+    case LTL_AND: LTL_DISPATCH_B(k_emit, LTL_AND, B, g, b, s, a); break;
+    case LTL_OR: LTL_DISPATCH_B(k_emit, LTL_OR, B, g, b, s, a); break;
+    case LTL_UNTIL: LTL_DISPATCH_B(k_emit, LTL_UNTIL, B, g, b, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_compact(int op, const ChunkArgs &a, hipStream_t s) {
+    const dim3 g = grid_tiles(a), b(256);
+    switch (op) {
+    case LTL_NOT: case LTL_NEXT: case LTL_FIN: case LTL_GLOB:
+        hipLaunchKernelGGL(k_compact<LTL_NOT>, g, b, 0, s, a); break;
+    case LTL_AND: case LTL_OR:
+        hipLaunchKernelGGL(k_compact<LTL_AND>, g, b, 0, s, a); break;
+    case LTL_UNTIL: hipLaunchKernelGGL(k_compact<LTL_UNTIL>, g, b, 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

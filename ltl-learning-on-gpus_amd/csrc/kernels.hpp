@@ -1,13 +1,15 @@
 // kernels.hpp -- device data layout and launch interface of the enumeration
 // kernels (gfx950, wave64).
 //
-// Store layout ("stripes").  A characteristic set (CS) over W traces is held
-// as nw packed words: trace i sits in slot i % TPW of word i / TPW, where a
-// slot is B bits (B = 8, 16, 32 or 64, the smallest power of two >= the
-// longest trace) and TPW = 64 / B.  Sets are grouped 64 to a stripe; inside
-// a stripe the layout is word-major, so word w of the 64 sets of a stripe is
-// one contiguous 512-byte row:  addr(s, w) = (s/64)*nw*64 + w*64 + s%64.
-// One lane handles one candidate; a wave reads or writes a full row.
+// Store layout.  A characteristic set (CS) over W traces is held as nw
+// packed words: trace i sits in slot i % TPW of word i / TPW, where a slot is
+// B bits (B = 8, 16, 32 or 64, the smallest power of two >= the longest
+// trace) and TPW = 64 / B.  The store is word-major: word w of set s is at
+// w * ws + s (ws = capacity rounded up to 64).  One lane handles one
+// candidate, so a wave reads or writes 64 consecutive sets of one word as a
+// contiguous 512-byte segment, and the waves of a launch sweep adjacent
+// segments of the same word array.  Summaries use the same layout with
+// 4 * nsw words per set.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -58,10 +60,11 @@ struct ChunkArgs {
     uint64_t *t_mask;         // per tile: survivor masks (2 for U)
     uint32_t *t_cnt;          // per tile: survivors
     uint32_t *t_off;          // per tile: exclusive prefix of t_cnt
+    uint32_t *t_list;         // per survivor (chunk order): candidate temp index
     uint8_t *m_op;
     int32_t *m_left, *m_right;
     uint64_t *m_fp2;
-    uint64_t *sum;            // summaries: 4 vectors of nsw words per set, stripe layout
+    uint64_t *sum;            // summaries: 4 vectors of nsw words per set, word-major
     const SumAux *saux;
     int64_t capacity;
     int64_t chunk_idx;
@@ -69,6 +72,7 @@ struct ChunkArgs {
     int64_t ls, rs;           // store index of the left / right level start
     int64_t a0, a1, b0, b1;
     int64_t nR;               // right level size (binary), 1 (unary)
+    int64_t ws;               // word stride of the store and summaries (sets)
     int32_t nw;
     int32_t hamming_k;
     uint32_t tmask;           // table slots - 1
@@ -81,6 +85,9 @@ struct ChunkArgs {
 hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_decide(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_finish(const ChunkArgs &a, hipStream_t s);
+// Survivor list (candidate temp index per output position), then emit with
+// one lane per output position.
+hipError_t launch_compact(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s);
 // Check-only pass over a whole block (no dedup, no store): atomicMin of the

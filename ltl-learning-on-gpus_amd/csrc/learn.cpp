@@ -47,6 +47,8 @@ struct PendingEvent {
     int kind;
     hipEvent_t a, b;
     uint64_t alg, min, items;
+    const long long *count;     // emit: survivors of the chunk (pinned), bytes per survivor below
+    uint64_t alg_per, min_per;
 };
 
 uint64_t splitmix(uint64_t x) {
@@ -57,6 +59,8 @@ uint64_t splitmix(uint64_t x) {
 }
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+constexpr int kCountSlots = 1 << 17;
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -74,6 +78,7 @@ struct ltl_ctx {
     // layout of the last call
     int W = 0, P = 0, A = 0, nw = 0, B = 64, TPW = 1;
     int64_t capacity = 0;
+    int64_t ws = 0;  // word stride of the word-major store (capacity rounded up)
     int64_t CH = 0;  // candidate slots per chunk
     uint32_t tmask = 0;
     WordAux *d_aux = nullptr;
@@ -83,7 +88,7 @@ struct ltl_ctx {
     int nsw = 0;
     uint8_t *d_op = nullptr;
     int32_t *d_left = nullptr, *d_right = nullptr;
-    uint32_t *t_slot = nullptr, *t_cnt = nullptr, *t_off = nullptr;
+    uint32_t *t_slot = nullptr, *t_cnt = nullptr, *t_off = nullptr, *t_list = nullptr;
     uint64_t *t_fp2 = nullptr, *t_mask = nullptr;
     void *d_scan = nullptr;
     size_t scan_bytes = 0;
@@ -93,6 +98,8 @@ struct ltl_ctx {
     ltl_kernel_stat stats[K_NKIND]{};
     std::vector<PendingEvent> pending;
     std::vector<hipEvent_t> free_events;
+    long long *h_counts = nullptr;  // pinned, one slot per profiled emit of a level
+    int n_counts = 0;
     int64_t chunk_counter = 0;
 };
 
@@ -136,7 +143,8 @@ extern "C" ltl_ctx *ltl_create(const ltl_options *opt, int32_t *status) {
     auto *c = new ltl_ctx;
     c->opt = o;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipHostMalloc((void **)&c->h_state, sizeof(DevState), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&c->h_state, sizeof(DevState), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->h_counts, kCountSlots * sizeof(long long), hipHostMallocDefault) != hipSuccess) {
         ltl_destroy(c);
         return fail(LTL_E_HIP);
     }
@@ -160,6 +168,7 @@ extern "C" void ltl_destroy(ltl_ctx *c) {
     for (auto e : c->free_events) hipEventDestroy(e);
     if (c->pool) hipFree(c->pool);
     if (c->h_state) hipHostFree(c->h_state);
+    if (c->h_counts) hipHostFree(c->h_counts);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -193,15 +202,15 @@ static int32_t ensure_pool(ltl_ctx *c) {
 static int32_t carve(ltl_ctx *c) {
     const size_t budget = c->pool_bytes;
     const size_t per_cs = (size_t)c->nw * 8 + (size_t)c->nsw * 32 + 17;
-    // chunk: per-candidate temporaries (14 B each) within 1/32 of the pool
+    // chunk: per-candidate temporaries (18 B each) within 1/32 of the pool
     int64_t ch = 1 << 26;
-    while (ch > 4096 && (size_t)ch * 14 > budget / 32) ch >>= 1;
+    while (ch > 4096 && (size_t)ch * 18 > budget / 32) ch >>= 1;
     c->CH = ch;
     const size_t tiles = (size_t)ch / 64;
     c->scan_bytes = scan_temp_bytes((int)tiles);
     size_t fixed = align_up((size_t)c->nw * sizeof(WordAux), 256) + align_up(sizeof(DevState), 256) +
                    align_up((size_t)c->nsw * sizeof(SumAux), 256) +
-                   align_up((size_t)ch * 4, 256) + align_up((size_t)ch * 8, 256) + align_up(tiles * 16, 256) +
+                   2 * align_up((size_t)ch * 4, 256) + align_up((size_t)ch * 8, 256) + align_up(tiles * 16, 256) +
                    2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) + 8 * 256;
     if (fixed >= budget) return LTL_E_OOM;
     const size_t R = budget - fixed;
@@ -236,6 +245,7 @@ static int32_t carve(ltl_ctx *c) {
     c->d_state = (DevState *)take(sizeof(DevState));
     c->d_saux = (SumAux *)take((size_t)c->nsw * sizeof(SumAux));
     c->t_slot = (uint32_t *)take((size_t)ch * 4);
+    c->t_list = (uint32_t *)take((size_t)ch * 4);
     c->t_fp2 = (uint64_t *)take((size_t)ch * 8);
     c->t_mask = (uint64_t *)take(tiles * 16);
     c->t_cnt = (uint32_t *)take(tiles * 4);
@@ -243,6 +253,7 @@ static int32_t carve(ltl_ctx *c) {
     c->d_scan = take(c->scan_bytes);
     c->d_tab = (uint64_t *)take(slots * 16);
     const int64_t capr = align_up((size_t)cap, 64);
+    c->ws = capr;
     c->d_op = (uint8_t *)take((size_t)capr);
     c->d_left = (int32_t *)take((size_t)capr * 4);
     c->d_right = (int32_t *)take((size_t)capr * 4);
@@ -271,6 +282,11 @@ static void drain_events(ltl_ctx *c) {
     for (auto &p : c->pending) {
         float ms = 0.f;
         hipEventElapsedTime(&ms, p.a, p.b);
+        if (p.count) {
+            p.items = (uint64_t)*p.count;
+            p.alg = p.items * p.alg_per;
+            p.min = p.items * p.min_per;
+        }
         auto &s = c->stats[p.kind];
         s.launches++;
         s.seconds += ms * 1e-3;
@@ -281,12 +297,13 @@ static void drain_events(ltl_ctx *c) {
         c->free_events.push_back(p.b);
     }
     c->pending.clear();
+    c->n_counts = 0;
 }
 
 template <class F>
 static hipError_t timed(ltl_ctx *c, int kind, uint64_t alg, uint64_t mn, uint64_t items, F &&launch) {
     if (!c->opt.profile) return launch();
-    PendingEvent pe{kind, get_event(c), get_event(c), alg, mn, items};
+    PendingEvent pe{kind, get_event(c), get_event(c), alg, mn, items, nullptr, 0, 0};
     hipEventRecord(pe.a, c->stream);
     hipError_t e = launch();
     hipEventRecord(pe.b, c->stream);
@@ -372,7 +389,19 @@ struct Run {
             if (e != hipSuccess) return e;
             return launch_finish(a, c->stream);
         }));
+        HIP_TRY(timed(c, K_SCAN, 0, 0, 0, [&] { return launch_compact(op, a, c->stream); }));
         HIP_TRY(timed(c, ek, 0, 0, 0, [&] { return launch_emit(op, c->B, a, c->stream); }));
+        if (c->opt.profile && c->n_counts < kCountSlots) {
+            // survivors of this chunk -> bytes of its emit (read after the level sync)
+            long long *slot = c->h_counts + c->n_counts++;
+            HIP_TRY(hipMemcpyAsync(slot, &c->d_state->chunk_total, sizeof(long long), hipMemcpyDeviceToHost,
+                                   c->stream));
+            PendingEvent &pe = c->pending.back();
+            pe.count = slot;
+            const uint64_t sv = (uint64_t)c->nsw * 32;
+            pe.alg_per = (bin ? 2 : 1) * cs + cs + sv + 17 + 8 + 4 + 12;
+            pe.min_per = cs + sv + 17 + 8 + 4 + 12;
+        }
         return 0;
     }
 
@@ -570,15 +599,15 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             if (i < P) x.tgt |= 1ull << (j * B);
         }
     }
-    std::vector<uint64_t> stripe0((size_t)c->nw * 64, 0);
+    std::vector<uint64_t> atom_words((size_t)c->nw * A, 0);  // [w][a]
     for (int a = 0; a < A; ++a)
         for (int i = 0; i < W; ++i)
-            stripe0[(size_t)(i / c->TPW) * 64 + a] |=
+            atom_words[(size_t)(i / c->TPW) * A + a] |=
                 (pb->atoms[(size_t)a * W + i] & len_mask(pb->lens[i])) << ((i % c->TPW) * B);
     // Summaries (b0, b1, nz, fl per trace, 64 traces per word) of the atoms,
     // and the per-summary-word target and valid masks.
     const int nsw = c->nsw;
-    std::vector<uint64_t> sum0((size_t)4 * nsw * 64, 0);
+    std::vector<uint64_t> sum0((size_t)4 * nsw * A, 0);  // [k*nsw + j][a]
     std::vector<SumAux> saux(nsw, SumAux{0, 0});
     for (int i = 0; i < W; ++i) {
         saux[i / 64].val |= 1ull << (i % 64);
@@ -589,10 +618,10 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             const uint64_t m = len_mask(pb->lens[i]), x = pb->atoms[(size_t)a * W + i] & m;
             const uint64_t bit = 1ull << (i % 64);
             const int j = i / 64;
-            if (x & 1) sum0[(size_t)(0 * nsw + j) * 64 + a] |= bit;
-            if (x & 2) sum0[(size_t)(1 * nsw + j) * 64 + a] |= bit;
-            if (x) sum0[(size_t)(2 * nsw + j) * 64 + a] |= bit;
-            if (x == m) sum0[(size_t)(3 * nsw + j) * 64 + a] |= bit;
+            if (x & 1) sum0[(size_t)(0 * nsw + j) * A + a] |= bit;
+            if (x & 2) sum0[(size_t)(1 * nsw + j) * A + a] |= bit;
+            if (x) sum0[(size_t)(2 * nsw + j) * A + a] |= bit;
+            if (x == m) sum0[(size_t)(3 * nsw + j) * A + a] |= bit;
         }
     DevState st0{};
     st0.found = kEmpty;
@@ -601,8 +630,10 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     hipStream_t s = c->stream;
     HIP_TRY(hipMemcpyAsync(c->d_aux, aux.data(), aux.size() * sizeof(WordAux), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->d_saux, saux.data(), saux.size() * sizeof(SumAux), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->d_store, stripe0.data(), stripe0.size() * 8, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->d_sum, sum0.data(), sum0.size() * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpy2DAsync(c->d_store, (size_t)c->ws * 8, atom_words.data(), (size_t)A * 8, (size_t)A * 8,
+                             (size_t)c->nw, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpy2DAsync(c->d_sum, (size_t)c->ws * 8, sum0.data(), (size_t)A * 8, (size_t)A * 8,
+                             (size_t)4 * nsw, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->d_state, &st0, sizeof(st0), hipMemcpyHostToDevice, s));
     HIP_TRY(launch_fill_u64(c->d_tab, kEmpty, 2 * ((size_t)c->tmask + 1), s));
 
@@ -610,9 +641,11 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     ChunkArgs &ba = run.base;
     ba.store = c->d_store; ba.aux = c->d_aux; ba.tab = c->d_tab; ba.st = c->d_state;
     ba.t_slot = c->t_slot; ba.t_fp2 = c->t_fp2; ba.t_mask = c->t_mask; ba.t_cnt = c->t_cnt; ba.t_off = c->t_off;
+    ba.t_list = c->t_list;
     ba.m_op = c->d_op; ba.m_left = c->d_left; ba.m_right = c->d_right; ba.m_fp2 = c->d_fp2;
     ba.sum = c->d_sum; ba.saux = c->d_saux; ba.nsw = nsw;
     ba.capacity = c->capacity;
+    ba.ws = c->ws;
     ba.nw = c->nw;
     ba.hamming_k = o.hamming_k;
     ba.tmask = c->tmask;
@@ -777,18 +810,14 @@ extern "C" int32_t ltl_store_read(ltl_ctx *c, int64_t first, int64_t count, uint
     if (first + count > c->capacity) return LTL_E_ARG;
     HIP_TRY(hipSetDevice(c->opt.device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    const int64_t s0 = first / 64, s1 = (first + count - 1) / 64;
-    const size_t stripe = (size_t)c->nw * 64;
-    std::vector<uint64_t> buf((size_t)(s1 - s0 + 1) * stripe);
-    HIP_TRY(hipMemcpy(buf.data(), c->d_store + (size_t)s0 * stripe, buf.size() * 8, hipMemcpyDeviceToHost));
+    std::vector<uint64_t> buf((size_t)c->nw * count);  // [w][k]
+    HIP_TRY(hipMemcpy2D(buf.data(), (size_t)count * 8, c->d_store + first, (size_t)c->ws * 8, (size_t)count * 8,
+                        (size_t)c->nw, hipMemcpyDeviceToHost));
     const uint64_t slot = c->B == 64 ? ~0ull : ((1ull << c->B) - 1);
     if (cs)
-        for (int64_t k = 0; k < count; ++k) {
-            const int64_t sidx = first + k;
-            const uint64_t *base = buf.data() + (size_t)(sidx / 64 - s0) * stripe + (sidx % 64);
+        for (int64_t k = 0; k < count; ++k)
             for (int i = 0; i < c->W; ++i)
-                cs[(size_t)k * c->W + i] = (base[(size_t)(i / c->TPW) * 64] >> ((i % c->TPW) * c->B)) & slot;
-        }
+                cs[(size_t)k * c->W + i] = (buf[(size_t)(i / c->TPW) * count + k] >> ((i % c->TPW) * c->B)) & slot;
     if (op) HIP_TRY(hipMemcpy(op, c->d_op + first, (size_t)count, hipMemcpyDeviceToHost));
     if (left) HIP_TRY(hipMemcpy(left, c->d_left + first, (size_t)count * 4, hipMemcpyDeviceToHost));
     if (right) HIP_TRY(hipMemcpy(right, c->d_right + first, (size_t)count * 4, hipMemcpyDeviceToHost));

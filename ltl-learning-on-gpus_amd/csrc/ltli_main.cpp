@@ -1,0 +1,139 @@
+// ltli_main.cpp -- command-line front end over the C ABI.
+//
+// Same arguments and stdout as the reference binary (code/ltli6463.cu:
+// 1575-1681), so that drivers which parse its output (code/dc.py:23-43, 192-194)
+// work unchanged:
+//
+//   ltli <input_file> <c1> .. <c8> <maxCost> <RlxUnqChkTyp> <NegType>
+//
+// stdout: the traces, "Cost Function: ...", "Number of Traces: N", the
+// run statistics, then a last line  LTL: "<formula>"  (or "not_found").
+// Differences: errors exit with a nonzero status (the reference always exits
+// 0), traces may number more than 64, and deduplication is exact for any
+// total trace length (RlxUnqChkTyp is validated and otherwise ignored).
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "../../include/ltl_amd.h"
+
+static int usage(const char *argv0) {
+    printf("Arguments should be in the form of\n");
+    printf("-----------------------------------------------------------------\n");
+    printf("%s <input_file_address> <c1> <c2> <c3> <c4> <c5> <c6> <c7> <c8> <maxCost> <RlxUnqChkTyp> <NegType>\n",
+           argv0);
+    printf("-----------------------------------------------------------------\n");
+    printf("\nFor example\n");
+    printf("-----------------------------------------------------------------\n");
+    printf("%s ./input.txt 1 1 1 1 1 1 1 1 500 3 2\n", argv0);
+    printf("-----------------------------------------------------------------\n");
+    return 2;
+}
+
+// Column letters of the alphabet line, in file order (the token letters are
+// printed in this order, code/ltli6463.cu:1531-1543, 1645-1652).
+static std::string file_alphabet(const char *path) {
+    std::ifstream f(path);
+    std::string line, last;
+    while (std::getline(f, line))
+        if (!line.empty() && line != "\r") last = line;
+    std::string a;
+    for (char c : last)
+        if (c >= 'a' && c <= 'z') a += c;
+    return a;
+}
+
+int main(int argc, char **argv) {
+    if (argc != 13) return usage(argv[0]);
+    bool bad = false;
+    for (int i = 2; i < argc - 2; ++i) {
+        const int v = std::atoi(argv[i]);
+        if (v <= 0 || v > SHRT_MAX) {
+            printf("Argument number %d, \"%s\", should be a positive short integer.\n", i, argv[i]);
+            bad = true;
+        }
+    }
+    const int rlx = std::atoi(argv[11]), neg = std::atoi(argv[12]);
+    if (rlx < 1 || rlx > 3) {
+        printf("Argument number 11, RlxUnqChkTyp = \"%s\", should be 1, 2, or 3.\n", argv[11]);
+        bad = true;
+    }
+    if (neg < 1 || neg > 2) {
+        printf("Argument number 12, NegType = \"%s\", should be 1, or 2.\n", argv[12]);
+        bad = true;
+    }
+    if (bad) return 2;
+
+    const int max_traces = 1 << 16;
+    std::vector<uint8_t> lens(max_traces);
+    std::vector<uint64_t> atoms((size_t)26 * max_traces);
+    char names[32];
+    int32_t P = 0, N = 0, A = 0;
+    int32_t rc = ltl_read_trace_file(argv[1], max_traces, &P, &N, &A, lens.data(), atoms.data(), names);
+    if (rc == LTL_E_ARG) {
+        printf("Failed to open the input file.\n");
+        return 1;
+    }
+    if (rc == LTL_E_LIMIT) {
+        printf("In this version, The input can have:\n");
+        printf("1) At most %d traces.\n", max_traces);
+        printf("2) Max(len(trace)) = 64.\n");
+        return 1;
+    }
+    const int W = P + N;
+
+    ltl_options opt;
+    ltl_default_options(&opt);
+    for (int i = 0; i < 8; ++i) opt.cost[i] = (uint16_t)std::atoi(argv[i + 2]);
+    opt.max_cost = (uint16_t)std::atoi(argv[10]);
+    opt.neg_type = (uint8_t)neg;
+    int32_t st = 0;
+    ltl_ctx *ctx = ltl_create(&opt, &st);
+    if (!ctx) {
+        printf("No usable GPU (error %d).\n", st);
+        return 1;
+    }
+    ltl_problem pb{P, N, A, 0, lens.data(), atoms.data(), names};
+    ltl_result res;
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    rc = ltl_learn(ctx, &pb, &res);
+    const auto t1 = std::chrono::high_resolution_clock::now();
+
+    const std::string alpha = file_alphabet(argv[1]);
+    auto print_traces = [&](const char *title, int from, int to) {
+        printf("\n%s: \n", title);
+        for (int t = from; t < to; ++t) {
+            printf("\t");
+            for (int k = 0; k < lens[t]; ++k) {
+                std::string s;
+                for (char c : alpha) {
+                    const int a = (int)(std::string(names).find(c));
+                    if (a >= 0 && ((atoms[(size_t)a * W + t] >> k) & 1)) { s += c; s += ", "; }
+                }
+                printf("{%s}\t", s.substr(0, s.size() >= 2 ? s.size() - 2 : 0).c_str());
+            }
+            printf("\n");
+        }
+    };
+    print_traces("Positive", 0, P);
+    print_traces("Negative", P, W);
+    printf("\nCost Function: p:%u, ~:%u, &:%u, |:%u, X:%u, F:%u, G:%u, U:%u", opt.cost[0], opt.cost[1],
+           opt.cost[2], opt.cost[3], opt.cost[4], opt.cost[5], opt.cost[6], opt.cost[7]);
+    printf("\nNumber of Traces: %d", W);
+    if (rc >= 0) {
+        const double us = (double)std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count();
+        printf("\nNumber of All LTLs: %lu", (unsigned long)res.all_ltls);
+        printf("\nCost of Final LTL: %d", res.cost);
+        printf("\nRunning Time: %f s", us * 0.000001);
+        printf("\n\nLTL: \"%s\"\n", res.formula ? res.formula : "not_found");
+    } else {
+        printf("\n\nLTL: \"see_the_error\"\n");
+    }
+    ltl_result_free(&res);
+    ltl_destroy(ctx);
+    return rc < 0 ? 1 : 0;
+}

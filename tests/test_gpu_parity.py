@@ -121,3 +121,14 @@ def test_config3_found(ltl, oracle, learner):
     inst = ltl.generate(3, seed=1)
     gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=6, hamming_k=inst.hamming_k)
     assert gpu["status"] == 1
+
+
+# Answers that are U formulas, at every slot width: the precheck decides U
+# candidates from a closed form (B = 32, 64) or the prefix scan (B = 8, 16).
+@pytest.mark.parametrize("precheck", [0, 1])
+@pytest.mark.parametrize("lens", [(2, 8), (9, 16), (17, 32), (33, 64), (60, 64)])
+@pytest.mark.parametrize("planted", ["(a)U(b)", "(X(a))U(b)", "(a)U((b)&(X(c)))", "~((a)U(b))"])
+def test_until_answers(ltl, oracle, learner, planted, lens, precheck):
+    inst = ltl.generate_custom(23 + lens[0], 40, 40, 3, lens[0], lens[1], 0.5, planted=planted)
+    gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=6, precheck=precheck)
+    assert gpu["status"] == 1
