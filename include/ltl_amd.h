@@ -136,6 +136,47 @@ int32_t ltl_learn(ltl_ctx *ctx, const ltl_problem *pb, ltl_result *res);
  * instances solved, or a negative LTL_E_* code. */
 int32_t ltl_learn_batch(ltl_ctx *ctx, const ltl_problem *pbs, int32_t n, ltl_result *res);
 
+/* Parameters of the batched launch (no reference counterpart).
+ *   tile         candidates per workgroup step = threads per workgroup
+ *                (256, 512 or 1024)
+ *   word_block   loads in flight per lane in the word loops (4 or 8)
+ *   workgroups   persistent workgroups (0 = 2 per CU, 1 per CU for tile 1024)
+ *   defer_limit  candidates of one cost level above which an instance is
+ *                handed to the whole-device path (ltl_learn) */
+typedef struct {
+    int32_t tile;
+    int32_t word_block;
+    int32_t workgroups;
+    int32_t reserved;
+    int64_t defer_limit;
+} ltl_batch_params;
+
+typedef struct {
+    ltl_batch_params params;
+    double seconds;      /* wall time of ltl_learn_batch with these params */
+    int32_t deferred;    /* instances handed to ltl_learn */
+    int32_t reserved;
+} ltl_batch_trial;
+
+typedef struct {
+    double seconds;         /* wall time of the last ltl_learn_batch */
+    double kernel_seconds;  /* the persistent launch alone */
+    int32_t n;              /* instances */
+    int32_t on_device;      /* finished inside the persistent launch */
+    int32_t deferred;       /* handed to ltl_learn */
+    int32_t host;           /* answered on the host (an atom separates) */
+} ltl_batch_stats;
+
+int32_t ltl_set_batch_params(ltl_ctx *ctx, const ltl_batch_params *p);
+int32_t ltl_get_batch_params(const ltl_ctx *ctx, ltl_batch_params *p);
+int32_t ltl_batch_last_stats(const ltl_ctx *ctx, ltl_batch_stats *s);
+
+/* Project autotuner: time ltl_learn_batch on these instances for a grid of
+ * (tile, word_block, defer_limit), keep the fastest in the context, and
+ * report up to max_trials trials.  Returns the number of trials. */
+int32_t ltl_autotune_batch(ltl_ctx *ctx, const ltl_problem *pbs, int32_t n, ltl_batch_trial *trials,
+                           int32_t max_trials);
+
 /* Free what a result owns (formula, nodes). */
 void ltl_result_free(ltl_result *res);
 

@@ -69,6 +69,23 @@ class KernelStat(ctypes.Structure):
                 ("min_bytes", ctypes.c_uint64), ("items", ctypes.c_uint64)]
 
 
+class BatchParams(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int32), ("word_block", ctypes.c_int32),
+                ("workgroups", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("defer_limit", ctypes.c_int64)]
+
+
+class BatchTrial(ctypes.Structure):
+    _fields_ = [("params", BatchParams), ("seconds", ctypes.c_double),
+                ("deferred", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [("seconds", ctypes.c_double), ("kernel_seconds", ctypes.c_double),
+                ("n", ctypes.c_int32), ("on_device", ctypes.c_int32),
+                ("deferred", ctypes.c_int32), ("host", ctypes.c_int32)]
+
+
 class GenInstance(ctypes.Structure):
     _fields_ = [("num_pos", ctypes.c_int32), ("num_neg", ctypes.c_int32),
                 ("num_atoms", ctypes.c_int32), ("max_cost", ctypes.c_int32),
@@ -79,6 +96,7 @@ class GenInstance(ctypes.Structure):
 
 
 EXPORTS = ["ltl_default_options", "ltl_create", "ltl_set_options", "ltl_learn", "ltl_learn_batch",
+           "ltl_set_batch_params", "ltl_get_batch_params", "ltl_batch_last_stats", "ltl_autotune_batch",
            "ltl_result_free", "ltl_destroy", "ltl_get_kernel_stats", "ltl_store_read", "ltl_levels",
            "ltl_read_trace_file", "ltl_eval_formula", "ltl_mismatches", "ltl_version",
            "ltl_gen_config", "ltl_gen_custom", "ltl_gen_free"]
@@ -102,6 +120,10 @@ def lib():
     L.ltl_learn.argtypes = [ctypes.c_void_p, P(Problem), P(Result)]
     L.ltl_learn_batch.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int32, P(Result)]
     L.ltl_result_free.argtypes = [P(Result)]
+    L.ltl_set_batch_params.argtypes = [ctypes.c_void_p, P(BatchParams)]
+    L.ltl_get_batch_params.argtypes = [ctypes.c_void_p, P(BatchParams)]
+    L.ltl_batch_last_stats.argtypes = [ctypes.c_void_p, P(BatchStats)]
+    L.ltl_autotune_batch.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int32, P(BatchTrial), ctypes.c_int32]
     L.ltl_destroy.argtypes = [ctypes.c_void_p]
     L.ltl_get_kernel_stats.argtypes = [ctypes.c_void_p, P(KernelStat), ctypes.c_int32]
     L.ltl_store_read.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, P(ctypes.c_uint64),
@@ -265,6 +287,57 @@ class Learner:
         if rc < 0:
             raise LtlError(f"ltl_learn -> {rc}")
         return out
+
+    @staticmethod
+    def _result(res):
+        return dict(status=res.status, cost=res.cost, all_ltls=res.all_ltls, found_id=res.found_id,
+                    n_stored=res.n_stored, on_the_fly=res.on_the_fly,
+                    formula=(res.formula or b"").decode(), seconds=res.seconds,
+                    nodes=[(res.nodes[i].op, res.nodes[i].atom, res.nodes[i].left, res.nodes[i].right)
+                           for i in range(res.n_nodes)])
+
+    def learn_batch(self, insts, **opts):
+        """ltl_learn_batch over a list of instances; returns a list of dicts."""
+        self.set_options(**opts)
+        n = len(insts)
+        pbs = (Problem * n)(*[i.problem() for i in insts])
+        res = (Result * n)()
+        rc = lib().ltl_learn_batch(self._ctx, pbs, n, res)
+        out = [self._result(res[i]) for i in range(n)]
+        for i in range(n):
+            lib().ltl_result_free(ctypes.byref(res[i]))
+        if rc < 0:
+            raise LtlError(f"ltl_learn_batch -> {rc}")
+        return out
+
+    def set_batch_params(self, tile=512, word_block=4, workgroups=0, defer_limit=1 << 22):
+        p = BatchParams(tile, word_block, workgroups, 0, defer_limit)
+        rc = lib().ltl_set_batch_params(self._ctx, ctypes.byref(p))
+        if rc != 0:
+            raise LtlError(f"ltl_set_batch_params -> {rc}")
+
+    def batch_params(self):
+        p = BatchParams()
+        lib().ltl_get_batch_params(self._ctx, ctypes.byref(p))
+        return dict(tile=p.tile, word_block=p.word_block, workgroups=p.workgroups, defer_limit=p.defer_limit)
+
+    def batch_stats(self):
+        s = BatchStats()
+        lib().ltl_batch_last_stats(self._ctx, ctypes.byref(s))
+        return dict(seconds=s.seconds, kernel_seconds=s.kernel_seconds, n=s.n, on_device=s.on_device,
+                    deferred=s.deferred, host=s.host)
+
+    def autotune_batch(self, insts, **opts):
+        """Project autotuner: returns the trials; the fastest stays set."""
+        self.set_options(**opts)
+        n = len(insts)
+        pbs = (Problem * n)(*[i.problem() for i in insts])
+        trials = (BatchTrial * 64)()
+        k = lib().ltl_autotune_batch(self._ctx, pbs, n, trials, 64)
+        if k < 0:
+            raise LtlError(f"ltl_autotune_batch -> {k}")
+        return [dict(tile=t.params.tile, word_block=t.params.word_block, defer_limit=t.params.defer_limit,
+                     seconds=t.seconds, deferred=t.deferred) for t in trials[:k]]
 
     def kernel_stats(self):
         arr = (KernelStat * 16)()

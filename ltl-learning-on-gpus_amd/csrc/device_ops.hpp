@@ -1,0 +1,167 @@
+// device_ops.hpp -- per-word operators, fingerprints, summaries and the
+// dedup-table insert shared by the chunk kernels and the batched kernel.
+//
+// Operators follow code/ltli6463.cu:203, 295, 385, 468, 551-556, 638-645,
+// 739-771, applied to B-bit trace slots: every shift is masked so that no bit
+// crosses a slot (bits at or above a trace's length stay zero).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/ltl_amd.h"
+#include "kernels.hpp"
+
+namespace ltl {
+
+
+template <int B>
+__host__ __device__ constexpr uint64_t rep(uint64_t per) {
+    uint64_t m = 0;
+    for (int j = 0; j < 64 / B; ++j) m |= per << (j * B);
+    return m;
+}
+
+// All bits of each B-bit slot except its top s bits (0 < s < B).
+template <int B>
+__host__ __device__ constexpr uint64_t keep(int s) {
+    return B == 64 ? (~0ull >> s) : rep<B>((1ull << (B - s)) - 1ull);
+}
+
+template <int B>
+__device__ __forceinline__ uint64_t f_fin(uint64_t x) {
+#pragma unroll
+    for (int s = 1; s < B; s <<= 1) x |= (x >> s) & keep<B>(s);
+    return x;
+}
+
+template <int B>
+__device__ __forceinline__ uint64_t f_until(uint64_t l, uint64_t r) {
+    uint64_t run = l, y = r;
+#pragma unroll
+    for (int s = 1; s < B; s <<= 1) {
+        y |= run & ((y >> s) & keep<B>(s));
+        run &= (run >> s) & keep<B>(s);
+    }
+    return y;
+}
+
+template <int OP, int B>
+__device__ __forceinline__ uint64_t apply1(uint64_t x, uint64_t lm) {
+    if constexpr (OP == LTL_NOT) return ~x & lm;
+    else if constexpr (OP == LTL_NEXT) return (x >> 1) & keep<B>(1);
+    else if constexpr (OP == LTL_FIN) return f_fin<B>(x);
+    else return x & ~f_fin<B>(~x & lm);
+}
+
+// NH compression step (Black et al., UMAC, CRYPTO 1999): the two 32-bit
+// halves plus key halves, multiplied to 64 bits, summed mod 2^64.
+__device__ __forceinline__ uint64_t nh(uint64_t c, uint64_t k) {
+    const uint32_t lo = (uint32_t)c + (uint32_t)k;
+    const uint32_t hi = (uint32_t)(c >> 32) + (uint32_t)(k >> 32);
+    return (uint64_t)lo * (uint64_t)hi;
+}
+
+struct Acc {
+    uint64_t f1 = 0, f2 = 0;
+    uint32_t mism = 0;
+};
+
+// Fold word c of a candidate: both fingerprints and the mismatch count.
+// Fields are read one by one from the (wave-uniform) per-word record, so
+// that only what is used occupies scalar registers.
+__device__ __forceinline__ void fold(Acc &a, uint64_t c, const WordAux *x) {
+    a.f1 += nh(c, x->k1);
+    a.f2 += nh(c, x->k2);
+    a.mism += (uint32_t)__popcll((c ^ x->tgt) & x->chk);
+}
+
+// Per-trace summaries of one packed word, OR-ed into the summary words at
+// bit offset sh: position 0 (b0), position 1 (b1), any position (nz) and
+// every position inside the trace (fl).  They decide the target check of
+// ~, X, F, G, & and | candidates without reading the full sets.
+template <int B>
+__device__ __forceinline__ void summarize(uint64_t c, uint64_t lm, int sh, uint64_t (&v)[4]) {
+    constexpr int TPW = 64 / B;
+    constexpr uint64_t SM = B == 64 ? ~0ull : ((1ull << B) - 1ull);
+#pragma unroll
+    for (int j = 0; j < TPW; ++j) {
+        const uint64_t x = (c >> (j * B)) & SM, m = (lm >> (j * B)) & SM;
+        const int at = sh + j;
+        v[0] |= (x & 1ull) << at;
+        v[1] |= ((x >> 1) & 1ull) << at;
+        v[2] |= (uint64_t)(x != 0) << at;
+        v[3] |= (uint64_t)(x == m) << at;
+    }
+}
+
+// Flush the summary words of set `pos` when word w closes summary word w/B.
+template <int B>
+__device__ __forceinline__ void flush_summary(uint64_t *z, int w, int nw, int nsw, int64_t ws, uint64_t (&v)[4],
+                                              bool on) {
+    if ((w % B) != B - 1 && w != nw - 1) return;
+    const int j = w / B;
+    if (on) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) z[(size_t)(k * nsw + j) * ws] = v[k];
+    }
+    v[0] = v[1] = v[2] = v[3] = 0;
+}
+
+// Table key from fingerprint 1: a 64-bit finalizer spreads the NH sum over
+// all bits (slot index = low bits); the empty key is never produced.
+__device__ __forceinline__ uint64_t table_key(uint64_t f) {
+    f ^= f >> 33; f *= 0xff51afd7ed558ccdull;
+    f ^= f >> 33; f *= 0xc4ceb9fe1a85ec53ull;
+    f ^= f >> 33;
+    return f == kEmpty ? kEmpty - 2 : f;
+}
+
+// Linear probing; key CAS from empty, then atomicMin of the value.
+__device__ inline uint32_t tab_insert(uint64_t *tab, uint32_t tmask, uint64_t key, uint64_t val, int *probe_fail) {
+    uint32_t p = (uint32_t)key & tmask;
+    for (uint32_t n = 0; n <= tmask; ++n) {
+        unsigned long long *kp = reinterpret_cast<unsigned long long *>(tab + 2 * (size_t)p);
+        unsigned long long k = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (k == kEmpty) {
+            k = atomicCAS(kp, (unsigned long long)kEmpty, (unsigned long long)key);
+            if (k == kEmpty) k = key;
+        }
+        if (k == key) {
+            // Values only decrease; a read that is already <= val makes the
+            // atomic a no-op.  Popular sets (the empty set under G, the full
+            // set under F, ...) are committed early, so their many duplicates
+            // become cached loads instead of serialized atomics on one slot.
+            unsigned long long cur = __hip_atomic_load(kp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((unsigned long long)val < cur) atomicMin(kp + 1, (unsigned long long)val);
+            return p;
+        }
+        p = (p + 1) & tmask;
+    }
+    *probe_fail = 1;
+    return 0;
+}
+
+// U: position 0 of l U r holds iff r has a set bit at or below the first
+// position where l is false: with n = ~l, n ^ (n - 1) masks the positions up
+// to and including n's lowest set bit.  Per slot (the subtraction must not
+// borrow across slots), 32-bit halves for B = 32, the whole word for B = 64.
+// Bits at or above the trace length are zero in l, so n's lowest set bit is
+// at most the length; r has no bits there.
+template <int B>
+__device__ __forceinline__ uint64_t until_at0(uint64_t l, uint64_t r) {
+    if constexpr (B == 64) {
+        const uint64_t n = ~l;
+        return (uint64_t)((r & (n ^ (n - 1))) != 0);
+    } else if constexpr (B == 32) {
+        const uint32_t n0 = ~(uint32_t)l, n1 = ~(uint32_t)(l >> 32);
+        const uint32_t h0 = ((uint32_t)r & (n0 ^ (n0 - 1))) != 0;
+        const uint32_t h1 = ((uint32_t)(r >> 32) & (n1 ^ (n1 - 1))) != 0;
+        return (uint64_t)h0 | ((uint64_t)h1 << 32);
+    } else {
+        return f_until<B>(l, r);  // narrow slots: the short prefix is cheaper
+    }
+}
+
+
+}  // namespace ltl

@@ -22,132 +22,10 @@
 
 #include "../../include/ltl_amd.h"
 #include "kernels.hpp"
+#include "device_ops.hpp"
 
 namespace ltl {
 namespace {
-
-template <int B>
-__host__ __device__ constexpr uint64_t rep(uint64_t per) {
-    uint64_t m = 0;
-    for (int j = 0; j < 64 / B; ++j) m |= per << (j * B);
-    return m;
-}
-
-// All bits of each B-bit slot except its top s bits (0 < s < B).
-template <int B>
-__host__ __device__ constexpr uint64_t keep(int s) {
-    return B == 64 ? (~0ull >> s) : rep<B>((1ull << (B - s)) - 1ull);
-}
-
-template <int B>
-__device__ __forceinline__ uint64_t f_fin(uint64_t x) {
-#pragma unroll
-    for (int s = 1; s < B; s <<= 1) x |= (x >> s) & keep<B>(s);
-    return x;
-}
-
-template <int B>
-__device__ __forceinline__ uint64_t f_until(uint64_t l, uint64_t r) {
-    uint64_t run = l, y = r;
-#pragma unroll
-    for (int s = 1; s < B; s <<= 1) {
-        y |= run & ((y >> s) & keep<B>(s));
-        run &= (run >> s) & keep<B>(s);
-    }
-    return y;
-}
-
-template <int OP, int B>
-__device__ __forceinline__ uint64_t apply1(uint64_t x, uint64_t lm) {
-    if constexpr (OP == LTL_NOT) return ~x & lm;
-    else if constexpr (OP == LTL_NEXT) return (x >> 1) & keep<B>(1);
-    else if constexpr (OP == LTL_FIN) return f_fin<B>(x);
-    else return x & ~f_fin<B>(~x & lm);
-}
-
-// NH compression step (Black et al., UMAC, CRYPTO 1999): the two 32-bit
-// halves plus key halves, multiplied to 64 bits, summed mod 2^64.
-__device__ __forceinline__ uint64_t nh(uint64_t c, uint64_t k) {
-    const uint32_t lo = (uint32_t)c + (uint32_t)k;
-    const uint32_t hi = (uint32_t)(c >> 32) + (uint32_t)(k >> 32);
-    return (uint64_t)lo * (uint64_t)hi;
-}
-
-struct Acc {
-    uint64_t f1 = 0, f2 = 0;
-    uint32_t mism = 0;
-};
-
-__device__ __forceinline__ void fold(Acc &a, uint64_t c, const WordAux &x) {
-    a.f1 += nh(c, x.k1);
-    a.f2 += nh(c, x.k2);
-    a.mism += (uint32_t)__popcll((c ^ x.tgt) & x.chk);
-}
-
-// Per-trace summaries of one packed word, OR-ed into the summary words at
-// bit offset sh: position 0 (b0), position 1 (b1), any position (nz) and
-// every position inside the trace (fl).  They decide the target check of
-// ~, X, F, G, & and | candidates without reading the full sets.
-template <int B>
-__device__ __forceinline__ void summarize(uint64_t c, uint64_t lm, int sh, uint64_t (&v)[4]) {
-    constexpr int TPW = 64 / B;
-    constexpr uint64_t SM = B == 64 ? ~0ull : ((1ull << B) - 1ull);
-#pragma unroll
-    for (int j = 0; j < TPW; ++j) {
-        const uint64_t x = (c >> (j * B)) & SM, m = (lm >> (j * B)) & SM;
-        const int at = sh + j;
-        v[0] |= (x & 1ull) << at;
-        v[1] |= ((x >> 1) & 1ull) << at;
-        v[2] |= (uint64_t)(x != 0) << at;
-        v[3] |= (uint64_t)(x == m) << at;
-    }
-}
-
-// Flush the summary words of set `pos` when word w closes summary word w/B.
-template <int B>
-__device__ __forceinline__ void flush_summary(const ChunkArgs &A, uint64_t *z, int w, uint64_t (&v)[4], bool on) {
-    if ((w % B) != B - 1 && w != A.nw - 1) return;
-    const int j = w / B;
-    if (on) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) z[(size_t)(k * A.nsw + j) * A.ws] = v[k];
-    }
-    v[0] = v[1] = v[2] = v[3] = 0;
-}
-
-// Table key from fingerprint 1: a 64-bit finalizer spreads the NH sum over
-// all bits (slot index = low bits); the empty key is never produced.
-__device__ __forceinline__ uint64_t table_key(uint64_t f) {
-    f ^= f >> 33; f *= 0xff51afd7ed558ccdull;
-    f ^= f >> 33; f *= 0xc4ceb9fe1a85ec53ull;
-    f ^= f >> 33;
-    return f == kEmpty ? kEmpty - 2 : f;
-}
-
-// Linear probing; key CAS from empty, then atomicMin of the value.
-__device__ uint32_t tab_insert(uint64_t *tab, uint32_t tmask, uint64_t key, uint64_t val, DevState *st) {
-    uint32_t p = (uint32_t)key & tmask;
-    for (uint32_t n = 0; n <= tmask; ++n) {
-        unsigned long long *kp = reinterpret_cast<unsigned long long *>(tab + 2 * (size_t)p);
-        unsigned long long k = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (k == kEmpty) {
-            k = atomicCAS(kp, (unsigned long long)kEmpty, (unsigned long long)key);
-            if (k == kEmpty) k = key;
-        }
-        if (k == key) {
-            // Values only decrease; a read that is already <= val makes the
-            // atomic a no-op.  Popular sets (the empty set under G, the full
-            // set under F, ...) are committed early, so their many duplicates
-            // become cached loads instead of serialized atomics on one slot.
-            unsigned long long cur = __hip_atomic_load(kp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((unsigned long long)val < cur) atomicMin(kp + 1, (unsigned long long)val);
-            return p;
-        }
-        p = (p + 1) & tmask;
-    }
-    st->probe_fail = 1;
-    return 0;
-}
 
 // Workgroups of a binary chunk are laid out so that consecutive workgroups
 // dispatched to one XCD (dispatch is round-robin over the 8 XCDs) walk the
@@ -206,7 +84,7 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     const uint64_t *pr = A.store + (size_t)sr;
     Acc c0, c1;
     auto step = [&](int w, uint64_t l, uint64_t r) {
-        const WordAux x = A.aux[w];
+        const WordAux *x = A.aux + w;
         if constexpr (OP == LTL_AND) fold(c0, l & r, x);
         else if constexpr (OP == LTL_OR) fold(c0, l | r, x);
         else {
@@ -234,12 +112,12 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     if (otf) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
     if constexpr (OP == LTL_UNTIL) {
-        A.t_slot[2 * ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, A.st);
+        A.t_slot[2 * ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
         A.t_fp2[2 * ix] = c0.f2;
-        A.t_slot[2 * ix + 1] = tab_insert(A.tab, A.tmask, table_key(c1.f1), kPending | (id0 + 1), A.st);
+        A.t_slot[2 * ix + 1] = tab_insert(A.tab, A.tmask, table_key(c1.f1), kPending | (id0 + 1), &A.st->probe_fail);
         A.t_fp2[2 * ix + 1] = c1.f2;
     } else {
-        A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, A.st);
+        A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
         A.t_fp2[ix] = c0.f2;
     }
 }
@@ -254,8 +132,8 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     const uint64_t *p = A.store + (size_t)s;
     Acc c0;
     auto step = [&](int w, uint64_t v) {
-        const WordAux x = A.aux[w];
-        fold(c0, apply1<OP, B>(v, x.lm), x);
+        const WordAux *x = A.aux + w;
+        fold(c0, apply1<OP, B>(v, x->lm), x);
     };
     constexpr int UB = 8;
     int w = 0;
@@ -272,7 +150,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
     if (otf) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
-    A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, A.st);
+    A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
     A.t_fp2[ix] = c0.f2;
 }
 
@@ -284,8 +162,8 @@ __global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms) {
     if (a >= n_atoms) return;
     const uint64_t *p = A.store + (size_t)a;
     Acc c0;
-    for (int w = 0; w < A.nw; ++w) fold(c0, p[(size_t)w * A.ws], A.aux[w]);
-    tab_insert(A.tab, A.tmask, table_key(c0.f1), (uint64_t)a, A.st);
+    for (int w = 0; w < A.nw; ++w) fold(c0, p[(size_t)w * A.ws], A.aux + w);
+    tab_insert(A.tab, A.tmask, table_key(c0.f1), (uint64_t)a, &A.st->probe_fail);
     A.m_fp2[a] = c0.f2;
     A.m_op[a] = LTL_ATOM;
     A.m_left[a] = -1;
@@ -439,7 +317,7 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A) {
         else c = apply1<OP, B>(l, lm);
         q[(size_t)w * A.ws] = c;
         summarize<B>(c, lm, (w % B) * TPW, v);
-        flush_summary<B>(A, z, w, v, true);
+        flush_summary<B>(z, w, A.nw, A.nsw, A.ws, v, true);
     };
     constexpr int UB = bin ? 4 : 8;
     int w = 0;
@@ -498,27 +376,6 @@ __global__ __launch_bounds__(256) void k_chk2(ChunkArgs A) {
     atomicMin(&A.st->found, (unsigned long long)(A.id_base + pair));
 }
 
-// U: position 0 of l U r holds iff r has a set bit at or below the first
-// position where l is false: with n = ~l, n ^ (n - 1) masks the positions up
-// to and including n's lowest set bit.  Per slot (the subtraction must not
-// borrow across slots), 32-bit halves for B = 32, the whole word for B = 64.
-// Bits at or above the trace length are zero in l, so n's lowest set bit is
-// at most the length; r has no bits there.
-template <int B>
-__device__ __forceinline__ uint64_t until_at0(uint64_t l, uint64_t r) {
-    if constexpr (B == 64) {
-        const uint64_t n = ~l;
-        return (uint64_t)((r & (n ^ (n - 1))) != 0);
-    } else if constexpr (B == 32) {
-        const uint32_t n0 = ~(uint32_t)l, n1 = ~(uint32_t)(l >> 32);
-        const uint32_t h0 = ((uint32_t)r & (n0 ^ (n0 - 1))) != 0;
-        const uint32_t h1 = ((uint32_t)(r >> 32) & (n1 ^ (n1 - 1))) != 0;
-        return (uint64_t)h0 | ((uint64_t)h1 << 32);
-    } else {
-        return f_until<B>(l, r);  // narrow slots: the short prefix is cheaper
-    }
-}
-
 template <int B>
 __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     if (run_stopped(A)) return;
@@ -529,9 +386,9 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     const uint64_t *pr = A.store + (size_t)sr;
     uint32_t m0 = 0, m1 = 0;
     auto step = [&](int w, uint64_t l, uint64_t r) {
-        const WordAux &x = A.aux[w];
-        m0 += (uint32_t)__popcll((until_at0<B>(l, r) ^ x.tgt) & x.chk);
-        m1 += (uint32_t)__popcll((until_at0<B>(r, l) ^ x.tgt) & x.chk);
+        const WordAux *x = A.aux + w;
+        m0 += (uint32_t)__popcll((until_at0<B>(l, r) ^ x->tgt) & x->chk);
+        m1 += (uint32_t)__popcll((until_at0<B>(r, l) ^ x->tgt) & x->chk);
     };
     constexpr int UB = 4;
     int w = 0;

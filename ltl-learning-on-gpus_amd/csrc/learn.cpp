@@ -27,6 +27,7 @@
 #include "../../include/ltl_amd.h"
 #include "formula.hpp"
 #include "kernels.hpp"
+#include "batch.hpp"
 
 using namespace ltl;
 
@@ -100,6 +101,9 @@ struct ltl_ctx {
     std::vector<hipEvent_t> free_events;
     long long *h_counts = nullptr;  // pinned, one slot per profiled emit of a level
     int n_counts = 0;
+    ltl_batch_params batch{512, 4, 0, 0, (int64_t)1 << 22};
+    ltl_batch_stats batch_stats{};
+    int n_cu = 0;
     int64_t chunk_counter = 0;
 };
 
@@ -149,6 +153,7 @@ extern "C" ltl_ctx *ltl_create(const ltl_options *opt, int32_t *status) {
         return fail(LTL_E_HIP);
     }
     for (int k = 0; k < K_NKIND; ++k) snprintf(c->stats[k].name, sizeof(c->stats[k].name), "%s", kKindName[k]);
+    if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, o.device) != hipSuccess) c->n_cu = 256;
     if (status) *status = 0;
     return c;
 }
@@ -310,6 +315,103 @@ static hipError_t timed(ltl_ctx *c, int kind, uint64_t alg, uint64_t mn, uint64_
     c->pending.push_back(pe);
     return e;
 }
+
+// ---------------------------------------------------------------------------
+// Per-instance host preparation (shared by ltl_learn and ltl_learn_batch)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Prepared {
+    int B, TPW, nw, nsw;
+    std::vector<WordAux> aux;          // [nw]
+    std::vector<SumAux> saux;          // [nsw]
+    std::vector<uint64_t> atom_words;  // [nw][A]
+    std::vector<uint64_t> atom_sums;   // [4*nsw][A]
+};
+
+int slot_width(const ltl_problem *pb) {
+    int maxlen = 1;
+    for (int i = 0; i < pb->num_pos + pb->num_neg; ++i) maxlen = std::max<int>(maxlen, pb->lens[i]);
+    int B = 8;
+    while (B < maxlen) B <<= 1;
+    return B;
+}
+
+void prepare_instance(const ltl_problem *pb, int B, Prepared &q) {
+    const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms;
+    q.B = B;
+    q.TPW = 64 / B;
+    q.nw = (W + q.TPW - 1) / q.TPW;
+    q.nsw = (W + 63) / 64;
+    q.aux.assign(q.nw, WordAux{});
+    const uint64_t seed = 0x5eed1e55c0ffee11ull;
+    for (int w = 0; w < q.nw; ++w) {
+        WordAux &x = q.aux[w];
+        memset(&x, 0, sizeof(x));
+        x.k1 = splitmix(seed ^ (2ull * w));
+        x.k2 = splitmix(~seed ^ (2ull * w + 1));
+        for (int j = 0; j < q.TPW; ++j) {
+            const int i = w * q.TPW + j;
+            if (i >= W) break;
+            x.lm |= len_mask(pb->lens[i]) << (j * B);
+            x.chk |= 1ull << (j * B);
+            if (i < P) x.tgt |= 1ull << (j * B);
+        }
+    }
+    q.atom_words.assign((size_t)q.nw * A, 0);
+    for (int a = 0; a < A; ++a)
+        for (int i = 0; i < W; ++i)
+            q.atom_words[(size_t)(i / q.TPW) * A + a] |=
+                (pb->atoms[(size_t)a * W + i] & len_mask(pb->lens[i])) << ((i % q.TPW) * B);
+    // summaries (b0, b1, nz, fl per trace, 64 traces per word) of the atoms
+    const int nsw = q.nsw;
+    q.atom_sums.assign((size_t)4 * nsw * A, 0);
+    q.saux.assign(nsw, SumAux{0, 0});
+    for (int i = 0; i < W; ++i) {
+        q.saux[i / 64].val |= 1ull << (i % 64);
+        if (i < P) q.saux[i / 64].tgt |= 1ull << (i % 64);
+    }
+    for (int a = 0; a < A; ++a)
+        for (int i = 0; i < W; ++i) {
+            const uint64_t m = len_mask(pb->lens[i]), x = pb->atoms[(size_t)a * W + i] & m;
+            const uint64_t bit = 1ull << (i % 64);
+            const int j = i / 64;
+            if (x & 1) q.atom_sums[(size_t)(0 * nsw + j) * A + a] |= bit;
+            if (x & 2) q.atom_sums[(size_t)(1 * nsw + j) * A + a] |= bit;
+            if (x) q.atom_sums[(size_t)(2 * nsw + j) * A + a] |= bit;
+            if (x == m) q.atom_sums[(size_t)(3 * nsw + j) * A + a] |= bit;
+        }
+}
+
+// Validate an instance; on an accepted atom fill *res (the reference checks
+// atoms on the host before any device work, code/ltli6463.cu:1100-1136).
+// Returns 1 if res is final, 0 to continue, or a negative status.
+int32_t precheck_host(const ltl_options &o, const ltl_problem *pb, ltl_result *res, std::string &names) {
+    const int P = pb->num_pos, N = pb->num_neg, W = P + N, A = pb->num_atoms;
+    if (P < 0 || N < 0 || W < 1 || A < 1 || A > 64 || !pb->lens || !pb->atoms) return LTL_E_ARG;
+    for (int i = 0; i < W; ++i)
+        if (pb->lens[i] < 1 || pb->lens[i] > 64) return LTL_E_LIMIT;
+    names = pb->names ? std::string(pb->names, A) : std::string();
+    if (names.empty()) for (int a = 0; a < A; ++a) names += (char)('a' + a);
+    res->cost = o.cost[0];
+    std::vector<uint64_t> cs(W);
+    for (int a = 0; a < A; ++a) {
+        for (int i = 0; i < W; ++i) cs[i] = pb->atoms[(size_t)a * W + i] & len_mask(pb->lens[i]);
+        if (ltl_mismatches(cs.data(), P, N) <= o.hamming_k) {
+            res->status = LTL_FOUND;
+            res->all_ltls = (uint64_t)a + 1;
+            res->found_id = a;
+            res->n_nodes = 1;
+            res->nodes = (ltl_node *)malloc(sizeof(ltl_node));
+            res->nodes[0] = {LTL_ATOM, a, -1, -1};
+            res->formula = strdup(std::string(1, names[a]).c_str());
+            return 1;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Enumeration
@@ -533,44 +635,28 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     if (!res) return LTL_E_ARG;
     memset(res, 0, sizeof(*res));
     res->found_id = -1;
-    if (!c || !pb || !pb->lens || !pb->atoms) return res->status = LTL_E_ARG;
+    if (!c || !pb) return res->status = LTL_E_ARG;
     const double t0 = now_s();
     const ltl_options &o = c->opt;
-    const int P = pb->num_pos, N = pb->num_neg, W = P + N, A = pb->num_atoms;
-    if (P < 0 || N < 0 || W < 1 || A < 1 || A > 64) return res->status = LTL_E_ARG;
-    int maxlen = 0;
-    for (int i = 0; i < W; ++i) {
-        if (pb->lens[i] < 1 || pb->lens[i] > 64) return res->status = LTL_E_LIMIT;
-        maxlen = std::max<int>(maxlen, pb->lens[i]);
-    }
-    std::string names = pb->names ? std::string(pb->names, A) : std::string();
-    if (names.empty()) for (int a = 0; a < A; ++a) names += (char)('a' + a);
-    const int c1 = o.cost[0], c2 = o.cost[1], c3 = o.cost[2], c4 = o.cost[3];
-    const int c5 = o.cost[4], c6 = o.cost[5], c7 = o.cost[6], c8 = o.cost[7];
-    const int maxc = o.max_cost;
-    res->cost = c1;
+    std::string names;
     for (auto &s : c->stats) { s.launches = 0; s.seconds = 0; s.alg_bytes = s.min_bytes = s.items = 0; }
-
-    // Atoms, checked in alphabet order before anything else
-    // (code/ltli6463.cu:1100-1136).
-    uint64_t all_ltls = 0;
-    std::vector<uint64_t> cs(W);
-    for (int a = 0; a < A; ++a) {
-        ++all_ltls;
-        for (int i = 0; i < W; ++i) cs[i] = pb->atoms[(size_t)a * W + i] & len_mask(pb->lens[i]);
-        if (ltl_mismatches(cs.data(), P, N) <= o.hamming_k) {
-            res->status = LTL_FOUND;
-            res->all_ltls = all_ltls;
-            res->found_id = a;
-            res->n_nodes = 1;
-            res->nodes = (ltl_node *)malloc(sizeof(ltl_node));
-            res->nodes[0] = {LTL_ATOM, a, -1, -1};
-            res->formula = strdup(std::string(1, names[a]).c_str());
+    {
+        const int32_t pre = precheck_host(o, pb, res, names);
+        if (pre < 0) return res->status = pre;
+        if (pre == 1) {
             res->seconds = now_s() - t0;
             c->W = 0;
             return res->status;
         }
     }
+    const int P = pb->num_pos, N = pb->num_neg, W = P + N, A = pb->num_atoms;
+    (void)N;
+    int maxlen = 0;
+    for (int i = 0; i < W; ++i) maxlen = std::max<int>(maxlen, pb->lens[i]);
+    const int c1 = o.cost[0], c2 = o.cost[1], c3 = o.cost[2], c4 = o.cost[3];
+    const int c5 = o.cost[4], c6 = o.cost[5], c7 = o.cost[6], c8 = o.cost[7];
+    const int maxc = o.max_cost;
+    const uint64_t all_ltls = (uint64_t)A;  // atoms (code/ltli6463.cu:1134)
 
     // Layout: B-bit trace slots, TPW traces per word.
     if (hipSetDevice(o.device) != hipSuccess) return res->status = LTL_E_HIP;
@@ -583,46 +669,13 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     if (int32_t e = carve(c)) return res->status = e;
     if (c->capacity < A) return res->status = LTL_E_OOM;
 
-    // Per-word keys and masks; atoms packed into stripe 0.
-    std::vector<WordAux> aux(c->nw);
-    const uint64_t seed = 0x5eed1e55c0ffee11ull;
-    for (int w = 0; w < c->nw; ++w) {
-        WordAux &x = aux[w];
-        memset(&x, 0, sizeof(x));
-        x.k1 = splitmix(seed ^ (2ull * w));
-        x.k2 = splitmix(~seed ^ (2ull * w + 1));
-        for (int j = 0; j < c->TPW; ++j) {
-            const int i = w * c->TPW + j;
-            if (i >= W) break;
-            x.lm |= len_mask(pb->lens[i]) << (j * B);
-            x.chk |= 1ull << (j * B);
-            if (i < P) x.tgt |= 1ull << (j * B);
-        }
-    }
-    std::vector<uint64_t> atom_words((size_t)c->nw * A, 0);  // [w][a]
-    for (int a = 0; a < A; ++a)
-        for (int i = 0; i < W; ++i)
-            atom_words[(size_t)(i / c->TPW) * A + a] |=
-                (pb->atoms[(size_t)a * W + i] & len_mask(pb->lens[i])) << ((i % c->TPW) * B);
-    // Summaries (b0, b1, nz, fl per trace, 64 traces per word) of the atoms,
-    // and the per-summary-word target and valid masks.
+    // Per-word keys and masks, summary masks, packed atoms and their summaries.
+    Prepared prep;
+    prepare_instance(pb, B, prep);
     const int nsw = c->nsw;
-    std::vector<uint64_t> sum0((size_t)4 * nsw * A, 0);  // [k*nsw + j][a]
-    std::vector<SumAux> saux(nsw, SumAux{0, 0});
-    for (int i = 0; i < W; ++i) {
-        saux[i / 64].val |= 1ull << (i % 64);
-        if (i < P) saux[i / 64].tgt |= 1ull << (i % 64);
-    }
-    for (int a = 0; a < A; ++a)
-        for (int i = 0; i < W; ++i) {
-            const uint64_t m = len_mask(pb->lens[i]), x = pb->atoms[(size_t)a * W + i] & m;
-            const uint64_t bit = 1ull << (i % 64);
-            const int j = i / 64;
-            if (x & 1) sum0[(size_t)(0 * nsw + j) * A + a] |= bit;
-            if (x & 2) sum0[(size_t)(1 * nsw + j) * A + a] |= bit;
-            if (x) sum0[(size_t)(2 * nsw + j) * A + a] |= bit;
-            if (x == m) sum0[(size_t)(3 * nsw + j) * A + a] |= bit;
-        }
+    std::vector<WordAux> &aux = prep.aux;
+    std::vector<SumAux> &saux = prep.saux;
+    std::vector<uint64_t> &atom_words = prep.atom_words, &sum0 = prep.atom_sums;
     DevState st0{};
     st0.found = kEmpty;
     st0.n_stored = A;
@@ -776,15 +829,210 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     return res->status;
 }
 
-extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n, ltl_result *res) {
-    if (!c || !pbs || !res || n < 0) return LTL_E_ARG;
-    int32_t solved = 0;
-    for (int32_t i = 0; i < n; ++i) {
-        int32_t st = ltl_learn(c, &pbs[i], &res[i]);
-        if (st < 0) return st;
-        solved += st == LTL_FOUND;
+// ---------------------------------------------------------------------------
+// Batched learning: one persistent launch over many instances.
+// ---------------------------------------------------------------------------
+static int32_t batch_fill_result(const BatchOut &bo, const std::string &names, ltl_result *r) {
+    r->cost = bo.cost;
+    r->all_ltls = bo.all_ltls;
+    r->found_id = bo.found_id;
+    r->n_stored = bo.n_stored;
+    if (bo.status == BATCH_FOUND) {
+        std::vector<ltl_node> nodes(bo.nodes, bo.nodes + bo.n_nodes);
+        r->n_nodes = bo.n_nodes;
+        r->nodes = (ltl_node *)malloc(nodes.size() * sizeof(ltl_node));
+        memcpy(r->nodes, nodes.data(), nodes.size() * sizeof(ltl_node));
+        r->formula = strdup(node_string(nodes, 0, names).c_str());
+        return r->status = LTL_FOUND;
     }
+    r->formula = strdup("not_found");
+    return r->status = LTL_NOT_FOUND;
+}
+
+extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n, ltl_result *res) {
+    if (!c || n < 0 || (n > 0 && (!pbs || !res))) return LTL_E_ARG;
+    const double t0 = now_s();
+    const ltl_options &o = c->opt;
+    ltl_batch_stats bs{};
+    bs.n = n;
+    std::vector<std::string> names(n);
+    std::vector<int> dev;
+    for (int i = 0; i < n; ++i) {
+        memset(&res[i], 0, sizeof(res[i]));
+        res[i].found_id = -1;
+        const int32_t pre = precheck_host(o, &pbs[i], &res[i], names[i]);
+        if (pre < 0) res[i].status = pre;
+        else if (pre == 1) bs.host++;
+        else dev.push_back(i);
+    }
+    std::vector<int> deferred;
+    if (!dev.empty() && (o.max_cost + 2 > kBatchLevels || o.dedup_mode != LTL_DEDUP_EXACT)) {
+        deferred = dev;
+        dev.clear();
+    }
+    if (!dev.empty()) {
+        if (hipSetDevice(o.device) != hipSuccess) return LTL_E_HIP;
+        const int m = (int)dev.size();
+        std::vector<BatchInst> inst(m);
+        std::vector<uint64_t> in;
+        std::vector<int64_t> work(m);
+        for (int k = 0; k < m; ++k) {
+            const ltl_problem *pb = &pbs[dev[k]];
+            Prepared q;
+            prepare_instance(pb, slot_width(pb), q);
+            BatchInst &I = inst[k];
+            memset(&I, 0, sizeof(I));
+            I.W = pb->num_pos + pb->num_neg; I.P = pb->num_pos; I.A = pb->num_atoms;
+            I.nw = q.nw; I.nsw = q.nsw; I.B = q.B;
+            I.max_cost = o.max_cost; I.neg_type = o.neg_type; I.hamming_k = o.hamming_k;
+            for (int j = 0; j < 8; ++j) I.cost[j] = o.cost[j];
+            auto put = [&](const void *p, size_t bytes) {
+                const int64_t off = (int64_t)in.size();
+                in.resize(in.size() + (bytes + 7) / 8);
+                memcpy(in.data() + off, p, bytes);
+                return off;
+            };
+            I.off_aux = put(q.aux.data(), q.aux.size() * sizeof(WordAux));
+            I.off_saux = put(q.saux.data(), q.saux.size() * sizeof(SumAux));
+            I.off_atoms = put(q.atom_words.data(), q.atom_words.size() * 8);
+            I.off_sums = put(q.atom_sums.data(), q.atom_sums.size() * 8);
+            work[k] = (int64_t)I.W * I.A * I.A;
+        }
+        // one launch per slot width; within it, largest instances first so
+        // that they do not form the tail
+        std::vector<int32_t> order(m);
+        for (int k = 0; k < m; ++k) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            if (inst[a].B != inst[b].B) return inst[a].B < inst[b].B;
+            return work[a] > work[b];
+        });
+
+        if (int32_t e = ensure_pool(c)) return e;
+        const ltl_batch_params &bp = c->batch;
+        const int tile = bp.tile == 256 || bp.tile == 1024 ? bp.tile : 512;
+        const int wb = bp.word_block == 8 ? 8 : 4;
+        int G = bp.workgroups > 0 ? bp.workgroups : (tile == 1024 ? 1 : 2) * std::max(c->n_cu, 1);
+        char *p = c->pool;
+        auto take = [&](size_t bytes) { char *q = p; p += align_up(bytes, 256); return q; };
+        BatchInst *d_inst = (BatchInst *)take(inst.size() * sizeof(BatchInst));
+        BatchOut *d_out = (BatchOut *)take((size_t)m * sizeof(BatchOut));
+        int32_t *d_order = (int32_t *)take((size_t)m * 4);
+        unsigned *d_next = (unsigned *)take(256);
+        uint64_t *d_in = (uint64_t *)take(in.size() * 8);
+        const size_t used = (size_t)(p - c->pool);
+        if (used + (size_t)G * (64ull << 20) > c->pool_bytes)
+            G = (int)std::max<size_t>(1, (c->pool_bytes - std::min(used, c->pool_bytes)) / (64ull << 20));
+        const uint64_t region = ((c->pool_bytes - used) / (size_t)G) / 4096 * 4096;
+        uint32_t tslots = 1u << 20;
+        while (tslots > 4096 && (uint64_t)tslots * 16 > region / 4) tslots >>= 1;
+        hipStream_t s = c->stream;
+        HIP_TRY(hipMemcpyAsync(d_inst, inst.data(), inst.size() * sizeof(BatchInst), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_in, in.data(), in.size() * 8, hipMemcpyHostToDevice, s));
+        HIP_TRY(launch_fill_regions(p, region, tslots, G, s));
+        BatchArgs ba{};
+        ba.inst = d_inst; ba.out = d_out; ba.in = d_in;
+        ba.regions = p; ba.region_bytes = region; ba.tslots = tslots; ba.next = d_next;
+        ba.defer_limit = bp.defer_limit > 0 ? bp.defer_limit : ((int64_t)1 << 22);
+        hipEvent_t e0 = get_event(c), e1 = get_event(c);
+        HIP_TRY(hipEventRecord(e0, s));
+        for (int g0 = 0; g0 < m;) {
+            int g1 = g0;
+            while (g1 < m && inst[order[g1]].B == inst[order[g0]].B) ++g1;
+            ba.order = d_order + g0;
+            ba.n_inst = g1 - g0;
+            HIP_TRY(hipMemsetAsync(d_next, 0, 4, s));
+            HIP_TRY(launch_batch(inst[order[g0]].B, tile, wb, std::min(G, g1 - g0), ba, s));
+            g0 = g1;
+        }
+        HIP_TRY(hipEventRecord(e1, s));
+        std::vector<BatchOut> out(m);
+        HIP_TRY(hipMemcpyAsync(out.data(), d_out, (size_t)m * sizeof(BatchOut), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        c->free_events.push_back(e0);
+        c->free_events.push_back(e1);
+        bs.kernel_seconds = ms * 1e-3;
+        for (int k = 0; k < m; ++k) {
+            const int i = dev[k];
+            const BatchOut &bo = out[k];
+            if (bo.status == BATCH_FOUND || bo.status == BATCH_NOT_FOUND) {
+                batch_fill_result(bo, names[i], &res[i]);
+                bs.on_device++;
+            } else if (bo.status == BATCH_ERROR && bo.error == 1) {
+                res[i].status = LTL_E_COLLISION;
+            } else {
+                deferred.push_back(i);
+            }
+        }
+    }
+    c->W = 0;  // the store of the last call is not readable after a batch
+    for (int i : deferred) {
+        ltl_learn(c, &pbs[i], &res[i]);
+        bs.deferred++;
+    }
+    int32_t solved = 0;
+    for (int i = 0; i < n; ++i) solved += res[i].status == LTL_FOUND;
+    bs.seconds = now_s() - t0;
+    c->batch_stats = bs;
     return solved;
+}
+
+extern "C" int32_t ltl_set_batch_params(ltl_ctx *c, const ltl_batch_params *p) {
+    if (!c || !p) return LTL_E_ARG;
+    if (p->tile != 256 && p->tile != 512 && p->tile != 1024) return LTL_E_ARG;
+    if (p->word_block != 4 && p->word_block != 8) return LTL_E_ARG;
+    if (p->workgroups < 0 || p->defer_limit < 0) return LTL_E_ARG;
+    c->batch = *p;
+    return 0;
+}
+
+extern "C" int32_t ltl_get_batch_params(const ltl_ctx *c, ltl_batch_params *p) {
+    if (!c || !p) return LTL_E_ARG;
+    *p = c->batch;
+    return 0;
+}
+
+extern "C" int32_t ltl_batch_last_stats(const ltl_ctx *c, ltl_batch_stats *s) {
+    if (!c || !s) return LTL_E_ARG;
+    *s = c->batch_stats;
+    return 0;
+}
+
+extern "C" int32_t ltl_autotune_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n, ltl_batch_trial *trials,
+                                      int32_t max_trials) {
+    if (!c || n <= 0 || !pbs) return LTL_E_ARG;
+    std::vector<ltl_result> res(n);
+    auto run = [&](const ltl_batch_params &p, ltl_batch_trial *t) -> int32_t {
+        if (int32_t e = ltl_set_batch_params(c, &p)) return e;
+        const int32_t r = ltl_learn_batch(c, pbs, n, res.data());
+        for (auto &x : res) ltl_result_free(&x);
+        if (r < 0) return r;
+        if (t) {
+            t->params = p;
+            t->seconds = c->batch_stats.seconds;
+            t->deferred = c->batch_stats.deferred;
+        }
+        return 0;
+    };
+    const ltl_batch_params keep = c->batch;
+    if (int32_t e = run(keep, nullptr)) return e;  // loads the code objects
+    std::vector<ltl_batch_trial> all;
+    for (int tile : {256, 512, 1024})
+        for (int wb : {4, 8})
+            for (int64_t lim : {(int64_t)1 << 20, (int64_t)1 << 22}) {
+                ltl_batch_params p{tile, wb, 0, 0, lim};
+                ltl_batch_trial t{};
+                if (int32_t e = run(p, &t)) return e;
+                all.push_back(t);
+            }
+    size_t best = 0;
+    for (size_t k = 1; k < all.size(); ++k)
+        if (all[k].seconds < all[best].seconds) best = k;
+    c->batch = all[best].params;
+    for (int32_t k = 0; k < max_trials && k < (int32_t)all.size(); ++k) trials[k] = all[k];
+    return (int32_t)all.size();
 }
 
 extern "C" int32_t ltl_get_kernel_stats(const ltl_ctx *c, ltl_kernel_stat *out, int32_t max) {
