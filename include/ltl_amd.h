@@ -208,6 +208,53 @@ int32_t ltl_store_read(ltl_ctx *ctx, int64_t first, int64_t count, uint64_t *cs,
 /* Level boundaries of the last call: start[c], end[c] for c < max. */
 int32_t ltl_levels(const ltl_ctx *ctx, int64_t *start, int64_t *end, int32_t max);
 
+/* ---- Learned operand ordering ------------------------------------------
+ * No counterpart in the reference (its order is fixed by the loops of
+ * code/ltli6463.cu:1209-1440).  A scorer is a BF16 MLP (24 -> 128 -> 128 -> 1,
+ * FP32 accumulation) run on the matrix cores.  With order = LTL_ORDER_LEARNED
+ * the check pass of every cost level tests the unary blocks, then the
+ * (64-set x 64-set) tile pairs of the binary blocks in descending score, in
+ * rounds; it stops at the first round that accepts a candidate.  Every lower
+ * level has been tested completely, so the cost is the same as in plain
+ * order; the formula is a separating formula of that cost, not necessarily
+ * the one with the smallest canonical id. */
+typedef struct ltl_scorer ltl_scorer;
+
+#define LTL_SCORER_FEATURES 24
+/* Parameter layout (floats): W1[128][32] (columns 24..31 unused), b1[128],
+ * W2[128][128], b2[128], w3[128], b3. */
+#define LTL_SCORER_PARAMS (128 * 32 + 128 + 128 * 128 + 128 + 128 + 1)
+
+/* Create a scorer on a device with seeded initial weights; NULL on error. */
+ltl_scorer *ltl_scorer_create(int32_t device, uint64_t seed, int32_t *status);
+void ltl_scorer_destroy(ltl_scorer *sc);
+int32_t ltl_scorer_get_params(ltl_scorer *sc, float *params);
+/* Replace the parameters; resets the optimizer state. */
+int32_t ltl_scorer_set_params(ltl_scorer *sc, const float *params);
+/* y[i] = score of feature row x[i*24 .. i*24+23]. */
+int32_t ltl_scorer_forward(ltl_scorer *sc, const float *x, int64_t n, float *y);
+/* Gradient of the mean squared error mean_i (y_i - t_i)^2 with respect to
+ * the parameters (layout above); *loss = that error.  The BF16 products
+ * are those of the forward pass. */
+int32_t ltl_scorer_grad(ltl_scorer *sc, const float *x, const float *t, int64_t n, float *grad, float *loss);
+/* Train on log rows (24 features, then the label; rows[i*25 + 24]) with
+ * Adam (beta 0.9 / 0.999), mini-batches of `batch` rows in a seeded
+ * shuffled order, `epochs` passes.  *loss = mean batch loss of the last pass. */
+int32_t ltl_scorer_train(ltl_scorer *sc, const float *rows, int64_t n, int32_t epochs, int32_t batch, float lr,
+                         uint64_t seed, float *loss);
+/* Attach a scorer to a context (same device); NULL detaches. */
+int32_t ltl_set_scorer(ltl_ctx *ctx, ltl_scorer *sc);
+
+/* Enumeration log: with max_rows > 0, every following ltl_learn call writes
+ * up to per_block rows per binary block of each level's check pass (tile
+ * pairs at an even stride over the block): the 24 features of the tile pair
+ * and, as the label, 1 - (fewest misclassified traces of any candidate in
+ * it) / W.  max_rows = 0 turns logging off. */
+int32_t ltl_set_log(ltl_ctx *ctx, int64_t max_rows, int32_t per_block);
+/* Copy up to max_rows rows (25 floats each) of the last call; returns the
+ * number of rows the call logged. */
+int64_t ltl_get_log(ltl_ctx *ctx, float *rows, int64_t max_rows);
+
 /* ---- Host helpers around the boundary (no GPU needed) ------------------ */
 
 /* Read a trace file in the reference format (code/ltli6463.cu:1500-1573):

@@ -99,7 +99,13 @@ EXPORTS = ["ltl_default_options", "ltl_create", "ltl_set_options", "ltl_learn", 
            "ltl_set_batch_params", "ltl_get_batch_params", "ltl_batch_last_stats", "ltl_autotune_batch",
            "ltl_result_free", "ltl_destroy", "ltl_get_kernel_stats", "ltl_store_read", "ltl_levels",
            "ltl_read_trace_file", "ltl_eval_formula", "ltl_mismatches", "ltl_version",
-           "ltl_gen_config", "ltl_gen_custom", "ltl_gen_free"]
+           "ltl_gen_config", "ltl_gen_custom", "ltl_gen_free",
+           "ltl_scorer_create", "ltl_scorer_destroy", "ltl_scorer_get_params", "ltl_scorer_set_params",
+           "ltl_scorer_forward", "ltl_scorer_grad", "ltl_scorer_train", "ltl_set_scorer", "ltl_set_log",
+           "ltl_get_log"]
+
+SCORER_FEATURES = 24
+SCORER_PARAMS = 128 * 32 + 128 + 128 * 128 + 128 + 128 + 1
 
 _lib = None
 
@@ -139,6 +145,20 @@ def lib():
                                  ctypes.c_int32, ctypes.c_int32, ctypes.c_double, ctypes.c_char_p,
                                  ctypes.c_int32, P(GenInstance)]
     L.ltl_gen_free.argtypes = [P(GenInstance)]
+    fp = P(ctypes.c_float)
+    L.ltl_scorer_create.argtypes = [ctypes.c_int32, ctypes.c_uint64, P(ctypes.c_int32)]
+    L.ltl_scorer_create.restype = ctypes.c_void_p
+    L.ltl_scorer_destroy.argtypes = [ctypes.c_void_p]
+    L.ltl_scorer_get_params.argtypes = [ctypes.c_void_p, fp]
+    L.ltl_scorer_set_params.argtypes = [ctypes.c_void_p, fp]
+    L.ltl_scorer_forward.argtypes = [ctypes.c_void_p, fp, ctypes.c_int64, fp]
+    L.ltl_scorer_grad.argtypes = [ctypes.c_void_p, fp, fp, ctypes.c_int64, fp, fp]
+    L.ltl_scorer_train.argtypes = [ctypes.c_void_p, fp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.c_float, ctypes.c_uint64, fp]
+    L.ltl_set_scorer.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.ltl_set_log.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32]
+    L.ltl_get_log.argtypes = [ctypes.c_void_p, fp, ctypes.c_int64]
+    L.ltl_get_log.restype = ctypes.c_int64
     _lib = L
     return L
 
@@ -340,11 +360,34 @@ class Learner:
                      seconds=t.seconds, deferred=t.deferred) for t in trials[:k]]
 
     def kernel_stats(self):
-        arr = (KernelStat * 16)()
-        n = lib().ltl_get_kernel_stats(self._ctx, arr, 16)
+        arr = (KernelStat * 32)()
+        n = lib().ltl_get_kernel_stats(self._ctx, arr, 32)
         return {arr[i].name.decode(): dict(launches=arr[i].launches, seconds=arr[i].seconds,
                                            alg_bytes=arr[i].alg_bytes, min_bytes=arr[i].min_bytes,
-                                           items=arr[i].items) for i in range(min(n, 16))}
+                                           items=arr[i].items) for i in range(min(n, 32))}
+
+    def set_scorer(self, scorer):
+        """Attach a Scorer (used with order=ORDER_LEARNED); None detaches."""
+        rc = lib().ltl_set_scorer(self._ctx, scorer._sc if scorer is not None else None)
+        if rc != 0:
+            raise LtlError(f"ltl_set_scorer -> {rc}")
+        self._scorer = scorer  # keep it alive while attached
+
+    def set_log(self, max_rows, per_block=64):
+        """Log up to per_block tile-pair rows per binary block of every level."""
+        rc = lib().ltl_set_log(self._ctx, int(max_rows), int(per_block))
+        if rc != 0:
+            raise LtlError(f"ltl_set_log -> {rc}")
+        self._log_cap = int(max_rows)
+
+    def get_log(self):
+        """Rows (n, 25) of the last call: 24 features, then the label."""
+        cap = getattr(self, "_log_cap", 0)
+        rows = np.zeros((max(cap, 1), SCORER_FEATURES + 1), dtype=np.float32)
+        n = lib().ltl_get_log(self._ctx, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), cap)
+        if n < 0:
+            raise LtlError(f"ltl_get_log -> {n}")
+        return rows[:min(n, cap)].copy()
 
     def levels(self, max_levels=64):
         s = np.zeros(max_levels, dtype=np.int64)
@@ -365,6 +408,75 @@ class Learner:
         if rc != 0:
             raise LtlError(f"ltl_store_read -> {rc}")
         return cs.reshape(count, W), op, left, right
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class Scorer:
+    """The learned-order scorer: BF16 MLP 24 -> 128 -> 128 -> 1 on MFMA."""
+
+    def __init__(self, seed=1, device=0):
+        st = ctypes.c_int32(0)
+        self._sc = lib().ltl_scorer_create(int(device), int(seed), ctypes.byref(st))
+        if not self._sc:
+            raise LtlError(f"ltl_scorer_create failed: {st.value}")
+
+    def close(self):
+        if self._sc:
+            lib().ltl_scorer_destroy(self._sc)
+            self._sc = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(SCORER_PARAMS, dtype=np.float32)
+        rc = lib().ltl_scorer_get_params(self._sc, _fp(p))
+        if rc != 0:
+            raise LtlError(f"ltl_scorer_get_params -> {rc}")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.shape != (SCORER_PARAMS,):
+            raise ValueError("parameter vector of the wrong size")
+        rc = lib().ltl_scorer_set_params(self._sc, _fp(p))
+        if rc != 0:
+            raise LtlError(f"ltl_scorer_set_params -> {rc}")
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, SCORER_FEATURES)
+        y = np.zeros(x.shape[0], dtype=np.float32)
+        rc = lib().ltl_scorer_forward(self._sc, _fp(x), x.shape[0], _fp(y))
+        if rc != 0:
+            raise LtlError(f"ltl_scorer_forward -> {rc}")
+        return y
+
+    def grad(self, x, t):
+        """(gradient of mean (y - t)^2 over the parameter vector, loss)."""
+        x = np.ascontiguousarray(x, dtype=np.float32).reshape(-1, SCORER_FEATURES)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        g = np.zeros(SCORER_PARAMS, dtype=np.float32)
+        loss = ctypes.c_float(0)
+        rc = lib().ltl_scorer_grad(self._sc, _fp(x), _fp(t), x.shape[0], _fp(g), ctypes.byref(loss))
+        if rc != 0:
+            raise LtlError(f"ltl_scorer_grad -> {rc}")
+        return g, loss.value
+
+    def train(self, rows, epochs=20, batch=1024, lr=1e-3, seed=1):
+        """Adam on log rows (n, 25); returns the mean batch loss of the last epoch."""
+        rows = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, SCORER_FEATURES + 1)
+        loss = ctypes.c_float(0)
+        rc = lib().ltl_scorer_train(self._sc, _fp(rows), rows.shape[0], int(epochs), int(batch), float(lr),
+                                    int(seed), ctypes.byref(loss))
+        if rc != 0:
+            raise LtlError(f"ltl_scorer_train -> {rc}")
+        return loss.value
 
 
 def _instance_from_tokens(pos, neg, alphabet):
@@ -407,3 +519,37 @@ def LTLI(costFun, maxCost, RlxUnqChkTyp, NegType, alphabet, pos, neg, learner=No
         if own:
             L.close()
     return r["formula"], r["cost"], r["all_ltls"]
+
+
+def collect_logs(learner=None, per_block=64, config1_seeds=range(1, 21), config2_max_cost=7,
+                 config3_max_cost=6, max_rows=1 << 20):
+    """Enumeration logs of seeded runs of configurations 1-3 (plain order):
+    rows (n, 25) of tile-pair features and labels for Scorer.train."""
+    own = learner is None
+    L = learner or Learner(budget=8 << 30)
+    out = []
+    try:
+        L.set_log(max_rows, per_block)
+        for s in config1_seeds:
+            L.learn(generate(1, seed=s), max_cost=10, order=ORDER_PLAIN)
+            out.append(L.get_log())
+        L.learn(generate(2, seed=1), max_cost=config2_max_cost, order=ORDER_PLAIN)
+        out.append(L.get_log())
+        inst3 = generate(3, seed=1)
+        L.learn(inst3, max_cost=config3_max_cost, hamming_k=inst3.hamming_k, order=ORDER_PLAIN)
+        out.append(L.get_log())
+        L.set_log(0, 0)
+    finally:
+        if own:
+            L.close()
+    return np.concatenate(out) if out else np.zeros((0, SCORER_FEATURES + 1), np.float32)
+
+
+def train_scorer(rows=None, seed=1, epochs=20, batch=512, lr=1e-3, learner=None):
+    """A Scorer trained on the device from enumeration logs (collect_logs()
+    when rows is None).  Returns (scorer, final mean batch loss, rows used)."""
+    if rows is None:
+        rows = collect_logs(learner=learner)
+    sc = Scorer(seed=seed)
+    loss = sc.train(rows, epochs=epochs, batch=batch, lr=lr, seed=seed)
+    return sc, loss, len(rows)

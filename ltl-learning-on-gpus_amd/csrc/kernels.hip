@@ -156,18 +156,30 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
 
 // Atoms: interned with committed values (store index = atom index), so that
 // the first of equal atoms wins (code/ltli6463.cu:144-173).
-template <int B>
+// One wave per atom, lanes over the words; the NH sums (mod 2^64) are
+// reduced across the wave.
 __global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms) {
-    const int a = (int)threadIdx.x;
-    if (a >= n_atoms) return;
+    const int a = (int)blockIdx.x, l = (int)threadIdx.x;
     const uint64_t *p = A.store + (size_t)a;
     Acc c0;
-    for (int w = 0; w < A.nw; ++w) fold(c0, p[(size_t)w * A.ws], A.aux + w);
+    for (int w = l; w < A.nw; w += 64) fold(c0, p[(size_t)w * A.ws], A.aux + w);
+    for (int o = 1; o < 64; o <<= 1) {
+        c0.f1 += __shfl_xor(c0.f1, o);
+        c0.f2 += __shfl_xor(c0.f2, o);
+    }
+    if (l != 0) return;
     tab_insert(A.tab, A.tmask, table_key(c0.f1), (uint64_t)a, &A.st->probe_fail);
     A.m_fp2[a] = c0.f2;
     A.m_op[a] = LTL_ATOM;
     A.m_left[a] = -1;
     A.m_right[a] = -1;
+}
+
+// dst[r * ws + c] = src[r * ncols + c]: rows of a contiguous staging copy
+// into the word-major store.
+__global__ void k_scatter(uint64_t *dst, int64_t ws, const uint64_t *src, int ncols, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[(i / ncols) * ws + i % ncols] = src[i];
 }
 
 // ---------------------------------------------------------------------------
@@ -344,16 +356,25 @@ template <int OP>
 __global__ __launch_bounds__(256) void k_chk1(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
-    if (!tile_unary(A, t) || !t.valid) return;
+    if (!tile_unary(A, t)) return;
     constexpr int VEC = OP == LTL_NOT ? 0 : OP == LTL_NEXT ? 1 : OP == LTL_FIN ? 2 : 3;
-    const uint64_t *p = A.sum + (size_t)(A.ls + t.a) + (size_t)(VEC * A.nsw) * A.ws;
-    uint32_t m = 0;
+    const uint64_t *p = A.sum + (size_t)(A.ls + (t.valid ? t.a : A.a0)) + (size_t)(VEC * A.nsw) * A.ws;
+    const uint32_t k = (uint32_t)A.hamming_k;
+    const bool anti = A.anti && t.valid;
+    uint32_t m = t.valid ? 0u : 0x7fffffffu;  // invalid lanes never accept
+    uint32_t seen = 0;                         // traces tested so far
     for (int j = 0; j < A.nsw; ++j) {
         uint64_t v = p[(size_t)j * A.ws];
         if (OP == LTL_NOT) v = ~v;
         m += (uint32_t)__popcll((v ^ A.saux[j].tgt) & A.saux[j].val);
+        seen += (uint32_t)__popcll(A.saux[j].val);
+        // Every lane of the wave already fails (and, with anti, classifies
+        // more than k traces correctly, so its complement fails too).
+        if (!__any(m <= k || (anti && seen - m <= k))) return;
     }
-    if (m <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)(A.id_base + (uint64_t)t.a));
+    const uint64_t id = A.id_base + (uint64_t)t.a;
+    if (m <= k) atomicMin(&A.st->found, (unsigned long long)id);
+    else if (anti && seen - m <= k) atomicMin(&A.st->neg_found, (unsigned long long)id);
 }
 
 // & and |: position 0 of the result is the AND / OR of the operands' b0.
@@ -365,13 +386,14 @@ __global__ __launch_bounds__(256) void k_chk2(ChunkArgs A) {
     const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
     const uint64_t *pl = A.sum + (size_t)sl;
     const uint64_t *pr = A.sum + (size_t)sr;
-    uint32_t m = 0;
+    uint32_t m = t.valid ? 0u : 0x7fffffffu;  // invalid lanes never accept
     for (int j = 0; j < A.nsw; ++j) {
         const uint64_t l = pl[(size_t)j * A.ws], r = pr[(size_t)j * A.ws];
         const uint64_t c = OP == LTL_AND ? (l & r) : (l | r);
         m += (uint32_t)__popcll((c ^ A.saux[j].tgt) & A.saux[j].val);
+        if (!__any(m <= (uint32_t)A.hamming_k)) return;  // every lane of the wave already fails
     }
-    if (!t.valid || m > (uint32_t)A.hamming_k) return;
+    if (m > (uint32_t)A.hamming_k) return;
     const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
     atomicMin(&A.st->found, (unsigned long long)(A.id_base + pair));
 }
@@ -384,7 +406,8 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
     const uint64_t *pl = A.store + (size_t)sl;
     const uint64_t *pr = A.store + (size_t)sr;
-    uint32_t m0 = 0, m1 = 0;
+    const uint32_t k = (uint32_t)A.hamming_k;
+    uint32_t m0 = t.valid ? 0u : 0x7fffffffu, m1 = m0;  // invalid lanes never accept
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = A.aux + w;
         m0 += (uint32_t)__popcll((until_at0<B>(l, r) ^ x->tgt) & x->chk);
@@ -398,12 +421,14 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
         for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) * A.ws]; r[u] = pr[(size_t)(w + u) * A.ws]; }
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
+        // Mismatch counts only grow: stop once every lane of the wave fails.
+        if (!__any(m0 <= k || m1 <= k)) return;
     }
     for (; w < A.nw; ++w) step(w, pl[(size_t)w * A.ws], pr[(size_t)w * A.ws]);
     if (!t.valid) return;
     const uint64_t id0 = A.id_base + 2 * ((uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b);
-    if (m0 <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
-    else if (m1 <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
+    if (m0 <= k) atomicMin(&A.st->found, (unsigned long long)id0);
+    else if (m1 <= k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
 }
 
 __global__ void k_fill(uint64_t *p, uint64_t v, size_t n) {
@@ -495,12 +520,17 @@ hipError_t launch_finish(const ChunkArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s) {
-    switch (B) {
-    case 8: hipLaunchKernelGGL(k_atoms<8>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
-    case 16: hipLaunchKernelGGL(k_atoms<16>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
-    case 32: hipLaunchKernelGGL(k_atoms<32>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
-    default: hipLaunchKernelGGL(k_atoms<64>, dim3(1), dim3(64), 0, s, a, n_atoms); break;
-    }
+    (void)B;
+    if (n_atoms <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_atoms, dim3((unsigned)n_atoms), dim3(64), 0, s, a, n_atoms);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(uint64_t *dst, int64_t ws, const uint64_t *src, int ncols, int64_t nrows, hipStream_t s) {
+    const int64_t n = nrows * ncols;
+    if (n <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_scatter, dim3(blocks), dim3(256), 0, s, dst, ws, src, ncols, n);
     return hipGetLastError();
 }
 

@@ -44,7 +44,8 @@ struct DevState {
     unsigned long long dropped;   // survivors not stored (overflow)
     int collision;                // fingerprint collision detected
     int probe_fail;               // table full
-    unsigned long long pad[2];
+    unsigned long long neg_found; // smallest id of a candidate whose complement is accepted
+    unsigned long long pad;
 };
 
 // One chunk of one (operator, split) block, in canonical order.  Binary:
@@ -79,6 +80,8 @@ struct ChunkArgs {
     int32_t nbs;              // 64-wide column stripes per row in the chunk
     int32_t n_tiles;          // tiles in the chunk
     int32_t nsw;              // summary words per vector: ceil(W / 64)
+    int32_t anti;             // check pass: also record candidates whose complement is accepted
+    int32_t pad_;
 };
 
 // Launchers (stream-ordered, asynchronous).
@@ -90,6 +93,8 @@ hipError_t launch_finish(const ChunkArgs &a, hipStream_t s);
 hipError_t launch_compact(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s);
+// dst[r * ws + c] = src[r * ncols + c] for r < nrows, c < ncols.
+hipError_t launch_scatter(uint64_t *dst, int64_t ws, const uint64_t *src, int ncols, int64_t nrows, hipStream_t s);
 // Check-only pass over a whole block (no dedup, no store): atomicMin of the
 // canonical id of every accepted candidate.
 hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s);

@@ -28,15 +28,18 @@
 #include "formula.hpp"
 #include "kernels.hpp"
 #include "batch.hpp"
+#include "scorer.hpp"
 
 using namespace ltl;
 
 namespace {
 
-enum { K_GEN1, K_GEN2, K_GENU, K_DECIDE, K_SCAN, K_EMIT1, K_EMIT2, K_EMITU, K_CHK1, K_CHK2, K_CHKU, K_NKIND };
+enum { K_GEN1, K_GEN2, K_GENU, K_DECIDE, K_SCAN, K_EMIT1, K_EMIT2, K_EMITU, K_CHK1, K_CHK2, K_CHKU,
+       K_TSTAT, K_SCORE, K_SORT, K_CHKR, K_LOG, K_NKIND };
 const char *kKindName[K_NKIND] = {"gen_unary", "gen_and_or", "gen_until", "decide",
                                   "scan_finish", "emit_unary", "emit_and_or", "emit_until",
-                                  "check_unary", "check_and_or", "check_until"};
+                                  "check_unary", "check_and_or", "check_until",
+                                  "tile_stats", "score_tiles", "sort_tiles", "check_ranked", "log_rows"};
 
 struct Block {
     uint64_t id_base, size;
@@ -91,6 +94,7 @@ struct ltl_ctx {
     int32_t *d_left = nullptr, *d_right = nullptr;
     uint32_t *t_slot = nullptr, *t_cnt = nullptr, *t_off = nullptr, *t_list = nullptr;
     uint64_t *t_fp2 = nullptr, *t_mask = nullptr;
+    uint64_t *d_stage = nullptr;  // atom words and summaries, contiguous, before the scatter
     void *d_scan = nullptr;
     size_t scan_bytes = 0;
     std::vector<int64_t> lvl_s, lvl_e;
@@ -105,6 +109,23 @@ struct ltl_ctx {
     ltl_batch_stats batch_stats{};
     int n_cu = 0;
     int64_t chunk_counter = 0;
+
+    // learned order and enumeration log
+    ltl_scorer *scorer = nullptr;
+    float *d_tstat = nullptr;     // tile statistics of the stored levels
+    int64_t tstat_cap = 0;        // tiles
+    int64_t *d_toff = nullptr;    // first tile of each level
+    int toff_cap = 0;
+    LBlock *d_blk = nullptr;      // binary blocks of the current level
+    int blk_cap = 0;
+    float *d_keys = nullptr;      // scores: 2 x ent_cap
+    uint32_t *d_vals = nullptr;   // tile-pair entries: 2 x ent_cap
+    int64_t ent_cap = 0;
+    void *d_sort = nullptr;
+    size_t sort_cap = 0;
+    float *d_log = nullptr;       // log rows
+    int64_t log_cap = 0, log_n = 0;
+    int32_t log_per_block = 0;
 };
 
 static int32_t hip_fail(hipError_t e) { return e == hipErrorOutOfMemory ? LTL_E_OOM : LTL_E_HIP; }
@@ -132,6 +153,7 @@ static int32_t check_options(const ltl_options *o) {
     if (o->dedup_mode != LTL_DEDUP_EXACT) return LTL_E_ARG;  // reference key modes: not on the device path
     if (o->hamming_k < 0) return LTL_E_ARG;
     if (o->hash_slots_log2 < 0 || o->hash_slots_log2 > 31) return LTL_E_ARG;
+    if (o->order != LTL_ORDER_PLAIN && o->order != LTL_ORDER_LEARNED) return LTL_E_ARG;
     return 0;
 }
 
@@ -172,6 +194,9 @@ extern "C" void ltl_destroy(ltl_ctx *c) {
     for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
     for (auto e : c->free_events) hipEventDestroy(e);
     if (c->pool) hipFree(c->pool);
+    for (void *p : {(void *)c->d_tstat, (void *)c->d_toff, (void *)c->d_blk, (void *)c->d_keys, (void *)c->d_vals,
+                    c->d_sort, (void *)c->d_log})
+        if (p) hipFree(p);
     if (c->h_state) hipHostFree(c->h_state);
     if (c->h_counts) hipHostFree(c->h_counts);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -216,7 +241,8 @@ static int32_t carve(ltl_ctx *c) {
     size_t fixed = align_up((size_t)c->nw * sizeof(WordAux), 256) + align_up(sizeof(DevState), 256) +
                    align_up((size_t)c->nsw * sizeof(SumAux), 256) +
                    2 * align_up((size_t)ch * 4, 256) + align_up((size_t)ch * 8, 256) + align_up(tiles * 16, 256) +
-                   2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) + 8 * 256;
+                   2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) +
+                   align_up((size_t)(c->nw + 4 * c->nsw) * c->A * 8, 256) + 9 * 256;
     if (fixed >= budget) return LTL_E_OOM;
     const size_t R = budget - fixed;
     // Table: load stays <= 1/2 with a full chunk pending (cap + ch <= slots/2).
@@ -256,6 +282,7 @@ static int32_t carve(ltl_ctx *c) {
     c->t_cnt = (uint32_t *)take(tiles * 4);
     c->t_off = (uint32_t *)take(tiles * 4);
     c->d_scan = take(c->scan_bytes);
+    c->d_stage = (uint64_t *)take((size_t)(c->nw + 4 * c->nsw) * c->A * 8);
     c->d_tab = (uint64_t *)take(slots * 16);
     const int64_t capr = align_up((size_t)cap, 64);
     c->ws = capr;
@@ -426,8 +453,14 @@ struct Run {
     uint64_t candidates = 0, alg_bytes = 0;
     bool otf = false;         // overflow seen at a previous level end
     bool check_only = false;  // blocks are tested only (precheck pass)
+    bool ranked = false;      // check pass: binary blocks go to the ranked rounds
+    bool logging = false;     // check pass: binary blocks are also logged
+    bool skip_binary = false; // check pass: binary blocks were tested ahead of the level
+    bool only_binary = false; // check pass: test the binary blocks only (lookahead)
     uint64_t checked = 0;
     std::vector<Block> blocks;
+    std::vector<LBlock> lblk;  // binary blocks of the current check pass
+    int64_t n_ent = 0;         // their tile pairs
 
     // Check-only pass over one block: no temporaries, so no chunking beyond
     // keeping the grid within limits.
@@ -509,7 +542,7 @@ struct Run {
 
     int32_t unary(int op, int64_t s, int64_t e) {
         const int64_t n = e - s;
-        if (n <= 0) return 0;
+        if (n <= 0 || (check_only && only_binary)) return 0;
         blocks.push_back({all_ltls, (uint64_t)n, op, s, n, 0, 1});
         const int64_t step = check_only ? ((int64_t)1 << 36) : c->CH;
         for (int64_t a0 = 0; a0 < n; a0 += step) {
@@ -524,12 +557,23 @@ struct Run {
         return 0;
     }
 
-    int32_t binary(int op, int64_t ls, int64_t le, int64_t rs, int64_t re) {
+    // Binary block of left level li and right level ri.
+    int32_t binary(int op, int li, int ri) {
+        const int64_t ls = c->lvl_s[li], le = c->lvl_e[li], rs = c->lvl_s[ri], re = c->lvl_e[ri];
         const int64_t nL = le - ls, nR = re - rs;
-        if (nL <= 0 || nR <= 0) return 0;
+        if (nL <= 0 || nR <= 0 || (check_only && skip_binary)) return 0;
         const int nd = op == LTL_UNTIL ? 2 : 1;
         const uint64_t size = (uint64_t)nL * (uint64_t)nR * nd;
         blocks.push_back({all_ltls, size, op, ls, nL, rs, nR});
+        if (check_only && (ranked || logging)) {
+            const int64_t nTR = (nR + 63) / 64;
+            lblk.push_back({op, li, ri, 0, ls, nL, rs, nR, all_ltls, nTR, n_ent});
+            n_ent += (nL + 63) / 64 * nTR;
+            if (ranked) {
+                all_ltls += size;
+                return 0;
+            }
+        }
         if (check_only) {
             const int64_t nbs = (nR + 63) / 64;
             const int64_t rows = std::max<int64_t>(4, ((int64_t)1 << 30) / nbs * 4);
@@ -638,6 +682,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     if (!c || !pb) return res->status = LTL_E_ARG;
     const double t0 = now_s();
     const ltl_options &o = c->opt;
+    if (o.order == LTL_ORDER_LEARNED && !c->scorer) return res->status = LTL_E_ARG;  // no scorer attached
     std::string names;
     for (auto &s : c->stats) { s.launches = 0; s.seconds = 0; s.alg_bytes = s.min_bytes = s.items = 0; }
     {
@@ -678,15 +723,20 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     std::vector<uint64_t> &atom_words = prep.atom_words, &sum0 = prep.atom_sums;
     DevState st0{};
     st0.found = kEmpty;
+    st0.neg_found = kEmpty;
     st0.n_stored = A;
     st0.overflow_chunk = LLONG_MAX;
     hipStream_t s = c->stream;
     HIP_TRY(hipMemcpyAsync(c->d_aux, aux.data(), aux.size() * sizeof(WordAux), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(c->d_saux, saux.data(), saux.size() * sizeof(SumAux), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpy2DAsync(c->d_store, (size_t)c->ws * 8, atom_words.data(), (size_t)A * 8, (size_t)A * 8,
-                             (size_t)c->nw, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpy2DAsync(c->d_sum, (size_t)c->ws * 8, sum0.data(), (size_t)A * 8, (size_t)A * 8,
-                             (size_t)4 * nsw, hipMemcpyHostToDevice, s));
+    // Atom words and summaries: one contiguous copy, then scattered into the
+    // word-major store (a strided copy from pageable memory goes row by row).
+    // (`stage` outlives the copy: the first level ends with a stream sync.)
+    std::vector<uint64_t> stage(atom_words);
+    stage.insert(stage.end(), sum0.begin(), sum0.end());
+    HIP_TRY(hipMemcpyAsync(c->d_stage, stage.data(), stage.size() * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_scatter(c->d_store, c->ws, c->d_stage, A, c->nw, s));
+    HIP_TRY(launch_scatter(c->d_sum, c->ws, c->d_stage + (size_t)c->nw * A, A, (int64_t)4 * nsw, s));
     HIP_TRY(hipMemcpyAsync(c->d_state, &st0, sizeof(st0), hipMemcpyHostToDevice, s));
     HIP_TRY(launch_fill_u64(c->d_tab, kEmpty, 2 * ((size_t)c->tmask + 1), s));
 
@@ -732,17 +782,137 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         if ((o.neg_type == 1 && cost - c2 >= c1) || (o.neg_type == 2 && cost - c2 == c1))
             if ((r = run.unary(LTL_NOT, Ls[cost - c2], Le[cost - c2]))) return r;
         for (int i = c1; 2 * i <= cost - c3; ++i)
-            if ((r = run.binary(LTL_AND, Ls[i], Le[i], Ls[cost - i - c3], Le[cost - i - c3]))) return r;
+            if ((r = run.binary(LTL_AND, i, cost - i - c3))) return r;
         for (int i = c1; 2 * i <= cost - c4; ++i)
-            if ((r = run.binary(LTL_OR, Ls[i], Le[i], Ls[cost - i - c4], Le[cost - i - c4]))) return r;
+            if ((r = run.binary(LTL_OR, i, cost - i - c4))) return r;
         if (cost - c5 >= c1 && (r = run.unary(LTL_NEXT, Ls[cost - c5], Le[cost - c5]))) return r;
         if (cost - c6 >= c1 && (r = run.unary(LTL_FIN, Ls[cost - c6], Le[cost - c6]))) return r;
         if (cost - c7 >= c1 && (r = run.unary(LTL_GLOB, Ls[cost - c7], Le[cost - c7]))) return r;
         for (int i = c1; 2 * i <= cost - c8; ++i)
-            if ((r = run.binary(LTL_UNTIL, Ls[i], Le[i], Ls[cost - i - c8], Le[cost - i - c8]))) return r;
+            if ((r = run.binary(LTL_UNTIL, i, cost - i - c8))) return r;
         return 0;
     };
+    // Learned order and log: tile statistics of the stored levels, the
+    // binary blocks of the level, then log rows and / or ranked rounds.
+    const bool learned = o.order == LTL_ORDER_LEARNED;
+    run.ranked = learned;
+    run.logging = c->log_cap > 0 && c->log_per_block > 0;
+    c->log_n = 0;
+    int tstat_upto = c1 - 1;
+    int64_t tstat_used = 0;
+    std::vector<int64_t> toff(nlev, 0);
+    if (learned || run.logging) {
+        const int64_t tiles = c->capacity / 64 + nlev + 1;
+        if (tiles > c->tstat_cap) {
+            if (c->d_tstat) hipFree(c->d_tstat);
+            c->d_tstat = nullptr;
+            c->tstat_cap = 0;
+            HIP_TRY(hipMalloc((void **)&c->d_tstat, (size_t)tiles * kTileStat * sizeof(float)));
+            c->tstat_cap = tiles;
+        }
+        if (nlev > c->toff_cap) {
+            if (c->d_toff) hipFree(c->d_toff);
+            c->d_toff = nullptr;
+            c->toff_cap = 0;
+            HIP_TRY(hipMalloc((void **)&c->d_toff, (size_t)nlev * sizeof(int64_t)));
+            c->toff_cap = nlev;
+        }
+    }
+    // Binary blocks of level `cost` recorded in run.lblk; levels up to `upto`
+    // are complete in the store.
+    auto ranked_level = [&](int cost, int upto) -> int32_t {
+        // tile statistics of the levels completed since the last call
+        if (tstat_upto < upto) {
+            for (int lv = tstat_upto + 1; lv <= upto; ++lv) {
+                const int64_t n = Le[lv] - Ls[lv];
+                toff[lv] = tstat_used;
+                if (n > 0) {
+                    HIP_TRY(timed(c, K_TSTAT, (uint64_t)n * nsw * 16, (uint64_t)n * nsw * 16, (uint64_t)n, [&] {
+                        return launch_tile_stats(ba, Ls[lv], n, P, W, c->d_tstat + tstat_used * kTileStat, s);
+                    }));
+                }
+                tstat_used += (n + 63) / 64;
+            }
+            tstat_upto = upto;
+            HIP_TRY(hipMemcpyAsync(c->d_toff, toff.data(), (size_t)nlev * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        }
+        const int nb = (int)run.lblk.size();
+        if (nb > c->blk_cap) {
+            HIP_TRY(hipStreamSynchronize(s));
+            if (c->d_blk) hipFree(c->d_blk);
+            c->d_blk = nullptr;
+            c->blk_cap = 0;
+            HIP_TRY(hipMalloc((void **)&c->d_blk, (size_t)std::max(nb, 64) * sizeof(LBlock)));
+            c->blk_cap = std::max(nb, 64);
+        }
+        HIP_TRY(hipMemcpyAsync(c->d_blk, run.lblk.data(), (size_t)nb * sizeof(LBlock), hipMemcpyHostToDevice, s));
+        const FeatCtx f{c->d_tstat, c->d_toff, (float)cost, (float)P / (float)W, W, P};
+        if (run.logging) {
+            for (int b = 0; b < nb && c->log_n < c->log_cap; ++b) {
+                const LBlock &K = run.lblk[b];
+                const int64_t E = (K.nL + 63) / 64 * K.nTR;
+                const int64_t rows = std::min<int64_t>({(int64_t)c->log_per_block, E, c->log_cap - c->log_n});
+                HIP_TRY(timed(c, K_LOG, 0, 0, (uint64_t)rows, [&] {
+                    return launch_log(B, ba, c->d_blk, b, f, rows, c->d_log + c->log_n * (kFeat + 1), s);
+                }));
+                c->log_n += rows;
+            }
+        }
+        if (!run.ranked) return 0;
+        const int64_t E = run.n_ent;
+        if (E > ((int64_t)1 << 31)) return LTL_E_LIMIT;
+        if (E > c->ent_cap) {
+            HIP_TRY(hipStreamSynchronize(s));
+            for (void *p : {(void *)c->d_keys, (void *)c->d_vals, c->d_sort})
+                if (p) hipFree(p);
+            c->d_keys = nullptr; c->d_vals = nullptr; c->d_sort = nullptr;
+            c->ent_cap = 0;
+            const int64_t cap = std::max<int64_t>(E, 1 << 16);
+            c->sort_cap = sort_temp_bytes(cap);
+            HIP_TRY(hipMalloc((void **)&c->d_keys, (size_t)cap * 2 * sizeof(float)));
+            HIP_TRY(hipMalloc((void **)&c->d_vals, (size_t)cap * 2 * sizeof(uint32_t)));
+            HIP_TRY(hipMalloc(&c->d_sort, std::max<size_t>(c->sort_cap, 256)));
+            c->ent_cap = cap;
+        }
+        float *k0 = c->d_keys, *k1 = c->d_keys + c->ent_cap;
+        uint32_t *v0 = c->d_vals, *v1 = c->d_vals + c->ent_cap;
+        HIP_TRY(timed(c, K_SCORE, 0, 0, (uint64_t)E, [&] {
+            return launch_score(c->scorer->m, c->d_blk, nb, f, E, k0, v0, s);
+        }));
+        HIP_TRY(timed(c, K_SORT, 0, 0, (uint64_t)E, [&] {
+            return launch_sort_desc(k0, k1, v0, v1, E, c->d_sort, c->sort_cap, s);
+        }));
+        // Rounds: the top 1/64 of the tile pairs (at least 256), then four
+        // times the previous round, until a round accepts a candidate.
+        int64_t r0 = 0, step = std::max<int64_t>(256, E / 64);
+        while (r0 < E) {
+            const int64_t n = std::min(step, E - r0);
+            HIP_TRY(timed(c, K_CHKR, 0, 0, (uint64_t)n * 4096, [&] {
+                return launch_check_list(B, ba, c->d_blk, nb, v1, r0, n, s);
+            }));
+            run.checked += (uint64_t)n * 4096;
+            r0 += n;
+            step *= 4;
+            HIP_TRY(hipMemcpyAsync(&h, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipStreamSynchronize(s));
+            if (h.found != kEmpty) break;
+        }
+        return 0;
+    };
+
     int64_t n_stored = A;
+    // Learned order, before any overflow: the binary blocks of level c + 1
+    // only combine levels <= c - 1, so they are tested (ranked, early exit)
+    // before level c is generated; an accepted candidate there is minimal,
+    // since every level up to c has been tested completely by then.  With
+    // neg_type 1 and unit-cost negation the check passes also record
+    // candidates whose complement is accepted: once level c has been tested
+    // completely without a hit, the negation of such a level-c candidate is
+    // an accepted formula of cost c + 1, and level c is never generated.
+    const bool anti_ok = learned && o.neg_type == 1 && c2 == 1;
+    int looked_ahead = -1;  // level whose binary blocks were tested early
+    int found_cost = -1;
+    bool neg_answer = false;
     for (cost = c1 + 1; cost <= maxc; ++cost) {
         if (run.otf) {
             const int dif = cost - shortage;  // code/ltli6463.cu:1202-1206
@@ -755,22 +925,54 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         // accepted first), so a check-only pass finds the same result as
         // generation with dedup, without generating the level.
         bool generate = true;
-        if (o.precheck || run.otf) {
+        const bool la = learned && !run.otf;
+        if (o.precheck || run.otf || learned) {
             const uint64_t save = run.all_ltls;
             const size_t nb = run.blocks.size();
             run.check_only = true;
+            run.skip_binary = la && looked_ahead == cost;
+            run.base.anti = la && anti_ok;
+            run.lblk.clear();
+            run.n_ent = 0;
             rc = level_blocks(cost);
-            run.check_only = false;
-            if (rc || (rc = sync_state())) break;
-            if (h.found != kEmpty) { found = true; Le[cost] = n_stored; break; }
+            if (!rc && !run.lblk.empty()) rc = ranked_level(cost, cost - 1);
+            if (!rc) rc = sync_state();
+            run.check_only = run.skip_binary = false;
+            run.base.anti = 0;
+            if (rc) break;
+            if (h.found != kEmpty) { found = true; found_cost = cost; Le[cost] = n_stored; break; }
+            if (la && anti_ok && h.neg_found != kEmpty && cost + 1 <= maxc) {
+                neg_answer = true;
+                found_cost = cost + 1;
+                Le[cost] = n_stored;
+                break;
+            }
+            if (la && cost + 1 <= maxc) {
+                if (launch_fill_u64(reinterpret_cast<uint64_t *>(&c->d_state->neg_found), kEmpty, 1, s) != hipSuccess) {
+                    rc = LTL_E_HIP;
+                    break;
+                }
+                run.check_only = run.only_binary = true;
+                run.base.anti = anti_ok;
+                run.lblk.clear();
+                run.n_ent = 0;
+                rc = level_blocks(cost + 1);
+                if (!rc && !run.lblk.empty()) rc = ranked_level(cost + 1, cost - 1);
+                if (!rc) rc = sync_state();
+                run.check_only = run.only_binary = false;
+                run.base.anti = 0;
+                if (rc) break;
+                looked_ahead = cost + 1;
+                if (h.found != kEmpty) { found = true; found_cost = cost + 1; Le[cost] = n_stored; break; }
+            }
             if (run.otf) generate = false;  // check-only level (overflow)
-            else { run.all_ltls = save; run.blocks.resize(nb); }
+            else if (!learned) { run.all_ltls = save; run.blocks.resize(nb); }
         }
         if (generate) {
             if ((rc = level_blocks(cost))) break;
             if ((rc = sync_state())) break;
             n_stored = h.n_stored;
-            if (h.found != kEmpty) { found = true; Le[cost] = n_stored; break; }
+            if (h.found != kEmpty) { found = true; found_cost = cost; Le[cost] = n_stored; break; }
             if (h.overflow_chunk != LLONG_MAX) run.otf = true;
         }
         Le[cost] = n_stored;
@@ -783,6 +985,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         return res->status = rc;
     }
     if (cost == maxc + 1) cost--;  // code/ltli6463.cu:1445
+    if (found || neg_answer) cost = found_cost;
     res->cost = cost;
     res->n_stored = n_stored;
     res->on_the_fly = run.otf || h.overflow_chunk != LLONG_MAX;
@@ -791,8 +994,9 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     res->hbm_bytes = run.alg_bytes;
     for (auto &k : c->stats) res->kernel_seconds += k.seconds;
 
-    if (found) {
-        const uint64_t id = h.found;
+    if (found || neg_answer) {
+        // Candidate id -> block -> operator and operand store indices.
+        const uint64_t id = neg_answer ? h.neg_found : h.found;
         const Block *blk = nullptr;
         for (const auto &b : run.blocks)
             if (id >= b.id_base && id < b.id_base + b.size) { blk = &b; break; }
@@ -815,6 +1019,14 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         std::vector<ltl_node> nodes;
         std::vector<int64_t> src;
         if (int32_t e = build_tree(c, blk->op, l, r, -1, nodes, src)) return res->status = e;
+        if (neg_answer) {
+            // The accepted formula is the negation of the recorded candidate.
+            for (auto &nd : nodes) {
+                if (nd.left >= 0) ++nd.left;
+                if (nd.right >= 0) ++nd.right;
+            }
+            nodes.insert(nodes.begin(), ltl_node{LTL_NOT, -1, 1, -1});
+        }
         res->n_nodes = (int32_t)nodes.size();
         res->nodes = (ltl_node *)malloc(nodes.size() * sizeof(ltl_node));
         memcpy(res->nodes, nodes.data(), nodes.size() * sizeof(ltl_node));
@@ -1039,6 +1251,41 @@ extern "C" int32_t ltl_get_kernel_stats(const ltl_ctx *c, ltl_kernel_stat *out, 
     if (!c) return LTL_E_ARG;
     for (int k = 0; k < K_NKIND && k < max; ++k) out[k] = c->stats[k];
     return K_NKIND;
+}
+
+extern "C" int32_t ltl_set_scorer(ltl_ctx *c, ltl_scorer *sc) {
+    if (!c) return LTL_E_ARG;
+    if (sc && sc->device != c->opt.device) return LTL_E_ARG;
+    c->scorer = sc;
+    return 0;
+}
+
+extern "C" int32_t ltl_set_log(ltl_ctx *c, int64_t max_rows, int32_t per_block) {
+    if (!c || max_rows < 0 || per_block < 0) return LTL_E_ARG;
+    HIP_TRY(hipSetDevice(c->opt.device));
+    if (max_rows > c->log_cap) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->d_log) hipFree(c->d_log);
+        c->d_log = nullptr;
+        c->log_cap = 0;
+        HIP_TRY(hipMalloc((void **)&c->d_log, (size_t)max_rows * (kFeat + 1) * sizeof(float)));
+    }
+    c->log_cap = max_rows;
+    c->log_per_block = per_block;
+    c->log_n = 0;
+    return 0;
+}
+
+extern "C" int64_t ltl_get_log(ltl_ctx *c, float *rows, int64_t max_rows) {
+    if (!c || max_rows < 0 || (max_rows && !rows)) return LTL_E_ARG;
+    const int64_t n = std::min(max_rows, c->log_n);
+    if (n > 0) {
+        HIP_TRY(hipSetDevice(c->opt.device));
+        HIP_TRY(hipMemcpyAsync(rows, c->d_log, (size_t)n * (kFeat + 1) * sizeof(float), hipMemcpyDeviceToHost,
+                               c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+    }
+    return c->log_n;
 }
 
 extern "C" int32_t ltl_levels(const ltl_ctx *c, int64_t *start, int64_t *end, int32_t max) {
