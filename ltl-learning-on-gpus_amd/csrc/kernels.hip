@@ -363,7 +363,8 @@ __global__ __launch_bounds__(256) void k_chk1(ChunkArgs A) {
     const bool anti = A.anti && t.valid;
     uint32_t m = t.valid ? 0u : 0x7fffffffu;  // invalid lanes never accept
     uint32_t seen = 0;                         // traces tested so far
-    for (int j = 0; j < A.nsw; ++j) {
+    for (int i = 0; i < A.nsw; ++i) {
+        const int j = interleave(i, A.nps, A.nsw);
         uint64_t v = p[(size_t)j * A.ws];
         if (OP == LTL_NOT) v = ~v;
         m += (uint32_t)__popcll((v ^ A.saux[j].tgt) & A.saux[j].val);
@@ -387,7 +388,8 @@ __global__ __launch_bounds__(256) void k_chk2(ChunkArgs A) {
     const uint64_t *pl = A.sum + (size_t)sl;
     const uint64_t *pr = A.sum + (size_t)sr;
     uint32_t m = t.valid ? 0u : 0x7fffffffu;  // invalid lanes never accept
-    for (int j = 0; j < A.nsw; ++j) {
+    for (int i = 0; i < A.nsw; ++i) {
+        const int j = interleave(i, A.nps, A.nsw);
         const uint64_t l = pl[(size_t)j * A.ws], r = pr[(size_t)j * A.ws];
         const uint64_t c = OP == LTL_AND ? (l & r) : (l | r);
         m += (uint32_t)__popcll((c ^ A.saux[j].tgt) & A.saux[j].val);
@@ -414,17 +416,25 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
         m1 += (uint32_t)__popcll((until_at0<B>(r, l) ^ x->tgt) & x->chk);
     };
     constexpr int UB = 4;
-    int w = 0;
-    for (; w + UB <= A.nw; w += UB) {
+    int i = 0;
+    for (; i + UB <= A.nw; i += UB) {
         uint64_t l[UB], r[UB];
+        int wu[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) * A.ws]; r[u] = pr[(size_t)(w + u) * A.ws]; }
+        for (int u = 0; u < UB; ++u) {
+            wu[u] = interleave(i + u, A.npw, A.nw);
+            l[u] = pl[(size_t)wu[u] * A.ws];
+            r[u] = pr[(size_t)wu[u] * A.ws];
+        }
 #pragma unroll
-        for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
+        for (int u = 0; u < UB; ++u) step(wu[u], l[u], r[u]);
         // Mismatch counts only grow: stop once every lane of the wave fails.
         if (!__any(m0 <= k || m1 <= k)) return;
     }
-    for (; w < A.nw; ++w) step(w, pl[(size_t)w * A.ws], pr[(size_t)w * A.ws]);
+    for (; i < A.nw; ++i) {
+        const int w = interleave(i, A.npw, A.nw);
+        step(w, pl[(size_t)w * A.ws], pr[(size_t)w * A.ws]);
+    }
     if (!t.valid) return;
     const uint64_t id0 = A.id_base + 2 * ((uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b);
     if (m0 <= k) atomicMin(&A.st->found, (unsigned long long)id0);

@@ -81,8 +81,21 @@ struct ChunkArgs {
     int32_t n_tiles;          // tiles in the chunk
     int32_t nsw;              // summary words per vector: ceil(W / 64)
     int32_t anti;             // check pass: also record candidates whose complement is accepted
+    int32_t npw, nps;         // words (summary words) holding positive traces: ceil(P / TPW), ceil(P / 64)
     int32_t pad_;
 };
+
+// Check passes visit words alternately from the positive and the negative
+// traces (sums do not depend on the order): a candidate that holds, or
+// fails, at position 0 of almost every trace then fails within a few words
+// instead of after all positive traces, and its wave stops early.  j-th
+// word of n, the first np of which hold positive traces.
+__host__ __device__ __forceinline__ int interleave(int j, int np, int n) {
+    const int m = np < n - np ? np : n - np;
+    if (j < 2 * m) return (j & 1) ? np + (j >> 1) : (j >> 1);
+    j -= 2 * m;
+    return np > n - np ? m + j : np + m + j;
+}
 
 // Launchers (stream-ordered, asynchronous).
 hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s);

@@ -235,6 +235,10 @@ static int32_t carve(ltl_ctx *c) {
     // chunk: per-candidate temporaries (18 B each) within 1/32 of the pool
     int64_t ch = 1 << 26;
     while (ch > 4096 && (size_t)ch * 18 > budget / 32) ch >>= 1;
+    // A fixed table keeps room for the stored sets: a chunk takes at most a
+    // quarter of its slots.
+    if (c->opt.hash_slots_log2)
+        while (ch > 4096 && ch > ((int64_t)1 << c->opt.hash_slots_log2) / 4) ch >>= 1;
     c->CH = ch;
     const size_t tiles = (size_t)ch / 64;
     c->scan_bytes = scan_temp_bytes((int)tiles);
@@ -747,6 +751,8 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     ba.t_list = c->t_list;
     ba.m_op = c->d_op; ba.m_left = c->d_left; ba.m_right = c->d_right; ba.m_fp2 = c->d_fp2;
     ba.sum = c->d_sum; ba.saux = c->d_saux; ba.nsw = nsw;
+    ba.npw = std::min(c->nw, (P + c->TPW - 1) / c->TPW);
+    ba.nps = std::min(nsw, (P + 63) / 64);
     ba.capacity = c->capacity;
     ba.ws = c->ws;
     ba.nw = c->nw;

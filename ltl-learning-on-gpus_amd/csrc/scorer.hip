@@ -422,17 +422,19 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
     if (op == LTL_UNTIL) {
         const uint64_t *pl = A.store + (size_t)sl, *pr = A.store + (size_t)sr;
         constexpr int UB = 4;
-        int w = 0;
-        for (; w + UB <= A.nw; w += UB) {
+        int i = 0;
+        for (; i + UB <= A.nw; i += UB) {
             uint64_t lv[UB], rv[UB];
+            int wu[UB];
 #pragma unroll
             for (int u = 0; u < UB; ++u) {
-                lv[u] = pl[(size_t)(w + u) * A.ws];
-                rv[u] = pr[(size_t)(w + u) * A.ws];
+                wu[u] = EARLY ? interleave(i + u, A.npw, A.nw) : i + u;
+                lv[u] = pl[(size_t)wu[u] * A.ws];
+                rv[u] = pr[(size_t)wu[u] * A.ws];
             }
 #pragma unroll
             for (int u = 0; u < UB; ++u) {
-                const WordAux *x = A.aux + w + u;
+                const WordAux *x = A.aux + wu[u];
                 m0 += (uint32_t)__popcll((until_at0<B>(lv[u], rv[u]) ^ x->tgt) & x->chk);
                 m1 += (uint32_t)__popcll((until_at0<B>(rv[u], lv[u]) ^ x->tgt) & x->chk);
                 seen += (uint32_t)__popcll(x->chk);
@@ -440,7 +442,8 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
             if (EARLY && !__any(m0 <= stop || m1 <= stop || (anti && (seen - m0 <= stop || seen - m1 <= stop))))
                 return;
         }
-        for (; w < A.nw; ++w) {
+        for (; i < A.nw; ++i) {
+            const int w = EARLY ? interleave(i, A.npw, A.nw) : i;
             const WordAux *x = A.aux + w;
             const uint64_t lv = pl[(size_t)w * A.ws], rv = pr[(size_t)w * A.ws];
             m0 += (uint32_t)__popcll((until_at0<B>(lv, rv) ^ x->tgt) & x->chk);
@@ -449,7 +452,8 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
         }
     } else {
         const uint64_t *pl = A.sum + (size_t)sl, *pr = A.sum + (size_t)sr;
-        for (int j = 0; j < A.nsw; ++j) {
+        for (int i = 0; i < A.nsw; ++i) {
+            const int j = EARLY ? interleave(i, A.nps, A.nsw) : i;
             const uint64_t a = pl[(size_t)j * A.ws], b = pr[(size_t)j * A.ws];
             const uint64_t v = op == LTL_AND ? (a & b) : (a | b);
             m0 += (uint32_t)__popcll((v ^ A.saux[j].tgt) & A.saux[j].val);
