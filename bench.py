@@ -83,6 +83,93 @@ def calibrate_copy(nbytes=1 << 30):
     print(json.dumps({"calibrate_copy_bytes": nbytes}), flush=True)
 
 
+def learned_vs_plain(ltl, L, args):
+    """Time to the minimal formula, plain vs learned order, at equal cost:
+    configuration 2 (steps as above) and configuration-5 instances.  The
+    scorer is trained on the device from logs of configurations 1-3 first
+    (outside the timed calls)."""
+    t = time.perf_counter()
+    rows = ltl.collect_logs(learner=L, per_block=256, config2_max_cost=8, config3_max_cost=6)
+    t_log = time.perf_counter() - t
+    t = time.perf_counter()
+    trained, loss, n_rows = ltl.train_scorer(rows, seed=2, epochs=20)
+    t_train = time.perf_counter() - t
+    untrained = ltl.Scorer(seed=5)
+
+    def best_of(inst, k, scorer, **opts):
+        L.set_scorer(scorer)
+        order = ltl.ORDER_PLAIN if scorer is None else ltl.ORDER_LEARNED
+        best = None
+        for _ in range(k):
+            r = L.learn(inst, order=order, **opts)
+            if best is None or r["seconds"] < best["seconds"]:
+                best = r
+        L.set_scorer(None)
+        return best
+
+    out = {"scorer": {"log_rows": int(n_rows), "log_s": round(t_log, 3), "train_s": round(t_train, 3),
+                      "final_loss": round(float(loss), 6), "trained_on": "configs 1-3 (seeded), plain order logs"}}
+    inst2 = ltl.generate(2, seed=1)
+    p = best_of(inst2, max(args.steps, 1), None, max_cost=args.max_cost)
+    q = best_of(inst2, max(args.steps, 1), trained, max_cost=args.max_cost)
+    assert q["status"] == p["status"] and q["cost"] == p["cost"]
+    out["config2"] = {"plain_s": round(p["seconds"], 4), "learned_s": round(q["seconds"], 4),
+                      "speedup": round(p["seconds"] / q["seconds"], 3), "cost": p["cost"],
+                      "plain_formula": p["formula"], "learned_formula": q["formula"]}
+    rows5, tot_p, tot_l, tot_u, faster, ranks = [], 0.0, 0.0, 0.0, 0, []
+    for idx in range(args.config5):
+        inst = ltl.generate(5, seed=1, index=idx)
+        p = best_of(inst, 2, None, max_cost=args.config5_cost)
+        q = best_of(inst, 2, trained, max_cost=args.config5_cost)
+        u = best_of(inst, 2, untrained, max_cost=args.config5_cost)
+        assert (q["status"], q["cost"]) == (p["status"], p["cost"]) == (u["status"], u["cost"])
+        tot_p += p["seconds"]
+        tot_l += q["seconds"]
+        tot_u += u["seconds"]
+        faster += q["seconds"] < p["seconds"]
+        rows5.append([idx, p["status"], p["cost"], round(p["seconds"] * 1e3, 2), round(q["seconds"] * 1e3, 2),
+                      round(u["seconds"] * 1e3, 2)])
+        if p["status"] == 1 and p["cost"] >= 4:
+            rk = ltl.first_hit_ranks(L, inst, {"trained": trained, "untrained": untrained}, args.config5_cost)
+            if rk:
+                ranks.append(rk)
+    out["config5"] = {"instances": args.config5, "max_cost": args.config5_cost,
+                      "plain_s": round(tot_p, 4), "learned_s": round(tot_l, 4), "learned_untrained_s": round(tot_u, 4),
+                      "learned_faster": faster,
+                      "per_instance_ms": "[index, status, cost, plain, learned, learned untrained]",
+                      "rows": rows5, "first_hit_rank": ranks}
+    trained.close()
+    untrained.close()
+    return out
+
+
+def batch_vs_loop(ltl, args):
+    """Configuration 4: 256 instances, one call per instance vs the batched
+    persistent launch with autotuned parameters."""
+    insts = [ltl.generate(4, seed=1, index=i) for i in range(256)]
+    mc = max(i.max_cost for i in insts)
+    L = ltl.Learner(budget=8 << 30)
+    try:
+        for i in insts[:8]:
+            L.learn(i, max_cost=mc)
+        t = time.perf_counter()
+        loop = [L.learn(i, max_cost=mc) for i in insts]
+        t_loop = time.perf_counter() - t
+        trials = L.autotune_batch(insts, max_cost=mc)
+        best = L.batch_params()
+        t = time.perf_counter()
+        res = L.learn_batch(insts, max_cost=mc)
+        t_batch = time.perf_counter() - t
+        st = L.batch_stats()
+    finally:
+        L.close()
+    same = sum(a["cost"] == b["cost"] and a["status"] == b["status"] for a, b in zip(loop, res))
+    return {"instances": len(insts), "max_cost": mc, "loop_s": round(t_loop, 4), "batched_s": round(t_batch, 4),
+            "loop_inst_per_s": round(len(insts) / t_loop, 1), "batched_inst_per_s": round(len(insts) / t_batch, 1),
+            "speedup": round(t_loop / t_batch, 2), "same_status_and_cost": same, "tuned": best,
+            "trials": len(trials), "on_device": st["on_device"], "deferred": st["deferred"], "host": st["host"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +180,9 @@ def main():
     ap.add_argument("--cpu-cost", type=int, default=8, help="levels timed for the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the learned-order and batch measurements")
+    ap.add_argument("--config5", type=int, default=16, help="configuration-5 instances compared")
+    ap.add_argument("--config5-cost", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run one 1 GiB device copy (PMC calibration)")
     args = ap.parse_args()
@@ -140,7 +230,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
         "config": {"workload": "config2: 500+500 traces, 5 atoms, lengths U{10..32}, per-atom rate "
                                "U[0.3,0.7], planted G(a->F b)&~(c U d), unit costs, max cost 14, "
-                               "2^28 dedup slots, seed 1",
+                               "2^28 dedup slots, seed 1, plain order",
                    "traces": inst.W, "atoms": int(inst.atoms.shape[0]), "max_cost": args.max_cost},
         "time_to_min_formula_s": round(wall / args.steps, 4),
         "min_cost": r0["cost"], "formula": r0["formula"], "uniques_per_step": r0["n_stored"],
@@ -150,6 +240,11 @@ def main():
         "roofline": roof,
     }
     L.close()
+    if not args.no_extra:
+        L = ltl.Learner(budget=int(args.budget_gb * 1e9), hash_slots_log2=28, max_cost=args.max_cost, neg_type=1)
+        out["learned_vs_plain"] = learned_vs_plain(ltl, L, args)
+        L.close()
+        out["config4_batch"] = batch_vs_loop(ltl, args)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         v, dt, u = cpu_baseline(inst, args.cpu_cost, threads)

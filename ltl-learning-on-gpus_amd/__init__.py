@@ -553,3 +553,28 @@ def train_scorer(rows=None, seed=1, epochs=20, batch=512, lr=1e-3, learner=None)
     sc = Scorer(seed=seed)
     loss = sc.train(rows, epochs=epochs, batch=batch, lr=lr, seed=seed)
     return sc, loss, len(rows)
+
+
+def first_hit_ranks(learner, inst, scorers, max_cost, max_rows=1 << 18, **opts):
+    """Ranking quality on the last level of a plain-order run: the level's
+    tile pairs (all of them, from the enumeration log) in canonical order and
+    in each scorer's descending order; returns, per order, the position of
+    the first tile pair that holds an accepted candidate (None if the run
+    found nothing in a binary block), and the number of tile pairs."""
+    learner.set_log(max_rows, max_rows)
+    try:
+        r = learner.learn(inst, max_cost=max_cost, order=ORDER_PLAIN, **opts)
+        rows = learner.get_log()
+    finally:
+        learner.set_log(0, 0)
+    if r["status"] != 1 or len(rows) == 0:
+        return None
+    last = rows[np.isclose(rows[:, 19] * 32, r["cost"])]
+    hit = last[:, 24] >= 1.0 - opts.get("hamming_k", 0) / inst.W - 1e-6
+    if not hit.any():
+        return None
+    out = {"tile_pairs": int(len(last)), "canonical": int(np.argmax(hit))}
+    for name, sc in scorers.items():
+        order = np.argsort(-sc.forward(last[:, :SCORER_FEATURES]), kind="stable")
+        out[name] = int(np.argmax(hit[order]))
+    return out

@@ -375,7 +375,7 @@ __global__ __launch_bounds__(256) void k_chk1(ChunkArgs A) {
     }
     const uint64_t id = A.id_base + (uint64_t)t.a;
     if (m <= k) atomicMin(&A.st->found, (unsigned long long)id);
-    else if (anti && seen - m <= k) atomicMin(&A.st->neg_found, (unsigned long long)id);
+    else if (anti && seen - m <= k) atomicMin(&A.st->neg_found[A.neg_slot], (unsigned long long)id);
 }
 
 // & and |: position 0 of the result is the AND / OR of the operands' b0.

@@ -44,8 +44,8 @@ struct DevState {
     unsigned long long dropped;   // survivors not stored (overflow)
     int collision;                // fingerprint collision detected
     int probe_fail;               // table full
-    unsigned long long neg_found; // smallest id of a candidate whose complement is accepted
-    unsigned long long pad;
+    unsigned long long neg_found[2];  // per level parity: smallest id of a candidate whose
+                                      // complement is accepted
 };
 
 // One chunk of one (operator, split) block, in canonical order.  Binary:
@@ -82,7 +82,7 @@ struct ChunkArgs {
     int32_t nsw;              // summary words per vector: ceil(W / 64)
     int32_t anti;             // check pass: also record candidates whose complement is accepted
     int32_t npw, nps;         // words (summary words) holding positive traces: ceil(P / TPW), ceil(P / 64)
-    int32_t pad_;
+    int32_t neg_slot;         // anti: neg_found entry written
 };
 
 // Check passes visit words alternately from the positive and the negative

@@ -45,6 +45,7 @@ struct Block {
     uint64_t id_base, size;
     int op;
     int64_t ls, nL, rs, nR;
+    int level;
 };
 
 struct PendingEvent {
@@ -465,6 +466,7 @@ struct Run {
     std::vector<Block> blocks;
     std::vector<LBlock> lblk;  // binary blocks of the current check pass
     int64_t n_ent = 0;         // their tile pairs
+    int cur_level = 0;         // level of the blocks being registered
 
     // Check-only pass over one block: no temporaries, so no chunking beyond
     // keeping the grid within limits.
@@ -547,7 +549,7 @@ struct Run {
     int32_t unary(int op, int64_t s, int64_t e) {
         const int64_t n = e - s;
         if (n <= 0 || (check_only && only_binary)) return 0;
-        blocks.push_back({all_ltls, (uint64_t)n, op, s, n, 0, 1});
+        blocks.push_back({all_ltls, (uint64_t)n, op, s, n, 0, 1, cur_level});
         const int64_t step = check_only ? ((int64_t)1 << 36) : c->CH;
         for (int64_t a0 = 0; a0 < n; a0 += step) {
             ChunkArgs a = base;
@@ -568,7 +570,7 @@ struct Run {
         if (nL <= 0 || nR <= 0 || (check_only && skip_binary)) return 0;
         const int nd = op == LTL_UNTIL ? 2 : 1;
         const uint64_t size = (uint64_t)nL * (uint64_t)nR * nd;
-        blocks.push_back({all_ltls, size, op, ls, nL, rs, nR});
+        blocks.push_back({all_ltls, size, op, ls, nL, rs, nR, cur_level});
         if (check_only && (ranked || logging)) {
             const int64_t nTR = (nR + 63) / 64;
             lblk.push_back({op, li, ri, 0, ls, nL, rs, nR, all_ltls, nTR, n_ent});
@@ -727,7 +729,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     std::vector<uint64_t> &atom_words = prep.atom_words, &sum0 = prep.atom_sums;
     DevState st0{};
     st0.found = kEmpty;
-    st0.neg_found = kEmpty;
+    st0.neg_found[0] = st0.neg_found[1] = kEmpty;
     st0.n_stored = A;
     st0.overflow_chunk = LLONG_MAX;
     hipStream_t s = c->stream;
@@ -785,6 +787,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     // (code/ltli6463.cu:1209-1440).
     auto level_blocks = [&](int cost) -> int32_t {
         int32_t r = 0;
+        run.cur_level = cost;
         if ((o.neg_type == 1 && cost - c2 >= c1) || (o.neg_type == 2 && cost - c2 == c1))
             if ((r = run.unary(LTL_NOT, Ls[cost - c2], Le[cost - c2]))) return r;
         for (int i = c1; 2 * i <= cost - c3; ++i)
@@ -824,24 +827,25 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             c->toff_cap = nlev;
         }
     }
-    // Binary blocks of level `cost` recorded in run.lblk; levels up to `upto`
-    // are complete in the store.
-    auto ranked_level = [&](int cost, int upto) -> int32_t {
-        // tile statistics of the levels completed since the last call
-        if (tstat_upto < upto) {
-            for (int lv = tstat_upto + 1; lv <= upto; ++lv) {
-                const int64_t n = Le[lv] - Ls[lv];
-                toff[lv] = tstat_used;
-                if (n > 0) {
-                    HIP_TRY(timed(c, K_TSTAT, (uint64_t)n * nsw * 16, (uint64_t)n * nsw * 16, (uint64_t)n, [&] {
-                        return launch_tile_stats(ba, Ls[lv], n, P, W, c->d_tstat + tstat_used * kTileStat, s);
-                    }));
-                }
-                tstat_used += (n + 63) / 64;
+    // Binary blocks of a level recorded in run.lblk; levels up to `upto` are
+    // complete in the store.
+    auto ensure_tstat = [&](int upto) -> int32_t {
+        if (tstat_upto >= upto) return 0;
+        for (int lv = tstat_upto + 1; lv <= upto; ++lv) {
+            const int64_t n = Le[lv] - Ls[lv];
+            toff[lv] = tstat_used;
+            if (n > 0) {
+                HIP_TRY(timed(c, K_TSTAT, (uint64_t)n * nsw * 16, (uint64_t)n * nsw * 16, (uint64_t)n, [&] {
+                    return launch_tile_stats(ba, Ls[lv], n, P, W, c->d_tstat + tstat_used * kTileStat, s);
+                }));
             }
-            tstat_upto = upto;
-            HIP_TRY(hipMemcpyAsync(c->d_toff, toff.data(), (size_t)nlev * sizeof(int64_t), hipMemcpyHostToDevice, s));
+            tstat_used += (n + 63) / 64;
         }
+        tstat_upto = upto;
+        HIP_TRY(hipMemcpyAsync(c->d_toff, toff.data(), (size_t)nlev * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        return 0;
+    };
+    auto upload_blocks = [&]() -> int32_t {
         const int nb = (int)run.lblk.size();
         if (nb > c->blk_cap) {
             HIP_TRY(hipStreamSynchronize(s));
@@ -852,19 +856,27 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             c->blk_cap = std::max(nb, 64);
         }
         HIP_TRY(hipMemcpyAsync(c->d_blk, run.lblk.data(), (size_t)nb * sizeof(LBlock), hipMemcpyHostToDevice, s));
-        const FeatCtx f{c->d_tstat, c->d_toff, (float)cost, (float)P / (float)W, W, P};
-        if (run.logging) {
-            for (int b = 0; b < nb && c->log_n < c->log_cap; ++b) {
-                const LBlock &K = run.lblk[b];
-                const int64_t E = (K.nL + 63) / 64 * K.nTR;
-                const int64_t rows = std::min<int64_t>({(int64_t)c->log_per_block, E, c->log_cap - c->log_n});
-                HIP_TRY(timed(c, K_LOG, 0, 0, (uint64_t)rows, [&] {
-                    return launch_log(B, ba, c->d_blk, b, f, rows, c->d_log + c->log_n * (kFeat + 1), s);
-                }));
-                c->log_n += rows;
-            }
+        return 0;
+    };
+    auto feat = [&](int cost) { return FeatCtx{c->d_tstat, c->d_toff, (float)cost, (float)P / (float)W, W, P}; };
+    auto log_level = [&](int cost, int upto) -> int32_t {
+        if (int32_t e = ensure_tstat(upto)) return e;
+        const FeatCtx f = feat(cost);
+        for (int b = 0; b < (int)run.lblk.size() && c->log_n < c->log_cap; ++b) {
+            const LBlock &K = run.lblk[b];
+            const int64_t E = (K.nL + 63) / 64 * K.nTR;
+            const int64_t rows = std::min<int64_t>({(int64_t)c->log_per_block, E, c->log_cap - c->log_n});
+            HIP_TRY(timed(c, K_LOG, 0, 0, (uint64_t)rows, [&] {
+                return launch_log(B, ba, c->d_blk, b, f, rows, c->d_log + c->log_n * (kFeat + 1), s);
+            }));
+            c->log_n += rows;
         }
-        if (!run.ranked) return 0;
+        return 0;
+    };
+    // Tile-pair order of the level in the second half of d_vals: by
+    // descending score, or canonical when one round covers the level.
+    constexpr int64_t kRankMin = 1024;
+    auto order_level = [&](int cost, int upto) -> int32_t {
         const int64_t E = run.n_ent;
         if (E > ((int64_t)1 << 31)) return LTL_E_LIMIT;
         if (E > c->ent_cap) {
@@ -882,28 +894,55 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         }
         float *k0 = c->d_keys, *k1 = c->d_keys + c->ent_cap;
         uint32_t *v0 = c->d_vals, *v1 = c->d_vals + c->ent_cap;
+        if (E <= kRankMin) {
+            HIP_TRY(launch_iota(v1, E, s));
+            return 0;
+        }
+        if (int32_t e = ensure_tstat(upto)) return e;
+        const FeatCtx f = feat(cost);
+        const int nb = (int)run.lblk.size();
         HIP_TRY(timed(c, K_SCORE, 0, 0, (uint64_t)E, [&] {
             return launch_score(c->scorer->m, c->d_blk, nb, f, E, k0, v0, s);
         }));
         HIP_TRY(timed(c, K_SORT, 0, 0, (uint64_t)E, [&] {
             return launch_sort_desc(k0, k1, v0, v1, E, c->d_sort, c->sort_cap, s);
         }));
-        // Rounds: the top 1/64 of the tile pairs (at least 256), then four
-        // times the previous round, until a round accepts a candidate.
-        int64_t r0 = 0, step = std::max<int64_t>(256, E / 64);
-        while (r0 < E) {
-            const int64_t n = std::min(step, E - r0);
-            HIP_TRY(timed(c, K_CHKR, 0, 0, (uint64_t)n * 4096, [&] {
-                return launch_check_list(B, ba, c->d_blk, nb, v1, r0, n, s);
-            }));
-            run.checked += (uint64_t)n * 4096;
-            r0 += n;
-            step *= 4;
+        return 0;
+    };
+    auto submit_round = [&](int64_t r0, int64_t n) -> int32_t {
+        if (n <= 0) return 0;
+        const int nb = (int)run.lblk.size();
+        const uint32_t *v1 = c->d_vals + c->ent_cap;
+        HIP_TRY(timed(c, K_CHKR, 0, 0, (uint64_t)n * 4096, [&] {
+            return launch_check_list(B, run.base, c->d_blk, nb, v1, r0, n, s);
+        }));
+        run.checked += (uint64_t)n * 4096;
+        return 0;
+    };
+    // First round of a ranked pass: the top 1/16 of the tile pairs (at
+    // least kRankMin), then the rest.
+    auto first_round = [&](int64_t E) { return std::min(E, std::max(kRankMin, E / 16)); };
+    // Log and / or test (ranked rounds, a sync after each) the recorded blocks.
+    auto ranked_level = [&](int cost, int upto) -> int32_t {
+        if (int32_t e = upload_blocks()) return e;
+        if (run.logging)
+            if (int32_t e = log_level(cost, upto)) return e;
+        if (!run.ranked) return 0;
+        if (int32_t e = order_level(cost, upto)) return e;
+        const int64_t E = run.n_ent, r1 = first_round(E);
+        for (const auto &rr : {std::make_pair((int64_t)0, r1), std::make_pair(r1, E)}) {
+            if (rr.second <= rr.first) continue;
+            if (int32_t e = submit_round(rr.first, rr.second - rr.first)) return e;
             HIP_TRY(hipMemcpyAsync(&h, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             if (h.found != kEmpty) break;
         }
         return 0;
+    };
+    auto level_of = [&](uint64_t id) -> int {
+        for (const auto &b : run.blocks)
+            if (id >= b.id_base && id < b.id_base + b.size) return b.level;
+        return -1;
     };
 
     int64_t n_stored = A;
@@ -919,6 +958,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     int looked_ahead = -1;  // level whose binary blocks were tested early
     int found_cost = -1;
     bool neg_answer = false;
+    uint64_t neg_id = kEmpty;
     for (cost = c1 + 1; cost <= maxc; ++cost) {
         if (run.otf) {
             const int dif = cost - shortage;  // code/ltli6463.cu:1202-1206
@@ -932,45 +972,87 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         // generation with dedup, without generating the level.
         bool generate = true;
         const bool la = learned && !run.otf;
-        if (o.precheck || run.otf || learned) {
+        if (la) {
+            // Level `cost`: unary blocks (binary ones too if they were not
+            // tested ahead), then the binary blocks of cost + 1, submitted
+            // together; one sync decides.
+            const int slot = cost & 1;
+            const bool ahead = cost + 1 <= maxc;
+            run.check_only = true;
+            run.skip_binary = looked_ahead == cost;
+            run.base.anti = anti_ok;
+            run.base.neg_slot = slot;
+            run.lblk.clear();
+            run.n_ent = 0;
+            rc = level_blocks(cost);
+            if (!rc && !run.lblk.empty()) rc = ranked_level(cost, cost - 1);
+            run.skip_binary = false;
+            if (!rc && h.found != kEmpty && !run.lblk.empty()) {  // found by the rounds above
+                found = true;
+                found_cost = cost;
+                Le[cost] = n_stored;
+                break;
+            }
+            int64_t E = 0, r1 = 0;
+            if (!rc && ahead) {
+                if (launch_fill_u64(reinterpret_cast<uint64_t *>(&c->d_state->neg_found[1 - slot]), kEmpty, 1, s) !=
+                    hipSuccess)
+                    rc = LTL_E_HIP;
+                run.only_binary = true;
+                run.base.neg_slot = 1 - slot;
+                run.lblk.clear();
+                run.n_ent = 0;
+                if (!rc) rc = level_blocks(cost + 1);
+                run.only_binary = false;
+                E = run.n_ent;
+                if (!rc && E > 0) {
+                    rc = upload_blocks();
+                    if (!rc && run.logging) rc = log_level(cost + 1, cost - 1);
+                    if (!rc) rc = order_level(cost + 1, cost - 1);
+                    r1 = first_round(E);
+                    if (!rc) rc = submit_round(0, r1);
+                }
+            }
+            if (!rc) rc = sync_state();
+            if (rc) break;
+            if (h.found != kEmpty) {
+                found = true;
+                found_cost = level_of(h.found);
+                Le[cost] = n_stored;
+                break;
+            }
+            if (anti_ok && ahead && h.neg_found[slot] != kEmpty) {
+                neg_answer = true;
+                neg_id = h.neg_found[slot];
+                found_cost = cost + 1;
+                Le[cost] = n_stored;
+                break;
+            }
+            if (r1 < E) {
+                if ((rc = submit_round(r1, E - r1)) || (rc = sync_state())) break;
+                if (h.found != kEmpty) {
+                    found = true;
+                    found_cost = cost + 1;
+                    Le[cost] = n_stored;
+                    break;
+                }
+            }
+            run.check_only = false;
+            run.base.anti = 0;
+            if (ahead) looked_ahead = cost + 1;
+            else generate = false;  // the last level is tested completely; its sets would serve no later level
+        } else if (o.precheck || run.otf || learned) {
             const uint64_t save = run.all_ltls;
             const size_t nb = run.blocks.size();
             run.check_only = true;
-            run.skip_binary = la && looked_ahead == cost;
-            run.base.anti = la && anti_ok;
             run.lblk.clear();
             run.n_ent = 0;
             rc = level_blocks(cost);
             if (!rc && !run.lblk.empty()) rc = ranked_level(cost, cost - 1);
             if (!rc) rc = sync_state();
-            run.check_only = run.skip_binary = false;
-            run.base.anti = 0;
+            run.check_only = false;
             if (rc) break;
             if (h.found != kEmpty) { found = true; found_cost = cost; Le[cost] = n_stored; break; }
-            if (la && anti_ok && h.neg_found != kEmpty && cost + 1 <= maxc) {
-                neg_answer = true;
-                found_cost = cost + 1;
-                Le[cost] = n_stored;
-                break;
-            }
-            if (la && cost + 1 <= maxc) {
-                if (launch_fill_u64(reinterpret_cast<uint64_t *>(&c->d_state->neg_found), kEmpty, 1, s) != hipSuccess) {
-                    rc = LTL_E_HIP;
-                    break;
-                }
-                run.check_only = run.only_binary = true;
-                run.base.anti = anti_ok;
-                run.lblk.clear();
-                run.n_ent = 0;
-                rc = level_blocks(cost + 1);
-                if (!rc && !run.lblk.empty()) rc = ranked_level(cost + 1, cost - 1);
-                if (!rc) rc = sync_state();
-                run.check_only = run.only_binary = false;
-                run.base.anti = 0;
-                if (rc) break;
-                looked_ahead = cost + 1;
-                if (h.found != kEmpty) { found = true; found_cost = cost + 1; Le[cost] = n_stored; break; }
-            }
             if (run.otf) generate = false;  // check-only level (overflow)
             else if (!learned) { run.all_ltls = save; run.blocks.resize(nb); }
         }
@@ -1002,7 +1084,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
 
     if (found || neg_answer) {
         // Candidate id -> block -> operator and operand store indices.
-        const uint64_t id = neg_answer ? h.neg_found : h.found;
+        const uint64_t id = neg_answer ? neg_id : h.found;
         const Block *blk = nullptr;
         for (const auto &b : run.blocks)
             if (id >= b.id_base && id < b.id_base + b.size) { blk = &b; break; }

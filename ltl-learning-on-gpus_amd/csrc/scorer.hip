@@ -283,6 +283,11 @@ __global__ void k_refresh(MlpDev M) {
     if (i < P_N) store_bf16(M, i, M.p[i]);
 }
 
+__global__ void k_iota(uint32_t *p, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = (uint32_t)i;
+}
+
 __global__ void k_zero(float *p, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0.f;
@@ -486,8 +491,9 @@ __global__ __launch_bounds__(256) void k_chk_list(ChunkArgs A, const LBlock *blk
     const uint64_t id = K.id_base + (K.op == LTL_UNTIL ? 2 * pair : pair);
     if (m0 <= k) atomicMin(&A.st->found, (unsigned long long)id);
     else if (K.op == LTL_UNTIL && m1 <= k) atomicMin(&A.st->found, (unsigned long long)(id + 1));
-    else if (anti && seen - m0 <= k) atomicMin(&A.st->neg_found, (unsigned long long)id);
-    else if (anti && K.op == LTL_UNTIL && seen - m1 <= k) atomicMin(&A.st->neg_found, (unsigned long long)(id + 1));
+    else if (anti && seen - m0 <= k) atomicMin(&A.st->neg_found[A.neg_slot], (unsigned long long)id);
+    else if (anti && K.op == LTL_UNTIL && seen - m1 <= k)
+        atomicMin(&A.st->neg_found[A.neg_slot], (unsigned long long)(id + 1));
 }
 
 // One workgroup per log row: the fewest mismatches over the tile pair's
@@ -583,6 +589,13 @@ hipError_t launch_log(int B, const ChunkArgs &a, const LBlock *blk, int b, const
                       float *rows, hipStream_t s) {
     if (n_rows <= 0) return hipSuccess;
     LTL_SCORER_B(k_log, B, dim3((unsigned)n_rows), s, a, blk, b, f, n_rows, rows);
+    return hipGetLastError();
+}
+
+hipError_t launch_iota(uint32_t *p, int64_t n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_iota, dim3(blocks), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

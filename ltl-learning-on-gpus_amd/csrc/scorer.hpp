@@ -93,6 +93,7 @@ hipError_t launch_log(int B, const ChunkArgs &a, const LBlock *blk, int b, const
 
 // Training and inference on feature rows (x: stride xs floats, t: stride ts).
 hipError_t launch_zero(float *p, int64_t n, hipStream_t s);
+hipError_t launch_iota(uint32_t *p, int64_t n, hipStream_t s);
 hipError_t launch_grad(const MlpDev &m, const float *x, int xs, const float *t, int ts, int64_t n, float inv_n,
                        hipStream_t s);
 hipError_t launch_adam(const MlpDev &m, float lr, float b1, float b2, float eps, int step, hipStream_t s);
