@@ -103,7 +103,7 @@ __device__ __forceinline__ void flush_summary(uint64_t *z, int w, int nw, int ns
     const int j = w / B;
     if (on) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k) z[(size_t)(k * nsw + j) * ws] = v[k];
+        for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(v[k], z + (size_t)(k * nsw + j) * ws);
     }
     v[0] = v[1] = v[2] = v[3] = 0;
 }

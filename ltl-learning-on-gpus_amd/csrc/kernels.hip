@@ -327,7 +327,7 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A) {
         else if constexpr (OP == LTL_OR) c = l | r;
         else if constexpr (OP == LTL_UNTIL) c = swap ? f_until<B>(r, l) : f_until<B>(l, r);
         else c = apply1<OP, B>(l, lm);
-        q[(size_t)w * A.ws] = c;
+        __builtin_nontemporal_store(c, q + (size_t)w * A.ws);  // streamed: not re-read in this chunk
         summarize<B>(c, lm, (w % B) * TPW, v);
         flush_summary<B>(z, w, A.nw, A.nsw, A.ws, v, true);
     };
