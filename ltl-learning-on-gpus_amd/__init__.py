@@ -24,6 +24,7 @@ FOUND, NOT_FOUND = 1, 0
 E_ARG, E_LIMIT, E_OOM, E_HIP, E_COLLISION, E_NODEVICE = -1, -2, -3, -4, -5, -6
 ATOM, NOT, AND, OR, NEXT, FIN, GLOB, UNTIL = range(8)
 ORDER_PLAIN, ORDER_LEARNED = 0, 1
+DEDUP_EXACT, DEDUP_REF1, DEDUP_REF2, DEDUP_REF3 = 0, 1, 2, 3
 
 
 class LtlError(RuntimeError):
@@ -250,12 +251,13 @@ def mismatches(inst, cs):
 
 
 def make_options(cost=(1,) * 8, max_cost=10, neg_type=1, hamming_k=0, order=ORDER_PLAIN,
-                 hash_slots_log2=0, precheck=1, capacity=0, budget=0, device=0, profile=0):
+                 hash_slots_log2=0, precheck=1, capacity=0, budget=0, device=0, profile=0, dedup_mode=0):
     o = Options()
     lib().ltl_default_options(ctypes.byref(o))
     for i, c in enumerate(cost):
         o.cost[i] = int(c)
     o.max_cost, o.neg_type, o.hamming_k, o.order = int(max_cost), int(neg_type), int(hamming_k), int(order)
+    o.dedup_mode = int(dedup_mode)
     o.hash_slots_log2, o.precheck, o.capacity = int(hash_slots_log2), int(precheck), int(capacity)
     o.hbm_budget_bytes, o.device, o.profile = int(budget), int(device), int(profile)
     return o
@@ -496,23 +498,26 @@ def _instance_from_tokens(pos, neg, alphabet):
     return Instance(len(pos), len(neg), lens, atoms, names)
 
 
-def LTLI(costFun, maxCost, RlxUnqChkTyp, NegType, alphabet, pos, neg, learner=None):
+def LTLI(costFun, maxCost, RlxUnqChkTyp, NegType, alphabet, pos, neg, learner=None, reference_keys=False):
     """Drop-in for the reference's LTLI() (code/ltli6463.cu:1043-1052).
 
     pos/neg: lists of traces, a trace a list of tokens, a token a string of
     the letters holding at that position.  Returns (formula, LTLcost,
     allLTLs) where formula is "not_found" or "see_the_error" on failure, as
-    in the reference.  RlxUnqChkTyp is accepted for signature parity; the
-    device path deduplicates exactly (the reference's relaxed keys only
-    matter above 126 total trace positions, code/ltli6463.cu:66-121).
+    in the reference.  Dedup is exact by default; reference_keys=True uses
+    the reference's relaxed uniqueness key RlxUnqChkTyp (1-3), which is lossy
+    above 126 total trace positions (code/ltli6463.cu:57-141).
     """
     inst = _instance_from_tokens(pos, neg, alphabet)
     if inst.W == 0 or int(inst.lens.max(initial=0)) > 64:
         return "see_the_error", int(costFun[0]), 0
+    if reference_keys and int(RlxUnqChkTyp) not in (1, 2, 3):
+        return "see_the_error", int(costFun[0]), 0
     own = learner is None
     L = learner or Learner(budget=1 << 30)
     try:
-        r = L.learn(inst, cost=tuple(costFun), max_cost=maxCost, neg_type=NegType)
+        r = L.learn(inst, cost=tuple(costFun), max_cost=maxCost, neg_type=NegType,
+                    dedup_mode=int(RlxUnqChkTyp) if reference_keys else DEDUP_EXACT)
     except LtlError:
         return "see_the_error", int(costFun[0]), 0
     finally:

@@ -76,6 +76,75 @@ __device__ __forceinline__ void fold(Acc &a, uint64_t c, const WordAux *x) {
     a.mism += (uint32_t)__popcll((c ^ x->tgt) & x->chk);
 }
 
+// The reference's relaxed uniqueness keys (code/ltli6463.cu:57-141), used
+// as dedup keys in modes 1-3 when the traces hold more than 126 positions:
+// key bit j of the reference is bit j of l (j < 63) or bit j - 63 of h.
+// Mode 1 takes every stride-th position of each trace, mode 2 the first
+// (126 len_i + lenSum) / lenSum positions, both until 126 bits are taken;
+// mode 3 XORs a 64-bit mixer of each trace into h (first half of the
+// traces) or l.  Folded word by word like the fingerprints.
+struct KeyAcc {
+    uint64_t h = 0, l = 0;
+};
+
+__device__ __forceinline__ void key_put(KeyAcc &k, uint64_t v, int j0, int cnt) {
+    constexpr uint64_t M63 = (1ull << 63) - 1ull;
+    if (j0 < 63) {
+        k.l |= (v << j0) & M63;
+        if (j0 + cnt > 63) k.h |= v >> (63 - j0);
+    } else {
+        k.h |= v << (j0 - 63);
+    }
+}
+
+template <int B>
+__device__ __forceinline__ void key_fold(KeyAcc &k, uint64_t c, int w, const ChunkArgs &A) {
+    constexpr int TPW = 64 / B;
+    constexpr uint64_t SM = B == 64 ? ~0ull : ((1ull << B) - 1ull);
+    if (A.kmode == 3) {
+#pragma unroll
+        for (int t = 0; t < TPW; ++t) {
+            const int i = w * TPW + t;
+            if (i >= A.W) break;
+            uint64_t x = (c >> (t * B)) & SM;
+            x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+            x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+            x ^= x >> 31;
+            if (i < A.W / 2) k.h ^= x; else k.l ^= x;
+        }
+        return;
+    }
+    if (w >= A.kw_end) return;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int i = w * TPW + t;
+        if (i >= A.W) break;
+        const uint32_t info = A.kinfo[i];
+        const int cnt = (int)(info & 0xffu), j0 = (int)(info >> 8);
+        if (cnt == 0) continue;
+        const uint64_t x = (c >> (t * B)) & SM;
+        if (A.kmode == 1) {
+            for (int m = 0; m < cnt; ++m)
+                if ((x >> (m * A.kstride)) & 1ull) key_put(k, 1ull, j0 + m, 1);
+        } else {
+            key_put(k, cnt >= 64 ? x : (x & ((1ull << cnt) - 1ull)), j0, cnt);
+        }
+    }
+}
+
+// Fingerprints of a relaxed key: equal keys give equal fingerprints; two
+// different keys share fingerprint 1 with probability 2^-64 (detected as a
+// collision through fingerprint 2).
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+__device__ __forceinline__ void key_finish(Acc &a, const KeyAcc &k) {
+    a.f1 = mix64(k.h ^ mix64(k.l + 0x9e3779b97f4a7c15ull));
+    a.f2 = mix64(k.l ^ mix64(k.h + 0xd1b54a32d192ed03ull));
+}
+
 // Per-trace summaries of one packed word, OR-ed into the summary words at
 // bit offset sh: position 0 (b0), position 1 (b1), any position (nz) and
 // every position inside the trace (fl).  They decide the target check of

@@ -73,7 +73,7 @@ __device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t) {
 // ---------------------------------------------------------------------------
 // gen
 // ---------------------------------------------------------------------------
-template <int OP, int B>
+template <int OP, int B, bool RX>
 __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
@@ -83,13 +83,21 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     const uint64_t *pl = A.store + (size_t)sl;
     const uint64_t *pr = A.store + (size_t)sr;
     Acc c0, c1;
+    KeyAcc k0, k1;
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = A.aux + w;
-        if constexpr (OP == LTL_AND) fold(c0, l & r, x);
-        else if constexpr (OP == LTL_OR) fold(c0, l | r, x);
-        else {
-            fold(c0, f_until<B>(l, r), x);
-            fold(c1, f_until<B>(r, l), x);
+        if constexpr (OP == LTL_UNTIL) {
+            const uint64_t u0 = f_until<B>(l, r), u1 = f_until<B>(r, l);
+            fold(c0, u0, x);
+            fold(c1, u1, x);
+            if constexpr (RX) {
+                key_fold<B>(k0, u0, w, A);
+                key_fold<B>(k1, u1, w, A);
+            }
+        } else {
+            const uint64_t v = OP == LTL_AND ? (l & r) : (l | r);
+            fold(c0, v, x);
+            if constexpr (RX) key_fold<B>(k0, v, w, A);
         }
     };
     // Batches of UB words: all loads of a batch are issued before use, so
@@ -105,6 +113,10 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     }
     for (; w < A.nw; ++w) step(w, pl[(size_t)w * A.ws], pr[(size_t)w * A.ws]);
     if (!t.valid) return;
+    if constexpr (RX) {
+        key_finish(c0, k0);
+        if constexpr (OP == LTL_UNTIL) key_finish(c1, k1);
+    }
     const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
     const uint64_t id0 = A.id_base + (OP == LTL_UNTIL ? 2 * pair : pair);
     if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
@@ -122,7 +134,7 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     }
 }
 
-template <int OP, int B>
+template <int OP, int B, bool RX>
 __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
@@ -131,9 +143,12 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     const int64_t s = A.ls + (t.valid ? t.a : A.a0);
     const uint64_t *p = A.store + (size_t)s;
     Acc c0;
+    KeyAcc k0;
     auto step = [&](int w, uint64_t v) {
         const WordAux *x = A.aux + w;
-        fold(c0, apply1<OP, B>(v, x->lm), x);
+        const uint64_t y = apply1<OP, B>(v, x->lm);
+        fold(c0, y, x);
+        if constexpr (RX) key_fold<B>(k0, y, w, A);
     };
     constexpr int UB = 8;
     int w = 0;
@@ -146,6 +161,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     }
     for (; w < A.nw; ++w) step(w, p[(size_t)w * A.ws]);
     if (!t.valid) return;
+    if constexpr (RX) key_finish(c0, k0);
     const uint64_t id0 = A.id_base + (uint64_t)t.a;
     if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
     if (otf) return;
@@ -158,15 +174,30 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
 // the first of equal atoms wins (code/ltli6463.cu:144-173).
 // One wave per atom, lanes over the words; the NH sums (mod 2^64) are
 // reduced across the wave.
-__global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms) {
+__global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms, int B) {
     const int a = (int)blockIdx.x, l = (int)threadIdx.x;
     const uint64_t *p = A.store + (size_t)a;
     Acc c0;
-    for (int w = l; w < A.nw; w += 64) fold(c0, p[(size_t)w * A.ws], A.aux + w);
+    KeyAcc k0;
+    for (int w = l; w < A.nw; w += 64) {
+        const uint64_t v = p[(size_t)w * A.ws];
+        fold(c0, v, A.aux + w);
+        if (A.kmode) {
+            switch (B) {
+            case 8: key_fold<8>(k0, v, w, A); break;
+            case 16: key_fold<16>(k0, v, w, A); break;
+            case 32: key_fold<32>(k0, v, w, A); break;
+            default: key_fold<64>(k0, v, w, A); break;
+            }
+        }
+    }
     for (int o = 1; o < 64; o <<= 1) {
         c0.f1 += __shfl_xor(c0.f1, o);
         c0.f2 += __shfl_xor(c0.f2, o);
+        k0.h ^= __shfl_xor(k0.h, o);  // contributions of different traces do not overlap (modes 1, 2)
+        k0.l ^= __shfl_xor(k0.l, o);
     }
+    if (A.kmode) key_finish(c0, k0);
     if (l != 0) return;
     tab_insert(A.tab, A.tmask, table_key(c0.f1), (uint64_t)a, &A.st->probe_fail);
     A.m_fp2[a] = c0.f2;
@@ -456,6 +487,14 @@ static dim3 grid_binary(const ChunkArgs &a) {
 }
 static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles + 3) / 4)); }
 
+#define LTL_DISPATCH_BX(KER, OP, B, RX, GRID, BLOCK, S, ...)                                      \
+    switch (B) {                                                                                    \
+    case 8: hipLaunchKernelGGL((KER<OP, 8, RX>), GRID, BLOCK, 0, S, __VA_ARGS__); break;            \
+    case 16: hipLaunchKernelGGL((KER<OP, 16, RX>), GRID, BLOCK, 0, S, __VA_ARGS__); break;          \
+    case 32: hipLaunchKernelGGL((KER<OP, 32, RX>), GRID, BLOCK, 0, S, __VA_ARGS__); break;          \
+    default: hipLaunchKernelGGL((KER<OP, 64, RX>), GRID, BLOCK, 0, S, __VA_ARGS__); break;          \
+    }
+
 #define LTL_DISPATCH_B(KER, OP, B, GRID, BLOCK, S, ...)                                    \
     switch (B) {                                                                            \
     case 8: hipLaunchKernelGGL((KER<OP, 8>), GRID, BLOCK, 0, S, __VA_ARGS__); break;        \
@@ -464,18 +503,23 @@ static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles +
     default: hipLaunchKernelGGL((KER<OP, 64>), GRID, BLOCK, 0, S, __VA_ARGS__); break;      \
     }
 
-hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s) {
+template <bool RX>
+static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s) {
     switch (op) {
-    case LTL_NOT: LTL_DISPATCH_B(k_gen1, LTL_NOT, B, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_NEXT: LTL_DISPATCH_B(k_gen1, LTL_NEXT, B, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_FIN: LTL_DISPATCH_B(k_gen1, LTL_FIN, B, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_GLOB: LTL_DISPATCH_B(k_gen1, LTL_GLOB, B, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_AND: LTL_DISPATCH_B(k_gen2, LTL_AND, B, grid_binary(a), dim3(256), s, a); break;
-    case LTL_OR: LTL_DISPATCH_B(k_gen2, LTL_OR, B, grid_binary(a), dim3(256), s, a); break;
-    case LTL_UNTIL: LTL_DISPATCH_B(k_gen2, LTL_UNTIL, B, grid_binary(a), dim3(256), s, a); break;
+    case LTL_NOT: LTL_DISPATCH_BX(k_gen1, LTL_NOT, B, RX, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_NEXT: LTL_DISPATCH_BX(k_gen1, LTL_NEXT, B, RX, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_FIN: LTL_DISPATCH_BX(k_gen1, LTL_FIN, B, RX, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_GLOB: LTL_DISPATCH_BX(k_gen1, LTL_GLOB, B, RX, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_AND: LTL_DISPATCH_BX(k_gen2, LTL_AND, B, RX, grid_binary(a), dim3(256), s, a); break;
+    case LTL_OR: LTL_DISPATCH_BX(k_gen2, LTL_OR, B, RX, grid_binary(a), dim3(256), s, a); break;
+    case LTL_UNTIL: LTL_DISPATCH_BX(k_gen2, LTL_UNTIL, B, RX, grid_binary(a), dim3(256), s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s) {
+    return a.kmode ? launch_gen_x<true>(op, B, a, s) : launch_gen_x<false>(op, B, a, s);
 }
 
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
@@ -530,9 +574,8 @@ hipError_t launch_finish(const ChunkArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s) {
-    (void)B;
     if (n_atoms <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_atoms, dim3((unsigned)n_atoms), dim3(64), 0, s, a, n_atoms);
+    hipLaunchKernelGGL(k_atoms, dim3((unsigned)n_atoms), dim3(64), 0, s, a, n_atoms, B);
     return hipGetLastError();
 }
 

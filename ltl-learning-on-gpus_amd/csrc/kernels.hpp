@@ -83,6 +83,12 @@ struct ChunkArgs {
     int32_t anti;             // check pass: also record candidates whose complement is accepted
     int32_t npw, nps;         // words (summary words) holding positive traces: ceil(P / TPW), ceil(P / 64)
     int32_t neg_slot;         // anti: neg_found entry written
+    // Reference relaxed uniqueness keys (dedup modes 1-3 above 126 trace
+    // positions): kmode 0 = exact (NH fingerprints), else the mode; per
+    // trace (first key bit << 8) | key bits taken (modes 1, 2); the stride
+    // of mode 1; words holding traces that take key bits.
+    const uint32_t *kinfo;
+    int32_t kmode, kstride, kw_end, W;
 };
 
 // Check passes visit words alternately from the positive and the negative

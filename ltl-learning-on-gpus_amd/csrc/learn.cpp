@@ -96,6 +96,7 @@ struct ltl_ctx {
     uint32_t *t_slot = nullptr, *t_cnt = nullptr, *t_off = nullptr, *t_list = nullptr;
     uint64_t *t_fp2 = nullptr, *t_mask = nullptr;
     uint64_t *d_stage = nullptr;  // atom words and summaries, contiguous, before the scatter
+    uint32_t *d_kinfo = nullptr;  // per trace: relaxed-key bit range (dedup modes 1-3)
     void *d_scan = nullptr;
     size_t scan_bytes = 0;
     std::vector<int64_t> lvl_s, lvl_e;
@@ -151,7 +152,7 @@ static int32_t check_options(const ltl_options *o) {
         if (o->cost[i] < 1 || o->cost[i] > SHRT_MAX) return LTL_E_ARG;
     if (o->max_cost < 1 || o->max_cost > SHRT_MAX) return LTL_E_ARG;
     if (o->neg_type < 1 || o->neg_type > 2) return LTL_E_ARG;
-    if (o->dedup_mode != LTL_DEDUP_EXACT) return LTL_E_ARG;  // reference key modes: not on the device path
+    if (o->dedup_mode > LTL_DEDUP_REF3) return LTL_E_ARG;
     if (o->hamming_k < 0) return LTL_E_ARG;
     if (o->hash_slots_log2 < 0 || o->hash_slots_log2 > 31) return LTL_E_ARG;
     if (o->order != LTL_ORDER_PLAIN && o->order != LTL_ORDER_LEARNED) return LTL_E_ARG;
@@ -247,7 +248,8 @@ static int32_t carve(ltl_ctx *c) {
                    align_up((size_t)c->nsw * sizeof(SumAux), 256) +
                    2 * align_up((size_t)ch * 4, 256) + align_up((size_t)ch * 8, 256) + align_up(tiles * 16, 256) +
                    2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) +
-                   align_up((size_t)(c->nw + 4 * c->nsw) * c->A * 8, 256) + 9 * 256;
+                   align_up((size_t)(c->nw + 4 * c->nsw) * c->A * 8, 256) + align_up((size_t)c->W * 4, 256) +
+                   10 * 256;
     if (fixed >= budget) return LTL_E_OOM;
     const size_t R = budget - fixed;
     // Table: load stays <= 1/2 with a full chunk pending (cap + ch <= slots/2).
@@ -288,6 +290,7 @@ static int32_t carve(ltl_ctx *c) {
     c->t_off = (uint32_t *)take(tiles * 4);
     c->d_scan = take(c->scan_bytes);
     c->d_stage = (uint64_t *)take((size_t)(c->nw + 4 * c->nsw) * c->A * 8);
+    c->d_kinfo = (uint32_t *)take((size_t)c->W * 4);
     c->d_tab = (uint64_t *)take(slots * 16);
     const int64_t capr = align_up((size_t)cap, 64);
     c->ws = capr;
@@ -737,7 +740,9 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     HIP_TRY(hipMemcpyAsync(c->d_saux, saux.data(), saux.size() * sizeof(SumAux), hipMemcpyHostToDevice, s));
     // Atom words and summaries: one contiguous copy, then scattered into the
     // word-major store (a strided copy from pageable memory goes row by row).
-    // (`stage` outlives the copy: the first level ends with a stream sync.)
+    // (`stage` and `kinfo` outlive their copies: the first level ends with a
+    // stream sync.)
+    std::vector<uint32_t> kinfo;
     std::vector<uint64_t> stage(atom_words);
     stage.insert(stage.end(), sum0.begin(), sum0.end());
     HIP_TRY(hipMemcpyAsync(c->d_stage, stage.data(), stage.size() * 8, hipMemcpyHostToDevice, s));
@@ -755,6 +760,36 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     ba.sum = c->d_sum; ba.saux = c->d_saux; ba.nsw = nsw;
     ba.npw = std::min(c->nw, (P + c->TPW - 1) / c->TPW);
     ba.nps = std::min(nsw, (P + 63) / 64);
+    // Relaxed keys (code/ltli6463.cu:57-141) only above 126 positions; below,
+    // every mode is the exact packing, which the exact dedup reproduces.
+    ba.W = W;
+    ba.kinfo = c->d_kinfo;
+    ba.kmode = 0;
+    {
+        int len_sum = 0;
+        for (int i = 0; i < W; ++i) len_sum += pb->lens[i];
+        if (o.dedup_mode != LTL_DEDUP_EXACT && len_sum > 126) {
+            kinfo.assign(W, 0);
+            const int stride = len_sum / 126;
+            int j = 0, last = -1;
+            for (int i = 0; i < W && j < 126; ++i) {
+                const int len = pb->lens[i];
+                int take = 0;
+                if (o.dedup_mode == LTL_DEDUP_REF1) take = (len + stride - 1) / stride;
+                else if (o.dedup_mode == LTL_DEDUP_REF2) take = (126 * len + len_sum) / len_sum;
+                const int cnt = std::min(take, 126 - j);
+                if (cnt > 0) {
+                    kinfo[i] = ((uint32_t)j << 8) | (uint32_t)cnt;
+                    last = i;
+                }
+                j += cnt;
+            }
+            ba.kmode = o.dedup_mode;
+            ba.kstride = stride;
+            ba.kw_end = last < 0 ? 0 : last / c->TPW + 1;
+            HIP_TRY(hipMemcpyAsync(c->d_kinfo, kinfo.data(), (size_t)W * 4, hipMemcpyHostToDevice, s));
+        }
+    }
     ba.capacity = c->capacity;
     ba.ws = c->ws;
     ba.nw = c->nw;

@@ -21,7 +21,8 @@ def to_oracle(O, inst):
 
 def check_parity(ltl, O, learner, inst, **opts):
     gpu = learner.learn(inst, **opts)
-    okw = {k: v for k, v in opts.items() if k in ("cost", "max_cost", "neg_type", "hamming_k", "capacity")}
+    okw = {k: v for k, v in opts.items() if k in ("cost", "max_cost", "neg_type", "hamming_k", "capacity",
+                                                  "dedup_mode")}
     ref = O.enumerate_(to_oracle(O, inst), keep_store=True, **okw)
     ctx = f"{inst.num_pos}+{inst.num_neg} opts={opts}"
     assert gpu["status"] == ref["status"], ctx
@@ -132,3 +133,49 @@ def test_until_answers(ltl, oracle, learner, planted, lens, precheck):
     inst = ltl.generate_custom(23 + lens[0], 40, 40, 3, lens[0], lens[1], 0.5, planted=planted)
     gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=6, precheck=precheck)
     assert gpu["status"] == 1
+
+
+# The reference's relaxed uniqueness keys (code/ltli6463.cu:57-141) above 126
+# trace positions: lossy dedup, reproduced set for set.  Shapes: (P, N, atoms,
+# min length, max length) with 30 * 5 .. 73 * 40 positions; one instance whose
+# traces hold <= 126 positions (every mode is then exact).
+RELAXED = [(15, 15, 3, 4, 8), (20, 13, 3, 9, 16), (33, 40, 3, 17, 32), (12, 12, 2, 40, 64), (8, 7, 3, 5, 8)]
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("shape", RELAXED, ids=lambda s: "x".join(map(str, s)))
+@pytest.mark.parametrize("planted", [9, 5])
+def test_relaxed_dedup_modes(ltl, oracle, learner, mode, shape, planted):
+    P, N, A, lo, hi = shape
+    inst = ltl.generate_custom(7 * P + N + mode, P, N, A, lo, hi, 0.5, planted_size=planted)
+    for precheck in (0, 1):
+        check_parity(ltl, oracle, learner, inst, max_cost=5 if planted == 9 else 7, dedup_mode=mode,
+                     precheck=precheck)
+
+
+def _mirror_instance(ltl, seed, pairs=6, lo=6, hi=11):
+    """Every trace has its a<->b swapped twin in the same half (positives =
+    first half), so the sets of phi and of phi with a, b exchanged are
+    permutations of each other within each half: mode 3 merges them, exact
+    dedup does not."""
+    rng = np.random.default_rng(seed)
+    cols = []
+    for _ in range(2):  # positives, negatives
+        for _ in range(pairs):
+            n = int(rng.integers(lo, hi + 1))
+            a, b, c = (int(rng.integers(0, 1 << n)) for _ in range(3))
+            cols += [(n, a, b, c), (n, b, a, c)]
+    W = len(cols)
+    lens = np.array([t[0] for t in cols], dtype=np.uint8)
+    atoms = np.array([[t[k] for t in cols] for k in (1, 2, 3)], dtype=np.uint64)
+    return ltl.Instance(2 * pairs, 2 * pairs, lens, atoms, "abc")
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_relaxed_mode3_merges_mirrored_sets(ltl, oracle, learner, seed):
+    inst = _mirror_instance(ltl, seed)
+    assert int(inst.lens.sum()) > 126
+    gpu, ref = check_parity(ltl, oracle, learner, inst, max_cost=5, dedup_mode=3)
+    exact = oracle.enumerate_(to_oracle(oracle, inst), max_cost=5)
+    if gpu["status"] == 0 and exact["status"] == 0:
+        assert gpu["n_stored"] < exact["n_stored"]  # the mode really merged sets
