@@ -180,6 +180,9 @@ int32_t ltl_autotune_batch(ltl_ctx *ctx, const ltl_problem *pbs, int32_t n, ltl_
 /* Free what a result owns (formula, nodes). */
 void ltl_result_free(ltl_result *res);
 
+/* Copy the context's current options. */
+int32_t ltl_get_options(const ltl_ctx *ctx, ltl_options *opt);
+
 /* Release the context and its device memory. */
 void ltl_destroy(ltl_ctx *ctx);
 
@@ -207,6 +210,20 @@ int32_t ltl_store_read(ltl_ctx *ctx, int64_t first, int64_t count, uint64_t *cs,
 
 /* Level boundaries of the last call: start[c], end[c] for c < max. */
 int32_t ltl_levels(const ltl_ctx *ctx, int64_t *start, int64_t *end, int32_t max);
+
+/* ---- Divide and conquer --------------------------------------------------
+ * Replaces the reference's Python drivers DetSplit / RandSplit
+ * (code/dc.py:188-249, :252-321) that call the CLI once per window: the
+ * traces are split recursively until a window of at most `window` traces is
+ * learned by ltl_learn (in process, the context's options), and the answers
+ * are combined with & and |.  RandSplit samples windows with a seeded
+ * SplitMix64 stream, reseeded on every call as dc.py:255 reseeds Python's
+ * random module.  res->formula separates all traces; res->cost is its cost
+ * under the context's costs (not minimal in general); res->all_ltls sums
+ * the sub-calls' counts; res->n_stored is the number of ltl_learn calls. */
+enum { LTL_DC_DET = 0, LTL_DC_RAND = 1 };
+int32_t ltl_learn_dc(ltl_ctx *ctx, const ltl_problem *pb, int32_t window, int32_t algo, uint64_t seed,
+                     ltl_result *res);
 
 /* ---- Learned operand ordering ------------------------------------------
  * No counterpart in the reference (its order is fixed by the loops of

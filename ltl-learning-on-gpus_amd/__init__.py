@@ -103,7 +103,9 @@ EXPORTS = ["ltl_default_options", "ltl_create", "ltl_set_options", "ltl_learn", 
            "ltl_gen_config", "ltl_gen_custom", "ltl_gen_free",
            "ltl_scorer_create", "ltl_scorer_destroy", "ltl_scorer_get_params", "ltl_scorer_set_params",
            "ltl_scorer_forward", "ltl_scorer_grad", "ltl_scorer_train", "ltl_set_scorer", "ltl_set_log",
-           "ltl_get_log"]
+           "ltl_get_log", "ltl_learn_dc", "ltl_get_options"]
+
+DC_DET, DC_RAND = 0, 1
 
 SCORER_FEATURES = 24
 SCORER_PARAMS = 128 * 32 + 128 + 128 * 128 + 128 + 128 + 1
@@ -160,6 +162,9 @@ def lib():
     L.ltl_set_log.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32]
     L.ltl_get_log.argtypes = [ctypes.c_void_p, fp, ctypes.c_int64]
     L.ltl_get_log.restype = ctypes.c_int64
+    L.ltl_learn_dc.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
+                               P(Result)]
+    L.ltl_get_options.argtypes = [ctypes.c_void_p, P(Options)]
     _lib = L
     return L
 
@@ -308,6 +313,20 @@ class Learner:
         lib().ltl_result_free(ctypes.byref(res))
         if rc < 0:
             raise LtlError(f"ltl_learn -> {rc}")
+        return out
+
+    def learn_dc(self, inst, window=64, algo=DC_DET, seed=0, **opts):
+        """Divide and conquer over windows of at most `window` traces
+        (ltl_learn_dc; the reference's DetSplit / RandSplit)."""
+        self.set_options(**opts)
+        pb = inst.problem()
+        res = Result()
+        rc = lib().ltl_learn_dc(self._ctx, ctypes.byref(pb), int(window), int(algo), int(seed), ctypes.byref(res))
+        out = self._result(res)
+        out["calls"] = res.n_stored
+        lib().ltl_result_free(ctypes.byref(res))
+        if rc < 0:
+            raise LtlError(f"ltl_learn_dc -> {rc}")
         return out
 
     @staticmethod

@@ -189,6 +189,12 @@ extern "C" int32_t ltl_set_options(ltl_ctx *c, const ltl_options *o) {
     return 0;
 }
 
+extern "C" int32_t ltl_get_options(const ltl_ctx *c, ltl_options *o) {
+    if (!c || !o) return LTL_E_ARG;
+    *o = c->opt;
+    return 0;
+}
+
 extern "C" void ltl_destroy(ltl_ctx *c) {
     if (!c) return;
     hipSetDevice(c->opt.device);
