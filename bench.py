@@ -143,12 +143,36 @@ def learned_vs_plain(ltl, L, args):
     return out
 
 
+def scorer_throughput(n):
+    """Forward and gradient of the scorer over n random feature rows (wall
+    time of the C ABI calls, host copies included; kernel times come from
+    a kernel trace of this run).  FLOPs per row: 2 * (32*128 + 128*128 + 128)
+    forward, twice that again for the backward products."""
+    import numpy as np
+    ltl = load_package()
+    sc = ltl.Scorer(seed=1)
+    rng = np.random.default_rng(0)
+    x = rng.uniform(0, 1, (n, ltl.SCORER_FEATURES)).astype(np.float32)
+    t = rng.uniform(0, 1, n).astype(np.float32)
+    sc.forward(x[:1024])
+    t0 = time.perf_counter()
+    sc.forward(x)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    sc.grad(x, t)
+    t_grad = time.perf_counter() - t0
+    sc.close()
+    fl = 2 * (32 * 128 + 128 * 128 + 128)
+    print(json.dumps({"scorer_rows": n, "forward_s": round(t_fwd, 4), "grad_s": round(t_grad, 4),
+                      "forward_flop": fl * n, "grad_flop": 3 * fl * n}), flush=True)
+
+
 def batch_vs_loop(ltl, args):
     """Configuration 4: 256 instances, one call per instance vs the batched
     persistent launch with autotuned parameters."""
     insts = [ltl.generate(4, seed=1, index=i) for i in range(256)]
     mc = max(i.max_cost for i in insts)
-    L = ltl.Learner(budget=8 << 30)
+    L = ltl.Learner(budget=int(args.budget_gb * 1e9))
     try:
         for i in insts[:8]:
             L.learn(i, max_cost=mc)
@@ -185,9 +209,14 @@ def main():
     ap.add_argument("--config5-cost", type=int, default=8)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run one 1 GiB device copy (PMC calibration)")
+    ap.add_argument("--scorer-rows", type=int, default=0,
+                    help="only run the scorer's forward and gradient on this many random rows")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
+        return
+    if args.scorer_rows:
+        scorer_throughput(args.scorer_rows)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "

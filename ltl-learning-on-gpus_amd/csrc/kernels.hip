@@ -64,9 +64,9 @@ __device__ __forceinline__ bool tile_binary(const ChunkArgs &A, Tile &t) {
 __device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t) {
     t.ti = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t.ti >= A.n_tiles) return false;
-    t.a = A.a0 + 64 * t.ti + (threadIdx.x & 63);
+    t.a = A.a0 - A.ush + 64 * t.ti + (threadIdx.x & 63);
     t.b = 0;
-    t.valid = t.a < A.a1;
+    t.valid = t.a >= A.a0 && t.a < A.a1;
     return true;
 }
 
@@ -80,8 +80,8 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     if (!tile_binary(A, t)) return;
     const bool otf = chunk_otf(A);
     const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
-    const uint64_t *pl = A.store + (size_t)sl;
-    const uint64_t *pr = A.store + (size_t)sr;
+    const uint64_t *pl = A.store + set_base(sl, A.nw);
+    const uint64_t *pr = A.store + set_base(sr, A.nw);
     Acc c0, c1;
     KeyAcc k0, k1;
     auto step = [&](int w, uint64_t l, uint64_t r) {
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     if (!tile_unary(A, t)) return;
     const bool otf = chunk_otf(A);
     const int64_t s = A.ls + (t.valid ? t.a : A.a0);
-    const uint64_t *p = A.store + (size_t)s;
+    const uint64_t *p = A.store + set_base(s, A.nw);
     Acc c0;
     KeyAcc k0;
     auto step = [&](int w, uint64_t v) {
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
 // reduced across the wave.
 __global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms, int B) {
     const int a = (int)blockIdx.x, l = (int)threadIdx.x;
-    const uint64_t *p = A.store + (size_t)a;
+    const uint64_t *p = A.store + set_base(a, A.nw);
     Acc c0;
     KeyAcc k0;
     for (int w = l; w < A.nw; w += 64) {
@@ -199,7 +199,9 @@ __global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms, int B) {
     }
     if (A.kmode) key_finish(c0, k0);
     if (l != 0) return;
-    tab_insert(A.tab, A.tmask, table_key(c0.f1), (uint64_t)a, &A.st->probe_fail);
+    const uint64_t key = table_key(c0.f1);
+    tab_insert(A.tab, A.tmask, key, (uint64_t)a, &A.st->probe_fail);
+    A.m_key[a] = key;
     A.m_fp2[a] = c0.f2;
     A.m_op[a] = LTL_ATOM;
     A.m_left[a] = -1;
@@ -229,7 +231,7 @@ __device__ __forceinline__ int64_t temp_index_of(const ChunkArgs &A, int op, uin
     }
     const int64_t a = (int64_t)rel;
     if (a < A.a0 || a >= A.a1) return -1;
-    return a - A.a0;
+    return a - A.a0 + A.ush;
 }
 
 template <int OP>
@@ -248,8 +250,8 @@ __global__ __launch_bounds__(256) void k_decide(ChunkArgs A) {
         const uint64_t pair = (uint64_t)a * (uint64_t)A.nR + (uint64_t)b;
         id0 = A.id_base + ND * pair;
     } else {
-        const int64_t a = A.a0 + ti * 64 + lane;
-        valid = a < A.a1;
+        const int64_t a = A.a0 - A.ush + ti * 64 + lane;
+        valid = a >= A.a0 && a < A.a1;
         id0 = A.id_base + (uint64_t)a;
     }
     const size_t ix = (size_t)ti * 64 + lane;
@@ -309,6 +311,7 @@ __device__ __forceinline__ void commit(const ChunkArgs &A, int64_t pos, size_t k
     A.m_left[pos] = l;
     A.m_right[pos] = r;
     A.m_fp2[pos] = A.t_fp2[k];
+    A.m_key[pos] = A.tab[2 * (size_t)A.t_slot[k]];
     A.tab[2 * (size_t)A.t_slot[k] + 1] = (uint64_t)pos;
 }
 
@@ -329,9 +332,13 @@ __global__ __launch_bounds__(256) void k_compact(ChunkArgs A) {
 template <int OP, int B>
 __global__ __launch_bounds__(256) void k_emit(ChunkArgs A) {
     if (run_stopped(A) || chunk_otf(A)) return;
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (i >= A.st->chunk_total) return;
-    const int64_t pos = A.st->chunk_base + i;
+    // Waves cover whole store tiles: lane l of wave v writes position
+    // 64 * (chunk_base / 64 + v) + l, so every store instruction of a wave
+    // fills one 512-byte segment of one tile.
+    const int64_t base = A.st->chunk_base;
+    const int64_t pos = (base & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = pos - base;
+    if (i < 0 || i >= A.st->chunk_total) return;
     if (pos >= A.capacity) return;
     constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
     constexpr int TPW = 64 / B;
@@ -344,12 +351,12 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A) {
         sl = A.ls + A.a0 + ti / A.nbs;
         sr = A.rs + A.b0 + (ti % A.nbs) * 64 + (ix % 64);
     } else {
-        sl = A.ls + A.a0 + ix;
+        sl = A.ls + A.a0 - A.ush + ix;
     }
-    const uint64_t *pl = A.store + (size_t)sl;
-    const uint64_t *pr = A.store + (size_t)(bin ? sr : sl);
-    uint64_t *q = A.store + (size_t)pos;
-    uint64_t *z = A.sum + (size_t)pos;
+    const uint64_t *pl = A.store + set_base(sl, A.nw);
+    const uint64_t *pr = A.store + set_base(bin ? sr : sl, A.nw);
+    uint64_t *q = A.store + set_base(pos, A.nw);
+    uint64_t *z = A.sum + set_base(pos, 4 * A.nsw);
     uint64_t v[4] = {0, 0, 0, 0};
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const uint64_t lm = A.aux[w].lm;
@@ -389,7 +396,7 @@ __global__ __launch_bounds__(256) void k_chk1(ChunkArgs A) {
     Tile t;
     if (!tile_unary(A, t)) return;
     constexpr int VEC = OP == LTL_NOT ? 0 : OP == LTL_NEXT ? 1 : OP == LTL_FIN ? 2 : 3;
-    const uint64_t *p = A.sum + (size_t)(A.ls + (t.valid ? t.a : A.a0)) + (size_t)(VEC * A.nsw) * A.ws;
+    const uint64_t *p = A.sum + set_base(A.ls + (t.valid ? t.a : A.a0), 4 * A.nsw) + (size_t)(VEC * A.nsw) * A.ws;
     const uint32_t k = (uint32_t)A.hamming_k;
     const bool anti = A.anti && t.valid;
     uint32_t m = t.valid ? 0u : 0x7fffffffu;  // invalid lanes never accept
@@ -416,8 +423,8 @@ __global__ __launch_bounds__(256) void k_chk2(ChunkArgs A) {
     Tile t;
     if (!tile_binary(A, t)) return;
     const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
-    const uint64_t *pl = A.sum + (size_t)sl;
-    const uint64_t *pr = A.sum + (size_t)sr;
+    const uint64_t *pl = A.sum + set_base(sl, 4 * A.nsw);
+    const uint64_t *pr = A.sum + set_base(sr, 4 * A.nsw);
     uint32_t m = t.valid ? 0u : 0x7fffffffu;  // invalid lanes never accept
     for (int i = 0; i < A.nsw; ++i) {
         const int j = interleave(i, A.nps, A.nsw);
@@ -437,8 +444,8 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     Tile t;
     if (!tile_binary(A, t)) return;
     const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
-    const uint64_t *pl = A.store + (size_t)sl;
-    const uint64_t *pr = A.store + (size_t)sr;
+    const uint64_t *pl = A.store + set_base(sl, A.nw);
+    const uint64_t *pr = A.store + set_base(sr, A.nw);
     const uint32_t k = (uint32_t)A.hamming_k;
     uint32_t m0 = t.valid ? 0u : 0x7fffffffu, m1 = m0;  // invalid lanes never accept
     auto step = [&](int w, uint64_t l, uint64_t r) {
@@ -470,6 +477,11 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     const uint64_t id0 = A.id_base + 2 * ((uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b);
     if (m0 <= k) atomicMin(&A.st->found, (unsigned long long)id0);
     else if (m1 <= k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
+}
+
+__global__ void k_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, int *probe_fail) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        tab_insert(tab, tmask, keys[i], (uint64_t)i, probe_fail);
 }
 
 __global__ void k_fill(uint64_t *p, uint64_t v, size_t n) {
@@ -526,7 +538,7 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
     // one lane per survivor; the survivor count is on the device, so the
     // grid covers every candidate of the chunk and idle lanes exit
     const int nd = op == LTL_UNTIL ? 2 : 1;
-    const dim3 g((unsigned)(((int64_t)a.n_tiles * 64 * nd + 255) / 256)), b(256);
+    const dim3 g((unsigned)(((int64_t)a.n_tiles * 64 * nd + 63 + 255) / 256)), b(256);
     switch (op) {
     case LTL_NOT: LTL_DISPATCH_B(k_emit, LTL_NOT, B, g, b, s, a); break;
     case LTL_NEXT: LTL_DISPATCH_B(k_emit, LTL_NEXT, B, g, b, s, a); break;
@@ -606,6 +618,14 @@ hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s) {
         break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, int *probe_fail,
+                         hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 16384);
+    hipLaunchKernelGGL(k_rehash, dim3(blocks), dim3(256), 0, s, tab, tmask, keys, n, probe_fail);
     return hipGetLastError();
 }
 

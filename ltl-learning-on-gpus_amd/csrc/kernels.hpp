@@ -4,12 +4,12 @@
 // Store layout.  A characteristic set (CS) over W traces is held as nw
 // packed words: trace i sits in slot i % TPW of word i / TPW, where a slot is
 // B bits (B = 8, 16, 32 or 64, the smallest power of two >= the longest
-// trace) and TPW = 64 / B.  The store is word-major: word w of set s is at
-// w * ws + s (ws = capacity rounded up to 64).  One lane handles one
+// trace) and TPW = 64 / B.  The store is tile-major: sets in tiles of 64,
+// word w of set s at ((s / 64) * nw + w) * 64 + s % 64.  One lane handles one
 // candidate, so a wave reads or writes 64 consecutive sets of one word as a
-// contiguous 512-byte segment, and the waves of a launch sweep adjacent
-// segments of the same word array.  Summaries use the same layout with
-// 4 * nsw words per set.
+// contiguous 512-byte segment, and the words of one tile follow each other
+// (a wave streams through 512-byte segments of one region, whatever the
+// capacity).  Summaries use the same layout with 4 * nsw words per set.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -18,6 +18,12 @@
 namespace ltl {
 
 constexpr uint64_t kEmpty = ~0ull;          // empty table key / value, no result
+
+// First word of set s in a tile-major array of `words` words per set; the
+// following words are 64 entries apart.
+__host__ __device__ __forceinline__ size_t set_base(int64_t s, int words) {
+    return (size_t)(s >> 6) * (size_t)words * 64 + (size_t)(s & 63);
+}
 constexpr uint64_t kPending = 1ull << 62;   // table value tag: candidate id
 
 // Per packed word: dedup hash keys and check masks (wave-uniform).
@@ -65,6 +71,7 @@ struct ChunkArgs {
     uint8_t *m_op;
     int32_t *m_left, *m_right;
     uint64_t *m_fp2;
+    uint64_t *m_key;          // per stored set: its table key (for rehashing a grown table)
     uint64_t *sum;            // summaries: 4 vectors of nsw words per set, word-major
     const SumAux *saux;
     int64_t capacity;
@@ -73,7 +80,7 @@ struct ChunkArgs {
     int64_t ls, rs;           // store index of the left / right level start
     int64_t a0, a1, b0, b1;
     int64_t nR;               // right level size (binary), 1 (unary)
-    int64_t ws;               // word stride of the store and summaries (sets)
+    int64_t ws;               // word stride within a tile of the store and summaries (64)
     int32_t nw;
     int32_t hamming_k;
     uint32_t tmask;           // table slots - 1
@@ -89,6 +96,10 @@ struct ChunkArgs {
     // of mode 1; words holding traces that take key bits.
     const uint32_t *kinfo;
     int32_t kmode, kstride, kw_end, W;
+    // Unary chunks: candidate a sits at temp index a - a0 + ush, with
+    // ush = (ls + a0) % 64, so that the 64 lanes of a wave read the 64 sets
+    // of one store tile.  0 for binary chunks.
+    int32_t ush, pad2_;
 };
 
 // Check passes visit words alternately from the positive and the negative
@@ -118,6 +129,9 @@ hipError_t launch_scatter(uint64_t *dst, int64_t ws, const uint64_t *src, int nc
 // canonical id of every accepted candidate.
 hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s);
+// Insert (keys[s], s) for s < n into an emptied table of tmask + 1 slots.
+hipError_t launch_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, int *probe_fail,
+                         hipStream_t s);
 // Exclusive scan of t_cnt into t_off (n elements); temp sized by scan_temp_bytes.
 size_t scan_temp_bytes(int n_max);
 hipError_t launch_scan(const uint32_t *in, uint32_t *out, int n, void *temp, size_t temp_bytes,

@@ -85,10 +85,11 @@ struct ltl_ctx {
     int64_t capacity = 0;
     int64_t ws = 0;  // word stride of the word-major store (capacity rounded up)
     int64_t CH = 0;  // candidate slots per chunk
-    uint32_t tmask = 0;
+    uint32_t tmask = 0;           // slots in use - 1 (the table grows between levels)
+    size_t tab_max_slots = 0;     // slots carved for the table
     WordAux *d_aux = nullptr;
     DevState *d_state = nullptr;
-    uint64_t *d_store = nullptr, *d_tab = nullptr, *d_fp2 = nullptr, *d_sum = nullptr;
+    uint64_t *d_store = nullptr, *d_tab = nullptr, *d_fp2 = nullptr, *d_sum = nullptr, *d_key = nullptr;
     SumAux *d_saux = nullptr;
     int nsw = 0;
     uint8_t *d_op = nullptr;
@@ -239,7 +240,7 @@ static int32_t ensure_pool(ltl_ctx *c) {
 
 static int32_t carve(ltl_ctx *c) {
     const size_t budget = c->pool_bytes;
-    const size_t per_cs = (size_t)c->nw * 8 + (size_t)c->nsw * 32 + 17;
+    const size_t per_cs = (size_t)c->nw * 8 + (size_t)c->nsw * 32 + 25;
     // chunk: per-candidate temporaries (18 B each) within 1/32 of the pool
     int64_t ch = 1 << 26;
     while (ch > 4096 && (size_t)ch * 18 > budget / 32) ch >>= 1;
@@ -282,6 +283,7 @@ static int32_t carve(ltl_ctx *c) {
     if (c->opt.capacity > 0) cap = std::min<int64_t>(cap, c->opt.capacity);
     c->capacity = cap;
     c->tmask = (uint32_t)(slots - 1);
+    c->tab_max_slots = slots;
 
     char *p = c->pool;
     auto take = [&](size_t n) { char *q = p; p += align_up(n, 256); return q; };
@@ -299,11 +301,12 @@ static int32_t carve(ltl_ctx *c) {
     c->d_kinfo = (uint32_t *)take((size_t)c->W * 4);
     c->d_tab = (uint64_t *)take(slots * 16);
     const int64_t capr = align_up((size_t)cap, 64);
-    c->ws = capr;
+    c->ws = 64;  // word stride inside a 64-set tile (tile-major store)
     c->d_op = (uint8_t *)take((size_t)capr);
     c->d_left = (int32_t *)take((size_t)capr * 4);
     c->d_right = (int32_t *)take((size_t)capr * 4);
     c->d_fp2 = (uint64_t *)take((size_t)capr * 8);
+    c->d_key = (uint64_t *)take((size_t)capr * 8);
     c->d_store = (uint64_t *)take((size_t)capr * c->nw * 8);
     c->d_sum = (uint64_t *)take((size_t)capr * c->nsw * 32);
     if ((size_t)(p - c->pool) > budget) return LTL_E_OOM;
@@ -467,6 +470,8 @@ struct Run {
     uint64_t candidates = 0, alg_bytes = 0;
     bool otf = false;         // overflow seen at a previous level end
     bool check_only = false;  // blocks are tested only (precheck pass)
+    bool count_only = false;  // blocks are only counted (candidates of a level)
+    uint64_t counted = 0;
     bool ranked = false;      // check pass: binary blocks go to the ranked rounds
     bool logging = false;     // check pass: binary blocks are also logged
     bool skip_binary = false; // check pass: binary blocks were tested ahead of the level
@@ -496,7 +501,8 @@ struct Run {
             kind = op == LTL_UNTIL ? K_CHKU : K_CHK2;
         } else {
             a.nbs = 1;
-            a.n_tiles = (int32_t)((a.a1 - a.a0 + 63) / 64);
+            a.ush = (int32_t)((a.ls + a.a0) & 63);
+            a.n_tiles = (int32_t)((a.a1 - a.a0 + a.ush + 63) / 64);
             cand = (uint64_t)(a.a1 - a.a0);
             alg = mn = cand * sv;
             kind = K_CHK1;
@@ -522,7 +528,8 @@ struct Run {
             mn = (uint64_t)(a.a1 - a.a0 + a.b1 - a.b0) * cs + cand * 20;
         } else {
             a.nbs = 1;
-            a.n_tiles = (int32_t)((a.a1 - a.a0 + 63) / 64);
+            a.ush = (int32_t)((a.ls + a.a0) & 63);
+            a.n_tiles = (int32_t)((a.a1 - a.a0 + a.ush + 63) / 64);
             cand = (uint64_t)(a.a1 - a.a0);
             alg = cand * (cs + 20);
             mn = alg;
@@ -558,8 +565,13 @@ struct Run {
     int32_t unary(int op, int64_t s, int64_t e) {
         const int64_t n = e - s;
         if (n <= 0 || (check_only && only_binary)) return 0;
+        if (count_only) {
+            counted += (uint64_t)n;
+            return 0;
+        }
         blocks.push_back({all_ltls, (uint64_t)n, op, s, n, 0, 1, cur_level});
-        const int64_t step = check_only ? ((int64_t)1 << 36) : c->CH;
+        // (a chunk's temporaries hold CH candidates, the tile shift up to 63)
+        const int64_t step = check_only ? ((int64_t)1 << 36) : c->CH - 64;
         for (int64_t a0 = 0; a0 < n; a0 += step) {
             ChunkArgs a = base;
             a.id_base = all_ltls;
@@ -579,6 +591,10 @@ struct Run {
         if (nL <= 0 || nR <= 0 || (check_only && skip_binary)) return 0;
         const int nd = op == LTL_UNTIL ? 2 : 1;
         const uint64_t size = (uint64_t)nL * (uint64_t)nR * nd;
+        if (count_only) {
+            counted += size;
+            return 0;
+        }
         blocks.push_back({all_ltls, size, op, ls, nL, rs, nR, cur_level});
         if (check_only && (ranked || logging)) {
             const int64_t nTR = (nR + 63) / 64;
@@ -755,6 +771,13 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     HIP_TRY(launch_scatter(c->d_store, c->ws, c->d_stage, A, c->nw, s));
     HIP_TRY(launch_scatter(c->d_sum, c->ws, c->d_stage + (size_t)c->nw * A, A, (int64_t)4 * nsw, s));
     HIP_TRY(hipMemcpyAsync(c->d_state, &st0, sizeof(st0), hipMemcpyHostToDevice, s));
+    // Without a fixed size the table starts small and grows before a level
+    // is generated (clearing 2^31 slots would cost milliseconds per call).
+    if (!o.hash_slots_log2) {
+        size_t slots = (size_t)1 << 16;
+        while (slots < c->tab_max_slots && slots < 4 * (size_t)A) slots <<= 1;
+        c->tmask = (uint32_t)(std::min(slots, c->tab_max_slots) - 1);
+    }
     HIP_TRY(launch_fill_u64(c->d_tab, kEmpty, 2 * ((size_t)c->tmask + 1), s));
 
     Run run{c, pb};
@@ -762,7 +785,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     ba.store = c->d_store; ba.aux = c->d_aux; ba.tab = c->d_tab; ba.st = c->d_state;
     ba.t_slot = c->t_slot; ba.t_fp2 = c->t_fp2; ba.t_mask = c->t_mask; ba.t_cnt = c->t_cnt; ba.t_off = c->t_off;
     ba.t_list = c->t_list;
-    ba.m_op = c->d_op; ba.m_left = c->d_left; ba.m_right = c->d_right; ba.m_fp2 = c->d_fp2;
+    ba.m_op = c->d_op; ba.m_left = c->d_left; ba.m_right = c->d_right; ba.m_fp2 = c->d_fp2; ba.m_key = c->d_key;
     ba.sum = c->d_sum; ba.saux = c->d_saux; ba.nsw = nsw;
     ba.npw = std::min(c->nw, (P + c->TPW - 1) / c->TPW);
     ba.nps = std::min(nsw, (P + 63) / 64);
@@ -1096,6 +1119,30 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             if (h.found != kEmpty) { found = true; found_cost = cost; Le[cost] = n_stored; break; }
             if (run.otf) generate = false;  // check-only level (overflow)
             else if (!learned) { run.all_ltls = save; run.blocks.resize(nb); }
+        }
+        if (generate && !o.hash_slots_log2) {
+            // Grow the table so that it stays at most half full with every
+            // set this level can add and a full chunk pending: a new table,
+            // the stored sets re-inserted from their keys.
+            run.count_only = true;
+            run.counted = 0;
+            rc = level_blocks(cost);
+            run.count_only = false;
+            if (rc) break;
+            const uint64_t need =
+                2 * (std::min<uint64_t>((uint64_t)c->capacity, (uint64_t)n_stored + run.counted) +
+                     std::min<uint64_t>((uint64_t)c->CH, run.counted));
+            size_t slots = (size_t)c->tmask + 1;
+            if (need > slots && slots < c->tab_max_slots) {
+                while (slots < need && slots < c->tab_max_slots) slots <<= 1;
+                c->tmask = (uint32_t)(slots - 1);
+                ba.tmask = c->tmask;
+                if (launch_fill_u64(c->d_tab, kEmpty, 2 * slots, s) != hipSuccess ||
+                    launch_rehash(c->d_tab, c->tmask, c->d_key, n_stored, &c->d_state->probe_fail, s) != hipSuccess) {
+                    rc = LTL_E_HIP;
+                    break;
+                }
+            }
         }
         if (generate) {
             if ((rc = level_blocks(cost))) break;
@@ -1434,14 +1481,19 @@ extern "C" int32_t ltl_store_read(ltl_ctx *c, int64_t first, int64_t count, uint
     if (first + count > c->capacity) return LTL_E_ARG;
     HIP_TRY(hipSetDevice(c->opt.device));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    std::vector<uint64_t> buf((size_t)c->nw * count);  // [w][k]
-    HIP_TRY(hipMemcpy2D(buf.data(), (size_t)count * 8, c->d_store + first, (size_t)c->ws * 8, (size_t)count * 8,
-                        (size_t)c->nw, hipMemcpyDeviceToHost));
+    // the whole tiles holding [first, first + count), one contiguous copy
+    const int64_t t0 = first / 64, t1 = (first + count - 1) / 64 + 1;
+    const size_t tile_words = (size_t)c->nw * 64;
+    std::vector<uint64_t> buf((size_t)(t1 - t0) * tile_words);
+    HIP_TRY(hipMemcpy(buf.data(), c->d_store + (size_t)t0 * tile_words, buf.size() * 8, hipMemcpyDeviceToHost));
     const uint64_t slot = c->B == 64 ? ~0ull : ((1ull << c->B) - 1);
     if (cs)
-        for (int64_t k = 0; k < count; ++k)
+        for (int64_t k = 0; k < count; ++k) {
+            const int64_t sidx = first + k;
+            const uint64_t *w0 = buf.data() + set_base(sidx, c->nw) - (size_t)t0 * tile_words;
             for (int i = 0; i < c->W; ++i)
-                cs[(size_t)k * c->W + i] = (buf[(size_t)(i / c->TPW) * count + k] >> ((i % c->TPW) * c->B)) & slot;
+                cs[(size_t)k * c->W + i] = (w0[(size_t)(i / c->TPW) * 64] >> ((i % c->TPW) * c->B)) & slot;
+        }
     if (op) HIP_TRY(hipMemcpy(op, c->d_op + first, (size_t)count, hipMemcpyDeviceToHost));
     if (left) HIP_TRY(hipMemcpy(left, c->d_left + first, (size_t)count * 4, hipMemcpyDeviceToHost));
     if (right) HIP_TRY(hipMemcpy(right, c->d_right + first, (size_t)count * 4, hipMemcpyDeviceToHost));

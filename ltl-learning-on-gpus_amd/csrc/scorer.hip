@@ -338,10 +338,11 @@ __global__ __launch_bounds__(256) void k_tile_stats(ChunkArgs A, int64_t first, 
     const bool ok = k < n;
     const int64_t s = first + (ok ? k : tile * 64);
     uint32_t p0 = 0, n0 = 0, pz = 0, nz = 0;
+    const uint64_t *zs = A.sum + set_base(s, 4 * A.nsw);
     for (int j = 0; j < A.nsw; ++j) {
         const uint64_t tg = A.saux[j].tgt, ng = A.saux[j].val & ~tg;
-        const uint64_t b0 = A.sum[(size_t)j * A.ws + s];
-        const uint64_t z = A.sum[(size_t)(2 * A.nsw + j) * A.ws + s];
+        const uint64_t b0 = zs[(size_t)j * A.ws];
+        const uint64_t z = zs[(size_t)(2 * A.nsw + j) * A.ws];
         p0 += __popcll(b0 & tg);
         n0 += __popcll(b0 & ng);
         pz += __popcll(z & tg);
@@ -425,7 +426,7 @@ template <int B, bool EARLY>
 __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl, int64_t sr, uint32_t stop,
                                           bool anti, uint32_t &m0, uint32_t &m1, uint32_t &seen) {
     if (op == LTL_UNTIL) {
-        const uint64_t *pl = A.store + (size_t)sl, *pr = A.store + (size_t)sr;
+        const uint64_t *pl = A.store + set_base(sl, A.nw), *pr = A.store + set_base(sr, A.nw);
         constexpr int UB = 4;
         int i = 0;
         for (; i + UB <= A.nw; i += UB) {
@@ -456,7 +457,7 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
             seen += (uint32_t)__popcll(x->chk);
         }
     } else {
-        const uint64_t *pl = A.sum + (size_t)sl, *pr = A.sum + (size_t)sr;
+        const uint64_t *pl = A.sum + set_base(sl, 4 * A.nsw), *pr = A.sum + set_base(sr, 4 * A.nsw);
         for (int i = 0; i < A.nsw; ++i) {
             const int j = EARLY ? interleave(i, A.nps, A.nsw) : i;
             const uint64_t a = pl[(size_t)j * A.ws], b = pr[(size_t)j * A.ws];
