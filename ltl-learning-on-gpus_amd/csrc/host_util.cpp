@@ -358,6 +358,12 @@ extern "C" int32_t ltl_read_trace_file(const char *path, int32_t max_traces, int
         }
         if (!cur.empty()) lines.push_back(cur);
     }
+    // Robust against what editors add: trailing blank lines (the reference
+    // reads the alphabet from the last line, code/ltli6463.cu:1511-1526),
+    // trailing blanks on a line, CR-LF line ends.
+    for (auto &l : lines)
+        while (!l.empty() && (l.back() == ' ' || l.back() == '\t')) l.pop_back();
+    while (!lines.empty() && lines.back().empty()) lines.pop_back();
     if (lines.empty()) return LTL_E_ARG;
     std::string alpha;
     bool inset[26] = {false};

@@ -144,3 +144,26 @@ def test_token_encoding_matches_reader(ltl, oracle):
     inst = ltl._instance_from_tokens(pos, neg, alpha)
     assert inst.names == ref.names
     assert np.array_equal(inst.atoms, ref.atoms) and np.array_equal(inst.lens, ref.lens)
+
+
+@pytest.mark.parametrize("variant", ["crlf", "trailing_blank_lines", "no_final_newline", "trailing_spaces"])
+def test_trace_reader_tolerates_editor_artifacts(ltl, tmp_path, variant):
+    """The same traces with CR-LF line ends, blank lines after the alphabet,
+    no final newline or blanks at the end of lines parse identically."""
+    src = os.path.join(GOLDEN, "traces", "Ab1.trace")
+    text = open(src).read()
+    lines = text.rstrip("\n").split("\n")
+    if variant == "crlf":
+        out = "\r\n".join(lines) + "\r\n"
+    elif variant == "trailing_blank_lines":
+        out = "\n".join(lines) + "\n\n\n"
+    elif variant == "no_final_newline":
+        out = "\n".join(lines)
+    else:
+        out = "\n".join(l + "  " for l in lines) + "\n"
+    p = tmp_path / "t.trace"
+    p.write_bytes(out.encode())
+    a, b = ltl.read_trace_file(src), ltl.read_trace_file(str(p))
+    assert (a.num_pos, a.num_neg, a.names) == (b.num_pos, b.num_neg, b.names)
+    assert np.array_equal(a.lens, b.lens)
+    assert np.array_equal(a.atoms, b.atoms)
