@@ -55,8 +55,8 @@ __device__ __forceinline__ bool tile_binary(const ChunkArgs &A, Tile &t) {
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     t.a = A.a0 + 4 * aq + wave;
     if (t.a >= A.a1) return false;
-    t.b = A.b0 + 64 * (int64_t)bs + (threadIdx.x & 63);
-    t.valid = t.b < A.b1;
+    t.b = A.b0 - A.ush + 64 * (int64_t)bs + (threadIdx.x & 63);
+    t.valid = t.b >= A.b0 && t.b < A.b1;
     t.ti = (t.a - A.a0) * A.nbs + bs;
     return true;
 }
@@ -225,8 +225,9 @@ __device__ __forceinline__ int64_t temp_index_of(const ChunkArgs &A, int op, uin
         const uint64_t pair = op == LTL_UNTIL ? rel >> 1 : rel;
         const int64_t a = (int64_t)(pair / (uint64_t)A.nR), b = (int64_t)(pair % (uint64_t)A.nR);
         if (a < A.a0 || a >= A.a1 || b < A.b0 || b >= A.b1) return -1;
-        const int64_t ti = (a - A.a0) * A.nbs + (b - A.b0) / 64;
-        const int64_t ix = ti * 64 + (b - A.b0) % 64;
+        const int64_t off = b - A.b0 + A.ush;
+        const int64_t ti = (a - A.a0) * A.nbs + off / 64;
+        const int64_t ix = ti * 64 + off % 64;
         return op == LTL_UNTIL ? 2 * ix + (int64_t)(rel & 1) : ix;
     }
     const int64_t a = (int64_t)rel;
@@ -245,8 +246,8 @@ __global__ __launch_bounds__(256) void k_decide(ChunkArgs A) {
     bool valid;
     uint64_t id0;
     if (bin) {
-        const int64_t a = A.a0 + ti / A.nbs, b = A.b0 + (ti % A.nbs) * 64 + lane;
-        valid = b < A.b1;
+        const int64_t a = A.a0 + ti / A.nbs, b = A.b0 - A.ush + (ti % A.nbs) * 64 + lane;
+        valid = b >= A.b0 && b < A.b1;
         const uint64_t pair = (uint64_t)a * (uint64_t)A.nR + (uint64_t)b;
         id0 = A.id_base + ND * pair;
     } else {
@@ -349,7 +350,7 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A) {
     if (bin) {
         const int64_t ti = ix / 64;
         sl = A.ls + A.a0 + ti / A.nbs;
-        sr = A.rs + A.b0 + (ti % A.nbs) * 64 + (ix % 64);
+        sr = A.rs + A.b0 - A.ush + (ti % A.nbs) * 64 + (ix % 64);
     } else {
         sl = A.ls + A.a0 - A.ush + ix;
     }

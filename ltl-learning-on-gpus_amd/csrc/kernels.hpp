@@ -96,9 +96,10 @@ struct ChunkArgs {
     // of mode 1; words holding traces that take key bits.
     const uint32_t *kinfo;
     int32_t kmode, kstride, kw_end, W;
-    // Unary chunks: candidate a sits at temp index a - a0 + ush, with
-    // ush = (ls + a0) % 64, so that the 64 lanes of a wave read the 64 sets
-    // of one store tile.  0 for binary chunks.
+    // Lane shift: unary chunks put candidate a at temp index a - a0 + ush
+    // with ush = (ls + a0) % 64; binary chunks put column b of a row at lane
+    // offset b - b0 + ush within the row's stripes, ush = (rs + b0) % 64.
+    // Either way the 64 lanes of a wave read the 64 sets of one store tile.
     int32_t ush, pad2_;
 };
 

@@ -491,7 +491,8 @@ struct Run {
         uint64_t cand, alg, mn;
         int kind;
         if (bin) {
-            a.nbs = (int32_t)((a.b1 - a.b0 + 63) / 64);
+            a.ush = (int32_t)((a.rs + a.b0) & 63);
+            a.nbs = (int32_t)((a.b1 - a.b0 + a.ush + 63) / 64);
             a.n_tiles = (int32_t)((a.a1 - a.a0) * a.nbs);
             const uint64_t pairs = (uint64_t)(a.a1 - a.a0) * (uint64_t)(a.b1 - a.b0);
             const uint64_t per = op == LTL_UNTIL ? cs : sv;
@@ -520,7 +521,8 @@ struct Run {
         uint64_t cand, alg, mn;
         const uint64_t cs = (uint64_t)c->nw * 8;
         if (bin) {
-            a.nbs = (int32_t)((a.b1 - a.b0 + 63) / 64);
+            a.ush = (int32_t)((a.rs + a.b0) & 63);
+            a.nbs = (int32_t)((a.b1 - a.b0 + a.ush + 63) / 64);
             a.n_tiles = (int32_t)((a.a1 - a.a0) * a.nbs);
             const uint64_t pairs = (uint64_t)(a.a1 - a.a0) * (uint64_t)(a.b1 - a.b0);
             cand = pairs * nd;
@@ -606,7 +608,7 @@ struct Run {
             }
         }
         if (check_only) {
-            const int64_t nbs = (nR + 63) / 64;
+            const int64_t nbs = (nR + 63 + 63) / 64;  // with the lane shift
             const int64_t rows = std::max<int64_t>(4, ((int64_t)1 << 30) / nbs * 4);
             for (int64_t a0 = 0; a0 < nL; a0 += rows) {
                 ChunkArgs a = base;
@@ -619,7 +621,7 @@ struct Run {
             all_ltls += size;
             return 0;
         }
-        const int64_t row_slots = (nR + 63) / 64 * 64 * nd;
+        const int64_t row_slots = (nR + 63 + 63) / 64 * 64 * nd;  // stripes of a row, with the lane shift
         if (row_slots <= c->CH) {
             const int64_t rows = std::max<int64_t>(1, c->CH / row_slots);
             for (int64_t a0 = 0; a0 < nL; a0 += rows) {
@@ -631,7 +633,7 @@ struct Run {
                 if (int32_t r = run_chunk(op, a)) return r;
             }
         } else {
-            const int64_t cols = std::max<int64_t>(64, c->CH / nd / 64 * 64);
+            const int64_t cols = std::max<int64_t>(64, (c->CH / nd - 64) / 64 * 64);
             for (int64_t a0 = 0; a0 < nL; ++a0)
                 for (int64_t b0 = 0; b0 < nR; b0 += cols) {
                     ChunkArgs a = base;
@@ -664,9 +666,21 @@ int32_t read_meta(ltl_ctx *c, int64_t k, Meta &m) {
 // Build the node array (breadth first) from the root triple and the store's
 // parent indices; store indices below the atom count are atoms.
 int32_t build_tree(ltl_ctx *c, int op, int32_t l, int32_t r, int atom, std::vector<ltl_node> &nodes,
-                   std::vector<int64_t> &src) {
+                   std::vector<int64_t> &src, int64_t n_stored) {
     nodes.clear();
     src.clear();
+    // Small stores: the parent arrays in one copy each instead of three
+    // synchronous copies per node.
+    std::vector<uint8_t> ops;
+    std::vector<int32_t> lefts, rights;
+    if (n_stored > 0 && n_stored <= ((int64_t)1 << 20)) {
+        ops.resize(n_stored);
+        lefts.resize(n_stored);
+        rights.resize(n_stored);
+        HIP_TRY(hipMemcpy(ops.data(), c->d_op, (size_t)n_stored, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(lefts.data(), c->d_left, (size_t)n_stored * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(rights.data(), c->d_right, (size_t)n_stored * 4, hipMemcpyDeviceToHost));
+    }
     nodes.push_back({op, op == LTL_ATOM ? atom : -1, -1, -1});
     src.push_back(-1);
     struct Q { int node; int32_t l, r; };
@@ -677,7 +691,13 @@ int32_t build_tree(ltl_ctx *c, int op, int32_t l, int32_t r, int atom, std::vect
         for (int s = 0; s < 2; ++s) {
             if (kids[s] < 0) continue;
             Meta m;
-            if (int32_t e = read_meta(c, kids[s], m)) return e;
+            if (kids[s] < (int64_t)ops.size()) {
+                m.op = ops[kids[s]];
+                m.l = lefts[kids[s]];
+                m.r = rights[kids[s]];
+            } else if (int32_t e = read_meta(c, kids[s], m)) {
+                return e;
+            }
             const int idx = (int)nodes.size();
             nodes.push_back({m.op, m.op == LTL_ATOM ? kids[s] : -1, -1, -1});
             src.push_back(kids[s]);
@@ -1194,7 +1214,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         }
         std::vector<ltl_node> nodes;
         std::vector<int64_t> src;
-        if (int32_t e = build_tree(c, blk->op, l, r, -1, nodes, src)) return res->status = e;
+        if (int32_t e = build_tree(c, blk->op, l, r, -1, nodes, src, n_stored)) return res->status = e;
         if (neg_answer) {
             // The accepted formula is the negation of the recorded candidate.
             for (auto &nd : nodes) {
