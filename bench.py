@@ -116,6 +116,13 @@ def learned_vs_plain(ltl, L, args):
     out["config2"] = {"plain_s": round(p["seconds"], 4), "learned_s": round(q["seconds"], 4),
                       "speedup": round(p["seconds"] / q["seconds"], 3), "cost": p["cost"],
                       "plain_formula": p["formula"], "learned_formula": q["formula"]}
+    inst3 = ltl.generate(3, seed=1)
+    p = best_of(inst3, 2, None, max_cost=inst3.max_cost, hamming_k=inst3.hamming_k)
+    q = best_of(inst3, 2, trained, max_cost=inst3.max_cost, hamming_k=inst3.hamming_k)
+    assert q["status"] == p["status"] and q["cost"] == p["cost"]
+    out["config3"] = {"plain_s": round(p["seconds"], 4), "learned_s": round(q["seconds"], 4),
+                      "speedup": round(p["seconds"] / q["seconds"], 3), "cost": p["cost"],
+                      "hamming_k": inst3.hamming_k, "plain_formula": p["formula"], "learned_formula": q["formula"]}
     rows5, tot_p, tot_l, tot_u, faster, ranks = [], 0.0, 0.0, 0.0, 0, []
     for idx in range(args.config5):
         inst = ltl.generate(5, seed=1, index=idx)

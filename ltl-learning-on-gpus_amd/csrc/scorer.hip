@@ -111,6 +111,74 @@ __device__ __forceinline__ void fwd16(const MlpDev &M, const uint16_t *xs, uint1
     }
 }
 
+// Forward pass of 64 samples (rows of xs) by one wave: four 16-row groups
+// share every weight fragment, so each B load feeds four products.  y[q][i]
+// is the output of sample 16q + 4g + i (every lane of group g).
+__device__ __forceinline__ void fwd64(const MlpDev &M, const uint16_t *xs, uint16_t *h1s, float (&y)[4][4]) {
+    const int l = threadIdx.x & 63, g = l >> 4, c = l & 15;
+    const float *p = M.p;
+    wave_sync();
+    bf16x8 a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] = ld8(xs + (16 * q + c) * XS + 8 * g);
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const bf16x8 b = ld8(M.w1b + (16 * t + c) * kIn + 8 * g);
+        const float bias = p[P_B1 + 16 * t + c];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            f32x4 z = {0.f, 0.f, 0.f, 0.f};
+            z = mma(a[q], b, z);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float v = z[i] + bias;
+                h1s[(16 * q + 4 * g + i) * HS + 16 * t + c] = bf16r(v > 0.f ? v : 0.f);
+            }
+        }
+    }
+    wave_sync();
+    f32x4 acc[4][8];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[q][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int kb = 0; kb < 4; ++kb) {
+        bf16x8 a2[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a2[q] = ld8(h1s + (16 * q + c) * HS + 32 * kb + 8 * g);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const bf16x8 b = ld8(M.w2b + (16 * t + c) * kHid + 32 * kb + 8 * g);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q][t] = mma(a2[q], b, acc[q][t]);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const float bias = p[P_B2 + 16 * t + c], w3 = p[P_W3 + 16 * t + c];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[q][t][i] + bias;
+                v = v > 0.f ? v : 0.f;
+                part[i] += v * w3;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            float s = part[i];
+            s += __shfl_xor(s, 1);
+            s += __shfl_xor(s, 2);
+            s += __shfl_xor(s, 4);
+            s += __shfl_xor(s, 8);
+            y[q][i] = s + p[P_B3];
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // training
 // ---------------------------------------------------------------------------
@@ -293,22 +361,23 @@ __global__ void k_zero(float *p, int64_t n) {
         p[i] = 0.f;
 }
 
-// Forward pass of feature rows (16 per wave).
-__global__ __launch_bounds__(256) void k_forward(MlpDev M, const float *X, int xsd, int64_t n, float *Y) {
-    __shared__ __align__(16) uint16_t xs[4][16 * XS];
-    __shared__ __align__(16) uint16_t hs[4][16 * HS];
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int64_t s0 = (int64_t)blockIdx.x * 64 + 16 * w;
-    if (l < 16) {
+// Forward pass of feature rows: one wave (workgroup) per 64 rows.
+__global__ __launch_bounds__(64) void k_forward(MlpDev M, const float *X, int xsd, int64_t n, float *Y) {
+    __shared__ __align__(16) uint16_t xs[64 * XS];
+    __shared__ __align__(16) uint16_t hs[64 * HS];
+    const int l = threadIdx.x;
+    const int64_t s0 = (int64_t)blockIdx.x * 64;
+    {
         const int64_t r = s0 + l;
-        for (int j = 0; j < 32; ++j) xs[w][l * XS + j] = bf16r(r < n && j < kFeat ? X[r * xsd + j] : 0.f);
+        for (int j = 0; j < 32; ++j) xs[l * XS + j] = bf16r(r < n && j < kFeat ? X[r * xsd + j] : 0.f);
     }
-    Fwd f;
-    fwd16(M, xs[w], hs[w], nullptr, 0, f);
+    float y[4][4];
+    fwd64(M, xs, hs, y);
     const int g = l >> 4, c = l & 15;
     if (c == 0)
-        for (int i = 0; i < 4; ++i)
-            if (s0 + 4 * g + i < n) Y[s0 + 4 * g + i] = f.y[i];
+        for (int q = 0; q < 4; ++q)
+            for (int i = 0; i < 4; ++i)
+                if (s0 + 16 * q + 4 * g + i < n) Y[s0 + 16 * q + 4 * g + i] = y[q][i];
 }
 
 // ---------------------------------------------------------------------------
@@ -385,13 +454,15 @@ __device__ void features(const LBlock &K, const FeatCtx &F, int64_t loc, float *
     x[23] = 1.f;
 }
 
-__global__ __launch_bounds__(256) void k_score(MlpDev M, const LBlock *blk, int nblk, FeatCtx F, int64_t n_ent,
-                                               float *keys, uint32_t *vals) {
-    __shared__ __align__(16) uint16_t xs[4][16 * XS];
-    __shared__ __align__(16) uint16_t hs[4][16 * HS];
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    const int64_t e0 = (int64_t)blockIdx.x * 64 + 16 * w;
-    if (l < 16) {
+// One wave (workgroup) per 64 tile pairs; each lane builds one entry's
+// features.
+__global__ __launch_bounds__(64) void k_score(MlpDev M, const LBlock *blk, int nblk, FeatCtx F, int64_t n_ent,
+                                              float *keys, uint32_t *vals) {
+    __shared__ __align__(16) uint16_t xs[64 * XS];
+    __shared__ __align__(16) uint16_t hs[64 * HS];
+    const int l = threadIdx.x;
+    const int64_t e0 = (int64_t)blockIdx.x * 64;
+    {
         const int64_t e = e0 + l;
         float x[kFeat];
         if (e < n_ent) {
@@ -400,19 +471,20 @@ __global__ __launch_bounds__(256) void k_score(MlpDev M, const LBlock *blk, int 
         } else {
             for (int j = 0; j < kFeat; ++j) x[j] = 0.f;
         }
-        for (int j = 0; j < 32; ++j) xs[w][l * XS + j] = j < kFeat ? bf16r(x[j]) : (uint16_t)0;
+        for (int j = 0; j < 32; ++j) xs[l * XS + j] = j < kFeat ? bf16r(x[j]) : (uint16_t)0;
     }
-    Fwd f;
-    fwd16(M, xs[w], hs[w], nullptr, 0, f);
+    float y[4][4];
+    fwd64(M, xs, hs, y);
     const int g = l >> 4, c = l & 15;
     if (c == 0)
-        for (int i = 0; i < 4; ++i) {
-            const int64_t e = e0 + 4 * g + i;
-            if (e < n_ent) {
-                keys[e] = f.y[i];
-                vals[e] = (uint32_t)e;
+        for (int q = 0; q < 4; ++q)
+            for (int i = 0; i < 4; ++i) {
+                const int64_t e = e0 + 16 * q + 4 * g + i;
+                if (e < n_ent) {
+                    keys[e] = y[q][i];
+                    vals[e] = (uint32_t)e;
+                }
             }
-        }
 }
 
 // ---------------------------------------------------------------------------
@@ -550,7 +622,7 @@ hipError_t launch_tile_stats(const ChunkArgs &a, int64_t first, int64_t n, int32
 hipError_t launch_score(const MlpDev &m, const LBlock *blk, int nblk, const FeatCtx &f, int64_t n_ent, float *keys,
                         uint32_t *vals, hipStream_t s) {
     if (n_ent <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_score, dim3((unsigned)((n_ent + 63) / 64)), dim3(256), 0, s, m, blk, nblk, f, n_ent, keys,
+    hipLaunchKernelGGL(k_score, dim3((unsigned)((n_ent + 63) / 64)), dim3(64), 0, s, m, blk, nblk, f, n_ent, keys,
                        vals);
     return hipGetLastError();
 }
@@ -627,7 +699,7 @@ hipError_t launch_refresh(const MlpDev &m, hipStream_t s) {
 
 hipError_t launch_forward(const MlpDev &m, const float *x, int xs, int64_t n, float *y, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_forward, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, s, m, x, xs, n, y);
+    hipLaunchKernelGGL(k_forward, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, m, x, xs, n, y);
     return hipGetLastError();
 }
 
