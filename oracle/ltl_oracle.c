@@ -47,6 +47,58 @@ uint64_t orc_op_until(uint64_t l, uint64_t r) {
     return y;
 }
 
+/* Multi-word traces (an extension: the reference stops at 63 positions,
+ * code/ltli6463.cu:1078).  Word k of a trace holds positions 64k..64k+63;
+ * the single-word operators apply per word, and what crosses a word boundary
+ * comes from the next higher word: X moves bit 0 of word k+1 into bit 63 of
+ * word k; F holds at every position of word k when a higher word has a set
+ * bit; G holds in word k only where every higher word is full; U holds on
+ * the top run of l's ones in word k when it holds at position 0 of word k+1. */
+static uint64_t word_mask(int len, int k) {
+    const int lo = 64 * k;
+    if (len <= lo) return 0;
+    return len - lo >= 64 ? ~0ull : ((1ull << (len - lo)) - 1ull);
+}
+
+void orc_trace_op(int32_t op, const uint64_t *x, const uint64_t *y, int32_t K, int32_t len, uint64_t *out) {
+    int carry = 0;
+    switch (op) {
+    case ORC_NOT:
+        for (int k = 0; k < K; ++k) out[k] = ~x[k] & word_mask(len, k);
+        return;
+    case ORC_AND:
+        for (int k = 0; k < K; ++k) out[k] = x[k] & y[k];
+        return;
+    case ORC_OR:
+        for (int k = 0; k < K; ++k) out[k] = x[k] | y[k];
+        return;
+    case ORC_NEXT:
+        for (int k = 0; k < K; ++k) out[k] = (x[k] >> 1) | (k + 1 < K ? x[k + 1] << 63 : 0ull);
+        return;
+    case ORC_FIN:
+        for (int k = K - 1; k >= 0; --k) {
+            out[k] = orc_op_fin(x[k]) | (carry ? word_mask(len, k) : 0ull);
+            carry |= x[k] != 0;
+        }
+        return;
+    case ORC_GLOB:
+        carry = 1;
+        for (int k = K - 1; k >= 0; --k) {
+            const uint64_t m = word_mask(len, k);
+            out[k] = carry ? x[k] & ~orc_op_fin(~x[k] & m) : 0ull;
+            carry = carry && x[k] == m;
+        }
+        return;
+    default:
+        for (int k = K - 1; k >= 0; --k) {
+            uint64_t u = orc_op_until(x[k], y[k]);
+            if (carry) u |= ~orc_op_fin(~x[k]);
+            out[k] = u;
+            carry = (int)(u & 1ull);
+        }
+    }
+}
+
 /* Target check (code/ltli6463.cu:240-242): positives must hold at position
  * 0, negatives must not.  Returned as a count so that a Hamming tolerance
  * can be applied (the exact reference check is count == 0). */
@@ -108,7 +160,8 @@ void orc_relaxed_key(const uint64_t *cs, const uint8_t *lens, int32_t W,
 typedef struct {
     const orc_problem *pb;
     const orc_options *opt;
-    int W, P, A;
+    int W, P, A;   /* W = words per set = T * K */
+    int T, K;      /* traces, words per trace */
     int len_sum;
     /* append-only store (code/ltli6463.cu:1166-1178) */
     uint64_t *words; uint8_t *op; int32_t *left, *right;
@@ -158,7 +211,7 @@ static void key_of(ectx *e, const uint64_t *cs, uint64_t *hv, uint64_t *k0, uint
     if (e->opt->dedup_mode == 0) {
         *hv = hash_words(cs, e->W); *k0 = 0; *k1 = 0;
     } else {
-        orc_relaxed_key(cs, e->pb->lens, e->W, e->opt->dedup_mode, e->len_sum, k0, k1);
+        orc_relaxed_key(cs, e->pb->lens, e->T, e->opt->dedup_mode, e->len_sum, k0, k1);
         uint64_t t[2] = {*k0, *k1};
         *hv = hash_words(t, 2);
     }
@@ -202,7 +255,17 @@ static int table_find(ectx *e, const uint64_t *cs, uint64_t hv, uint64_t k0, uin
 }
 
 static int accept(ectx *e, const uint64_t *cs) {
-    return orc_mismatches(cs, e->P, e->W - e->P) <= e->opt->hamming_k;
+    if (e->K == 1) return orc_mismatches(cs, e->P, e->T - e->P) <= e->opt->hamming_k;
+    int32_t m = 0;
+    for (int i = 0; i < e->T; ++i) m += (int32_t)(((cs[(size_t)i * e->K] & 1ull) != 0) != (i < e->P));
+    return m <= e->opt->hamming_k;
+}
+
+/* Operator op on every trace of a K-word set. */
+static void apply_set(const ectx *e, int op, const uint64_t *x, const uint64_t *y, uint64_t *out) {
+    const int K = e->K;
+    for (int i = 0; i < e->T; ++i)
+        orc_trace_op(op, x + (size_t)i * K, y ? y + (size_t)i * K : NULL, K, e->pb->lens[i], out + (size_t)i * K);
 }
 
 /* One candidate, as one thread of any operator kernel does it
@@ -238,14 +301,16 @@ static int unary_block(ectx *e, int op, int64_t s, int64_t t) {
     int rc = 0;
     for (int64_t k = s; k < t && !rc; ++k) {
         const uint64_t *x = e->words + (size_t)k * W;
-        for (int i = 0; i < W; ++i) {
-            switch (op) {
-            case ORC_NOT:  e->cand[i] = orc_op_not(x[i], lens[i]); break;
-            case ORC_NEXT: e->cand[i] = orc_op_next(x[i]); break;
-            case ORC_FIN:  e->cand[i] = orc_op_fin(x[i]); break;
-            default:       e->cand[i] = orc_op_glob(x[i], lens[i]); break;
+        if (e->K > 1) apply_set(e, op, x, NULL, e->cand);
+        else
+            for (int i = 0; i < W; ++i) {
+                switch (op) {
+                case ORC_NOT:  e->cand[i] = orc_op_not(x[i], lens[i]); break;
+                case ORC_NEXT: e->cand[i] = orc_op_next(x[i]); break;
+                case ORC_FIN:  e->cand[i] = orc_op_fin(x[i]); break;
+                default:       e->cand[i] = orc_op_glob(x[i], lens[i]); break;
+                }
             }
-        }
         rc = consider(e, e->cand, op, (int32_t)k, -1, base + (uint64_t)(k - s));
     }
     e->all_ltls += (uint64_t)(t - s);
@@ -262,11 +327,13 @@ static int binary_block(ectx *e, int op, int64_t ls, int64_t le, int64_t rs, int
             const uint64_t pair = (uint64_t)((a - ls) * nR + (b - rs));
             if (op == ORC_UNTIL) {
                 const uint64_t *x = e->words + (size_t)a * W, *y = e->words + (size_t)b * W;
-                for (int i = 0; i < W; ++i) e->cand[i] = orc_op_until(x[i], y[i]);
+                if (e->K > 1) apply_set(e, ORC_UNTIL, x, y, e->cand);
+                else for (int i = 0; i < W; ++i) e->cand[i] = orc_op_until(x[i], y[i]);
                 rc = consider(e, e->cand, op, (int32_t)a, (int32_t)b, base + 2 * pair);
                 if (rc) break;
                 x = e->words + (size_t)a * W; y = e->words + (size_t)b * W;
-                for (int i = 0; i < W; ++i) e->cand[i] = orc_op_until(y[i], x[i]);
+                if (e->K > 1) apply_set(e, ORC_UNTIL, y, x, e->cand);
+                else for (int i = 0; i < W; ++i) e->cand[i] = orc_op_until(y[i], x[i]);
                 rc = consider(e, e->cand, op, (int32_t)b, (int32_t)a, base + 2 * pair + 1);
             } else {
                 const uint64_t *x = e->words + (size_t)a * W, *y = e->words + (size_t)b * W;
@@ -297,11 +364,13 @@ static void cand_compute(ectx *e, int op, int64_t ls, int64_t nR, int64_t rs, in
         const uint64_t *x = e->words + (size_t)a * W, *y = e->words + (size_t)b * W;
         if (op == ORC_AND) for (int i = 0; i < W; ++i) out[i] = x[i] & y[i];
         else if (op == ORC_OR) for (int i = 0; i < W; ++i) out[i] = x[i] | y[i];
+        else if (e->K > 1) apply_set(e, ORC_UNTIL, dir ? y : x, dir ? x : y, out);
         else if (!dir) for (int i = 0; i < W; ++i) out[i] = orc_op_until(x[i], y[i]);
         else for (int i = 0; i < W; ++i) out[i] = orc_op_until(y[i], x[i]);
         *l = (int32_t)(dir ? b : a); *r = (int32_t)(dir ? a : b);
     } else {
         const uint64_t *x = e->words + (size_t)(ls + idx) * W;
+        if (e->K > 1) { apply_set(e, op, x, NULL, out); *l = (int32_t)(ls + idx); *r = -1; return; }
         for (int i = 0; i < W; ++i) {
             switch (op) {
             case ORC_NOT:  out[i] = orc_op_not(x[i], lens[i]); break;
@@ -494,10 +563,14 @@ static double now_s(void) {
 int32_t orc_enumerate(const orc_problem *pb, const orc_options *opt, orc_result *res) {
     memset(res, 0, sizeof(*res));
     res->found_id = -1; res->root_op = -1; res->root_left = -1; res->root_right = -1;
-    const int W = pb->num_pos + pb->num_neg;
-    if (W <= 0 || pb->num_atoms <= 0) return res->status = -1;
-    for (int i = 0; i < W; ++i) if (pb->lens[i] < 1 || pb->lens[i] > 64) return res->status = -1;
+    const int T = pb->num_pos + pb->num_neg;
+    const int K = pb->words_per_trace > 1 ? pb->words_per_trace : 1;
+    const int W = T * K;
+    if (T <= 0 || pb->num_atoms <= 0 || K > 4) return res->status = -1;
+    for (int i = 0; i < T; ++i) if (pb->lens[i] < 1 || pb->lens[i] > 64 * K) return res->status = -1;
     for (int i = 0; i < 8; ++i) if (opt->cost[i] <= 0) return res->status = -1;
+    /* the reference's relaxed keys are defined on one word per trace */
+    if (K > 1 && opt->dedup_mode != 0) return res->status = -1;
     const int c1 = opt->cost[0], c2 = opt->cost[1], c3 = opt->cost[2], c4 = opt->cost[3];
     const int c5 = opt->cost[4], c6 = opt->cost[5], c7 = opt->cost[6], c8 = opt->cost[7];
     const int maxc = opt->max_cost;
@@ -505,7 +578,8 @@ int32_t orc_enumerate(const orc_problem *pb, const orc_options *opt, orc_result 
     double t0 = now_s();
     ectx e; memset(&e, 0, sizeof(e));
     e.pb = pb; e.opt = opt; e.W = W; e.P = pb->num_pos; e.A = pb->num_atoms;
-    for (int i = 0; i < W; ++i) e.len_sum += pb->lens[i];
+    e.T = T; e.K = K;
+    for (int i = 0; i < T; ++i) e.len_sum += pb->lens[i];
     e.cand = (uint64_t *)malloc((size_t)W * sizeof(uint64_t));
     res->n_levels = (maxc > c1 ? maxc : c1) + 2;
     e.lvl_s = (int64_t *)calloc((size_t)res->n_levels, sizeof(int64_t));
@@ -678,12 +752,66 @@ static uint64_t eval_pos(const node *n, const orc_problem *pb, int t, int len) {
     return out;
 }
 
+/* Value at position k of a trace of length len, from the operands' values
+ * per position (the definitions above, for any length). */
+static uint8_t pos_value(int op, const uint8_t *a, const uint8_t *b, int k, int len) {
+    switch (op) {
+    case ORC_NOT: return (uint8_t)!a[k];
+    case ORC_AND: return (uint8_t)(a[k] && b[k]);
+    case ORC_OR: return (uint8_t)(a[k] || b[k]);
+    case ORC_NEXT: return (uint8_t)(k + 1 < len && a[k + 1]);
+    case ORC_FIN:
+        for (int j = k; j < len; ++j) if (a[j]) return 1;
+        return 0;
+    case ORC_GLOB:
+        for (int j = k; j < len; ++j) if (!a[j]) return 0;
+        return 1;
+    default:
+        /* the first j >= k with b@j decides, unless a fails before it */
+        for (int j = k; j < len; ++j) {
+            if (b[j]) return 1;
+            if (!a[j]) return 0;
+        }
+        return 0;
+    }
+}
+
+static void unpack_bits(const uint64_t *w, int len, uint8_t *out) {
+    for (int k = 0; k < len; ++k) out[k] = (uint8_t)((w[k >> 6] >> (k & 63)) & 1ull);
+}
+static void pack_bits(const uint8_t *v, int len, int K, uint64_t *w) {
+    for (int k = 0; k < K; ++k) w[k] = 0;
+    for (int k = 0; k < len; ++k) if (v[k]) w[k >> 6] |= 1ull << (k & 63);
+}
+
+/* Per-position values of n on trace t of a K-word problem (len <= 255). */
+static void eval_bits(const node *n, const orc_problem *pb, int t, int len, int K, uint8_t *out) {
+    const int T = pb->num_pos + pb->num_neg;
+    if (n->op == ORC_ATOM) {
+        unpack_bits(pb->atoms + ((size_t)n->atom * T + t) * K, len, out);
+        return;
+    }
+    uint8_t a[256], b[256] = {0};
+    eval_bits(n->a, pb, t, len, K, a);
+    if (n->b) eval_bits(n->b, pb, t, len, K, b);
+    for (int k = 0; k < len; ++k) out[k] = pos_value(n->op, a, b, k, len);
+}
+
 int32_t orc_eval_formula(const char *formula, const orc_problem *pb, uint64_t *out_cs) {
     const char *p = formula;
     node *n = parse_f(&p, pb);
     if (!n || *p) { free_node(n); return -1; }
     const int W = pb->num_pos + pb->num_neg;
-    for (int t = 0; t < W; ++t) out_cs[t] = eval_pos(n, pb, t, pb->lens[t]);
+    const int K = pb->words_per_trace > 1 ? pb->words_per_trace : 1;
+    if (K == 1) {
+        for (int t = 0; t < W; ++t) out_cs[t] = eval_pos(n, pb, t, pb->lens[t]);
+    } else {
+        uint8_t v[256];
+        for (int t = 0; t < W; ++t) {
+            eval_bits(n, pb, t, pb->lens[t], K, v);
+            pack_bits(v, pb->lens[t], K, out_cs + (size_t)t * K);
+        }
+    }
     free_node(n);
     return 0;
 }
@@ -698,7 +826,7 @@ int32_t orc_formula_cost(const char *formula, const int32_t *cost) {
     static const char letters[] = "abcdefghijklmnopqrstuvwxyz";
     uint8_t lens[1] = {1};
     uint64_t atoms[26] = {0};
-    orc_problem pb = {1, 0, 26, lens, atoms, letters};
+    orc_problem pb = {1, 0, 26, lens, atoms, letters, 1, 0};
     const char *p = formula;
     node *n = parse_f(&p, &pb);
     if (!n || *p) { free_node(n); return -1; }
@@ -743,8 +871,30 @@ static uint64_t bf_binary(int op, uint64_t a, uint64_t b, int len) {
     return out;
 }
 
+/* One trace of K words through an operator, per position (b = NULL for a
+ * unary operator). */
+static void bf_trace(int op, const uint64_t *a, const uint64_t *b, int len, int K, uint64_t *out) {
+    if (K == 1) {
+        out[0] = b ? bf_binary(op, a[0], b[0], len) : bf_unary(op, a[0], len);
+        return;
+    }
+    uint8_t x[256], y[256] = {0}, v[256];
+    unpack_bits(a, len, x);
+    if (b) unpack_bits(b, len, y);
+    for (int k = 0; k < len; ++k) v[k] = pos_value(op, x, y, k, len);
+    pack_bits(v, len, K, out);
+}
+
+static int32_t mism_stride(const uint64_t *cs, int P, int N, int K) {
+    int32_t m = 0;
+    for (int i = 0; i < P + N; ++i) m += (int32_t)(((cs[(size_t)i * K] & 1ull) != 0) != (i < P));
+    return m;
+}
+
 int32_t orc_bruteforce_min_cost(const orc_problem *pb, const orc_options *opt, int64_t max_formulas) {
-    const int W = pb->num_pos + pb->num_neg;
+    const int T = pb->num_pos + pb->num_neg;
+    const int K = pb->words_per_trace > 1 ? pb->words_per_trace : 1;
+    const int W = T * K;   /* words per formula */
     const int c1 = opt->cost[0], maxc = opt->max_cost;
     if (maxc < c1) return -1;
     uint64_t **lv = (uint64_t **)calloc((size_t)maxc + 1, sizeof(uint64_t *));
@@ -771,7 +921,7 @@ int32_t orc_bruteforce_min_cost(const orc_problem *pb, const orc_options *opt, i
         lv[c] = (uint64_t *)malloc((size_t)(n ? n : 1) * W * 8);
         int64_t k = 0;
 #define BF_EMIT() do { memcpy(lv[c] + (size_t)k * W, tmp, (size_t)W * 8); ++k; \
-            if (orc_mismatches(tmp, pb->num_pos, pb->num_neg) <= opt->hamming_k) result = c; } while (0)
+            if (mism_stride(tmp, pb->num_pos, pb->num_neg, K) <= opt->hamming_k) result = c; } while (0)
         if (c == c1)
             for (int a = 0; a < pb->num_atoms; ++a) {
                 for (int t = 0; t < W; ++t) tmp[t] = pb->atoms[(size_t)a * W + t];
@@ -783,7 +933,9 @@ int32_t orc_bruteforce_min_cost(const orc_problem *pb, const orc_options *opt, i
             int64_t m = cnt[sub];
             if (un[u] == ORC_NOT && opt->neg_type == 2) { if (sub != c1) continue; m = pb->num_atoms; }
             for (int64_t f = 0; f < m; ++f) {
-                for (int t = 0; t < W; ++t) tmp[t] = bf_unary(un[u], lv[sub][(size_t)f * W + t], pb->lens[t]);
+                for (int t = 0; t < T; ++t)
+                    bf_trace(un[u], lv[sub] + (size_t)f * W + (size_t)t * K, NULL, pb->lens[t], K,
+                             tmp + (size_t)t * K);
                 BF_EMIT();
             }
         }
@@ -792,8 +944,9 @@ int32_t orc_bruteforce_min_cost(const orc_problem *pb, const orc_options *opt, i
                 int j = c - opt->cost[bn[u]] - i;
                 for (int64_t f = 0; f < cnt[i]; ++f)
                     for (int64_t g = 0; g < cnt[j]; ++g) {
-                        for (int t = 0; t < W; ++t)
-                            tmp[t] = bf_binary(bn[u], lv[i][(size_t)f * W + t], lv[j][(size_t)g * W + t], pb->lens[t]);
+                        for (int t = 0; t < T; ++t)
+                            bf_trace(bn[u], lv[i] + (size_t)f * W + (size_t)t * K,
+                                     lv[j] + (size_t)g * W + (size_t)t * K, pb->lens[t], K, tmp + (size_t)t * K);
                         BF_EMIT();
                     }
             }

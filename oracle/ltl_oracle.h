@@ -8,7 +8,9 @@
  * It restates, in plain C, the algorithm of the reference learner
  * (code/ltli6463.cu of MojtabaValizadeh/ltl-learning-on-gpus):
  *   - CS encoding: one uint64 per trace, bit k = "holds at position k"
- *     (code/ltli6463.cu:1100-1133);
+ *     (code/ltli6463.cu:1100-1133); traces longer than 64 positions take K
+ *     words each (an extension: the reference stops at 63 positions,
+ *     code/ltli6463.cu:1078);
  *   - the seven operators (code/ltli6463.cu:203, 295, 385, 468, 551-556,
  *     638-645, 739-771);
  *   - the target check (code/ltli6463.cu:240-243), extended with a Hamming
@@ -44,9 +46,12 @@ enum { ORC_ATOM = 0, ORC_NOT = 1, ORC_AND = 2, ORC_OR = 3,
 
 typedef struct {
     int32_t num_pos, num_neg, num_atoms;
-    const uint8_t  *lens;   /* [W], 1..64 */
-    const uint64_t *atoms;  /* [num_atoms * W], atom-major */
+    const uint8_t  *lens;   /* [W], 1..64 * K */
+    const uint64_t *atoms;  /* [num_atoms * W * K], atom-major, then trace,
+                               then word (word k = positions 64k..64k+63) */
     const char     *names;  /* num_atoms letters (may be NULL: 'a'+i) */
+    int32_t words_per_trace; /* K: 0 or 1 = one word per trace, up to 4 */
+    int32_t pad_;
 } orc_problem;
 
 typedef struct {
@@ -86,6 +91,10 @@ uint64_t orc_op_fin(uint64_t x);
 uint64_t orc_op_glob(uint64_t x, int len);
 uint64_t orc_op_until(uint64_t l, uint64_t r);
 
+/* Operator op on one trace of K words (word k = positions 64k..64k+63,
+ * length len <= 64K); y is the right operand of & | U, else unused. */
+void orc_trace_op(int32_t op, const uint64_t *x, const uint64_t *y, int32_t K, int32_t len, uint64_t *out);
+
 /* Misclassified-trace count of a CS (bit 0 vs label; positives first). */
 int32_t orc_mismatches(const uint64_t *cs, int32_t num_pos, int32_t num_neg);
 
@@ -97,7 +106,7 @@ int32_t orc_enumerate(const orc_problem *pb, const orc_options *opt, orc_result 
 void orc_result_free(orc_result *res);
 
 /* Textbook per-position evaluation of a formula in the reference grammar.
- * Returns 0 and fills out_cs[W], or a negative parse error. */
+ * Returns 0 and fills out_cs[W * K], or a negative parse error. */
 int32_t orc_eval_formula(const char *formula, const orc_problem *pb, uint64_t *out_cs);
 /* Symbol count (= cost under unit costs), or negative on parse error. */
 int32_t orc_formula_cost(const char *formula, const int32_t *cost);

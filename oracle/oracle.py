@@ -20,7 +20,8 @@ class Problem(ctypes.Structure):
                 ("num_atoms", ctypes.c_int32),
                 ("lens", ctypes.POINTER(ctypes.c_uint8)),
                 ("atoms", ctypes.POINTER(ctypes.c_uint64)),
-                ("names", ctypes.c_char_p)]
+                ("names", ctypes.c_char_p),
+                ("words_per_trace", ctypes.c_int32), ("pad_", ctypes.c_int32)]
 
 
 class Options(ctypes.Structure):
@@ -86,6 +87,9 @@ def lib():
             getattr(L, f).restype = ctypes.c_uint64
         L.orc_op_until.argtypes = [ctypes.c_uint64, ctypes.c_uint64]
         L.orc_op_until.restype = ctypes.c_uint64
+        L.orc_trace_op.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64),
+                                   ctypes.POINTER(ctypes.c_uint64), ctypes.c_int32, ctypes.c_int32,
+                                   ctypes.POINTER(ctypes.c_uint64)]
         _lib = L
     return _lib
 
@@ -95,12 +99,13 @@ def _u64p(a):
 
 
 class Instance:
-    """A learning instance: atom CS words (atom-major), lengths, class sizes."""
+    """A learning instance: atom CS words (atom-major, then trace, then the K
+    words of a trace), lengths, class sizes."""
 
-    def __init__(self, num_pos, num_neg, lens, atoms, names=None):
-        self.num_pos, self.num_neg = int(num_pos), int(num_neg)
+    def __init__(self, num_pos, num_neg, lens, atoms, names=None, K=1):
+        self.num_pos, self.num_neg, self.K = int(num_pos), int(num_neg), int(K)
         self.lens = np.ascontiguousarray(lens, dtype=np.uint8)
-        self.atoms = np.ascontiguousarray(atoms, dtype=np.uint64).reshape(-1, self.W)
+        self.atoms = np.ascontiguousarray(atoms, dtype=np.uint64).reshape(-1, self.W * self.K)
         self.names = names or "".join(chr(ord("a") + i) for i in range(self.atoms.shape[0]))
 
     @property
@@ -111,7 +116,7 @@ class Instance:
         self._names_b = self.names.encode()
         return Problem(self.num_pos, self.num_neg, self.atoms.shape[0],
                        self.lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-                       _u64p(self.atoms), self._names_b)
+                       _u64p(self.atoms), self._names_b, self.K, 0)
 
 
 def options(cost=(1,) * 8, max_cost=10, neg_type=1, hamming_k=0, dedup_mode=0,
@@ -140,7 +145,7 @@ def enumerate_(inst, **kw):
         out["level_start"] = np.ctypeslib.as_array(res.level_start, (nl,)).copy()
         out["level_end"] = np.ctypeslib.as_array(res.level_end, (nl,)).copy()
     if opt.keep_store and res.status >= 0 and res.n_stored:
-        n, W = res.n_stored, inst.W
+        n, W = res.n_stored, inst.W * inst.K
         out["words"] = np.ctypeslib.as_array(res.words, (n * W,)).reshape(n, W).copy()
         out["op"] = np.ctypeslib.as_array(res.op, (n,)).copy()
         out["left"] = np.ctypeslib.as_array(res.left, (n,)).copy()
@@ -151,7 +156,7 @@ def enumerate_(inst, **kw):
 
 def eval_formula(inst, formula):
     """Textbook per-position evaluation; returns the CS words or None."""
-    out = np.zeros(inst.W, dtype=np.uint64)
+    out = np.zeros(inst.W * inst.K, dtype=np.uint64)
     pb = inst.problem()
     rc = lib().orc_eval_formula(formula.encode(), ctypes.byref(pb), _u64p(out))
     return out if rc == 0 else None
@@ -163,8 +168,17 @@ def formula_cost(formula, cost=(1,) * 8):
 
 
 def mismatches(inst, cs):
-    cs = np.ascontiguousarray(cs, dtype=np.uint64)
+    cs = np.ascontiguousarray(cs, dtype=np.uint64)[::inst.K].copy()
     return lib().orc_mismatches(_u64p(cs), inst.num_pos, inst.num_neg)
+
+
+def trace_op(op, x, y, K, length):
+    """Operator op on one trace of K words (y: right operand of & | U)."""
+    x = np.ascontiguousarray(x, dtype=np.uint64)
+    y = np.ascontiguousarray(y if y is not None else np.zeros(K), dtype=np.uint64)
+    out = np.zeros(K, dtype=np.uint64)
+    lib().orc_trace_op(int(op), _u64p(x), _u64p(y), int(K), int(length), _u64p(out))
+    return out
 
 
 def bruteforce_min_cost(inst, max_formulas=2_000_000, **kw):
