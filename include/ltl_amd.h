@@ -53,14 +53,20 @@ enum { LTL_ORDER_PLAIN = 0, LTL_ORDER_LEARNED = 1 };
  * alphabet set (code/ltli6463.cu:1043-1052): the atom characteristic sets
  * are given already encoded, one uint64 per trace, bit k set iff the atom
  * holds at position k (the encoding of code/ltli6463.cu:1100-1133).
- * Traces 0..num_pos-1 are positive, the rest negative. */
+ * Traces 0..num_pos-1 are positive, the rest negative.
+ * Traces longer than 64 positions (the reference stops at 63,
+ * code/ltli6463.cu:1078) take words_per_trace = K words each, 2 <= K <= 4:
+ * word k holds positions 64k..64k+63, atoms[(a*W + i)*K + k], lengths up to
+ * 64K (and 255).  K = 0 or 1 is the one-word layout.  Relaxed dedup modes
+ * and the batched launch take one-word instances only (ltl_learn_batch
+ * hands the others to ltl_learn). */
 typedef struct {
     int32_t num_pos;
     int32_t num_neg;
     int32_t num_atoms;
-    int32_t reserved;
-    const uint8_t  *lens;   /* [num_pos+num_neg], each 1..64 */
-    const uint64_t *atoms;  /* [num_atoms * (num_pos+num_neg)], atom-major */
+    int32_t words_per_trace;
+    const uint8_t  *lens;   /* [num_pos+num_neg], each 1..64*K */
+    const uint64_t *atoms;  /* [num_atoms * (num_pos+num_neg) * K], atom-major */
     const char     *names;  /* num_atoms single-letter names, or NULL */
 } ltl_problem;
 
@@ -204,7 +210,8 @@ typedef struct {
 int32_t ltl_get_kernel_stats(const ltl_ctx *ctx, ltl_kernel_stat *out, int32_t max);
 
 /* Read stored sets [first, first+count) of the last call, unpacked to one
- * uint64 per trace (cs[k*W + i]); op/left/right may be NULL. */
+ * uint64 per trace (cs[k*W + i]), or K per trace (cs[(k*W + i)*K + j]) for
+ * K words per trace; op/left/right may be NULL. */
 int32_t ltl_store_read(ltl_ctx *ctx, int64_t first, int64_t count, uint64_t *cs,
                        uint8_t *op, int32_t *left, int32_t *right);
 
@@ -282,12 +289,23 @@ int32_t ltl_read_trace_file(const char *path, int32_t max_traces,
                             int32_t *num_pos, int32_t *num_neg, int32_t *num_atoms,
                             uint8_t *lens, uint64_t *atoms, char *names);
 
+/* The same for traces of up to 64*words_per_trace positions (at most 255;
+ * the reference stops at 63, code/ltli6463.cu:1078): atoms[26*max_traces*K]
+ * in the K-word layout of ltl_problem (stride W*K per atom after the call).
+ * *max_len receives the longest trace, so that a caller can repack to one
+ * word when it is at most 64. */
+int32_t ltl_read_trace_file_words(const char *path, int32_t max_traces, int32_t words_per_trace,
+                                  int32_t *num_pos, int32_t *num_neg, int32_t *num_atoms,
+                                  int32_t *max_len, uint8_t *lens, uint64_t *atoms, char *names);
+
 /* Evaluate a formula in the reference output grammar on every trace
  * (bit-parallel; the product counterpart of code/dc.py:136-186).  Fills
- * out_cs[W]; returns 0 or LTL_E_ARG on a parse error. */
+ * out_cs[W * K] (K = words per trace); returns 0 or LTL_E_ARG on a parse
+ * error. */
 int32_t ltl_eval_formula(const char *formula, const ltl_problem *pb, uint64_t *out_cs);
 
-/* Misclassified traces of a characteristic set (0 = separates). */
+/* Misclassified traces of a characteristic set, one word per trace (for K
+ * words per trace pass every K-th word; 0 = separates). */
 int32_t ltl_mismatches(const uint64_t *cs, int32_t num_pos, int32_t num_neg);
 
 /* Library version string. */

@@ -33,7 +33,7 @@ class LtlError(RuntimeError):
 
 class Problem(ctypes.Structure):
     _fields_ = [("num_pos", ctypes.c_int32), ("num_neg", ctypes.c_int32),
-                ("num_atoms", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("num_atoms", ctypes.c_int32), ("words_per_trace", ctypes.c_int32),
                 ("lens", ctypes.POINTER(ctypes.c_uint8)),
                 ("atoms", ctypes.POINTER(ctypes.c_uint64)),
                 ("names", ctypes.c_char_p)]
@@ -99,7 +99,7 @@ class GenInstance(ctypes.Structure):
 EXPORTS = ["ltl_default_options", "ltl_create", "ltl_set_options", "ltl_learn", "ltl_learn_batch",
            "ltl_set_batch_params", "ltl_get_batch_params", "ltl_batch_last_stats", "ltl_autotune_batch",
            "ltl_result_free", "ltl_destroy", "ltl_get_kernel_stats", "ltl_store_read", "ltl_levels",
-           "ltl_read_trace_file", "ltl_eval_formula", "ltl_mismatches", "ltl_version",
+           "ltl_read_trace_file", "ltl_read_trace_file_words", "ltl_eval_formula", "ltl_mismatches", "ltl_version",
            "ltl_gen_config", "ltl_gen_custom", "ltl_gen_free",
            "ltl_scorer_create", "ltl_scorer_destroy", "ltl_scorer_get_params", "ltl_scorer_set_params",
            "ltl_scorer_forward", "ltl_scorer_grad", "ltl_scorer_train", "ltl_set_scorer", "ltl_set_log",
@@ -140,6 +140,9 @@ def lib():
     L.ltl_levels.argtypes = [ctypes.c_void_p, P(ctypes.c_int64), P(ctypes.c_int64), ctypes.c_int32]
     L.ltl_read_trace_file.argtypes = [ctypes.c_char_p, ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32),
                                       P(ctypes.c_int32), P(ctypes.c_uint8), P(ctypes.c_uint64), ctypes.c_char_p]
+    L.ltl_read_trace_file_words.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, P(ctypes.c_int32),
+                                            P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32),
+                                            P(ctypes.c_uint8), P(ctypes.c_uint64), ctypes.c_char_p]
     L.ltl_eval_formula.argtypes = [ctypes.c_char_p, P(Problem), P(ctypes.c_uint64)]
     L.ltl_mismatches.argtypes = [P(ctypes.c_uint64), ctypes.c_int32, ctypes.c_int32]
     L.ltl_version.restype = ctypes.c_char_p
@@ -174,14 +177,15 @@ def _u64p(a):
 
 
 class Instance:
-    """Atom characteristic sets (atom-major, one uint64 per trace), lengths,
-    class sizes (positives first) and atom letters."""
+    """Atom characteristic sets (atom-major, one uint64 per trace, or K per
+    trace for traces longer than 64 positions: word k = positions
+    64k..64k+63), lengths, class sizes (positives first) and atom letters."""
 
     def __init__(self, num_pos, num_neg, lens, atoms, names=None, planted=None,
-                 max_cost=None, hamming_k=0, neg_type=1):
-        self.num_pos, self.num_neg = int(num_pos), int(num_neg)
+                 max_cost=None, hamming_k=0, neg_type=1, K=1):
+        self.num_pos, self.num_neg, self.K = int(num_pos), int(num_neg), int(K)
         self.lens = np.ascontiguousarray(lens, dtype=np.uint8)
-        self.atoms = np.ascontiguousarray(atoms, dtype=np.uint64).reshape(-1, self.W)
+        self.atoms = np.ascontiguousarray(atoms, dtype=np.uint64).reshape(-1, self.W * self.K)
         self.names = names or "".join(chr(ord("a") + i) for i in range(self.atoms.shape[0]))
         self.planted, self.max_cost, self.hamming_k, self.neg_type = planted, max_cost, hamming_k, neg_type
 
@@ -191,7 +195,7 @@ class Instance:
 
     def problem(self):
         self._names_b = self.names.encode()
-        return Problem(self.num_pos, self.num_neg, self.atoms.shape[0], 0,
+        return Problem(self.num_pos, self.num_neg, self.atoms.shape[0], self.K,
                        self.lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
                        _u64p(self.atoms), self._names_b)
 
@@ -227,22 +231,25 @@ def generate_custom(seed, num_pos, num_neg, num_atoms, len_min, len_max, rate=0.
 
 
 def read_trace_file(path, max_traces=4096):
-    """Reference trace-file format (code/ltli6463.cu:1500-1573)."""
+    """Reference trace-file format (code/ltli6463.cu:1500-1573), traces of up
+    to 255 positions (the reference stops at 63, :1078): one word per trace
+    when the longest fits 64 positions, else K = ceil(longest / 64)."""
     lens = np.zeros(max_traces, dtype=np.uint8)
-    atoms = np.zeros(26 * max_traces, dtype=np.uint64)
+    atoms = np.zeros(26 * max_traces * 4, dtype=np.uint64)
     names = ctypes.create_string_buffer(32)
-    P, N, A = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-    rc = lib().ltl_read_trace_file(path.encode(), max_traces, ctypes.byref(P), ctypes.byref(N),
-                                   ctypes.byref(A), lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)),
-                                   _u64p(atoms), names)
+    P, N, A, M = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    rc = lib().ltl_read_trace_file_words(path.encode(), max_traces, 4, ctypes.byref(P), ctypes.byref(N),
+                                         ctypes.byref(A), ctypes.byref(M),
+                                         lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _u64p(atoms), names)
     if rc != 0:
         raise LtlError(f"{path}: error {rc}")
-    W = P.value + N.value
-    return Instance(P.value, N.value, lens[:W], atoms[:A.value * W].reshape(A.value, W), names.value.decode())
+    W, K = P.value + N.value, (max(M.value, 1) + 63) // 64
+    words = atoms[:A.value * W * 4].reshape(A.value, W, 4)[:, :, :K]
+    return Instance(P.value, N.value, lens[:W], words.reshape(A.value, W * K), names.value.decode(), K=K)
 
 
 def eval_formula(inst, formula):
-    out = np.zeros(inst.W, dtype=np.uint64)
+    out = np.zeros(inst.W * inst.K, dtype=np.uint64)
     pb = inst.problem()
     rc = lib().ltl_eval_formula(formula.encode(), ctypes.byref(pb), _u64p(out))
     if rc != 0:
@@ -251,7 +258,7 @@ def eval_formula(inst, formula):
 
 
 def mismatches(inst, cs):
-    cs = np.ascontiguousarray(cs, dtype=np.uint64)
+    cs = np.ascontiguousarray(np.asarray(cs, dtype=np.uint64)[::inst.K])
     return lib().ltl_mismatches(_u64p(cs), inst.num_pos, inst.num_neg)
 
 
@@ -418,6 +425,8 @@ class Learner:
         return s[:n], e[:n]
 
     def store(self, first, count, W):
+        """Stored sets [first, first + count): W words each (traces x words
+        per trace)."""
         cs = np.zeros(count * W, dtype=np.uint64)
         op = np.zeros(count, dtype=np.uint8)
         left = np.zeros(count, dtype=np.int32)
@@ -501,20 +510,24 @@ class Scorer:
 
 
 def _instance_from_tokens(pos, neg, alphabet):
-    """Encode the reference's token traces (code/ltli6463.cu:1100-1133)."""
+    """Encode the reference's token traces (code/ltli6463.cu:1100-1133); K
+    words per trace when the longest trace exceeds 64 positions."""
     names = "".join(sorted(set(alphabet)))
     traces = list(pos) + list(neg)
     W = len(traces)
-    lens = np.array([len(t) for t in traces], dtype=np.uint8)
-    atoms = np.zeros((len(names), W), dtype=np.uint64)
+    longest = max((len(t) for t in traces), default=1)
+    K = (max(longest, 1) + 63) // 64
+    lens = np.array([min(len(t), 255) for t in traces], dtype=np.uint8)
+    atoms = np.zeros((len(names), W, K), dtype=np.uint64)
     for a, ch in enumerate(names):
         for i, tr in enumerate(traces):
             w = 0
             for k, tok in enumerate(tr):
                 if ch in tok:
                     w |= 1 << k
-            atoms[a, i] = w
-    return Instance(len(pos), len(neg), lens, atoms, names)
+            for j in range(K):
+                atoms[a, i, j] = (w >> (64 * j)) & 0xFFFFFFFFFFFFFFFF
+    return Instance(len(pos), len(neg), lens, atoms.reshape(len(names), W * K), names, K=K)
 
 
 def LTLI(costFun, maxCost, RlxUnqChkTyp, NegType, alphabet, pos, neg, learner=None, reference_keys=False):
@@ -527,8 +540,10 @@ def LTLI(costFun, maxCost, RlxUnqChkTyp, NegType, alphabet, pos, neg, learner=No
     the reference's relaxed uniqueness key RlxUnqChkTyp (1-3), which is lossy
     above 126 total trace positions (code/ltli6463.cu:57-141).
     """
+    if any(len(t) > 255 for t in list(pos) + list(neg)):
+        return "see_the_error", int(costFun[0]), 0
     inst = _instance_from_tokens(pos, neg, alphabet)
-    if inst.W == 0 or int(inst.lens.max(initial=0)) > 64:
+    if inst.W == 0 or (inst.K > 1 and reference_keys):
         return "see_the_error", int(costFun[0]), 0
     if reference_keys and int(RlxUnqChkTyp) not in (1, 2, 3):
         return "see_the_error", int(costFun[0]), 0

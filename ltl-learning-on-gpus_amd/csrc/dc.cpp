@@ -38,7 +38,7 @@ struct Driver {
     uint64_t seed;
     uint64_t rng = 0;
     std::string names;
-    int W, A;
+    int W, A, K;  // traces, atoms, words per trace
     int32_t err = 0;
     uint64_t all_ltls = 0;
     int64_t calls = 0;
@@ -53,10 +53,10 @@ struct Driver {
             err = LTL_E_ARG;
             return nullptr;
         }
-        std::vector<uint64_t> cs(W);
-        eval_formula(*tree, W, pb->lens, pb->atoms, cs.data());
+        std::vector<uint64_t> cs((size_t)W * K);
+        eval_formula(*tree, W, K, pb->lens, pb->atoms, cs.data());
         std::vector<uint8_t> m(W);
-        for (int i = 0; i < W; ++i) m[i] = (uint8_t)(cs[i] & 1ull);
+        for (int i = 0; i < W; ++i) m[i] = (uint8_t)(cs[(size_t)i * K] & 1ull);
         return &cache.emplace(f, std::move(m)).first->second;
     }
     bool all_of(const std::string &f, const Idx &s) {
@@ -92,13 +92,15 @@ struct Driver {
     std::string solve(const Idx &pos, const Idx &neg) {
         const int P = (int)pos.size(), N = (int)neg.size(), w = P + N;
         std::vector<uint8_t> lens(w);
-        std::vector<uint64_t> atoms((size_t)A * w);
+        std::vector<uint64_t> atoms((size_t)A * w * K);
         for (int k = 0; k < w; ++k) {
             const int t = k < P ? pos[k] : neg[k - P];
             lens[k] = pb->lens[t];
-            for (int a = 0; a < A; ++a) atoms[(size_t)a * w + k] = pb->atoms[(size_t)a * W + t];
+            for (int a = 0; a < A; ++a)
+                for (int j = 0; j < K; ++j)
+                    atoms[((size_t)a * w + k) * K + j] = pb->atoms[((size_t)a * W + t) * K + j];
         }
-        ltl_problem sub{P, N, A, 0, lens.data(), atoms.data(), names.c_str()};
+        ltl_problem sub{P, N, A, K, lens.data(), atoms.data(), names.c_str()};
         ltl_result r;
         const int32_t st = ltl_learn(ctx, &sub, &r);
         ++calls;
@@ -255,6 +257,7 @@ extern "C" int32_t ltl_learn_dc(ltl_ctx *ctx, const ltl_problem *pb, int32_t win
     Driver d{ctx, pb, window, algo, seed};
     d.W = pb->num_pos + pb->num_neg;
     d.A = pb->num_atoms;
+    d.K = pb->words_per_trace > 1 ? pb->words_per_trace : 1;
     d.names = pb->names ? std::string(pb->names, pb->num_atoms)
                         : std::string("abcdefghijklmnopqrstuvwxyz", pb->num_atoms);
     Idx pos(pb->num_pos), neg(pb->num_neg);

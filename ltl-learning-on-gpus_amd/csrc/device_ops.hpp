@@ -232,5 +232,87 @@ __device__ __forceinline__ uint64_t until_at0(uint64_t l, uint64_t r) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Traces longer than 64 positions: K words per trace, word k (positions
+// 64k..64k+63) of trace i is word i*K + k of a set.  The 64-bit operators
+// apply per word; what crosses a word boundary comes from the next higher
+// word, so a trace's words are combined from the top down: X moves bit 0 of
+// word k+1 into bit 63 of word k; F holds on all of word k once a higher word
+// has a set bit; G holds in word k only while every higher word is full; U
+// holds on the top run of l's ones in word k when it holds at position 0 of
+// word k+1.  Restated from the per-position definitions
+// (code/ltli6463.cu:468, 551-556, 638-645, 739-771 for one word).
+// ---------------------------------------------------------------------------
+template <int OP, int K>
+__device__ __forceinline__ void trace_unary(const uint64_t (&x)[K], const WordAux *aux, uint64_t (&y)[K]) {
+    if constexpr (OP == LTL_NOT) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) y[k] = ~x[k] & aux[k].lm;
+    } else if constexpr (OP == LTL_NEXT) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) y[k] = (x[k] >> 1) | (k + 1 < K ? x[k + 1] << 63 : 0ull);
+    } else if constexpr (OP == LTL_FIN) {
+        uint64_t any = 0;  // all ones once a higher word has a set bit
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k) {
+            y[k] = f_fin<64>(x[k]) | (any & aux[k].lm);
+            any |= 0ull - (uint64_t)(x[k] != 0);
+        }
+    } else {
+        uint64_t all = ~0ull;  // all ones while every higher word is full
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k) {
+            const uint64_t lm = aux[k].lm;
+            y[k] = all & x[k] & ~f_fin<64>(~x[k] & lm);
+            all &= 0ull - (uint64_t)(x[k] == lm);
+        }
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void trace_until(const uint64_t (&l)[K], const uint64_t (&r)[K], uint64_t (&y)[K]) {
+    uint64_t c = 0;  // all ones when the result holds at the next word's first position
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k) {
+        y[k] = f_until<64>(l[k], r[k]) | (c & ~f_fin<64>(~l[k]));
+        c = 0ull - (y[k] & 1ull);
+    }
+}
+
+// Summary bits of trace `at` (of its summary word) from its K result words.
+template <int K>
+__device__ __forceinline__ void summarize_trace(const uint64_t (&y)[K], const WordAux *aux, int at,
+                                                uint64_t (&v)[4]) {
+    bool nz = false, fl = true;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        nz |= y[k] != 0;
+        fl &= y[k] == aux[k].lm;
+    }
+    v[0] |= (y[0] & 1ull) << at;
+    v[1] |= ((y[0] >> 1) & 1ull) << at;
+    v[2] |= (uint64_t)nz << at;
+    v[3] |= (uint64_t)fl << at;
+}
+
+// Position 0 of l U r (h0) and of r U l (h1) on one trace of K words at pl,
+// pr (word stride ws): decided in the first word where the left operand has
+// a zero or the right operand a one at or before that zero.
+__device__ __forceinline__ void until_at0_trace(const uint64_t *pl, const uint64_t *pr, int64_t ws, int K,
+                                                uint32_t &h0, uint32_t &h1) {
+    bool d0 = false, d1 = false;
+    h0 = h1 = 0;
+    for (int k = 0; k < K && !(d0 && d1); ++k) {
+        const uint64_t l = pl[(size_t)k * ws], r = pr[(size_t)k * ws], nl = ~l, nr = ~r;
+        if (!d0) {
+            if (r & (nl ^ (nl - 1))) { h0 = 1; d0 = true; }
+            else if (nl) d0 = true;
+        }
+        if (!d1) {
+            if (l & (nr ^ (nr - 1))) { h1 = 1; d1 = true; }
+            else if (nr) d1 = true;
+        }
+    }
+}
 
 }  // namespace ltl

@@ -2,7 +2,8 @@
 //
 // Grammar (code/ltli6463.cu:877-928): a letter, ~(f), X(f), F(f), G(f),
 // (f)&(g), (f)|(g), (f)U(g).  Evaluation is bit-parallel on one uint64 per
-// trace, with the same operator semantics as the device kernels.
+// trace, or K per trace for traces longer than 64 positions, with the same
+// operator semantics as the device kernels.
 #pragma once
 
 #include <cstdint>
@@ -29,6 +30,55 @@ inline uint64_t op_until(uint64_t l, uint64_t r) {
     return y;
 }
 
+// Positions 64k..64k+63 of a trace of length len.
+inline uint64_t word_mask(int len, int k) {
+    const int lo = 64 * k;
+    if (len <= lo) return 0;
+    return len - lo >= 64 ? ~0ull : ((1ull << (len - lo)) - 1ull);
+}
+
+// Operator op on one trace of K words (word k = positions 64k..64k+63); y
+// is the right operand of & | U.  What crosses a word boundary comes from
+// the next higher word: X takes its bit 0, F holds on all of word k below a
+// set bit there, G only while the higher words are full, U on the top run
+// of l's ones when it holds at the higher word's first position.
+inline void trace_op(int op, const uint64_t *x, const uint64_t *y, int K, int len, uint64_t *out) {
+    uint64_t c = 0;
+    switch (op) {
+    case 1:  // ~
+        for (int k = 0; k < K; ++k) out[k] = ~x[k] & word_mask(len, k);
+        break;
+    case 2:  // &
+        for (int k = 0; k < K; ++k) out[k] = x[k] & y[k];
+        break;
+    case 3:  // |
+        for (int k = 0; k < K; ++k) out[k] = x[k] | y[k];
+        break;
+    case 4:  // X
+        for (int k = 0; k < K; ++k) out[k] = (x[k] >> 1) | (k + 1 < K ? x[k + 1] << 63 : 0ull);
+        break;
+    case 5:  // F
+        for (int k = K - 1; k >= 0; --k) {
+            out[k] = suffix_or(x[k]) | (c ? word_mask(len, k) : 0ull);
+            c |= x[k];
+        }
+        break;
+    case 6:  // G
+        c = 1;
+        for (int k = K - 1; k >= 0; --k) {
+            const uint64_t m = word_mask(len, k);
+            out[k] = c ? x[k] & ~suffix_or(~x[k] & m) : 0ull;
+            c = c && x[k] == m;
+        }
+        break;
+    default:  // U
+        for (int k = K - 1; k >= 0; --k) {
+            out[k] = op_until(x[k], y[k]) | (c ? ~suffix_or(~x[k]) : 0ull);
+            c = out[k] & 1ull;
+        }
+    }
+}
+
 struct Node {
     int op = 0;     // LTL_* code
     int atom = -1;
@@ -41,5 +91,7 @@ std::string print_formula(const Node &n, const std::string &names);
 int formula_size(const Node &n, const uint16_t *cost);
 // cs[t] for every trace t; atoms is atom-major [A][W].
 void eval_formula(const Node &n, int W, const uint8_t *lens, const uint64_t *atoms, uint64_t *cs);
+// K words per trace: cs[t*K + k]; atoms [A][W][K].
+void eval_formula(const Node &n, int W, int K, const uint8_t *lens, const uint64_t *atoms, uint64_t *cs);
 
 }  // namespace ltl

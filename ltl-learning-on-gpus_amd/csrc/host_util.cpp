@@ -89,6 +89,26 @@ int formula_size(const Node &n, const uint16_t *cost) {
     return c;
 }
 
+void eval_formula(const Node &n, int W, int K, const uint8_t *lens, const uint64_t *atoms, uint64_t *cs) {
+    if (K <= 1) {
+        eval_formula(n, W, lens, atoms, cs);
+        return;
+    }
+    const size_t nw = (size_t)W * K;
+    if (n.op == LTL_ATOM) {
+        for (int t = 0; t < W; ++t)
+            for (int k = 0; k < K; ++k)
+                cs[(size_t)t * K + k] = atoms[(size_t)n.atom * nw + (size_t)t * K + k] & word_mask(lens[t], k);
+        return;
+    }
+    std::vector<uint64_t> a(nw), b;
+    eval_formula(*n.a, W, K, lens, atoms, a.data());
+    if (n.b) { b.resize(nw); eval_formula(*n.b, W, K, lens, atoms, b.data()); }
+    for (int t = 0; t < W; ++t)
+        trace_op(n.op, a.data() + (size_t)t * K, n.b ? b.data() + (size_t)t * K : nullptr, K, lens[t],
+                 cs + (size_t)t * K);
+}
+
 void eval_formula(const Node &n, int W, const uint8_t *lens, const uint64_t *atoms, uint64_t *cs) {
     if (n.op == LTL_ATOM) {
         for (int t = 0; t < W; ++t) cs[t] = atoms[(size_t)n.atom * W + t] & len_mask(lens[t]);
@@ -341,9 +361,12 @@ extern "C" void ltl_gen_free(ltl_gen_instance *inst) {
 // last line (letters a-z, columns in file order); the atoms are the sorted
 // letters; a '1' in column j puts letter j into the position's token.
 // ---------------------------------------------------------------------------
-extern "C" int32_t ltl_read_trace_file(const char *path, int32_t max_traces, int32_t *num_pos,
-                                       int32_t *num_neg, int32_t *num_atoms, uint8_t *lens,
-                                       uint64_t *atoms, char *names) {
+extern "C" int32_t ltl_read_trace_file_words(const char *path, int32_t max_traces, int32_t words_per_trace,
+                                             int32_t *num_pos, int32_t *num_neg, int32_t *num_atoms,
+                                             int32_t *max_len, uint8_t *lens, uint64_t *atoms, char *names) {
+    const int K = words_per_trace;
+    if (!path || max_traces < 0 || K < 1 || K > 4) return LTL_E_ARG;
+    const int cap = std::min(64 * K, 255);
     std::ifstream in(path, std::ios::binary);
     if (!in) return LTL_E_ARG;
     std::stringstream ss;
@@ -374,30 +397,43 @@ extern "C" int32_t ltl_read_trace_file(const char *path, int32_t max_traces, int
         if (inset[c]) { names[A] = (char)('a' + c); col_of[c] = A++; }
     names[A] = 0;
     *num_atoms = A;
-    int W = 0, P = -1, section = 0;
+    int W = 0, P = -1, section = 0, longest = 0;
+    const size_t row = (size_t)max_traces * K;  // atom stride while reading
     for (const auto &line : lines) {
         if (section >= 2) break;
         if (line == "---") { if (section == 0) P = W; ++section; continue; }
         if (W >= max_traces) return LTL_E_LIMIT;
-        uint64_t bits[26] = {0};
+        uint64_t bits[26][4] = {{0}};
         int pos = 0, j = 0;
         for (char c : line) {
             if (c == ';') { ++pos; j = 0; }
             else if (c == ',') continue;
             else {
-                if (c == '1' && j < (int)alpha.size() && pos < 64) bits[col_of[alpha[j] - 'a']] |= 1ull << pos;
+                if (c == '1' && j < (int)alpha.size() && pos < cap)
+                    bits[col_of[alpha[j] - 'a']][pos >> 6] |= 1ull << (pos & 63);
                 ++j;
             }
         }
-        if (pos + 1 > 64) return LTL_E_LIMIT;
+        if (pos + 1 > cap) return LTL_E_LIMIT;
         lens[W] = (uint8_t)(pos + 1);
-        for (int a = 0; a < A; ++a) atoms[(size_t)a * max_traces + W] = bits[a];
+        longest = std::max(longest, pos + 1);
+        for (int a = 0; a < A; ++a)
+            for (int k = 0; k < K; ++k) atoms[(size_t)a * row + (size_t)W * K + k] = bits[a][k];
         ++W;
     }
     if (P < 0) P = W;
     *num_pos = P; *num_neg = W - P;
-    for (int a = 1; a < A; ++a) memmove(atoms + (size_t)a * W, atoms + (size_t)a * max_traces, (size_t)W * 8);
+    if (max_len) *max_len = longest;
+    for (int a = 1; a < A; ++a)
+        memmove(atoms + (size_t)a * W * K, atoms + (size_t)a * row, (size_t)W * K * 8);
     return 0;
+}
+
+extern "C" int32_t ltl_read_trace_file(const char *path, int32_t max_traces, int32_t *num_pos,
+                                       int32_t *num_neg, int32_t *num_atoms, uint8_t *lens,
+                                       uint64_t *atoms, char *names) {
+    return ltl_read_trace_file_words(path, max_traces, 1, num_pos, num_neg, num_atoms, nullptr, lens, atoms,
+                                     names);
 }
 
 extern "C" int32_t ltl_eval_formula(const char *formula, const ltl_problem *pb, uint64_t *out_cs) {
@@ -405,7 +441,7 @@ extern "C" int32_t ltl_eval_formula(const char *formula, const ltl_problem *pb, 
     std::string names = pb->names ? std::string(pb->names, pb->num_atoms) : std::string(kLetters, pb->num_atoms);
     auto f = parse_formula(formula, names);
     if (!f) return LTL_E_ARG;
-    eval_formula(*f, pb->num_pos + pb->num_neg, pb->lens, pb->atoms, out_cs);
+    eval_formula(*f, pb->num_pos + pb->num_neg, pb->words_per_trace, pb->lens, pb->atoms, out_cs);
     return 0;
 }
 

@@ -73,6 +73,38 @@ __device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t) {
 // ---------------------------------------------------------------------------
 // gen
 // ---------------------------------------------------------------------------
+// After the last word: test the candidate(s) of this lane, then intern
+// fingerprint 1 with the candidate id as pending value (unless the chunk is
+// check-only).
+template <int OP>
+__device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, const Acc &c0, const Acc &c1,
+                                          bool otf) {
+    const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
+    const uint64_t id0 = A.id_base + (OP == LTL_UNTIL ? 2 * pair : pair);
+    if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
+    if (OP == LTL_UNTIL && c1.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
+    if (otf) return;
+    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
+    if constexpr (OP == LTL_UNTIL) {
+        A.t_slot[2 * ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
+        A.t_fp2[2 * ix] = c0.f2;
+        A.t_slot[2 * ix + 1] = tab_insert(A.tab, A.tmask, table_key(c1.f1), kPending | (id0 + 1), &A.st->probe_fail);
+        A.t_fp2[2 * ix + 1] = c1.f2;
+    } else {
+        A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
+        A.t_fp2[ix] = c0.f2;
+    }
+}
+
+__device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, const Acc &c0, bool otf) {
+    const uint64_t id0 = A.id_base + (uint64_t)t.a;
+    if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
+    if (otf) return;
+    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
+    A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
+    A.t_fp2[ix] = c0.f2;
+}
+
 template <int OP, int B, bool RX>
 __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     if (run_stopped(A)) return;
@@ -117,21 +149,7 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
         key_finish(c0, k0);
         if constexpr (OP == LTL_UNTIL) key_finish(c1, k1);
     }
-    const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
-    const uint64_t id0 = A.id_base + (OP == LTL_UNTIL ? 2 * pair : pair);
-    if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
-    if (OP == LTL_UNTIL && c1.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
-    if (otf) return;
-    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
-    if constexpr (OP == LTL_UNTIL) {
-        A.t_slot[2 * ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
-        A.t_fp2[2 * ix] = c0.f2;
-        A.t_slot[2 * ix + 1] = tab_insert(A.tab, A.tmask, table_key(c1.f1), kPending | (id0 + 1), &A.st->probe_fail);
-        A.t_fp2[2 * ix + 1] = c1.f2;
-    } else {
-        A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
-        A.t_fp2[ix] = c0.f2;
-    }
+    gen2_tail<OP>(A, t, c0, c1, otf);
 }
 
 template <int OP, int B, bool RX>
@@ -162,12 +180,64 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     for (; w < A.nw; ++w) step(w, p[(size_t)w * A.ws]);
     if (!t.valid) return;
     if constexpr (RX) key_finish(c0, k0);
-    const uint64_t id0 = A.id_base + (uint64_t)t.a;
-    if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
-    if (otf) return;
-    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
-    A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
-    A.t_fp2[ix] = c0.f2;
+    gen1_tail(A, t, c0, otf);
+}
+
+// Multi-word traces (B = 0): K words per trace, combined from the top word
+// down.  & and | are word-wise and use k_gen2<OP, 64, false>.
+template <int OP, int K>
+__global__ __launch_bounds__(256) void k_gen1m(ChunkArgs A) {
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_unary(A, t)) return;
+    const bool otf = chunk_otf(A);
+    const int64_t s = A.ls + (t.valid ? t.a : A.a0);
+    const uint64_t *p = A.store + set_base(s, A.nw);
+    const int T = A.nw / K;
+    Acc c0;
+#pragma unroll 2
+    for (int i = 0; i < T; ++i) {
+        uint64_t x[K], y[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) x[k] = p[(size_t)(i * K + k) * A.ws];
+        const WordAux *aux = A.aux + i * K;
+        trace_unary<OP, K>(x, aux, y);
+#pragma unroll
+        for (int k = 0; k < K; ++k) fold(c0, y[k], aux + k);
+    }
+    if (!t.valid) return;
+    gen1_tail(A, t, c0, otf);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_gen2m(ChunkArgs A) {  // U
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_binary(A, t)) return;
+    const bool otf = chunk_otf(A);
+    const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
+    const uint64_t *pl = A.store + set_base(sl, A.nw);
+    const uint64_t *pr = A.store + set_base(sr, A.nw);
+    const int T = A.nw / K;
+    Acc c0, c1;
+    for (int i = 0; i < T; ++i) {
+        uint64_t l[K], r[K], u0[K], u1[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            l[k] = pl[(size_t)(i * K + k) * A.ws];
+            r[k] = pr[(size_t)(i * K + k) * A.ws];
+        }
+        trace_until<K>(l, r, u0);
+        trace_until<K>(r, l, u1);
+        const WordAux *aux = A.aux + i * K;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            fold(c0, u0[k], aux + k);
+            fold(c1, u1[k], aux + k);
+        }
+    }
+    if (!t.valid) return;
+    gen2_tail<LTL_UNTIL>(A, t, c0, c1, otf);
 }
 
 // Atoms: interned with committed values (store index = atom index), so that
@@ -387,6 +457,62 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A) {
     else commit(A, pos, k, OP, (int32_t)sl, -1);
 }
 
+// Multi-word traces (B = 0): one trace (K words) per step; its summary bits
+// come from all K words.
+template <int OP, int K>
+__global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    const int64_t base = A.st->chunk_base;
+    const int64_t pos = (base & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t rel = pos - base;
+    if (rel < 0 || rel >= A.st->chunk_total) return;
+    if (pos >= A.capacity) return;
+    constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
+    const uint32_t k = A.t_list[rel];         // temp index of the candidate
+    const uint32_t ix = OP == LTL_UNTIL ? k >> 1 : k;
+    const bool swap = OP == LTL_UNTIL && (k & 1);
+    int64_t sl, sr = -1;
+    if (bin) {
+        const int64_t ti = ix / 64;
+        sl = A.ls + A.a0 + ti / A.nbs;
+        sr = A.rs + A.b0 - A.ush + (ti % A.nbs) * 64 + (ix % 64);
+    } else {
+        sl = A.ls + A.a0 - A.ush + ix;
+    }
+    const uint64_t *pl = A.store + set_base(swap ? sr : sl, A.nw);
+    const uint64_t *pr = A.store + set_base(bin ? (swap ? sl : sr) : sl, A.nw);
+    uint64_t *q = A.store + set_base(pos, A.nw);
+    uint64_t *z = A.sum + set_base(pos, 4 * A.nsw);
+    const int T = A.nw / K;
+    uint64_t v[4] = {0, 0, 0, 0};
+    for (int i = 0; i < T; ++i) {
+        uint64_t x[K], r[K], y[K];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            x[kk] = pl[(size_t)(i * K + kk) * A.ws];
+            r[kk] = bin ? pr[(size_t)(i * K + kk) * A.ws] : 0ull;
+        }
+        const WordAux *aux = A.aux + i * K;
+        if constexpr (OP == LTL_AND) {
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) y[kk] = x[kk] & r[kk];
+        } else if constexpr (OP == LTL_OR) {
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) y[kk] = x[kk] | r[kk];
+        } else if constexpr (OP == LTL_UNTIL) {
+            trace_until<K>(x, r, y);
+        } else {
+            trace_unary<OP, K>(x, aux, y);
+        }
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) __builtin_nontemporal_store(y[kk], q + (size_t)(i * K + kk) * A.ws);
+        summarize_trace<K>(y, aux, i & 63, v);
+        flush_summary<64>(z, i, T, A.nsw, A.ws, v, true);
+    }
+    if (bin) commit(A, pos, k, OP, (int32_t)(swap ? sr : sl), (int32_t)(swap ? sl : sr));
+    else commit(A, pos, k, OP, (int32_t)sl, -1);
+}
+
 // ---------------------------------------------------------------------------
 // check-only pass (no dedup, no store)
 // ---------------------------------------------------------------------------
@@ -480,6 +606,34 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     else if (m1 <= k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
 }
 
+// Multi-word traces (B = 0): position 0 of U from the first word of each
+// trace that decides it, traces in interleaved order.
+__global__ __launch_bounds__(256) void k_chkUm(ChunkArgs A) {
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_binary(A, t)) return;
+    const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
+    const uint64_t *pl = A.store + set_base(sl, A.nw);
+    const uint64_t *pr = A.store + set_base(sr, A.nw);
+    const uint32_t k = (uint32_t)A.hamming_k;
+    const int K = A.K, T = A.nw / K;
+    uint32_t m0 = t.valid ? 0u : 0x7fffffffu, m1 = m0;  // invalid lanes never accept
+    for (int j = 0; j < T; ++j) {
+        const int i = interleave(j, A.npw, T);
+        uint32_t h0, h1;
+        until_at0_trace(pl + (size_t)i * K * A.ws, pr + (size_t)i * K * A.ws, A.ws, K, h0, h1);
+        const uint32_t tg = (uint32_t)(A.aux[i * K].tgt & 1ull);
+        m0 += h0 ^ tg;
+        m1 += h1 ^ tg;
+        // Mismatch counts only grow: stop once every lane of the wave fails.
+        if ((j & 3) == 3 && !__any(m0 <= k || m1 <= k)) return;
+    }
+    if (!t.valid) return;
+    const uint64_t id0 = A.id_base + 2 * ((uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b);
+    if (m0 <= k) atomicMin(&A.st->found, (unsigned long long)id0);
+    else if (m1 <= k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
+}
+
 __global__ void k_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, int *probe_fail) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         tab_insert(tab, tmask, keys[i], (uint64_t)i, probe_fail);
@@ -516,6 +670,36 @@ static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles +
     default: hipLaunchKernelGGL((KER<OP, 64>), GRID, BLOCK, 0, S, __VA_ARGS__); break;      \
     }
 
+#define LTL_DISPATCH_K(KER, OP, K, GRID, S, ...)                                            \
+    switch (K) {                                                                            \
+    case 2: hipLaunchKernelGGL((KER<OP, 2>), GRID, dim3(256), 0, S, __VA_ARGS__); break;    \
+    case 3: hipLaunchKernelGGL((KER<OP, 3>), GRID, dim3(256), 0, S, __VA_ARGS__); break;    \
+    case 4: hipLaunchKernelGGL((KER<OP, 4>), GRID, dim3(256), 0, S, __VA_ARGS__); break;    \
+    default: return hipErrorInvalidValue;                                                   \
+    }
+
+// Multi-word traces (B = 0, a.K words per trace).
+static hipError_t launch_gen_mw(int op, const ChunkArgs &a, hipStream_t s) {
+    switch (op) {
+    case LTL_NOT: LTL_DISPATCH_K(k_gen1m, LTL_NOT, a.K, grid_tiles(a), s, a); break;
+    case LTL_NEXT: LTL_DISPATCH_K(k_gen1m, LTL_NEXT, a.K, grid_tiles(a), s, a); break;
+    case LTL_FIN: LTL_DISPATCH_K(k_gen1m, LTL_FIN, a.K, grid_tiles(a), s, a); break;
+    case LTL_GLOB: LTL_DISPATCH_K(k_gen1m, LTL_GLOB, a.K, grid_tiles(a), s, a); break;
+    case LTL_AND: hipLaunchKernelGGL((k_gen2<LTL_AND, 64, false>), grid_binary(a), dim3(256), 0, s, a); break;
+    case LTL_OR: hipLaunchKernelGGL((k_gen2<LTL_OR, 64, false>), grid_binary(a), dim3(256), 0, s, a); break;
+    case LTL_UNTIL:
+        switch (a.K) {
+        case 2: hipLaunchKernelGGL(k_gen2m<2>, grid_binary(a), dim3(256), 0, s, a); break;
+        case 3: hipLaunchKernelGGL(k_gen2m<3>, grid_binary(a), dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL(k_gen2m<4>, grid_binary(a), dim3(256), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <bool RX>
 static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s) {
     switch (op) {
@@ -532,6 +716,7 @@ static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s)
 }
 
 hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s) {
+    if (B == 0) return a.kmode ? hipErrorInvalidValue : launch_gen_mw(op, a, s);
     return a.kmode ? launch_gen_x<true>(op, B, a, s) : launch_gen_x<false>(op, B, a, s);
 }
 
@@ -540,6 +725,19 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
     // grid covers every candidate of the chunk and idle lanes exit
     const int nd = op == LTL_UNTIL ? 2 : 1;
     const dim3 g((unsigned)(((int64_t)a.n_tiles * 64 * nd + 63 + 255) / 256)), b(256);
+    if (B == 0) {
+        switch (op) {
+        case LTL_NOT: LTL_DISPATCH_K(k_emitm, LTL_NOT, a.K, g, s, a); break;
+        case LTL_NEXT: LTL_DISPATCH_K(k_emitm, LTL_NEXT, a.K, g, s, a); break;
+        case LTL_FIN: LTL_DISPATCH_K(k_emitm, LTL_FIN, a.K, g, s, a); break;
+        case LTL_GLOB: LTL_DISPATCH_K(k_emitm, LTL_GLOB, a.K, g, s, a); break;
+        case LTL_AND: LTL_DISPATCH_K(k_emitm, LTL_AND, a.K, g, s, a); break;
+        case LTL_OR: LTL_DISPATCH_K(k_emitm, LTL_OR, a.K, g, s, a); break;
+        case LTL_UNTIL: LTL_DISPATCH_K(k_emitm, LTL_UNTIL, a.K, g, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (op) {
     case LTL_NOT: LTL_DISPATCH_B(k_emit, LTL_NOT, B, g, b, s, a); break;
     case LTL_NEXT: LTL_DISPATCH_B(k_emit, LTL_NEXT, B, g, b, s, a); break;
@@ -611,6 +809,7 @@ hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s) {
     case LTL_OR: hipLaunchKernelGGL(k_chk2<LTL_OR>, grid_binary(a), b, 0, s, a); break;
     case LTL_UNTIL:
         switch (B) {
+        case 0: hipLaunchKernelGGL(k_chkUm, grid_binary(a), b, 0, s, a); break;
         case 8: hipLaunchKernelGGL(k_chkU<8>, grid_binary(a), b, 0, s, a); break;
         case 16: hipLaunchKernelGGL(k_chkU<16>, grid_binary(a), b, 0, s, a); break;
         case 32: hipLaunchKernelGGL(k_chkU<32>, grid_binary(a), b, 0, s, a); break;

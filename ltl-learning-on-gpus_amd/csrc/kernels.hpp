@@ -10,6 +10,8 @@
 // contiguous 512-byte segment, and the words of one tile follow each other
 // (a wave streams through 512-byte segments of one region, whatever the
 // capacity).  Summaries use the same layout with 4 * nsw words per set.
+// Traces longer than 64 positions (slot code B = 0): K words per trace, word
+// k (positions 64k..64k+63) of trace i is word i * K + k of the set.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -100,7 +102,8 @@ struct ChunkArgs {
     // with ush = (ls + a0) % 64; binary chunks put column b of a row at lane
     // offset b - b0 + ush within the row's stripes, ush = (rs + b0) % 64.
     // Either way the 64 lanes of a wave read the 64 sets of one store tile.
-    int32_t ush, pad2_;
+    int32_t ush;
+    int32_t K;                // words per trace (B = 0), else 1
 };
 
 // Check passes visit words alternately from the positive and the negative

@@ -81,7 +81,7 @@ struct ltl_ctx {
     DevState *h_state = nullptr;  // pinned
 
     // layout of the last call
-    int W = 0, P = 0, A = 0, nw = 0, B = 64, TPW = 1;
+    int W = 0, P = 0, A = 0, nw = 0, B = 64, TPW = 1, K = 1;  // B = 0: K words per trace
     int64_t capacity = 0;
     int64_t ws = 0;  // word stride of the word-major store (capacity rounded up)
     int64_t CH = 0;  // candidate slots per chunk
@@ -373,7 +373,11 @@ struct Prepared {
     std::vector<uint64_t> atom_sums;   // [4*nsw][A]
 };
 
+int words_per_trace(const ltl_problem *pb) { return pb->words_per_trace > 1 ? pb->words_per_trace : 1; }
+
+// Slot width B of one-word traces, or 0 for K words per trace.
 int slot_width(const ltl_problem *pb) {
+    if (words_per_trace(pb) > 1) return 0;
     int maxlen = 1;
     for (int i = 0; i < pb->num_pos + pb->num_neg; ++i) maxlen = std::max<int>(maxlen, pb->lens[i]);
     int B = 8;
@@ -381,7 +385,60 @@ int slot_width(const ltl_problem *pb) {
     return B;
 }
 
+// K words per trace (B = 0): word k of trace i is word i * K + k; only word
+// 0 of a trace takes part in the target check.
+void prepare_multiword(const ltl_problem *pb, Prepared &q) {
+    const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms, K = words_per_trace(pb);
+    q.B = 0;
+    q.TPW = 1;
+    q.nw = W * K;
+    q.nsw = (W + 63) / 64;
+    q.aux.assign(q.nw, WordAux{});
+    const uint64_t seed = 0x5eed1e55c0ffee11ull;
+    for (int w = 0; w < q.nw; ++w) {
+        WordAux &x = q.aux[w];
+        memset(&x, 0, sizeof(x));
+        const int i = w / K, k = w % K;
+        x.k1 = splitmix(seed ^ (2ull * w));
+        x.k2 = splitmix(~seed ^ (2ull * w + 1));
+        x.lm = word_mask(pb->lens[i], k);
+        x.chk = k == 0 ? 1ull : 0ull;
+        x.tgt = k == 0 && i < P ? 1ull : 0ull;
+    }
+    q.atom_words.assign((size_t)q.nw * A, 0);
+    const int nsw = q.nsw;
+    q.atom_sums.assign((size_t)4 * nsw * A, 0);
+    q.saux.assign(nsw, SumAux{0, 0});
+    for (int i = 0; i < W; ++i) {
+        q.saux[i / 64].val |= 1ull << (i % 64);
+        if (i < P) q.saux[i / 64].tgt |= 1ull << (i % 64);
+    }
+    for (int a = 0; a < A; ++a)
+        for (int i = 0; i < W; ++i) {
+            bool nz = false, fl = true;
+            uint64_t x0 = 0;
+            for (int k = 0; k < K; ++k) {
+                const uint64_t m = word_mask(pb->lens[i], k);
+                const uint64_t x = pb->atoms[((size_t)a * W + i) * K + k] & m;
+                q.atom_words[(size_t)(i * K + k) * A + a] = x;
+                if (k == 0) x0 = x;
+                nz |= x != 0;
+                fl &= x == m;
+            }
+            const uint64_t bit = 1ull << (i % 64);
+            const int j = i / 64;
+            if (x0 & 1) q.atom_sums[(size_t)(0 * nsw + j) * A + a] |= bit;
+            if (x0 & 2) q.atom_sums[(size_t)(1 * nsw + j) * A + a] |= bit;
+            if (nz) q.atom_sums[(size_t)(2 * nsw + j) * A + a] |= bit;
+            if (fl) q.atom_sums[(size_t)(3 * nsw + j) * A + a] |= bit;
+        }
+}
+
 void prepare_instance(const ltl_problem *pb, int B, Prepared &q) {
+    if (B == 0) {
+        prepare_multiword(pb, q);
+        return;
+    }
     const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms;
     q.B = B;
     q.TPW = 64 / B;
@@ -431,16 +488,18 @@ void prepare_instance(const ltl_problem *pb, int B, Prepared &q) {
 // atoms on the host before any device work, code/ltli6463.cu:1100-1136).
 // Returns 1 if res is final, 0 to continue, or a negative status.
 int32_t precheck_host(const ltl_options &o, const ltl_problem *pb, ltl_result *res, std::string &names) {
-    const int P = pb->num_pos, N = pb->num_neg, W = P + N, A = pb->num_atoms;
-    if (P < 0 || N < 0 || W < 1 || A < 1 || A > 64 || !pb->lens || !pb->atoms) return LTL_E_ARG;
+    const int P = pb->num_pos, N = pb->num_neg, W = P + N, A = pb->num_atoms, K = words_per_trace(pb);
+    if (P < 0 || N < 0 || W < 1 || A < 1 || A > 64 || !pb->lens || !pb->atoms || K > 4) return LTL_E_ARG;
     for (int i = 0; i < W; ++i)
-        if (pb->lens[i] < 1 || pb->lens[i] > 64) return LTL_E_LIMIT;
+        if (pb->lens[i] < 1 || pb->lens[i] > 64 * K) return LTL_E_LIMIT;
+    // the reference's relaxed keys are defined on one word per trace
+    if (K > 1 && o.dedup_mode != LTL_DEDUP_EXACT) return LTL_E_ARG;
     names = pb->names ? std::string(pb->names, A) : std::string();
     if (names.empty()) for (int a = 0; a < A; ++a) names += (char)('a' + a);
     res->cost = o.cost[0];
     std::vector<uint64_t> cs(W);
     for (int a = 0; a < A; ++a) {
-        for (int i = 0; i < W; ++i) cs[i] = pb->atoms[(size_t)a * W + i] & len_mask(pb->lens[i]);
+        for (int i = 0; i < W; ++i) cs[i] = pb->atoms[((size_t)a * W + i) * K] & word_mask(pb->lens[i], 0);
         if (ltl_mismatches(cs.data(), P, N) <= o.hamming_k) {
             res->status = LTL_FOUND;
             res->all_ltls = (uint64_t)a + 1;
@@ -754,12 +813,15 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     const int maxc = o.max_cost;
     const uint64_t all_ltls = (uint64_t)A;  // atoms (code/ltli6463.cu:1134)
 
-    // Layout: B-bit trace slots, TPW traces per word.
+    // Layout: B-bit trace slots, TPW traces per word; or K words per trace
+    // (B = 0) for traces longer than 64 positions.
     if (hipSetDevice(o.device) != hipSuccess) return res->status = LTL_E_HIP;
+    const int K = words_per_trace(pb);
     int B = 8;
     while (B < maxlen) B <<= 1;
-    c->B = B; c->TPW = 64 / B; c->W = W; c->P = P; c->A = A;
-    c->nw = (W + c->TPW - 1) / c->TPW;
+    if (K > 1) B = 0;
+    c->B = B; c->TPW = K > 1 ? 1 : 64 / B; c->K = K; c->W = W; c->P = P; c->A = A;
+    c->nw = K > 1 ? W * K : (W + c->TPW - 1) / c->TPW;
     c->nsw = (W + 63) / 64;
     if (int32_t e = ensure_pool(c)) return res->status = e;
     if (int32_t e = carve(c)) return res->status = e;
@@ -807,7 +869,8 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     ba.t_list = c->t_list;
     ba.m_op = c->d_op; ba.m_left = c->d_left; ba.m_right = c->d_right; ba.m_fp2 = c->d_fp2; ba.m_key = c->d_key;
     ba.sum = c->d_sum; ba.saux = c->d_saux; ba.nsw = nsw;
-    ba.npw = std::min(c->nw, (P + c->TPW - 1) / c->TPW);
+    ba.npw = K > 1 ? P : std::min(c->nw, (P + c->TPW - 1) / c->TPW);  // traces when K > 1
+    ba.K = K;
     ba.nps = std::min(nsw, (P + 63) / 64);
     // Relaxed keys (code/ltli6463.cu:57-141) only above 126 positions; below,
     // every mode is the exact packing, which the exact dedup reproduces.
@@ -1278,6 +1341,14 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
         deferred = dev;
         dev.clear();
     }
+    // traces longer than 64 positions: the whole-device path
+    for (size_t k = 0; k < dev.size();)
+        if (words_per_trace(&pbs[dev[k]]) > 1) {
+            deferred.push_back(dev[k]);
+            dev.erase(dev.begin() + (long)k);
+        } else {
+            ++k;
+        }
     if (!dev.empty()) {
         if (hipSetDevice(o.device) != hipSuccess) return LTL_E_HIP;
         const int m = (int)dev.size();
@@ -1506,8 +1577,13 @@ extern "C" int32_t ltl_store_read(ltl_ctx *c, int64_t first, int64_t count, uint
     const size_t tile_words = (size_t)c->nw * 64;
     std::vector<uint64_t> buf((size_t)(t1 - t0) * tile_words);
     HIP_TRY(hipMemcpy(buf.data(), c->d_store + (size_t)t0 * tile_words, buf.size() * 8, hipMemcpyDeviceToHost));
-    const uint64_t slot = c->B == 64 ? ~0ull : ((1ull << c->B) - 1);
-    if (cs)
+    const uint64_t slot = c->B == 64 || c->B == 0 ? ~0ull : ((1ull << c->B) - 1);
+    if (cs && c->B == 0)
+        for (int64_t k = 0; k < count; ++k) {
+            const uint64_t *w0 = buf.data() + set_base(first + k, c->nw) - (size_t)t0 * tile_words;
+            for (int w = 0; w < c->nw; ++w) cs[(size_t)k * c->nw + w] = w0[(size_t)w * 64];
+        }
+    else if (cs)
         for (int64_t k = 0; k < count; ++k) {
             const int64_t sidx = first + k;
             const uint64_t *w0 = buf.data() + set_base(sidx, c->nw) - (size_t)t0 * tile_words;

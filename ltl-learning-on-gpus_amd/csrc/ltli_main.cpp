@@ -11,6 +11,7 @@
 // Differences: errors exit with a nonzero status (the reference always exits
 // 0), traces may number more than 64, and deduplication is exact for any
 // total trace length (RlxUnqChkTyp is validated and otherwise ignored).
+#include <algorithm>
 #include <chrono>
 #include <climits>
 #include <cstdio>
@@ -68,12 +69,16 @@ int main(int argc, char **argv) {
     }
     if (bad) return 2;
 
+    // Traces of up to 255 positions (the reference stops at 63,
+    // code/ltli6463.cu:1078): read in four words per trace, then repacked to
+    // the fewest words that hold the longest trace (one word up to 64).
     const int max_traces = 1 << 16;
     std::vector<uint8_t> lens(max_traces);
-    std::vector<uint64_t> atoms((size_t)26 * max_traces);
+    std::vector<uint64_t> atoms((size_t)26 * max_traces * 4);
     char names[32];
-    int32_t P = 0, N = 0, A = 0;
-    int32_t rc = ltl_read_trace_file(argv[1], max_traces, &P, &N, &A, lens.data(), atoms.data(), names);
+    int32_t P = 0, N = 0, A = 0, longest = 0;
+    int32_t rc = ltl_read_trace_file_words(argv[1], max_traces, 4, &P, &N, &A, &longest, lens.data(), atoms.data(),
+                                           names);
     if (rc == LTL_E_ARG) {
         printf("Failed to open the input file.\n");
         return 1;
@@ -81,10 +86,13 @@ int main(int argc, char **argv) {
     if (rc == LTL_E_LIMIT) {
         printf("In this version, The input can have:\n");
         printf("1) At most %d traces.\n", max_traces);
-        printf("2) Max(len(trace)) = 64.\n");
+        printf("2) Max(len(trace)) = 255.\n");
         return 1;
     }
     const int W = P + N;
+    const int K = (std::max(longest, 1) + 63) / 64;
+    for (size_t x = 0; x < (size_t)A * W; ++x)
+        for (int k = 0; k < K; ++k) atoms[x * K + k] = atoms[x * 4 + k];
 
     ltl_options opt;
     ltl_default_options(&opt);
@@ -97,7 +105,7 @@ int main(int argc, char **argv) {
         printf("No usable GPU (error %d).\n", st);
         return 1;
     }
-    ltl_problem pb{P, N, A, 0, lens.data(), atoms.data(), names};
+    ltl_problem pb{P, N, A, K, lens.data(), atoms.data(), names};
     ltl_result res;
     const auto t0 = std::chrono::high_resolution_clock::now();
     rc = ltl_learn(ctx, &pb, &res);
@@ -112,7 +120,10 @@ int main(int argc, char **argv) {
                 std::string s;
                 for (char c : alpha) {
                     const int a = (int)(std::string(names).find(c));
-                    if (a >= 0 && ((atoms[(size_t)a * W + t] >> k) & 1)) { s += c; s += ", "; }
+                    if (a >= 0 && ((atoms[((size_t)a * W + t) * K + k / 64] >> (k % 64)) & 1)) {
+                        s += c;
+                        s += ", ";
+                    }
                 }
                 printf("{%s}\t", s.substr(0, s.size() >= 2 ? s.size() - 2 : 0).c_str());
             }

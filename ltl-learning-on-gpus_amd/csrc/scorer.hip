@@ -497,7 +497,23 @@ __global__ __launch_bounds__(64) void k_score(MlpDev M, const LBlock *blk, int n
 template <int B, bool EARLY>
 __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl, int64_t sr, uint32_t stop,
                                           bool anti, uint32_t &m0, uint32_t &m1, uint32_t &seen) {
-    if (op == LTL_UNTIL) {
+    if (B == 0 && op == LTL_UNTIL) {
+        // multi-word traces: one trace at a time, decided in its first words
+        const uint64_t *pl = A.store + set_base(sl, A.nw), *pr = A.store + set_base(sr, A.nw);
+        const int K = A.K, T = A.nw / K;
+        for (int j = 0; j < T; ++j) {
+            const int i = EARLY ? interleave(j, A.npw, T) : j;
+            uint32_t h0, h1;
+            until_at0_trace(pl + (size_t)i * K * A.ws, pr + (size_t)i * K * A.ws, A.ws, K, h0, h1);
+            const uint32_t tg = (uint32_t)(A.aux[i * K].tgt & 1ull);
+            m0 += h0 ^ tg;
+            m1 += h1 ^ tg;
+            ++seen;
+            if (EARLY && (j & 3) == 3 &&
+                !__any(m0 <= stop || m1 <= stop || (anti && (seen - m0 <= stop || seen - m1 <= stop))))
+                return;
+        }
+    } else if (op == LTL_UNTIL) {
         const uint64_t *pl = A.store + set_base(sl, A.nw), *pr = A.store + set_base(sr, A.nw);
         constexpr int UB = 4;
         int i = 0;
@@ -513,8 +529,8 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
 #pragma unroll
             for (int u = 0; u < UB; ++u) {
                 const WordAux *x = A.aux + wu[u];
-                m0 += (uint32_t)__popcll((until_at0<B>(lv[u], rv[u]) ^ x->tgt) & x->chk);
-                m1 += (uint32_t)__popcll((until_at0<B>(rv[u], lv[u]) ^ x->tgt) & x->chk);
+                m0 += (uint32_t)__popcll((until_at0<B ? B : 64>(lv[u], rv[u]) ^ x->tgt) & x->chk);
+                m1 += (uint32_t)__popcll((until_at0<B ? B : 64>(rv[u], lv[u]) ^ x->tgt) & x->chk);
                 seen += (uint32_t)__popcll(x->chk);
             }
             if (EARLY && !__any(m0 <= stop || m1 <= stop || (anti && (seen - m0 <= stop || seen - m1 <= stop))))
@@ -524,8 +540,8 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
             const int w = EARLY ? interleave(i, A.npw, A.nw) : i;
             const WordAux *x = A.aux + w;
             const uint64_t lv = pl[(size_t)w * A.ws], rv = pr[(size_t)w * A.ws];
-            m0 += (uint32_t)__popcll((until_at0<B>(lv, rv) ^ x->tgt) & x->chk);
-            m1 += (uint32_t)__popcll((until_at0<B>(rv, lv) ^ x->tgt) & x->chk);
+            m0 += (uint32_t)__popcll((until_at0<B ? B : 64>(lv, rv) ^ x->tgt) & x->chk);
+            m1 += (uint32_t)__popcll((until_at0<B ? B : 64>(rv, lv) ^ x->tgt) & x->chk);
             seen += (uint32_t)__popcll(x->chk);
         }
     } else {
@@ -643,6 +659,7 @@ hipError_t launch_sort_desc(const float *kin, float *kout, const uint32_t *vin, 
 
 #define LTL_SCORER_B(KER, B, GRID, S, ...)                                          \
     switch (B) {                                                                    \
+    case 0: hipLaunchKernelGGL(KER<0>, GRID, dim3(256), 0, S, __VA_ARGS__); break;   \
     case 8: hipLaunchKernelGGL(KER<8>, GRID, dim3(256), 0, S, __VA_ARGS__); break;   \
     case 16: hipLaunchKernelGGL(KER<16>, GRID, dim3(256), 0, S, __VA_ARGS__); break; \
     case 32: hipLaunchKernelGGL(KER<32>, GRID, dim3(256), 0, S, __VA_ARGS__); break; \
