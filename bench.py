@@ -13,7 +13,10 @@ wall time; `ms_per_step` = time to the minimal formula.  `roofline` is the
 dominant kernel's algorithmic bytes per launch over its HIP-event duration
 against the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (the
 reference enumeration restated in C, OpenMP) on a bounded sample of the same
-instance (levels up to cost 8).
+instance (levels up to cost 8).  The extras (skipped with --no-extra) time
+the learned order against the plain one (configs 2 and 5), the batched launch
+against one call per instance (config 4), and an instance of traces longer
+than 64 positions (4 words per trace).
 
     python bench.py [--gpus 1] [--steps 3] [--warmup 1]
 """
@@ -201,6 +204,51 @@ def batch_vs_loop(ltl, args):
             "trials": len(trials), "on_device": st["on_device"], "deferred": st["deferred"], "host": st["host"]}
 
 
+def long_instance(ltl, seed, P, N, A, lo, hi, planted, pool=4000):
+    """P + N traces of U{lo..hi} positions (K = ceil(hi / 64) words each),
+    per-atom bit rates U[0.3, 0.7], labelled by `planted` (product host
+    evaluator) from a seeded pool."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    K = (hi + 63) // 64
+    lens = rng.integers(lo, hi + 1, size=pool).astype(np.uint8)
+    rates = rng.uniform(0.3, 0.7, size=A)
+    bits = rng.random((A, pool, 64 * K)) < rates[:, None, None]
+    bits &= np.arange(64 * K)[None, None, :] < lens[None, :, None].astype(np.int64)
+    shifts = np.arange(64, dtype=np.uint64)
+    words = (bits.reshape(A, pool, K, 64).astype(np.uint64) << shifts).sum(axis=3, dtype=np.uint64)
+    every = ltl.Instance(pool, 0, lens, words.reshape(A, pool * K), K=K)
+    holds = (ltl.eval_formula(every, planted)[::K] & np.uint64(1)).astype(bool)
+    pi, ni = np.nonzero(holds)[0][:P], np.nonzero(~holds)[0][:N]
+    idx = np.concatenate([pi, ni])
+    return ltl.Instance(len(pi), len(ni), lens[idx], words[:, idx, :].reshape(A, len(idx) * K), K=K)
+
+
+def long_traces(ltl, args):
+    """Traces longer than 64 positions (K = 4 words per trace): 100 + 100
+    traces of U{65..250} positions, 5 atoms, config-2's planted formula."""
+    inst = long_instance(ltl, 1, 100, 100, 5, 65, 250, "(G((~(a))|(F(b))))&(~((c)U(d)))")
+    L = ltl.Learner(budget=int(args.budget_gb * 1e9), profile=1, hash_slots_log2=28, max_cost=14, neg_type=1)
+    try:
+        L.learn(inst)
+        t = time.perf_counter()
+        r = L.learn(inst)
+        dt = time.perf_counter() - t
+        stats = L.kernel_stats()
+    finally:
+        L.close()
+    dom = max(stats, key=lambda k: stats[k]["seconds"])
+    d = stats[dom]
+    return {"workload": "100+100 traces, 5 atoms, lengths U{65..250} (4 words per trace), per-atom rate "
+                        "U[0.3,0.7], planted G(a->F b)&~(c U d), unit costs, max cost 14, seed 1, plain order",
+            "words_per_trace": inst.K, "value": round((r["n_stored"] - 5) / dt, 1), "unit": "unique CS/s",
+            "seconds": round(dt, 4), "min_cost": r["cost"], "formula": r["formula"],
+            "uniques": r["n_stored"], "store_overflow": bool(r["on_the_fly"]),
+            "kernels_ms": {k: round(v["seconds"] * 1e3, 3) for k, v in stats.items() if v["launches"]},
+            "dominant": {"kernel": dom, "achieved_gbs": round(d["alg_bytes"] / d["seconds"] / 1e9, 1),
+                         "frac": round(d["alg_bytes"] / d["seconds"] / HBM_PEAK, 4)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,7 +266,12 @@ def main():
                     help="only run one 1 GiB device copy (PMC calibration)")
     ap.add_argument("--scorer-rows", type=int, default=0,
                     help="only run the scorer's forward and gradient on this many random rows")
+    ap.add_argument("--long-traces", action="store_true",
+                    help="only run the instance of traces longer than 64 positions")
     args = ap.parse_args()
+    if args.long_traces:
+        print(json.dumps(long_traces(load_package(), args)), flush=True)
+        return
     if args.calibrate_copy:
         calibrate_copy()
         return
@@ -281,6 +334,7 @@ def main():
         out["learned_vs_plain"] = learned_vs_plain(ltl, L, args)
         L.close()
         out["config4_batch"] = batch_vs_loop(ltl, args)
+        out["long_traces"] = long_traces(ltl, args)
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         v, dt, u = cpu_baseline(inst, args.cpu_cost, threads)
