@@ -263,6 +263,130 @@ __device__ __forceinline__ uint64_t until_at0(uint64_t l, uint64_t r) {
 }
 
 // ---------------------------------------------------------------------------
+// Packed layout (B = 1): whole traces packed into words, position p of a
+// trace of length L at bit o + L - 1 - p.  Per word, lm covers the traces'
+// bits and inner every bit but each trace's lowest (its last position).
+// The operators become masks and one carry chain (restated from the
+// per-position definitions; code/ltli6463.cu:203, 468, 551-556, 638-645,
+// 739-771 for one trace per slot):
+//   X f = (f << 1) & inner               (the next position is the next lower bit)
+//   l U r: Y_b = r_b | (l_b & inner_b & Y_{b-1}), the carry out of bit b of
+//          ((l & inner) | r) + r -- a trace's lowest bit has inner = 0, so no
+//          carry crosses from one trace into the next
+//   F f = true U f,  G f = ~F ~f
+// ---------------------------------------------------------------------------
+// Y_b = g_b | (p_b & Y_{b-1}): the carries into the bits of (p | g) + g are
+// sum ^ a ^ b; shifted down one they are the carries out of each bit.
+__device__ __forceinline__ uint64_t pk_chain(uint64_t p, uint64_t g) {
+    const uint64_t a = p | g;
+    unsigned c1, c2;  // carries out of bits 31 and 63
+    const uint32_t lo = __builtin_addc((uint32_t)a, (uint32_t)g, 0u, &c1);
+    const uint32_t hi = __builtin_addc((uint32_t)(a >> 32), (uint32_t)(g >> 32), c1, &c2);
+    const uint64_t s = (uint64_t)lo | ((uint64_t)hi << 32);
+    return ((s ^ a ^ g) >> 1) | ((uint64_t)c2 << 63);
+}
+
+__device__ __forceinline__ uint64_t pk_until(uint64_t l, uint64_t r, uint64_t inner) {
+    return pk_chain(l & inner, r);
+}
+
+template <int OP>
+__device__ __forceinline__ uint64_t pk_unary(uint64_t x, const WordAux *a) {
+    if constexpr (OP == LTL_NOT) return ~x & a->lm;
+    else if constexpr (OP == LTL_NEXT) return (x << 1) & a->inner;
+    else if constexpr (OP == LTL_FIN) return pk_chain(a->inner, x);
+    else return ~pk_chain(a->inner, ~x & a->lm) & a->lm;
+}
+
+// Summary bits of the traces of one packed word: positions 0 (b0) and 1
+// (b1), any position (nz = F at 0) and every position (fl = G at 0), taken
+// at the traces' position-0 bits (chk, wave-uniform) in packing order and
+// OR-ed in from bit `sbit` of the word's summary word (vl / vh: low and high
+// halves of the four summary words).  Bit positions are wave-uniform, so the
+// loop runs on the scalar unit and each bit moves with one 32-bit extract and
+// one shift-or.
+__device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint32_t (&vl)[4], uint32_t (&vh)[4]) {
+    const uint64_t inner = x->inner, lm = x->lm;
+    const uint64_t u[4] = {c, (c << 1) & inner, pk_chain(inner, c), ~pk_chain(inner, ~c & lm) & lm};
+    uint64_t m = x->chk;
+    int k = sinfo_bit(x->sinfo);
+    while (m) {
+        const int b = 63 - __builtin_clzll(m);  // traces from the top bit down
+        m ^= 1ull << b;
+        uint32_t bit[4];
+        if (b >= 32) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bit[j] = __builtin_amdgcn_ubfe((uint32_t)(u[j] >> 32), (uint32_t)(b - 32), 1u);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bit[j] = __builtin_amdgcn_ubfe((uint32_t)u[j], (uint32_t)b, 1u);
+        }
+        if (k >= 32) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) vh[j] |= bit[j] << (k - 32);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) vl[j] |= bit[j] << k;
+        }
+        ++k;
+    }
+}
+
+// Relaxed keys over the traces of one packed word (each trace back in the
+// reference's bit order, bit k = position k).
+__device__ __forceinline__ void key_fold_pk(KeyAcc &k, uint64_t c, int w, const WordAux *x, const ChunkArgs &A) {
+    if (A.kmode != 3 && w >= A.kw_end) return;
+    const uint64_t si = x->sinfo;
+    const int t0 = (int)sinfo_word(si) * 64 + sinfo_bit(si), n = sinfo_ntr(si);
+    for (int q = 0; q < n; ++q) {
+        const uint64_t rec = A.ptrace[t0 + q];
+        const int i = (int)(uint32_t)rec, off = (int)((rec >> 32) & 63), len = (int)((rec >> 40) & 127);
+        const uint64_t seg = (c >> off) & (len >= 64 ? ~0ull : ((1ull << len) - 1ull));
+        uint64_t v = __builtin_bitreverse64(seg) >> (64 - len);
+        if (A.kmode == 3) {
+            v = (v ^ (v >> 30)) * 0xbf58476d1ce4e5b9ull;
+            v = (v ^ (v >> 27)) * 0x94d049bb133111ebull;
+            v ^= v >> 31;
+            if (i < A.W / 2) k.h ^= v; else k.l ^= v;
+            continue;
+        }
+        const uint32_t info = A.kinfo[i];
+        const int cnt = (int)(info & 0xffu), j0 = (int)(info >> 8);
+        if (cnt == 0) continue;
+        if (A.kmode == 1) {
+            for (int m = 0; m < cnt; ++m)
+                if ((v >> (m * A.kstride)) & 1ull) key_put(k, 1ull, j0 + m, 1);
+        } else {
+            key_put(k, cnt >= 64 ? v : (v & ((1ull << cnt) - 1ull)), j0, cnt);
+        }
+    }
+}
+
+// Layout dispatch for the chunk kernels: B = 1 packed, else B-bit slots.
+template <int OP, int B>
+__device__ __forceinline__ uint64_t op_unary(uint64_t x, const WordAux *a) {
+    if constexpr (B == 1) return pk_unary<OP>(x, a);
+    else return apply1<OP, B>(x, a->lm);
+}
+template <int B>
+__device__ __forceinline__ uint64_t op_until(uint64_t l, uint64_t r, const WordAux *a) {
+    if constexpr (B == 1) return pk_until(l, r, a->inner);
+    else return f_until<B>(l, r);
+}
+// Position 0 of l U r wherever a trace starts (other bits are masked by the
+// caller's chk).
+template <int B>
+__device__ __forceinline__ uint64_t op_until_at0(uint64_t l, uint64_t r, const WordAux *a) {
+    if constexpr (B == 1) return pk_until(l, r, a->inner);
+    else return until_at0<(B == 0 ? 64 : B)>(l, r);
+}
+template <int B>
+__device__ __forceinline__ void op_key_fold(KeyAcc &k, uint64_t c, int w, const WordAux *a, const ChunkArgs &A) {
+    if constexpr (B == 1) key_fold_pk(k, c, w, a, A);
+    else key_fold<B>(k, c, w, A);
+}
+
+// ---------------------------------------------------------------------------
 // Traces longer than 64 positions: K words per trace, word k (positions
 // 64k..64k+63) of trace i is word i*K + k of a set.  The 64-bit operators
 // apply per word; what crosses a word boundary comes from the next higher

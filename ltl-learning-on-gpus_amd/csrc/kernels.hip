@@ -119,17 +119,17 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = A.aux + w;
         if constexpr (OP == LTL_UNTIL) {
-            const uint64_t u0 = f_until<B>(l, r), u1 = f_until<B>(r, l);
+            const uint64_t u0 = op_until<B>(l, r, x), u1 = op_until<B>(r, l, x);
             fold(c0, u0, x);
             fold(c1, u1, x);
             if constexpr (RX) {
-                key_fold<B>(k0, u0, w, A);
-                key_fold<B>(k1, u1, w, A);
+                op_key_fold<B>(k0, u0, w, x, A);
+                op_key_fold<B>(k1, u1, w, x, A);
             }
         } else {
             const uint64_t v = OP == LTL_AND ? (l & r) : (l | r);
             fold(c0, v, x);
-            if constexpr (RX) key_fold<B>(k0, v, w, A);
+            if constexpr (RX) op_key_fold<B>(k0, v, w, x, A);
         }
     };
     // Batches of UB words: all loads of a batch are issued before use, so
@@ -164,9 +164,9 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     KeyAcc k0;
     auto step = [&](int w, uint64_t v) {
         const WordAux *x = A.aux + w;
-        const uint64_t y = apply1<OP, B>(v, x->lm);
+        const uint64_t y = op_unary<OP, B>(v, x);
         fold(c0, y, x);
-        if constexpr (RX) key_fold<B>(k0, y, w, A);
+        if constexpr (RX) op_key_fold<B>(k0, y, w, x, A);
     };
     constexpr int UB = 8;
     int w = 0;
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(256) void k_gen2m(ChunkArgs A) {  // U
 // the first of equal atoms wins (code/ltli6463.cu:144-173).
 // One wave per atom, lanes over the words; the NH sums (mod 2^64) are
 // reduced across the wave.
-__global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms, int B) {
+__global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms) {
     const int a = (int)blockIdx.x, l = (int)threadIdx.x;
     const uint64_t *p = A.store + set_base(a, A.nw);
     Acc c0;
@@ -252,14 +252,7 @@ __global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms, int B) {
     for (int w = l; w < A.nw; w += 64) {
         const uint64_t v = p[(size_t)w * A.ws];
         fold(c0, v, A.aux + w);
-        if (A.kmode) {
-            switch (B) {
-            case 8: key_fold<8>(k0, v, w, A); break;
-            case 16: key_fold<16>(k0, v, w, A); break;
-            case 32: key_fold<32>(k0, v, w, A); break;
-            default: key_fold<64>(k0, v, w, A); break;
-            }
-        }
+        if (A.kmode) key_fold_pk(k0, v, w, A.aux + w, A);  // relaxed keys: packed layout only
     }
     for (int o = 1; o < 64; o <<= 1) {
         c0.f1 += __shfl_xor(c0.f1, o);
@@ -401,7 +394,10 @@ __global__ __launch_bounds__(256) void k_compact(ChunkArgs A) {
 }
 
 template <int OP, int B>
-__global__ __launch_bounds__(256) void k_emit(ChunkArgs A) {
+// `aux` is A.aux as a separate restrict parameter: the per-word records are
+// then known not to be written through the store pointer, and are read with
+// scalar loads instead of vector loads that wait behind the data loads.
+__global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__restrict__ aux) {
     if (run_stopped(A) || chunk_otf(A)) return;
     // Waves cover whole store tiles: lane l of wave v writes position
     // 64 * (chunk_base / 64 + v) + l, so every store instruction of a wave
@@ -429,16 +425,31 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A) {
     uint64_t *q = A.store + set_base(pos, A.nw);
     uint64_t *z = A.sum + set_base(pos, 4 * A.nsw);
     uint64_t v[4] = {0, 0, 0, 0};
+    uint32_t vl[4] = {0, 0, 0, 0}, vh[4] = {0, 0, 0, 0};  // packed layout
     auto step = [&](int w, uint64_t l, uint64_t r) {
-        const uint64_t lm = A.aux[w].lm;
+        const WordAux *x = aux + w;
         uint64_t c;
         if constexpr (OP == LTL_AND) c = l & r;
         else if constexpr (OP == LTL_OR) c = l | r;
-        else if constexpr (OP == LTL_UNTIL) c = swap ? f_until<B>(r, l) : f_until<B>(l, r);
-        else c = apply1<OP, B>(l, lm);
+        else if constexpr (OP == LTL_UNTIL) c = swap ? op_until<B>(r, l, x) : op_until<B>(l, r, x);
+        else c = op_unary<OP, B>(l, x);
         __builtin_nontemporal_store(c, q + (size_t)w * A.ws);  // streamed: not re-read in this chunk
-        summarize<B>(c, lm, (w % B) * TPW, v);
-        flush_summary<B>(z, w, A.nw, A.nsw, A.ws, v, true);
+        if constexpr (B == 1) {
+            // packed: the summary word closes with the last word of its group
+            summarize_pk(c, x, vl, vh);
+            const uint64_t si = x->sinfo;
+            if (sinfo_last(si)) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    __builtin_nontemporal_store((uint64_t)vl[j] | ((uint64_t)vh[j] << 32),
+                                                z + (size_t)(j * A.nsw + sinfo_word(si)) * A.ws);
+                    vl[j] = vh[j] = 0;
+                }
+            }
+        } else {
+            summarize<B>(c, x->lm, (w % B) * TPW, v);
+            flush_summary<B>(z, w, A.nw, A.nsw, A.ws, v, true);
+        }
     };
     constexpr int UB = bin ? 4 : 8;
     int w = 0;
@@ -577,8 +588,8 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     uint32_t m0 = t.valid ? 0u : 0x7fffffffu, m1 = m0;  // invalid lanes never accept
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = A.aux + w;
-        m0 += (uint32_t)__popcll((until_at0<B>(l, r) ^ x->tgt) & x->chk);
-        m1 += (uint32_t)__popcll((until_at0<B>(r, l) ^ x->tgt) & x->chk);
+        m0 += (uint32_t)__popcll((op_until_at0<B>(l, r, x) ^ x->tgt) & x->chk);
+        m1 += (uint32_t)__popcll((op_until_at0<B>(r, l, x) ^ x->tgt) & x->chk);
     };
     constexpr int UB = 4;
     int i = 0;
@@ -654,21 +665,15 @@ static dim3 grid_binary(const ChunkArgs &a) {
 }
 static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles + 3) / 4)); }
 
-#define LTL_DISPATCH_BX(KER, OP, B, RX, GRID, BLOCK, S, ...)                                      \
-    switch (B) {                                                                                    \
-    case 8: hipLaunchKernelGGL((KER<OP, 8, RX>), GRID, BLOCK, 0, S, __VA_ARGS__); break;            \
-    case 16: hipLaunchKernelGGL((KER<OP, 16, RX>), GRID, BLOCK, 0, S, __VA_ARGS__); break;          \
-    case 32: hipLaunchKernelGGL((KER<OP, 32, RX>), GRID, BLOCK, 0, S, __VA_ARGS__); break;          \
-    default: hipLaunchKernelGGL((KER<OP, 64, RX>), GRID, BLOCK, 0, S, __VA_ARGS__); break;          \
-    }
+// The chunk kernels run on the packed layout (B = 1); K words per trace
+// (B = 0) has kernels of its own.
+#define LTL_DISPATCH_BX(KER, OP, B, RX, GRID, BLOCK, S, ...)                          \
+    if ((B) != 1) return hipErrorInvalidValue;                                          \
+    hipLaunchKernelGGL((KER<OP, 1, RX>), GRID, BLOCK, 0, S, __VA_ARGS__);
 
-#define LTL_DISPATCH_B(KER, OP, B, GRID, BLOCK, S, ...)                                    \
-    switch (B) {                                                                            \
-    case 8: hipLaunchKernelGGL((KER<OP, 8>), GRID, BLOCK, 0, S, __VA_ARGS__); break;        \
-    case 16: hipLaunchKernelGGL((KER<OP, 16>), GRID, BLOCK, 0, S, __VA_ARGS__); break;      \
-    case 32: hipLaunchKernelGGL((KER<OP, 32>), GRID, BLOCK, 0, S, __VA_ARGS__); break;      \
-    default: hipLaunchKernelGGL((KER<OP, 64>), GRID, BLOCK, 0, S, __VA_ARGS__); break;      \
-    }
+#define LTL_DISPATCH_B(KER, OP, B, GRID, BLOCK, S, ...)                               \
+    if ((B) != 1) return hipErrorInvalidValue;                                          \
+    hipLaunchKernelGGL((KER<OP, 1>), GRID, BLOCK, 0, S, __VA_ARGS__);
 
 #define LTL_DISPATCH_K(KER, OP, K, GRID, S, ...)                                            \
     switch (K) {                                                                            \
@@ -739,13 +744,13 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
         return hipGetLastError();
     }
     switch (op) {
-    case LTL_NOT: LTL_DISPATCH_B(k_emit, LTL_NOT, B, g, b, s, a); break;
-    case LTL_NEXT: LTL_DISPATCH_B(k_emit, LTL_NEXT, B, g, b, s, a); break;
-    case LTL_FIN: LTL_DISPATCH_B(k_emit, LTL_FIN, B, g, b, s, a); break;
-    case LTL_GLOB: LTL_DISPATCH_B(k_emit, LTL_GLOB, B, g, b, s, a); break;
-    case LTL_AND: LTL_DISPATCH_B(k_emit, LTL_AND, B, g, b, s, a); break;
-    case LTL_OR: LTL_DISPATCH_B(k_emit, LTL_OR, B, g, b, s, a); break;
-    case LTL_UNTIL: LTL_DISPATCH_B(k_emit, LTL_UNTIL, B, g, b, s, a); break;
+    case LTL_NOT: LTL_DISPATCH_B(k_emit, LTL_NOT, B, g, b, s, a, a.aux); break;
+    case LTL_NEXT: LTL_DISPATCH_B(k_emit, LTL_NEXT, B, g, b, s, a, a.aux); break;
+    case LTL_FIN: LTL_DISPATCH_B(k_emit, LTL_FIN, B, g, b, s, a, a.aux); break;
+    case LTL_GLOB: LTL_DISPATCH_B(k_emit, LTL_GLOB, B, g, b, s, a, a.aux); break;
+    case LTL_AND: LTL_DISPATCH_B(k_emit, LTL_AND, B, g, b, s, a, a.aux); break;
+    case LTL_OR: LTL_DISPATCH_B(k_emit, LTL_OR, B, g, b, s, a, a.aux); break;
+    case LTL_UNTIL: LTL_DISPATCH_B(k_emit, LTL_UNTIL, B, g, b, s, a, a.aux); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -786,7 +791,8 @@ hipError_t launch_finish(const ChunkArgs &a, hipStream_t s) {
 
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s) {
     if (n_atoms <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_atoms, dim3((unsigned)n_atoms), dim3(64), 0, s, a, n_atoms, B);
+    (void)B;
+    hipLaunchKernelGGL(k_atoms, dim3((unsigned)n_atoms), dim3(64), 0, s, a, n_atoms);
     return hipGetLastError();
 }
 
@@ -810,10 +816,8 @@ hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s) {
     case LTL_UNTIL:
         switch (B) {
         case 0: hipLaunchKernelGGL(k_chkUm, grid_binary(a), b, 0, s, a); break;
-        case 8: hipLaunchKernelGGL(k_chkU<8>, grid_binary(a), b, 0, s, a); break;
-        case 16: hipLaunchKernelGGL(k_chkU<16>, grid_binary(a), b, 0, s, a); break;
-        case 32: hipLaunchKernelGGL(k_chkU<32>, grid_binary(a), b, 0, s, a); break;
-        default: hipLaunchKernelGGL(k_chkU<64>, grid_binary(a), b, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_chkU<1>, grid_binary(a), b, 0, s, a); break;
+        default: return hipErrorInvalidValue;
         }
         break;
     default: return hipErrorInvalidValue;

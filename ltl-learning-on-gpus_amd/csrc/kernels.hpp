@@ -12,6 +12,13 @@
 // capacity).  Summaries use the same layout with 4 * nsw words per set.
 // Traces longer than 64 positions (slot code B = 0): K words per trace, word
 // k (positions 64k..64k+63) of trace i is word i * K + k of the set.
+// Packed layout (slot code B = 1, every instance of traces of at most 64
+// positions): whole traces packed into words (first fit by decreasing length,
+// within groups of 64 consecutive traces), a trace of length L at bits
+// [o, o + L) with position p at bit o + L - 1 - p, so that "the next
+// position" is the next lower bit and the until recurrence runs upward like a
+// carry.  Summary bit t of a set belongs to the t-th trace in packing order
+// (a permutation of the traces within each group of 64).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -34,8 +41,24 @@ struct WordAux {
     uint64_t lm;      // bits inside the traces (length mask)
     uint64_t chk;     // position 0 of every valid slot
     uint64_t tgt;     // position 0 of every positive slot
-    uint64_t pad[3];
+    // packed layout only: bits whose next position is in the same trace
+    // (every trace bit but the last position's); and, in one word so that it
+    // is read with one scalar load, the summary word of the word's traces
+    // (bits 0-31), the summary bit of its first trace (32-39), its trace
+    // count (40-47) and whether it is the last word of its group of 64
+    // traces (bit 48)
+    uint64_t inner;
+    uint64_t sinfo;
+    uint64_t pad_;
 };
+
+__host__ __device__ __forceinline__ uint64_t make_sinfo(uint32_t sword, int sbit, int ntr, bool last) {
+    return (uint64_t)sword | ((uint64_t)sbit << 32) | ((uint64_t)ntr << 40) | ((uint64_t)last << 48);
+}
+__host__ __device__ __forceinline__ uint32_t sinfo_word(uint64_t i) { return (uint32_t)i; }
+__host__ __device__ __forceinline__ int sinfo_bit(uint64_t i) { return (int)((i >> 32) & 0xff); }
+__host__ __device__ __forceinline__ int sinfo_ntr(uint64_t i) { return (int)((i >> 40) & 0xff); }
+__host__ __device__ __forceinline__ bool sinfo_last(uint64_t i) { return (i >> 48) & 1; }
 
 // Per summary word j (traces 64j..64j+63): positive traces and valid traces.
 struct SumAux {
@@ -97,6 +120,7 @@ struct ChunkArgs {
     // trace (first key bit << 8) | key bits taken (modes 1, 2); the stride
     // of mode 1; words holding traces that take key bits.
     const uint32_t *kinfo;
+    const uint64_t *ptrace;   // packed layout: per packed trace, trace | bit offset << 32 | length << 40
     int32_t kmode, kstride, kw_end, W;
     // Lane shift: unary chunks put candidate a at temp index a - a0 + ush
     // with ush = (ls + a0) % 64; binary chunks put column b of a row at lane

@@ -81,7 +81,9 @@ struct ltl_ctx {
     DevState *h_state = nullptr;  // pinned
 
     // layout of the last call
-    int W = 0, P = 0, A = 0, nw = 0, B = 64, TPW = 1, K = 1;  // B = 0: K words per trace
+    int W = 0, P = 0, A = 0, nw = 0, B = 1, TPW = 1, K = 1;  // B = 1: packed, B = 0: K words per trace
+    std::vector<int> pk_word, pk_off;  // packed layout: word and bit offset of each trace
+    std::vector<uint8_t> lens;         // trace lengths of the last call
     int64_t capacity = 0;
     int64_t ws = 0;  // word stride of the word-major store (capacity rounded up)
     int64_t CH = 0;  // candidate slots per chunk
@@ -98,6 +100,7 @@ struct ltl_ctx {
     uint64_t *t_fp2 = nullptr, *t_mask = nullptr;
     uint64_t *d_stage = nullptr;  // atom words and summaries, contiguous, before the scatter
     uint32_t *d_kinfo = nullptr;  // per trace: relaxed-key bit range (dedup modes 1-3)
+    uint64_t *d_ptrace = nullptr; // packed layout: per packed trace, trace | offset << 32 | length << 40
     void *d_scan = nullptr;
     size_t scan_bytes = 0;
     std::vector<int64_t> lvl_s, lvl_e;
@@ -256,7 +259,7 @@ static int32_t carve(ltl_ctx *c) {
                    2 * align_up((size_t)ch * 4, 256) + align_up((size_t)ch * 8, 256) + align_up(tiles * 16, 256) +
                    2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) +
                    align_up((size_t)(c->nw + 4 * c->nsw) * c->A * 8, 256) + align_up((size_t)c->W * 4, 256) +
-                   10 * 256;
+                   align_up((size_t)c->W * 8, 256) + 11 * 256;
     if (fixed >= budget) return LTL_E_OOM;
     const size_t R = budget - fixed;
     // Table: load stays <= 1/2 with a full chunk pending (cap + ch <= slots/2).
@@ -299,6 +302,7 @@ static int32_t carve(ltl_ctx *c) {
     c->d_scan = take(c->scan_bytes);
     c->d_stage = (uint64_t *)take((size_t)(c->nw + 4 * c->nsw) * c->A * 8);
     c->d_kinfo = (uint32_t *)take((size_t)c->W * 4);
+    c->d_ptrace = (uint64_t *)take((size_t)c->W * 8);
     c->d_tab = (uint64_t *)take(slots * 16);
     const int64_t capr = align_up((size_t)cap, 64);
     c->ws = 64;  // word stride inside a 64-set tile (tile-major store)
@@ -371,6 +375,9 @@ struct Prepared {
     std::vector<SumAux> saux;          // [nsw]
     std::vector<uint64_t> atom_words;  // [nw][A]
     std::vector<uint64_t> atom_sums;   // [4*nsw][A]
+    // packed layout (B = 1)
+    std::vector<uint64_t> ptrace;      // per packed trace: trace | offset << 32 | length << 40
+    std::vector<int> pk_word, pk_off;  // per trace: word and bit offset
 };
 
 int words_per_trace(const ltl_problem *pb) { return pb->words_per_trace > 1 ? pb->words_per_trace : 1; }
@@ -383,6 +390,85 @@ int slot_width(const ltl_problem *pb) {
     int B = 8;
     while (B < maxlen) B <<= 1;
     return B;
+}
+
+// Packed layout (B = 1): the traces of each group of 64 consecutive traces
+// go into words by first fit in decreasing length order; the traces of a
+// word take its bits from the top down (the first at [64 - L1, 64)), each
+// with its positions in descending bit order.  The packing order (group by
+// group, word by word, top down) numbers the summary bits.
+void prepare_packed(const ltl_problem *pb, Prepared &q) {
+    const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms;
+    const uint64_t seed = 0x5eed1e55c0ffee11ull;
+    q.B = 1;
+    q.TPW = 1;
+    q.nsw = (W + 63) / 64;
+    q.aux.clear();
+    q.pk_word.assign(W, 0);
+    q.pk_off.assign(W, 0);
+    q.ptrace.assign(W, 0);
+    std::vector<int> order;  // packed index -> trace
+    order.reserve(W);
+    for (int g = 0; g < q.nsw; ++g) {
+        const int t0 = 64 * g, t1 = std::min(W, t0 + 64);
+        std::vector<int> idx;
+        for (int i = t0; i < t1; ++i) idx.push_back(i);
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return pb->lens[a] > pb->lens[b]; });
+        std::vector<std::vector<int>> bins;
+        std::vector<int> used;
+        for (int i : idx) {
+            size_t b = 0;
+            while (b < bins.size() && used[b] + pb->lens[i] > 64) ++b;
+            if (b == bins.size()) { bins.emplace_back(); used.push_back(0); }
+            bins[b].push_back(i);
+            used[b] += pb->lens[i];
+        }
+        for (size_t b = 0; b < bins.size(); ++b) {
+            const int w = (int)q.aux.size();
+            WordAux x;
+            memset(&x, 0, sizeof(x));
+            x.k1 = splitmix(seed ^ (2ull * w));
+            x.k2 = splitmix(~seed ^ (2ull * w + 1));
+            x.sinfo = make_sinfo((uint32_t)g, (int)(order.size() - (size_t)t0), (int)bins[b].size(),
+                                 b + 1 == bins.size());
+            int top = 64;
+            for (int i : bins[b]) {
+                const int L = pb->lens[i], o = top - L;
+                top = o;
+                const uint64_t m = len_mask(L) << o;
+                x.lm |= m;
+                x.inner |= m & ~(1ull << o);
+                x.chk |= 1ull << (o + L - 1);
+                if (i < P) x.tgt |= 1ull << (o + L - 1);
+                q.pk_word[i] = w;
+                q.pk_off[i] = o;
+                q.ptrace[order.size()] = (uint64_t)(uint32_t)i | ((uint64_t)o << 32) | ((uint64_t)L << 40);
+                order.push_back(i);
+            }
+            q.aux.push_back(x);
+        }
+    }
+    q.nw = (int)q.aux.size();
+    q.saux.assign(q.nsw, SumAux{0, 0});
+    for (int t = 0; t < W; ++t) {
+        q.saux[t / 64].val |= 1ull << (t % 64);
+        if (order[t] < P) q.saux[t / 64].tgt |= 1ull << (t % 64);
+    }
+    q.atom_words.assign((size_t)q.nw * A, 0);
+    const int nsw = q.nsw;
+    q.atom_sums.assign((size_t)4 * nsw * A, 0);
+    for (int a = 0; a < A; ++a)
+        for (int t = 0; t < W; ++t) {
+            const int i = order[t], L = pb->lens[i];
+            const uint64_t m = len_mask(L), x = pb->atoms[(size_t)a * W + i] & m;
+            q.atom_words[(size_t)q.pk_word[i] * A + a] |= (__builtin_bitreverse64(x) >> (64 - L)) << q.pk_off[i];
+            const uint64_t bit = 1ull << (t % 64);
+            const int j = t / 64;
+            if (x & 1) q.atom_sums[(size_t)(0 * nsw + j) * A + a] |= bit;
+            if (x & 2) q.atom_sums[(size_t)(1 * nsw + j) * A + a] |= bit;
+            if (x) q.atom_sums[(size_t)(2 * nsw + j) * A + a] |= bit;
+            if (x == m) q.atom_sums[(size_t)(3 * nsw + j) * A + a] |= bit;
+        }
 }
 
 // K words per trace (B = 0): word k of trace i is word i * K + k; only word
@@ -437,6 +523,10 @@ void prepare_multiword(const ltl_problem *pb, Prepared &q) {
 void prepare_instance(const ltl_problem *pb, int B, Prepared &q) {
     if (B == 0) {
         prepare_multiword(pb, q);
+        return;
+    }
+    if (B == 1) {
+        prepare_packed(pb, q);
         return;
     }
     const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms;
@@ -813,23 +903,24 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     const int maxc = o.max_cost;
     const uint64_t all_ltls = (uint64_t)A;  // atoms (code/ltli6463.cu:1134)
 
-    // Layout: B-bit trace slots, TPW traces per word; or K words per trace
-    // (B = 0) for traces longer than 64 positions.
+    // Layout: traces packed whole into words (B = 1), or K words per trace
+    // (B = 0) for traces longer than 64 positions.  Per-word keys and masks,
+    // summary masks, packed atoms and their summaries.
     if (hipSetDevice(o.device) != hipSuccess) return res->status = LTL_E_HIP;
+    (void)maxlen;
     const int K = words_per_trace(pb);
-    int B = 8;
-    while (B < maxlen) B <<= 1;
-    if (K > 1) B = 0;
-    c->B = B; c->TPW = K > 1 ? 1 : 64 / B; c->K = K; c->W = W; c->P = P; c->A = A;
-    c->nw = K > 1 ? W * K : (W + c->TPW - 1) / c->TPW;
-    c->nsw = (W + 63) / 64;
+    const int B = K > 1 ? 0 : 1;
+    Prepared prep;
+    prepare_instance(pb, B, prep);
+    c->B = B; c->TPW = 1; c->K = K; c->W = W; c->P = P; c->A = A;
+    c->nw = prep.nw;
+    c->nsw = prep.nsw;
+    c->pk_word.swap(prep.pk_word);
+    c->pk_off.swap(prep.pk_off);
+    c->lens.assign(pb->lens, pb->lens + W);
     if (int32_t e = ensure_pool(c)) return res->status = e;
     if (int32_t e = carve(c)) return res->status = e;
     if (c->capacity < A) return res->status = LTL_E_OOM;
-
-    // Per-word keys and masks, summary masks, packed atoms and their summaries.
-    Prepared prep;
-    prepare_instance(pb, B, prep);
     const int nsw = c->nsw;
     std::vector<WordAux> &aux = prep.aux;
     std::vector<SumAux> &saux = prep.saux;
@@ -869,9 +960,17 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     ba.t_list = c->t_list;
     ba.m_op = c->d_op; ba.m_left = c->d_left; ba.m_right = c->d_right; ba.m_fp2 = c->d_fp2; ba.m_key = c->d_key;
     ba.sum = c->d_sum; ba.saux = c->d_saux; ba.nsw = nsw;
-    ba.npw = K > 1 ? P : std::min(c->nw, (P + c->TPW - 1) / c->TPW);  // traces when K > 1
-    ba.K = K;
     ba.nps = std::min(nsw, (P + 63) / 64);
+    if (K > 1) {
+        ba.npw = P;  // traces
+    } else {
+        ba.npw = 0;  // words of the groups of positive traces
+        while (ba.npw < c->nw && (int)sinfo_word(aux[ba.npw].sinfo) < ba.nps) ++ba.npw;
+    }
+    ba.K = K;
+    ba.ptrace = c->d_ptrace;
+    if (B == 1)
+        HIP_TRY(hipMemcpyAsync(c->d_ptrace, prep.ptrace.data(), (size_t)W * 8, hipMemcpyHostToDevice, s));
     // Relaxed keys (code/ltli6463.cu:57-141) only above 126 positions; below,
     // every mode is the exact packing, which the exact dedup reproduces.
     ba.W = W;
@@ -883,22 +982,21 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         if (o.dedup_mode != LTL_DEDUP_EXACT && len_sum > 126) {
             kinfo.assign(W, 0);
             const int stride = len_sum / 126;
-            int j = 0, last = -1;
+            int j = 0;
             for (int i = 0; i < W && j < 126; ++i) {
                 const int len = pb->lens[i];
                 int take = 0;
                 if (o.dedup_mode == LTL_DEDUP_REF1) take = (len + stride - 1) / stride;
                 else if (o.dedup_mode == LTL_DEDUP_REF2) take = (126 * len + len_sum) / len_sum;
                 const int cnt = std::min(take, 126 - j);
-                if (cnt > 0) {
-                    kinfo[i] = ((uint32_t)j << 8) | (uint32_t)cnt;
-                    last = i;
-                }
+                if (cnt > 0) kinfo[i] = ((uint32_t)j << 8) | (uint32_t)cnt;
                 j += cnt;
             }
             ba.kmode = o.dedup_mode;
             ba.kstride = stride;
-            ba.kw_end = last < 0 ? 0 : last / c->TPW + 1;
+            ba.kw_end = 0;  // words holding a trace that takes key bits
+            for (int i = 0; i < W; ++i)
+                if (kinfo[i] & 0xffu) ba.kw_end = std::max(ba.kw_end, c->pk_word[i] + 1);
             HIP_TRY(hipMemcpyAsync(c->d_kinfo, kinfo.data(), (size_t)W * 4, hipMemcpyHostToDevice, s));
         }
     }
@@ -1577,18 +1675,21 @@ extern "C" int32_t ltl_store_read(ltl_ctx *c, int64_t first, int64_t count, uint
     const size_t tile_words = (size_t)c->nw * 64;
     std::vector<uint64_t> buf((size_t)(t1 - t0) * tile_words);
     HIP_TRY(hipMemcpy(buf.data(), c->d_store + (size_t)t0 * tile_words, buf.size() * 8, hipMemcpyDeviceToHost));
-    const uint64_t slot = c->B == 64 || c->B == 0 ? ~0ull : ((1ull << c->B) - 1);
-    if (cs && c->B == 0)
+    if (cs && c->B == 1)
         for (int64_t k = 0; k < count; ++k) {
+            // packed: position p of trace i at bit off + len - 1 - p of its word
             const uint64_t *w0 = buf.data() + set_base(first + k, c->nw) - (size_t)t0 * tile_words;
-            for (int w = 0; w < c->nw; ++w) cs[(size_t)k * c->nw + w] = w0[(size_t)w * 64];
+            for (int i = 0; i < c->W; ++i) {
+                const int L = c->lens[i];
+                const uint64_t seg = (w0[(size_t)c->pk_word[i] * 64] >> c->pk_off[i]) & len_mask(L);
+                cs[(size_t)k * c->W + i] = __builtin_bitreverse64(seg) >> (64 - L);
+            }
         }
     else if (cs)
         for (int64_t k = 0; k < count; ++k) {
-            const int64_t sidx = first + k;
-            const uint64_t *w0 = buf.data() + set_base(sidx, c->nw) - (size_t)t0 * tile_words;
-            for (int i = 0; i < c->W; ++i)
-                cs[(size_t)k * c->W + i] = (w0[(size_t)(i / c->TPW) * 64] >> ((i % c->TPW) * c->B)) & slot;
+            // K words per trace: the store order is the caller's
+            const uint64_t *w0 = buf.data() + set_base(first + k, c->nw) - (size_t)t0 * tile_words;
+            for (int w = 0; w < c->nw; ++w) cs[(size_t)k * c->nw + w] = w0[(size_t)w * 64];
         }
     if (op) HIP_TRY(hipMemcpy(op, c->d_op + first, (size_t)count, hipMemcpyDeviceToHost));
     if (left) HIP_TRY(hipMemcpy(left, c->d_left + first, (size_t)count * 4, hipMemcpyDeviceToHost));

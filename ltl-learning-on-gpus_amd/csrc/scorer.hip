@@ -529,8 +529,8 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
 #pragma unroll
             for (int u = 0; u < UB; ++u) {
                 const WordAux *x = A.aux + wu[u];
-                m0 += (uint32_t)__popcll((until_at0<B ? B : 64>(lv[u], rv[u]) ^ x->tgt) & x->chk);
-                m1 += (uint32_t)__popcll((until_at0<B ? B : 64>(rv[u], lv[u]) ^ x->tgt) & x->chk);
+                m0 += (uint32_t)__popcll((op_until_at0<B>(lv[u], rv[u], x) ^ x->tgt) & x->chk);
+                m1 += (uint32_t)__popcll((op_until_at0<B>(rv[u], lv[u], x) ^ x->tgt) & x->chk);
                 seen += (uint32_t)__popcll(x->chk);
             }
             if (EARLY && !__any(m0 <= stop || m1 <= stop || (anti && (seen - m0 <= stop || seen - m1 <= stop))))
@@ -540,8 +540,8 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
             const int w = EARLY ? interleave(i, A.npw, A.nw) : i;
             const WordAux *x = A.aux + w;
             const uint64_t lv = pl[(size_t)w * A.ws], rv = pr[(size_t)w * A.ws];
-            m0 += (uint32_t)__popcll((until_at0<B ? B : 64>(lv, rv) ^ x->tgt) & x->chk);
-            m1 += (uint32_t)__popcll((until_at0<B ? B : 64>(rv, lv) ^ x->tgt) & x->chk);
+            m0 += (uint32_t)__popcll((op_until_at0<B>(lv, rv, x) ^ x->tgt) & x->chk);
+            m1 += (uint32_t)__popcll((op_until_at0<B>(rv, lv, x) ^ x->tgt) & x->chk);
             seen += (uint32_t)__popcll(x->chk);
         }
     } else {
@@ -660,10 +660,8 @@ hipError_t launch_sort_desc(const float *kin, float *kout, const uint32_t *vin, 
 #define LTL_SCORER_B(KER, B, GRID, S, ...)                                          \
     switch (B) {                                                                    \
     case 0: hipLaunchKernelGGL(KER<0>, GRID, dim3(256), 0, S, __VA_ARGS__); break;   \
-    case 8: hipLaunchKernelGGL(KER<8>, GRID, dim3(256), 0, S, __VA_ARGS__); break;   \
-    case 16: hipLaunchKernelGGL(KER<16>, GRID, dim3(256), 0, S, __VA_ARGS__); break; \
-    case 32: hipLaunchKernelGGL(KER<32>, GRID, dim3(256), 0, S, __VA_ARGS__); break; \
-    default: hipLaunchKernelGGL(KER<64>, GRID, dim3(256), 0, S, __VA_ARGS__); break; \
+    case 1: hipLaunchKernelGGL(KER<1>, GRID, dim3(256), 0, S, __VA_ARGS__); break;   \
+    default: return hipErrorInvalidValue;                                           \
     }
 
 hipError_t launch_check_list(int B, const ChunkArgs &a, const LBlock *blk, int nblk, const uint32_t *ent, int64_t r0,
