@@ -36,7 +36,7 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 }
 
 __device__ __forceinline__ bool run_stopped(const ChunkArgs &A) {
-    return __hip_atomic_load(&A.st->found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kEmpty;
+    return __hip_atomic_load(&A.st->found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < A.id_end;
 }
 __device__ __forceinline__ bool chunk_otf(const ChunkArgs &A) {
     return __hip_atomic_load(&A.st->overflow_chunk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < A.chunk_idx;
@@ -181,6 +181,48 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     if (!t.valid) return;
     if constexpr (RX) key_finish(c0, k0);
     gen1_tail(A, t, c0, otf);
+}
+
+// X, F and G of one operand block in one pass over the operand words (the
+// three blocks read the same level when their costs are equal): three
+// candidates per operand, three fingerprint pairs, three inserts.  The table
+// keeps the smallest id, and X ids < F ids < G ids, so the dedup outcome is
+// the one of three consecutive passes.  Packed layout, exact keys.
+__global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t slots) {
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_unary(A, t)) return;
+    const bool otf = chunk_otf(A);
+    const int64_t s = A.ls + (t.valid ? t.a : A.a0);
+    const uint64_t *p = A.store + set_base(s, A.nw);
+    Acc c[3];
+    auto step = [&](int w, uint64_t v) {
+        const WordAux *x = A.aux + w;
+        fold(c[0], pk_unary<LTL_NEXT>(v, x), x);
+        fold(c[1], pk_unary<LTL_FIN>(v, x), x);
+        fold(c[2], pk_unary<LTL_GLOB>(v, x), x);
+    };
+    constexpr int UB = 8;
+    int w = 0;
+    for (; w + UB <= A.nw; w += UB) {
+        uint64_t v[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) v[u] = p[(size_t)(w + u) * A.ws];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) step(w + u, v[u]);
+    }
+    for (; w < A.nw; ++w) step(w, p[(size_t)w * A.ws]);
+    if (!t.valid) return;
+    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const uint64_t id = A.id_base + (uint64_t)j * (uint64_t)n + (uint64_t)t.a;
+        if (c[j].mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id);
+        if (otf) continue;
+        const size_t k = (size_t)j * (size_t)slots + ix;
+        A.t_slot[k] = tab_insert(A.tab, A.tmask, table_key(c[j].f1), kPending | id, &A.st->probe_fail);
+        A.t_fp2[k] = c[j].f2;
+    }
 }
 
 // Multi-word traces (B = 0): K words per trace, combined from the top word
@@ -717,6 +759,12 @@ static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s)
     case LTL_UNTIL: LTL_DISPATCH_BX(k_gen2, LTL_UNTIL, B, RX, grid_binary(a), dim3(256), s, a); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, hipStream_t s) {
+    if (a.kmode) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gen_xfg, grid_tiles(a), dim3(256), 0, s, a, n, slots);
     return hipGetLastError();
 }
 

@@ -128,6 +128,11 @@ struct ChunkArgs {
     // Either way the 64 lanes of a wave read the 64 sets of one store tile.
     int32_t ush;
     int32_t K;                // words per trace (B = 0), else 1
+    // The chunk's kernels stop once an accepted candidate with an id below
+    // id_end is known (all ids when id_end is kEmpty; fused X/F/G blocks set
+    // each block's end, so that the blocks before an accepted one still
+    // store their survivors, as they would one after the other).
+    uint64_t id_end = kEmpty;
 };
 
 // Check passes visit words alternately from the positive and the negative
@@ -144,6 +149,10 @@ __host__ __device__ __forceinline__ int interleave(int j, int np, int n) {
 
 // Launchers (stream-ordered, asynchronous).
 hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s);
+// X, F and G of one operand block in one pass (packed layout, exact keys):
+// block j in {X, F, G} has ids id_base + j * n and its temporaries at offset
+// j * slots (t_slot, t_fp2).
+hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, hipStream_t s);
 hipError_t launch_decide(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_finish(const ChunkArgs &a, hipStream_t s);
 // Survivor list (candidate temp index per output position), then emit with

@@ -688,12 +688,21 @@ struct Run {
         candidates += cand;
         alg_bytes += alg;
         const int gk = op == LTL_UNTIL ? K_GENU : bin ? K_GEN2 : K_GEN1;
-        const int ek = op == LTL_UNTIL ? K_EMITU : bin ? K_EMIT2 : K_EMIT1;
         HIP_TRY(timed(c, gk, alg, mn, cand, [&] { return launch_gen(op, c->B, a, c->stream); }));
         if (otf) return 0;
+        return store_chunk(op, a, cand);
+    }
+
+    // decide, scan, finish, compact and emit of a generated chunk (its
+    // temporaries at a.t_slot, a.t_fp2, ...).
+    int32_t store_chunk(int op, const ChunkArgs &a, uint64_t cand) {
+        ltl_ctx *c = this->c;
+        const bool bin = op == LTL_AND || op == LTL_OR || op == LTL_UNTIL;
+        const uint64_t cs = (uint64_t)c->nw * 8;
+        const int ek = op == LTL_UNTIL ? K_EMITU : bin ? K_EMIT2 : K_EMIT1;
         HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_decide(op, a, c->stream); }));
         HIP_TRY(timed(c, K_SCAN, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles, [&] {
-            hipError_t e = launch_scan(c->t_cnt, c->t_off, a.n_tiles, c->d_scan, c->scan_bytes, c->stream);
+            hipError_t e = launch_scan(a.t_cnt, a.t_off, a.n_tiles, c->d_scan, c->scan_bytes, c->stream);
             if (e != hipSuccess) return e;
             return launch_finish(a, c->stream);
         }));
@@ -710,6 +719,55 @@ struct Run {
             pe.alg_per = (bin ? 2 : 1) * cs + cs + sv + 17 + 8 + 4 + 12;
             pe.min_per = cs + sv + 17 + 8 + 4 + 12;
         }
+        return 0;
+    }
+
+    // X, F and G of one operand level (equal costs) in one generation pass,
+    // then the three blocks' decide ... emit one after the other, in
+    // canonical order; each block's kernels stop only for an accepted
+    // candidate up to its own end.  Falls back to three passes when the
+    // three blocks do not fit one chunk's temporaries, in check-only or
+    // counting passes, and for relaxed keys or multi-word traces.
+    int32_t unary_xfg(int64_t s, int64_t e) {
+        const int64_t n = e - s;
+        const int64_t tiles = (n + 63 + 63) / 64, slots = tiles * 64;
+        if (n <= 0 || check_only || count_only || base.kmode || c->B != 1 || 3 * slots > c->CH) {
+            int32_t r = unary(LTL_NEXT, s, e);
+            if (!r) r = unary(LTL_FIN, s, e);
+            if (!r) r = unary(LTL_GLOB, s, e);
+            return r;
+        }
+        const int ops[3] = {LTL_NEXT, LTL_FIN, LTL_GLOB};
+        for (int j = 0; j < 3; ++j) blocks.push_back({all_ltls + (uint64_t)j * n, (uint64_t)n, ops[j], s, n, 0, 1, cur_level});
+        ChunkArgs a = base;
+        a.id_base = all_ltls;
+        a.ls = s; a.rs = 0; a.nR = 1;
+        a.a0 = 0; a.a1 = n;
+        a.b0 = a.b1 = 0;
+        a.nbs = 1;
+        a.ush = (int32_t)((a.ls + a.a0) & 63);
+        a.n_tiles = (int32_t)((n + a.ush + 63) / 64);
+        a.chunk_idx = ++c->chunk_counter;
+        const uint64_t cs = (uint64_t)c->nw * 8;
+        const uint64_t alg = (uint64_t)n * (cs + 60);
+        candidates += 3 * (uint64_t)n;
+        alg_bytes += alg;
+        HIP_TRY(timed(c, K_GEN1, alg, alg, 3 * (uint64_t)n, [&] { return launch_gen_xfg(a, n, slots, c->stream); }));
+        if (!otf)
+            for (int j = 0; j < 3; ++j) {
+                ChunkArgs b = a;
+                b.id_base = all_ltls + (uint64_t)j * n;
+                b.id_end = b.id_base + (uint64_t)n;
+                if (j) b.chunk_idx = ++c->chunk_counter;
+                b.t_slot = c->t_slot + j * slots;
+                b.t_fp2 = c->t_fp2 + j * slots;
+                b.t_list = c->t_list + j * slots;
+                b.t_mask = c->t_mask + j * tiles;
+                b.t_cnt = c->t_cnt + j * tiles;
+                b.t_off = c->t_off + j * tiles;
+                if (int32_t r = store_chunk(ops[j], b, (uint64_t)n)) return r;
+            }
+        all_ltls += 3 * (uint64_t)n;
         return 0;
     }
 
@@ -1039,9 +1097,13 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             if ((r = run.binary(LTL_AND, i, cost - i - c3))) return r;
         for (int i = c1; 2 * i <= cost - c4; ++i)
             if ((r = run.binary(LTL_OR, i, cost - i - c4))) return r;
-        if (cost - c5 >= c1 && (r = run.unary(LTL_NEXT, Ls[cost - c5], Le[cost - c5]))) return r;
-        if (cost - c6 >= c1 && (r = run.unary(LTL_FIN, Ls[cost - c6], Le[cost - c6]))) return r;
-        if (cost - c7 >= c1 && (r = run.unary(LTL_GLOB, Ls[cost - c7], Le[cost - c7]))) return r;
+        if (c5 == c6 && c6 == c7) {
+            if (cost - c5 >= c1 && (r = run.unary_xfg(Ls[cost - c5], Le[cost - c5]))) return r;
+        } else {
+            if (cost - c5 >= c1 && (r = run.unary(LTL_NEXT, Ls[cost - c5], Le[cost - c5]))) return r;
+            if (cost - c6 >= c1 && (r = run.unary(LTL_FIN, Ls[cost - c6], Le[cost - c6]))) return r;
+            if (cost - c7 >= c1 && (r = run.unary(LTL_GLOB, Ls[cost - c7], Le[cost - c7]))) return r;
+        }
         for (int i = c1; 2 * i <= cost - c8; ++i)
             if ((r = run.binary(LTL_UNTIL, i, cost - i - c8))) return r;
         return 0;
