@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
             flush_summary<B>(z, w, A.nw, A.nsw, A.ws, v, true);
         }
     };
-    constexpr int UB = bin ? 4 : 8;
+    constexpr int UB = bin ? 8 : 16;
     int w = 0;
     for (; w + UB <= A.nw; w += UB) {
         uint64_t l[UB], r[UB];
