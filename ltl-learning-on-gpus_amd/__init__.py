@@ -358,7 +358,7 @@ class Learner:
             raise LtlError(f"ltl_learn_batch -> {rc}")
         return out
 
-    def set_batch_params(self, tile=512, word_block=4, workgroups=0, defer_limit=1 << 22):
+    def set_batch_params(self, tile=1024, word_block=4, workgroups=0, defer_limit=1 << 18):
         p = BatchParams(tile, word_block, workgroups, 0, defer_limit)
         rc = lib().ltl_set_batch_params(self._ctx, ctypes.byref(p))
         if rc != 0:

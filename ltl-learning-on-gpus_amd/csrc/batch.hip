@@ -117,8 +117,8 @@ __device__ void check_one(const Inst &X, int op, int64_t sl, int64_t sr, uint32_
         for (int w = 0; w < X.nw; ++w) {
             const WordAux *x = X.aux + w;
             const uint64_t l = pl[(size_t)w * X.ws], r = pr[(size_t)w * X.ws];
-            m0 += (uint32_t)__popcll((until_at0<B>(l, r) ^ x->tgt) & x->chk);
-            m1 += (uint32_t)__popcll((until_at0<B>(r, l) ^ x->tgt) & x->chk);
+            m0 += (uint32_t)__popcll((op_until_at0<B>(l, r, x) ^ x->tgt) & x->chk);
+            m1 += (uint32_t)__popcll((op_until_at0<B>(r, l, x) ^ x->tgt) & x->chk);
         }
     } else if (op == LTL_AND || op == LTL_OR) {
         const uint64_t *pl = X.sum + sl, *pr = X.sum + sr;
@@ -139,11 +139,11 @@ __device__ void check_one(const Inst &X, int op, int64_t sl, int64_t sr, uint32_
 }
 
 template <int OP, int B>
-__device__ __forceinline__ uint64_t combine(uint64_t l, uint64_t r, uint64_t lm, bool swap) {
+__device__ __forceinline__ uint64_t combine(uint64_t l, uint64_t r, const WordAux *x, bool swap) {
     if constexpr (OP == LTL_AND) return l & r;
     else if constexpr (OP == LTL_OR) return l | r;
-    else if constexpr (OP == LTL_UNTIL) return swap ? f_until<B>(r, l) : f_until<B>(l, r);
-    else return apply1<OP, B>(l, lm);
+    else if constexpr (OP == LTL_UNTIL) return swap ? op_until<B>(r, l, x) : op_until<B>(l, r, x);
+    else return op_unary<OP, B>(l, x);
 }
 
 // Fingerprints of the candidate(s) of one lane: c0 = op(l, r), c1 = r U l.
@@ -153,8 +153,8 @@ __device__ void fingerprint(const Inst &X, int64_t sl, int64_t sr, Acc &c0, Acc 
     const uint64_t *pl = X.store + sl, *pr = X.store + sr;
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = X.aux + w;
-        fold(c0, combine<OP, B>(l, r, (OP == LTL_NOT || OP == LTL_GLOB) ? x->lm : 0, false), x);
-        if constexpr (OP == LTL_UNTIL) fold(c1, f_until<B>(r, l), x);
+        fold(c0, combine<OP, B>(l, r, x, false), x);
+        if constexpr (OP == LTL_UNTIL) fold(c1, op_until<B>(r, l, x), x);
     };
     int w = 0;
     for (; w + UB <= X.nw; w += UB) {
@@ -180,12 +180,25 @@ __device__ void emit_one(const Inst &X, int64_t pos, int64_t sl, int64_t sr, boo
     const uint64_t *pl = X.store + sl, *pr = X.store + sr;
     uint64_t *q = X.store + pos, *z = X.sum + pos;
     uint64_t v[4] = {0, 0, 0, 0};
+    uint32_t vl[4] = {0, 0, 0, 0}, vh[4] = {0, 0, 0, 0};  // packed layout
     auto step = [&](int w, uint64_t l, uint64_t r) {
-        const uint64_t lm = X.aux[w].lm;
-        const uint64_t c = combine<OP, B>(l, r, lm, swap);
+        const WordAux *x = X.aux + w;
+        const uint64_t c = combine<OP, B>(l, r, x, swap);
         q[(size_t)w * X.ws] = c;
-        summarize<B>(c, lm, (w % B) * TPW, v);
-        flush_summary<B>(z, w, X.nw, X.nsw, X.ws, v, true);
+        if constexpr (B == 1) {
+            summarize_pk(c, x, vl, vh);
+            const uint64_t si = x->sinfo;
+            if (sinfo_last(si)) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    z[(size_t)(j * X.nsw + sinfo_word(si)) * X.ws] = (uint64_t)vl[j] | ((uint64_t)vh[j] << 32);
+                    vl[j] = vh[j] = 0;
+                }
+            }
+        } else {
+            summarize<B>(c, x->lm, (w % B) * TPW, v);
+            flush_summary<B>(z, w, X.nw, X.nsw, X.ws, v, true);
+        }
     };
     int w = 0;
     for (; w + UB <= X.nw; w += UB) {
@@ -505,12 +518,9 @@ static void launch_b(int tile, int word_block, const dim3 &g, const BatchArgs &a
 
 hipError_t launch_batch(int B, int tile, int word_block, int workgroups, const BatchArgs &a, hipStream_t s) {
     const dim3 g((unsigned)workgroups);
-    switch (B) {
-    case 8: launch_b<8>(tile, word_block, g, a, s); break;
-    case 16: launch_b<16>(tile, word_block, g, a, s); break;
-    case 32: launch_b<32>(tile, word_block, g, a, s); break;
-    default: launch_b<64>(tile, word_block, g, a, s); break;
-    }
+    // instances in the packed layout (traces of at most 64 positions)
+    if (B != 1) return hipErrorInvalidValue;
+    launch_b<1>(tile, word_block, g, a, s);
     return hipGetLastError();
 }
 

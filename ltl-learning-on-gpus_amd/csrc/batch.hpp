@@ -57,7 +57,7 @@ struct BatchArgs {
 
 // tile = threads per workgroup (candidates per batch step): 256, 512, 1024;
 // word_block = loads in flight per lane in the word loops: 4 or 8.
-// Instances of one slot width B per launch (G.order lists them).
+// B = 1: the packed layout of the chunk kernels (G.order lists the instances).
 hipError_t launch_batch(int B, int tile, int word_block, int workgroups, const BatchArgs &a, hipStream_t s);
 hipError_t launch_fill_regions(char *regions, uint64_t region_bytes, uint32_t tslots, int n_regions,
                                hipStream_t s);

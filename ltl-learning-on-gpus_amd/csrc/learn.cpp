@@ -111,7 +111,7 @@ struct ltl_ctx {
     std::vector<hipEvent_t> free_events;
     long long *h_counts = nullptr;  // pinned, one slot per profiled emit of a level
     int n_counts = 0;
-    ltl_batch_params batch{512, 4, 0, 0, (int64_t)1 << 22};
+    ltl_batch_params batch{1024, 4, 0, 0, (int64_t)1 << 18};
     ltl_batch_stats batch_stats{};
     int n_cu = 0;
     int64_t chunk_counter = 0;
@@ -382,15 +382,8 @@ struct Prepared {
 
 int words_per_trace(const ltl_problem *pb) { return pb->words_per_trace > 1 ? pb->words_per_trace : 1; }
 
-// Slot width B of one-word traces, or 0 for K words per trace.
-int slot_width(const ltl_problem *pb) {
-    if (words_per_trace(pb) > 1) return 0;
-    int maxlen = 1;
-    for (int i = 0; i < pb->num_pos + pb->num_neg; ++i) maxlen = std::max<int>(maxlen, pb->lens[i]);
-    int B = 8;
-    while (B < maxlen) B <<= 1;
-    return B;
-}
+// Layout code: 1 = traces packed whole into words, 0 = K words per trace.
+int slot_width(const ltl_problem *pb) { return words_per_trace(pb) > 1 ? 0 : 1; }
 
 // Packed layout (B = 1): the traces of each group of 64 consecutive traces
 // go into words by first fit in decreasing length order; the traces of a
@@ -520,58 +513,10 @@ void prepare_multiword(const ltl_problem *pb, Prepared &q) {
         }
 }
 
+// Layout code B: 1 = packed words, 0 = K words per trace.
 void prepare_instance(const ltl_problem *pb, int B, Prepared &q) {
-    if (B == 0) {
-        prepare_multiword(pb, q);
-        return;
-    }
-    if (B == 1) {
-        prepare_packed(pb, q);
-        return;
-    }
-    const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms;
-    q.B = B;
-    q.TPW = 64 / B;
-    q.nw = (W + q.TPW - 1) / q.TPW;
-    q.nsw = (W + 63) / 64;
-    q.aux.assign(q.nw, WordAux{});
-    const uint64_t seed = 0x5eed1e55c0ffee11ull;
-    for (int w = 0; w < q.nw; ++w) {
-        WordAux &x = q.aux[w];
-        memset(&x, 0, sizeof(x));
-        x.k1 = splitmix(seed ^ (2ull * w));
-        x.k2 = splitmix(~seed ^ (2ull * w + 1));
-        for (int j = 0; j < q.TPW; ++j) {
-            const int i = w * q.TPW + j;
-            if (i >= W) break;
-            x.lm |= len_mask(pb->lens[i]) << (j * B);
-            x.chk |= 1ull << (j * B);
-            if (i < P) x.tgt |= 1ull << (j * B);
-        }
-    }
-    q.atom_words.assign((size_t)q.nw * A, 0);
-    for (int a = 0; a < A; ++a)
-        for (int i = 0; i < W; ++i)
-            q.atom_words[(size_t)(i / q.TPW) * A + a] |=
-                (pb->atoms[(size_t)a * W + i] & len_mask(pb->lens[i])) << ((i % q.TPW) * B);
-    // summaries (b0, b1, nz, fl per trace, 64 traces per word) of the atoms
-    const int nsw = q.nsw;
-    q.atom_sums.assign((size_t)4 * nsw * A, 0);
-    q.saux.assign(nsw, SumAux{0, 0});
-    for (int i = 0; i < W; ++i) {
-        q.saux[i / 64].val |= 1ull << (i % 64);
-        if (i < P) q.saux[i / 64].tgt |= 1ull << (i % 64);
-    }
-    for (int a = 0; a < A; ++a)
-        for (int i = 0; i < W; ++i) {
-            const uint64_t m = len_mask(pb->lens[i]), x = pb->atoms[(size_t)a * W + i] & m;
-            const uint64_t bit = 1ull << (i % 64);
-            const int j = i / 64;
-            if (x & 1) q.atom_sums[(size_t)(0 * nsw + j) * A + a] |= bit;
-            if (x & 2) q.atom_sums[(size_t)(1 * nsw + j) * A + a] |= bit;
-            if (x) q.atom_sums[(size_t)(2 * nsw + j) * A + a] |= bit;
-            if (x == m) q.atom_sums[(size_t)(3 * nsw + j) * A + a] |= bit;
-        }
+    if (B == 0) prepare_multiword(pb, q);
+    else prepare_packed(pb, q);
 }
 
 // Validate an instance; on an accepted atom fill *res (the reference checks
@@ -1537,8 +1482,8 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
             I.off_sums = put(q.atom_sums.data(), q.atom_sums.size() * 8);
             work[k] = (int64_t)I.W * I.A * I.A;
         }
-        // one launch per slot width; within it, largest instances first so
-        // that they do not form the tail
+        // largest instances first so that they do not form the tail (one
+        // launch per layout code; every instance here is packed)
         std::vector<int32_t> order(m);
         for (int k = 0; k < m; ++k) order[k] = k;
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -1660,7 +1605,7 @@ extern "C" int32_t ltl_autotune_batch(ltl_ctx *c, const ltl_problem *pbs, int32_
     std::vector<ltl_batch_trial> all;
     for (int tile : {256, 512, 1024})
         for (int wb : {4, 8})
-            for (int64_t lim : {(int64_t)1 << 20, (int64_t)1 << 22}) {
+            for (int64_t lim : {(int64_t)1 << 16, (int64_t)1 << 18, (int64_t)1 << 20}) {
                 ltl_batch_params p{tile, wb, 0, 0, lim};
                 ltl_batch_trial t{};
                 if (int32_t e = run(p, &t)) return e;
