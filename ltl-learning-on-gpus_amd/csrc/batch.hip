@@ -109,7 +109,6 @@ __device__ __forceinline__ uint64_t block_size(int op, int ls, int le, int rs, i
 
 // Check-only test of one candidate pair / operand.  Returns mismatches of
 // the first (m0) and, for U, the swapped (m1) candidate.
-template <int B>
 __device__ void check_one(const Inst &X, int op, int64_t sl, int64_t sr, uint32_t &m0, uint32_t &m1) {
     m0 = m1 = 0;
     if (op == LTL_UNTIL) {
@@ -117,8 +116,8 @@ __device__ void check_one(const Inst &X, int op, int64_t sl, int64_t sr, uint32_
         for (int w = 0; w < X.nw; ++w) {
             const WordAux *x = X.aux + w;
             const uint64_t l = pl[(size_t)w * X.ws], r = pr[(size_t)w * X.ws];
-            m0 += (uint32_t)__popcll((op_until_at0<B>(l, r, x) ^ x->tgt) & x->chk);
-            m1 += (uint32_t)__popcll((op_until_at0<B>(r, l, x) ^ x->tgt) & x->chk);
+            m0 += (uint32_t)__popcll((pk_until(l, r, x->inner) ^ x->tgt) & x->chk);
+            m1 += (uint32_t)__popcll((pk_until(r, l, x->inner) ^ x->tgt) & x->chk);
         }
     } else if (op == LTL_AND || op == LTL_OR) {
         const uint64_t *pl = X.sum + sl, *pr = X.sum + sr;
@@ -138,23 +137,23 @@ __device__ void check_one(const Inst &X, int op, int64_t sl, int64_t sr, uint32_
     }
 }
 
-template <int OP, int B>
+template <int OP>
 __device__ __forceinline__ uint64_t combine(uint64_t l, uint64_t r, const WordAux *x, bool swap) {
     if constexpr (OP == LTL_AND) return l & r;
     else if constexpr (OP == LTL_OR) return l | r;
-    else if constexpr (OP == LTL_UNTIL) return swap ? op_until<B>(r, l, x) : op_until<B>(l, r, x);
-    else return op_unary<OP, B>(l, x);
+    else if constexpr (OP == LTL_UNTIL) return swap ? pk_until(r, l, x->inner) : pk_until(l, r, x->inner);
+    else return pk_unary<OP>(l, x);
 }
 
 // Fingerprints of the candidate(s) of one lane: c0 = op(l, r), c1 = r U l.
-template <int OP, int B, int UB>
+template <int OP, int UB>
 __device__ void fingerprint(const Inst &X, int64_t sl, int64_t sr, Acc &c0, Acc &c1) {
     constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
     const uint64_t *pl = X.store + sl, *pr = X.store + sr;
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = X.aux + w;
-        fold(c0, combine<OP, B>(l, r, x, false), x);
-        if constexpr (OP == LTL_UNTIL) fold(c1, op_until<B>(r, l, x), x);
+        fold(c0, combine<OP>(l, r, x, false), x);
+        if constexpr (OP == LTL_UNTIL) fold(c1, pk_until(r, l, x->inner), x);
     };
     int w = 0;
     for (; w + UB <= X.nw; w += UB) {
@@ -172,32 +171,26 @@ __device__ void fingerprint(const Inst &X, int64_t sl, int64_t sr, Acc &c0, Acc 
 
 // Write the survivor at pos: set words, summaries, parent indices,
 // fingerprint 2, table slot; commit the table value to pos.
-template <int OP, int B, int UB>
+template <int OP, int UB>
 __device__ void emit_one(const Inst &X, int64_t pos, int64_t sl, int64_t sr, bool swap, uint32_t slot,
                          uint64_t f2) {
     constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
-    constexpr int TPW = 64 / B;
     const uint64_t *pl = X.store + sl, *pr = X.store + sr;
     uint64_t *q = X.store + pos, *z = X.sum + pos;
-    uint64_t v[4] = {0, 0, 0, 0};
-    uint32_t vl[4] = {0, 0, 0, 0}, vh[4] = {0, 0, 0, 0};  // packed layout
+    uint32_t vl[4] = {0, 0, 0, 0}, vh[4] = {0, 0, 0, 0};
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = X.aux + w;
-        const uint64_t c = combine<OP, B>(l, r, x, swap);
+        const uint64_t c = combine<OP>(l, r, x, swap);
         q[(size_t)w * X.ws] = c;
-        if constexpr (B == 1) {
-            summarize_pk(c, x, vl, vh);
-            const uint64_t si = x->sinfo;
-            if (sinfo_last(si)) {
+        // the summary word closes with the last word of its group
+        summarize_pk(c, x, vl, vh);
+        const uint64_t si = x->sinfo;
+        if (sinfo_last(si)) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    z[(size_t)(j * X.nsw + sinfo_word(si)) * X.ws] = (uint64_t)vl[j] | ((uint64_t)vh[j] << 32);
-                    vl[j] = vh[j] = 0;
-                }
+            for (int j = 0; j < 4; ++j) {
+                z[(size_t)(j * X.nsw + sinfo_word(si)) * X.ws] = (uint64_t)vl[j] | ((uint64_t)vh[j] << 32);
+                vl[j] = vh[j] = 0;
             }
-        } else {
-            summarize<B>(c, x->lm, (w % B) * TPW, v);
-            flush_summary<B>(z, w, X.nw, X.nsw, X.ws, v, true);
         }
     };
     int w = 0;
@@ -222,7 +215,7 @@ __device__ void emit_one(const Inst &X, int64_t pos, int64_t sl, int64_t sr, boo
 
 // Generate one block; returns true if the instance must stop (deferred or
 // error; S.coll / S.pfail / overflow).
-template <int OP, int B, int T, int UB>
+template <int OP, int T, int UB>
 __device__ bool gen_block(const Inst &X, Shared<T> &S, int ls, int le, int rs, int re, uint64_t base_id,
                           bool *overflow) {
     constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
@@ -240,7 +233,7 @@ __device__ bool gen_block(const Inst &X, Shared<T> &S, int ls, int le, int rs, i
         uint64_t f20 = 0, f21 = 0;
         if (valid) {
             Acc c0, c1;
-            fingerprint<OP, B, UB>(X, sl, sr, c0, c1);
+            fingerprint<OP, UB>(X, sl, sr, c0, c1);
             slot0 = tab_insert(X.tab, X.tmask, table_key(c0.f1), kPending | id0, &S.pfail);
             f20 = c0.f2;
             S.fp2[2 * threadIdx.x] = f20;
@@ -272,8 +265,8 @@ __device__ bool gen_block(const Inst &X, Shared<T> &S, int ls, int le, int rs, i
         int total = 0;
         const int off = wg_scan<T>(S, (int)s0 + (int)s1, &total);
         if (S.n + total > X.cap) { *overflow = true; return true; }
-        if (s0) emit_one<OP, B, UB>(X, S.n + off, sl, sr, false, slot0, f20);
-        if (s1) emit_one<OP, B, UB>(X, S.n + off + (s0 ? 1 : 0), sl, sr, true, slot1, f21);
+        if (s0) emit_one<OP, UB>(X, S.n + off, sl, sr, false, slot0, f20);
+        if (s1) emit_one<OP, UB>(X, S.n + off + (s0 ? 1 : 0), sl, sr, true, slot1, f21);
         __syncthreads();
         if (threadIdx.x == 0) S.n += total;
         __syncthreads();
@@ -282,21 +275,21 @@ __device__ bool gen_block(const Inst &X, Shared<T> &S, int ls, int le, int rs, i
     return false;
 }
 
-template <int B, int T, int UB>
+template <int T, int UB>
 __device__ bool gen_any(int op, const Inst &X, Shared<T> &S, int ls, int le, int rs, int re, uint64_t base_id,
                         bool *overflow) {
     switch (op) {
-    case LTL_NOT: return gen_block<LTL_NOT, B, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
-    case LTL_AND: return gen_block<LTL_AND, B, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
-    case LTL_OR: return gen_block<LTL_OR, B, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
-    case LTL_NEXT: return gen_block<LTL_NEXT, B, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
-    case LTL_FIN: return gen_block<LTL_FIN, B, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
-    case LTL_GLOB: return gen_block<LTL_GLOB, B, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
-    default: return gen_block<LTL_UNTIL, B, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
+    case LTL_NOT: return gen_block<LTL_NOT, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
+    case LTL_AND: return gen_block<LTL_AND, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
+    case LTL_OR: return gen_block<LTL_OR, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
+    case LTL_NEXT: return gen_block<LTL_NEXT, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
+    case LTL_FIN: return gen_block<LTL_FIN, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
+    case LTL_GLOB: return gen_block<LTL_GLOB, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
+    default: return gen_block<LTL_UNTIL, T, UB>(X, S, ls, le, rs, re, base_id, overflow);
     }
 }
 
-template <int B, int T, int UB>
+template <int T, int UB>
 __device__ void run_instance(const BatchArgs &G, const BatchInst &I, BatchOut &O, Shared<T> &S, char *region) {
     Inst X;
     X.W = I.W; X.P = I.P; X.A = I.A; X.nw = I.nw; X.nsw = I.nsw;
@@ -379,7 +372,7 @@ __device__ void run_instance(const BatchArgs &G, const BatchInst &I, BatchOut &O
                     if (t < npairs) {
                         const int64_t sl = ls + t / nR, sr = bin ? rs + t % nR : sl;
                         uint32_t m0, m1;
-                        check_one<B>(X, op, sl, sr, m0, m1);
+                        check_one(X, op, sl, sr, m0, m1);
                         const uint64_t id = base + (uint64_t)(op == LTL_UNTIL ? 2 * t : t);
                         if (m0 <= (uint32_t)X.hk) atomicMin(&S.found, (unsigned long long)id);
                         else if (op == LTL_UNTIL && m1 <= (uint32_t)X.hk) atomicMin(&S.found, (unsigned long long)(id + 1));
@@ -410,7 +403,7 @@ __device__ void run_instance(const BatchArgs &G, const BatchInst &I, BatchOut &O
             // generate the level
             bool overflow = false;
             const bool stop = level_blocks(X, S.lvl_s, S.lvl_e, cost, [&](int op, int ls, int le, int rs, int re) {
-                const bool r = gen_any<B, T, UB>(op, X, S, ls, le, rs, re, all_ltls, &overflow);
+                const bool r = gen_any<T, UB>(op, X, S, ls, le, rs, re, all_ltls, &overflow);
                 all_ltls += block_size(op, ls, le, rs, re);
                 return r;
             });
@@ -476,7 +469,7 @@ __device__ void run_instance(const BatchArgs &G, const BatchInst &I, BatchOut &O
     __syncthreads();
 }
 
-template <int B, int T, int UB>
+template <int T, int UB>
 __global__ __launch_bounds__(T) void k_batch(BatchArgs G) {
     __shared__ Shared<T> S;
     char *region = G.regions + (size_t)blockIdx.x * G.region_bytes;
@@ -487,7 +480,7 @@ __global__ __launch_bounds__(T) void k_batch(BatchArgs G) {
         __syncthreads();
         if (q >= G.n_inst) return;
         const int i = G.order[q];
-        run_instance<B, T, UB>(G, G.inst[i], G.out[i], S, region);
+        run_instance<T, UB>(G, G.inst[i], G.out[i], S, region);
     }
 }
 
@@ -501,9 +494,8 @@ __global__ void k_fill_regions(char *regions, uint64_t region_bytes, uint32_t ts
 
 }  // namespace
 
-template <int B>
 static void launch_b(int tile, int word_block, const dim3 &g, const BatchArgs &a, hipStream_t s) {
-#define LTL_BATCH(T, UB) hipLaunchKernelGGL((k_batch<B, T, UB>), g, dim3(T), 0, s, a)
+#define LTL_BATCH(T, UB) hipLaunchKernelGGL((k_batch<T, UB>), g, dim3(T), 0, s, a)
     if (word_block == 8) {
         if (tile == 256) LTL_BATCH(256, 8);
         else if (tile == 512) LTL_BATCH(512, 8);
@@ -520,7 +512,7 @@ hipError_t launch_batch(int B, int tile, int word_block, int workgroups, const B
     const dim3 g((unsigned)workgroups);
     // instances in the packed layout (traces of at most 64 positions)
     if (B != 1) return hipErrorInvalidValue;
-    launch_b<1>(tile, word_block, g, a, s);
+    launch_b(tile, word_block, g, a, s);
     return hipGetLastError();
 }
 

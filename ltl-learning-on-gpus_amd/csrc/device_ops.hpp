@@ -2,8 +2,10 @@
 // dedup-table insert shared by the chunk kernels and the batched kernel.
 //
 // Operators follow code/ltli6463.cu:203, 295, 385, 468, 551-556, 638-645,
-// 739-771, applied to B-bit trace slots: every shift is masked so that no bit
-// crosses a slot (bits at or above a trace's length stay zero).
+// 739-771, restated for the two layouts: traces packed whole into words (a
+// mask per word keeps every shift and carry inside its trace; bits outside
+// the traces stay zero), and K words per trace for traces longer than 64
+// positions.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,23 +17,10 @@
 namespace ltl {
 
 
-template <int B>
-__host__ __device__ constexpr uint64_t rep(uint64_t per) {
-    uint64_t m = 0;
-    for (int j = 0; j < 64 / B; ++j) m |= per << (j * B);
-    return m;
-}
-
-// All bits of each B-bit slot except its top s bits (0 < s < B).
-template <int B>
-__host__ __device__ constexpr uint64_t keep(int s) {
-    return B == 64 ? (~0ull >> s) : rep<B>((1ull << (B - s)) - 1ull);
-}
-
-template <int B>
-__device__ __forceinline__ uint64_t f_fin(uint64_t x) {
+// Suffix OR of a 64-bit word: bit p set iff some bit at or above p is.
+__device__ __forceinline__ uint64_t suffix_or64(uint64_t x) {
 #pragma unroll
-    for (int s = 1; s < B; s <<= 1) x |= (x >> s) & keep<B>(s);
+    for (int s = 1; s < 64; s <<= 1) x |= x >> s;
     return x;
 }
 
@@ -47,41 +36,6 @@ __device__ __forceinline__ uint64_t until_carry64(uint64_t l, uint64_t r, uint64
     const unsigned __int128 t = (unsigned __int128)a + R + cin;
     const uint64_t s = (uint64_t)t, cout = (uint64_t)(t >> 64);
     return __builtin_bitreverse64(((s ^ a ^ R) >> 1) | (cout << 63));
-}
-
-// The same on one 32-bit slot (the carry out of bit 31 is bit 32 of the
-// 64-bit sum, so it lands in bit 31 after the shift).
-__device__ __forceinline__ uint32_t until_carry32(uint32_t l, uint32_t r) {
-    const uint32_t L = __builtin_bitreverse32(l), R = __builtin_bitreverse32(r), a = L | R;
-    const uint64_t t = (uint64_t)a + R;
-    return __builtin_bitreverse32((uint32_t)((t ^ a ^ R) >> 1));
-}
-
-template <int B>
-__device__ __forceinline__ uint64_t f_until(uint64_t l, uint64_t r) {
-    if constexpr (B == 64) {
-        return until_carry64(l, r, 0);
-    } else if constexpr (B == 32) {
-        return (uint64_t)until_carry32((uint32_t)l, (uint32_t)r) |
-               ((uint64_t)until_carry32((uint32_t)(l >> 32), (uint32_t)(r >> 32)) << 32);
-    } else {
-        // narrow slots: the doubling prefix (3 or 4 steps), masked per slot
-        uint64_t run = l, y = r;
-#pragma unroll
-        for (int s = 1; s < B; s <<= 1) {
-            y |= run & ((y >> s) & keep<B>(s));
-            run &= (run >> s) & keep<B>(s);
-        }
-        return y;
-    }
-}
-
-template <int OP, int B>
-__device__ __forceinline__ uint64_t apply1(uint64_t x, uint64_t lm) {
-    if constexpr (OP == LTL_NOT) return ~x & lm;
-    else if constexpr (OP == LTL_NEXT) return (x >> 1) & keep<B>(1);
-    else if constexpr (OP == LTL_FIN) return f_fin<B>(x);
-    else return x & ~f_fin<B>(~x & lm);
 }
 
 // NH compression step (Black et al., UMAC, CRYPTO 1999): the two 32-bit
@@ -127,41 +81,6 @@ __device__ __forceinline__ void key_put(KeyAcc &k, uint64_t v, int j0, int cnt) 
     }
 }
 
-template <int B>
-__device__ __forceinline__ void key_fold(KeyAcc &k, uint64_t c, int w, const ChunkArgs &A) {
-    constexpr int TPW = 64 / B;
-    constexpr uint64_t SM = B == 64 ? ~0ull : ((1ull << B) - 1ull);
-    if (A.kmode == 3) {
-#pragma unroll
-        for (int t = 0; t < TPW; ++t) {
-            const int i = w * TPW + t;
-            if (i >= A.W) break;
-            uint64_t x = (c >> (t * B)) & SM;
-            x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
-            x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
-            x ^= x >> 31;
-            if (i < A.W / 2) k.h ^= x; else k.l ^= x;
-        }
-        return;
-    }
-    if (w >= A.kw_end) return;
-#pragma unroll
-    for (int t = 0; t < TPW; ++t) {
-        const int i = w * TPW + t;
-        if (i >= A.W) break;
-        const uint32_t info = A.kinfo[i];
-        const int cnt = (int)(info & 0xffu), j0 = (int)(info >> 8);
-        if (cnt == 0) continue;
-        const uint64_t x = (c >> (t * B)) & SM;
-        if (A.kmode == 1) {
-            for (int m = 0; m < cnt; ++m)
-                if ((x >> (m * A.kstride)) & 1ull) key_put(k, 1ull, j0 + m, 1);
-        } else {
-            key_put(k, cnt >= 64 ? x : (x & ((1ull << cnt) - 1ull)), j0, cnt);
-        }
-    }
-}
-
 // Fingerprints of a relaxed key: equal keys give equal fingerprints; two
 // different keys share fingerprint 1 with probability 2^-64 (detected as a
 // collision through fingerprint 2).
@@ -175,35 +94,13 @@ __device__ __forceinline__ void key_finish(Acc &a, const KeyAcc &k) {
     a.f2 = mix64(k.l ^ mix64(k.h + 0xd1b54a32d192ed03ull));
 }
 
-// Per-trace summaries of one packed word, OR-ed into the summary words at
-// bit offset sh: position 0 (b0), position 1 (b1), any position (nz) and
-// every position inside the trace (fl).  They decide the target check of
-// ~, X, F, G, & and | candidates without reading the full sets.
-template <int B>
-__device__ __forceinline__ void summarize(uint64_t c, uint64_t lm, int sh, uint64_t (&v)[4]) {
-    constexpr int TPW = 64 / B;
-    constexpr uint64_t SM = B == 64 ? ~0ull : ((1ull << B) - 1ull);
+// Flush the summary words of a set when trace i (of n) closes summary word
+// i / 64, and clear them.
+__device__ __forceinline__ void flush_summary(uint64_t *z, int i, int n, int nsw, int64_t ws, uint64_t (&v)[4]) {
+    if ((i % 64) != 63 && i != n - 1) return;
+    const int j = i / 64;
 #pragma unroll
-    for (int j = 0; j < TPW; ++j) {
-        const uint64_t x = (c >> (j * B)) & SM, m = (lm >> (j * B)) & SM;
-        const int at = sh + j;
-        v[0] |= (x & 1ull) << at;
-        v[1] |= ((x >> 1) & 1ull) << at;
-        v[2] |= (uint64_t)(x != 0) << at;
-        v[3] |= (uint64_t)(x == m) << at;
-    }
-}
-
-// Flush the summary words of set `pos` when word w closes summary word w/B.
-template <int B>
-__device__ __forceinline__ void flush_summary(uint64_t *z, int w, int nw, int nsw, int64_t ws, uint64_t (&v)[4],
-                                              bool on) {
-    if ((w % B) != B - 1 && w != nw - 1) return;
-    const int j = w / B;
-    if (on) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(v[k], z + (size_t)(k * nsw + j) * ws);
-    }
+    for (int k = 0; k < 4; ++k) __builtin_nontemporal_store(v[k], z + (size_t)(k * nsw + j) * ws);
     v[0] = v[1] = v[2] = v[3] = 0;
 }
 
@@ -239,27 +136,6 @@ __device__ inline uint32_t tab_insert(uint64_t *tab, uint32_t tmask, uint64_t ke
     }
     *probe_fail = 1;
     return 0;
-}
-
-// U: position 0 of l U r holds iff r has a set bit at or below the first
-// position where l is false: with n = ~l, n ^ (n - 1) masks the positions up
-// to and including n's lowest set bit.  Per slot (the subtraction must not
-// borrow across slots), 32-bit halves for B = 32, the whole word for B = 64.
-// Bits at or above the trace length are zero in l, so n's lowest set bit is
-// at most the length; r has no bits there.
-template <int B>
-__device__ __forceinline__ uint64_t until_at0(uint64_t l, uint64_t r) {
-    if constexpr (B == 64) {
-        const uint64_t n = ~l;
-        return (uint64_t)((r & (n ^ (n - 1))) != 0);
-    } else if constexpr (B == 32) {
-        const uint32_t n0 = ~(uint32_t)l, n1 = ~(uint32_t)(l >> 32);
-        const uint32_t h0 = ((uint32_t)r & (n0 ^ (n0 - 1))) != 0;
-        const uint32_t h1 = ((uint32_t)(r >> 32) & (n1 ^ (n1 - 1))) != 0;
-        return (uint64_t)h0 | ((uint64_t)h1 << 32);
-    } else {
-        return f_until<B>(l, r);  // narrow slots: the short prefix is cheaper
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -362,30 +238,6 @@ __device__ __forceinline__ void key_fold_pk(KeyAcc &k, uint64_t c, int w, const 
     }
 }
 
-// Layout dispatch for the chunk kernels: B = 1 packed, else B-bit slots.
-template <int OP, int B>
-__device__ __forceinline__ uint64_t op_unary(uint64_t x, const WordAux *a) {
-    if constexpr (B == 1) return pk_unary<OP>(x, a);
-    else return apply1<OP, B>(x, a->lm);
-}
-template <int B>
-__device__ __forceinline__ uint64_t op_until(uint64_t l, uint64_t r, const WordAux *a) {
-    if constexpr (B == 1) return pk_until(l, r, a->inner);
-    else return f_until<B>(l, r);
-}
-// Position 0 of l U r wherever a trace starts (other bits are masked by the
-// caller's chk).
-template <int B>
-__device__ __forceinline__ uint64_t op_until_at0(uint64_t l, uint64_t r, const WordAux *a) {
-    if constexpr (B == 1) return pk_until(l, r, a->inner);
-    else return until_at0<(B == 0 ? 64 : B)>(l, r);
-}
-template <int B>
-__device__ __forceinline__ void op_key_fold(KeyAcc &k, uint64_t c, int w, const WordAux *a, const ChunkArgs &A) {
-    if constexpr (B == 1) key_fold_pk(k, c, w, a, A);
-    else key_fold<B>(k, c, w, A);
-}
-
 // ---------------------------------------------------------------------------
 // Traces longer than 64 positions: K words per trace, word k (positions
 // 64k..64k+63) of trace i is word i*K + k of a set.  The 64-bit operators
@@ -409,7 +261,7 @@ __device__ __forceinline__ void trace_unary(const uint64_t (&x)[K], const WordAu
         uint64_t any = 0;  // all ones once a higher word has a set bit
 #pragma unroll
         for (int k = K - 1; k >= 0; --k) {
-            y[k] = f_fin<64>(x[k]) | (any & aux[k].lm);
+            y[k] = suffix_or64(x[k]) | (any & aux[k].lm);
             any |= 0ull - (uint64_t)(x[k] != 0);
         }
     } else {
@@ -417,7 +269,7 @@ __device__ __forceinline__ void trace_unary(const uint64_t (&x)[K], const WordAu
 #pragma unroll
         for (int k = K - 1; k >= 0; --k) {
             const uint64_t lm = aux[k].lm;
-            y[k] = all & x[k] & ~f_fin<64>(~x[k] & lm);
+            y[k] = all & x[k] & ~suffix_or64(~x[k] & lm);
             all &= 0ull - (uint64_t)(x[k] == lm);
         }
     }

@@ -16,8 +16,9 @@
 //           summaries and parent indices; commit its table value.
 // This replaces the reference's per-thread operator kernels + Thrust
 // compaction (code/ltli6463.cu:176-873, 994-1035).  Operators follow
-// code/ltli6463.cu:203, 295, 385, 468, 551-556, 638-645, 739-771, applied
-// to B-bit trace slots: every shift is masked so that no bit crosses a slot.
+// code/ltli6463.cu:203, 295, 385, 468, 551-556, 638-645, 739-771, on words
+// of whole packed traces (masks and one carry chain, device_ops.hpp), or on
+// K words per trace for traces longer than 64 positions.
 #include <hipcub/hipcub.hpp>
 
 #include "../../include/ltl_amd.h"
@@ -105,7 +106,7 @@ __device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, con
     A.t_fp2[ix] = c0.f2;
 }
 
-template <int OP, int B, bool RX>
+template <int OP, bool RX>
 __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
@@ -119,17 +120,17 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = A.aux + w;
         if constexpr (OP == LTL_UNTIL) {
-            const uint64_t u0 = op_until<B>(l, r, x), u1 = op_until<B>(r, l, x);
+            const uint64_t u0 = pk_until(l, r, x->inner), u1 = pk_until(r, l, x->inner);
             fold(c0, u0, x);
             fold(c1, u1, x);
             if constexpr (RX) {
-                op_key_fold<B>(k0, u0, w, x, A);
-                op_key_fold<B>(k1, u1, w, x, A);
+                key_fold_pk(k0, u0, w, x, A);
+                key_fold_pk(k1, u1, w, x, A);
             }
         } else {
             const uint64_t v = OP == LTL_AND ? (l & r) : (l | r);
             fold(c0, v, x);
-            if constexpr (RX) op_key_fold<B>(k0, v, w, x, A);
+            if constexpr (RX) key_fold_pk(k0, v, w, x, A);
         }
     };
     // Batches of UB words: all loads of a batch are issued before use, so
@@ -152,7 +153,7 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     gen2_tail<OP>(A, t, c0, c1, otf);
 }
 
-template <int OP, int B, bool RX>
+template <int OP, bool RX>
 __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
@@ -164,9 +165,9 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     KeyAcc k0;
     auto step = [&](int w, uint64_t v) {
         const WordAux *x = A.aux + w;
-        const uint64_t y = op_unary<OP, B>(v, x);
+        const uint64_t y = pk_unary<OP>(v, x);
         fold(c0, y, x);
-        if constexpr (RX) op_key_fold<B>(k0, y, w, x, A);
+        if constexpr (RX) key_fold_pk(k0, y, w, x, A);
     };
     constexpr int UB = 8;
     int w = 0;
@@ -226,7 +227,7 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
 }
 
 // Multi-word traces (B = 0): K words per trace, combined from the top word
-// down.  & and | are word-wise and use k_gen2<OP, 64, false>.
+// down.  & and | are word-wise and use k_gen2<OP, false>.
 template <int OP, int K>
 __global__ __launch_bounds__(256) void k_gen1m(ChunkArgs A) {
     if (run_stopped(A)) return;
@@ -435,10 +436,10 @@ __global__ __launch_bounds__(256) void k_compact(ChunkArgs A) {
     if (m1 & bit) A.t_list[p0 + ((m0 & bit) ? 1 : 0)] = ND * ix + 1;
 }
 
-template <int OP, int B>
 // `aux` is A.aux as a separate restrict parameter: the per-word records are
 // then known not to be written through the store pointer, and are read with
 // scalar loads instead of vector loads that wait behind the data loads.
+template <int OP>
 __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__restrict__ aux) {
     if (run_stopped(A) || chunk_otf(A)) return;
     // Waves cover whole store tiles: lane l of wave v writes position
@@ -450,7 +451,6 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
     if (i < 0 || i >= A.st->chunk_total) return;
     if (pos >= A.capacity) return;
     constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
-    constexpr int TPW = 64 / B;
     const uint32_t k = A.t_list[i];           // temp index of the candidate
     const uint32_t ix = OP == LTL_UNTIL ? k >> 1 : k;
     const bool swap = OP == LTL_UNTIL && (k & 1);
@@ -466,31 +466,25 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
     const uint64_t *pr = A.store + set_base(bin ? sr : sl, A.nw);
     uint64_t *q = A.store + set_base(pos, A.nw);
     uint64_t *z = A.sum + set_base(pos, 4 * A.nsw);
-    uint64_t v[4] = {0, 0, 0, 0};
-    uint32_t vl[4] = {0, 0, 0, 0}, vh[4] = {0, 0, 0, 0};  // packed layout
+    uint32_t vl[4] = {0, 0, 0, 0}, vh[4] = {0, 0, 0, 0};
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = aux + w;
         uint64_t c;
         if constexpr (OP == LTL_AND) c = l & r;
         else if constexpr (OP == LTL_OR) c = l | r;
-        else if constexpr (OP == LTL_UNTIL) c = swap ? op_until<B>(r, l, x) : op_until<B>(l, r, x);
-        else c = op_unary<OP, B>(l, x);
+        else if constexpr (OP == LTL_UNTIL) c = swap ? pk_until(r, l, x->inner) : pk_until(l, r, x->inner);
+        else c = pk_unary<OP>(l, x);
         __builtin_nontemporal_store(c, q + (size_t)w * A.ws);  // streamed: not re-read in this chunk
-        if constexpr (B == 1) {
-            // packed: the summary word closes with the last word of its group
-            summarize_pk(c, x, vl, vh);
-            const uint64_t si = x->sinfo;
-            if (sinfo_last(si)) {
+        // the summary word closes with the last word of its group
+        summarize_pk(c, x, vl, vh);
+        const uint64_t si = x->sinfo;
+        if (sinfo_last(si)) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    __builtin_nontemporal_store((uint64_t)vl[j] | ((uint64_t)vh[j] << 32),
-                                                z + (size_t)(j * A.nsw + sinfo_word(si)) * A.ws);
-                    vl[j] = vh[j] = 0;
-                }
+            for (int j = 0; j < 4; ++j) {
+                __builtin_nontemporal_store((uint64_t)vl[j] | ((uint64_t)vh[j] << 32),
+                                            z + (size_t)(j * A.nsw + sinfo_word(si)) * A.ws);
+                vl[j] = vh[j] = 0;
             }
-        } else {
-            summarize<B>(c, x->lm, (w % B) * TPW, v);
-            flush_summary<B>(z, w, A.nw, A.nsw, A.ws, v, true);
         }
     };
     constexpr int UB = bin ? 8 : 16;
@@ -560,7 +554,7 @@ __global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) __builtin_nontemporal_store(y[kk], q + (size_t)(i * K + kk) * A.ws);
         summarize_trace<K>(y, aux, i & 63, v);
-        flush_summary<64>(z, i, T, A.nsw, A.ws, v, true);
+        flush_summary(z, i, T, A.nsw, A.ws, v);
     }
     if (bin) commit(A, pos, k, OP, (int32_t)(swap ? sr : sl), (int32_t)(swap ? sl : sr));
     else commit(A, pos, k, OP, (int32_t)sl, -1);
@@ -618,7 +612,6 @@ __global__ __launch_bounds__(256) void k_chk2(ChunkArgs A) {
     atomicMin(&A.st->found, (unsigned long long)(A.id_base + pair));
 }
 
-template <int B>
 __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
@@ -630,8 +623,8 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     uint32_t m0 = t.valid ? 0u : 0x7fffffffu, m1 = m0;  // invalid lanes never accept
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = A.aux + w;
-        m0 += (uint32_t)__popcll((op_until_at0<B>(l, r, x) ^ x->tgt) & x->chk);
-        m1 += (uint32_t)__popcll((op_until_at0<B>(r, l, x) ^ x->tgt) & x->chk);
+        m0 += (uint32_t)__popcll((pk_until(l, r, x->inner) ^ x->tgt) & x->chk);
+        m1 += (uint32_t)__popcll((pk_until(r, l, x->inner) ^ x->tgt) & x->chk);
     };
     constexpr int UB = 4;
     int i = 0;
@@ -711,11 +704,11 @@ static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles +
 // (B = 0) has kernels of its own.
 #define LTL_DISPATCH_BX(KER, OP, B, RX, GRID, BLOCK, S, ...)                          \
     if ((B) != 1) return hipErrorInvalidValue;                                          \
-    hipLaunchKernelGGL((KER<OP, 1, RX>), GRID, BLOCK, 0, S, __VA_ARGS__);
+    hipLaunchKernelGGL((KER<OP, RX>), GRID, BLOCK, 0, S, __VA_ARGS__);
 
 #define LTL_DISPATCH_B(KER, OP, B, GRID, BLOCK, S, ...)                               \
     if ((B) != 1) return hipErrorInvalidValue;                                          \
-    hipLaunchKernelGGL((KER<OP, 1>), GRID, BLOCK, 0, S, __VA_ARGS__);
+    hipLaunchKernelGGL((KER<OP>), GRID, BLOCK, 0, S, __VA_ARGS__);
 
 #define LTL_DISPATCH_K(KER, OP, K, GRID, S, ...)                                            \
     switch (K) {                                                                            \
@@ -732,8 +725,8 @@ static hipError_t launch_gen_mw(int op, const ChunkArgs &a, hipStream_t s) {
     case LTL_NEXT: LTL_DISPATCH_K(k_gen1m, LTL_NEXT, a.K, grid_tiles(a), s, a); break;
     case LTL_FIN: LTL_DISPATCH_K(k_gen1m, LTL_FIN, a.K, grid_tiles(a), s, a); break;
     case LTL_GLOB: LTL_DISPATCH_K(k_gen1m, LTL_GLOB, a.K, grid_tiles(a), s, a); break;
-    case LTL_AND: hipLaunchKernelGGL((k_gen2<LTL_AND, 64, false>), grid_binary(a), dim3(256), 0, s, a); break;
-    case LTL_OR: hipLaunchKernelGGL((k_gen2<LTL_OR, 64, false>), grid_binary(a), dim3(256), 0, s, a); break;
+    case LTL_AND: hipLaunchKernelGGL((k_gen2<LTL_AND, false>), grid_binary(a), dim3(256), 0, s, a); break;
+    case LTL_OR: hipLaunchKernelGGL((k_gen2<LTL_OR, false>), grid_binary(a), dim3(256), 0, s, a); break;
     case LTL_UNTIL:
         switch (a.K) {
         case 2: hipLaunchKernelGGL(k_gen2m<2>, grid_binary(a), dim3(256), 0, s, a); break;
@@ -864,7 +857,7 @@ hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s) {
     case LTL_UNTIL:
         switch (B) {
         case 0: hipLaunchKernelGGL(k_chkUm, grid_binary(a), b, 0, s, a); break;
-        case 1: hipLaunchKernelGGL(k_chkU<1>, grid_binary(a), b, 0, s, a); break;
+        case 1: hipLaunchKernelGGL(k_chkU, grid_binary(a), b, 0, s, a); break;
         default: return hipErrorInvalidValue;
         }
         break;

@@ -2,9 +2,8 @@
 // kernels (gfx950, wave64).
 //
 // Store layout.  A characteristic set (CS) over W traces is held as nw
-// packed words: trace i sits in slot i % TPW of word i / TPW, where a slot is
-// B bits (B = 8, 16, 32 or 64, the smallest power of two >= the longest
-// trace) and TPW = 64 / B.  The store is tile-major: sets in tiles of 64,
+// words (the packed layout below, or K words per trace).  The store is
+// tile-major: sets in tiles of 64,
 // word w of set s at ((s / 64) * nw + w) * 64 + s % 64.  One lane handles one
 // candidate, so a wave reads or writes 64 consecutive sets of one word as a
 // contiguous 512-byte segment, and the words of one tile follow each other
@@ -113,7 +112,8 @@ struct ChunkArgs {
     int32_t n_tiles;          // tiles in the chunk
     int32_t nsw;              // summary words per vector: ceil(W / 64)
     int32_t anti;             // check pass: also record candidates whose complement is accepted
-    int32_t npw, nps;         // words (summary words) holding positive traces: ceil(P / TPW), ceil(P / 64)
+    int32_t npw, nps;         // words of the groups of positive traces (K words per trace: positive
+                              // traces); summary words holding positive traces, ceil(P / 64)
     int32_t neg_slot;         // anti: neg_found entry written
     // Reference relaxed uniqueness keys (dedup modes 1-3 above 126 trace
     // positions): kmode 0 = exact (NH fingerprints), else the mode; per

@@ -81,7 +81,7 @@ struct ltl_ctx {
     DevState *h_state = nullptr;  // pinned
 
     // layout of the last call
-    int W = 0, P = 0, A = 0, nw = 0, B = 1, TPW = 1, K = 1;  // B = 1: packed, B = 0: K words per trace
+    int W = 0, P = 0, A = 0, nw = 0, B = 1, K = 1;  // B = 1: packed, B = 0: K words per trace
     std::vector<int> pk_word, pk_off;  // packed layout: word and bit offset of each trace
     std::vector<uint8_t> lens;         // trace lengths of the last call
     int64_t capacity = 0;
@@ -370,7 +370,7 @@ static hipError_t timed(ltl_ctx *c, int kind, uint64_t alg, uint64_t mn, uint64_
 namespace {
 
 struct Prepared {
-    int B, TPW, nw, nsw;
+    int B, nw, nsw;
     std::vector<WordAux> aux;          // [nw]
     std::vector<SumAux> saux;          // [nsw]
     std::vector<uint64_t> atom_words;  // [nw][A]
@@ -394,7 +394,6 @@ void prepare_packed(const ltl_problem *pb, Prepared &q) {
     const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms;
     const uint64_t seed = 0x5eed1e55c0ffee11ull;
     q.B = 1;
-    q.TPW = 1;
     q.nsw = (W + 63) / 64;
     q.aux.clear();
     q.pk_word.assign(W, 0);
@@ -469,7 +468,6 @@ void prepare_packed(const ltl_problem *pb, Prepared &q) {
 void prepare_multiword(const ltl_problem *pb, Prepared &q) {
     const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms, K = words_per_trace(pb);
     q.B = 0;
-    q.TPW = 1;
     q.nw = W * K;
     q.nsw = (W + 63) / 64;
     q.aux.assign(q.nw, WordAux{});
@@ -915,7 +913,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     const int B = K > 1 ? 0 : 1;
     Prepared prep;
     prepare_instance(pb, B, prep);
-    c->B = B; c->TPW = 1; c->K = K; c->W = W; c->P = P; c->A = A;
+    c->B = B; c->K = K; c->W = W; c->P = P; c->A = A;
     c->nw = prep.nw;
     c->nsw = prep.nsw;
     c->pk_word.swap(prep.pk_word);

@@ -529,8 +529,8 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
 #pragma unroll
             for (int u = 0; u < UB; ++u) {
                 const WordAux *x = A.aux + wu[u];
-                m0 += (uint32_t)__popcll((op_until_at0<B>(lv[u], rv[u], x) ^ x->tgt) & x->chk);
-                m1 += (uint32_t)__popcll((op_until_at0<B>(rv[u], lv[u], x) ^ x->tgt) & x->chk);
+                m0 += (uint32_t)__popcll((pk_until(lv[u], rv[u], x->inner) ^ x->tgt) & x->chk);
+                m1 += (uint32_t)__popcll((pk_until(rv[u], lv[u], x->inner) ^ x->tgt) & x->chk);
                 seen += (uint32_t)__popcll(x->chk);
             }
             if (EARLY && !__any(m0 <= stop || m1 <= stop || (anti && (seen - m0 <= stop || seen - m1 <= stop))))
@@ -540,8 +540,8 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
             const int w = EARLY ? interleave(i, A.npw, A.nw) : i;
             const WordAux *x = A.aux + w;
             const uint64_t lv = pl[(size_t)w * A.ws], rv = pr[(size_t)w * A.ws];
-            m0 += (uint32_t)__popcll((op_until_at0<B>(lv, rv, x) ^ x->tgt) & x->chk);
-            m1 += (uint32_t)__popcll((op_until_at0<B>(rv, lv, x) ^ x->tgt) & x->chk);
+            m0 += (uint32_t)__popcll((pk_until(lv, rv, x->inner) ^ x->tgt) & x->chk);
+            m1 += (uint32_t)__popcll((pk_until(rv, lv, x->inner) ^ x->tgt) & x->chk);
             seen += (uint32_t)__popcll(x->chk);
         }
     } else {
