@@ -51,13 +51,14 @@ struct Acc {
     uint32_t mism = 0;
 };
 
-// Fold word c of a candidate: both fingerprints and the mismatch count.
-// Fields are read one by one from the (wave-uniform) per-word record, so
-// that only what is used occupies scalar registers.
+// Fold word c of a candidate: both fingerprints and (CHK) the mismatch
+// count.  Fields are read one by one from the (wave-uniform) per-word
+// record, so that only what is used occupies scalar registers.
+template <bool CHK = true>
 __device__ __forceinline__ void fold(Acc &a, uint64_t c, const WordAux *x) {
     a.f1 += nh(c, x->k1);
     a.f2 += nh(c, x->k2);
-    a.mism += (uint32_t)__popcll((c ^ x->tgt) & x->chk);
+    if constexpr (CHK) a.mism += (uint32_t)__popcll((c ^ x->tgt) & x->chk);
 }
 
 // The reference's relaxed uniqueness keys (code/ltli6463.cu:57-141), used

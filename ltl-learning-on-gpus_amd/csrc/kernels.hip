@@ -74,16 +74,17 @@ __device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t) {
 // ---------------------------------------------------------------------------
 // gen
 // ---------------------------------------------------------------------------
-// After the last word: test the candidate(s) of this lane, then intern
-// fingerprint 1 with the candidate id as pending value (unless the chunk is
-// check-only).
-template <int OP>
+// After the last word: test the candidate(s) of this lane (CHK: the level
+// was not tested by a check pass beforehand), then intern fingerprint 1 with
+// the candidate id as pending value (unless the chunk is check-only).
+template <int OP, bool CHK>
 __device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, const Acc &c0, const Acc &c1,
                                           bool otf) {
     const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
     const uint64_t id0 = A.id_base + (OP == LTL_UNTIL ? 2 * pair : pair);
-    if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
-    if (OP == LTL_UNTIL && c1.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
+    if (CHK && c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
+    if (CHK && OP == LTL_UNTIL && c1.mism <= (uint32_t)A.hamming_k)
+        atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
     if (otf) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
     if constexpr (OP == LTL_UNTIL) {
@@ -97,16 +98,17 @@ __device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, con
     }
 }
 
+template <bool CHK>
 __device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, const Acc &c0, bool otf) {
     const uint64_t id0 = A.id_base + (uint64_t)t.a;
-    if (c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
+    if (CHK && c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
     if (otf) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
     A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
     A.t_fp2[ix] = c0.f2;
 }
 
-template <int OP, bool RX>
+template <int OP, bool RX, bool CHK>
 __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
@@ -121,15 +123,15 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
         const WordAux *x = A.aux + w;
         if constexpr (OP == LTL_UNTIL) {
             const uint64_t u0 = pk_until(l, r, x->inner), u1 = pk_until(r, l, x->inner);
-            fold(c0, u0, x);
-            fold(c1, u1, x);
+            fold<CHK>(c0, u0, x);
+            fold<CHK>(c1, u1, x);
             if constexpr (RX) {
                 key_fold_pk(k0, u0, w, x, A);
                 key_fold_pk(k1, u1, w, x, A);
             }
         } else {
             const uint64_t v = OP == LTL_AND ? (l & r) : (l | r);
-            fold(c0, v, x);
+            fold<CHK>(c0, v, x);
             if constexpr (RX) key_fold_pk(k0, v, w, x, A);
         }
     };
@@ -150,10 +152,10 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
         key_finish(c0, k0);
         if constexpr (OP == LTL_UNTIL) key_finish(c1, k1);
     }
-    gen2_tail<OP>(A, t, c0, c1, otf);
+    gen2_tail<OP, CHK>(A, t, c0, c1, otf);
 }
 
-template <int OP, bool RX>
+template <int OP, bool RX, bool CHK>
 __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     auto step = [&](int w, uint64_t v) {
         const WordAux *x = A.aux + w;
         const uint64_t y = pk_unary<OP>(v, x);
-        fold(c0, y, x);
+        fold<CHK>(c0, y, x);
         if constexpr (RX) key_fold_pk(k0, y, w, x, A);
     };
     constexpr int UB = 8;
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     for (; w < A.nw; ++w) step(w, p[(size_t)w * A.ws]);
     if (!t.valid) return;
     if constexpr (RX) key_finish(c0, k0);
-    gen1_tail(A, t, c0, otf);
+    gen1_tail<CHK>(A, t, c0, otf);
 }
 
 // X, F and G of one operand block in one pass over the operand words (the
@@ -189,6 +191,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
 // candidates per operand, three fingerprint pairs, three inserts.  The table
 // keeps the smallest id, and X ids < F ids < G ids, so the dedup outcome is
 // the one of three consecutive passes.  Packed layout, exact keys.
+template <bool CHK>
 __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t slots) {
     if (run_stopped(A)) return;
     Tile t;
@@ -199,9 +202,9 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
     Acc c[3];
     auto step = [&](int w, uint64_t v) {
         const WordAux *x = A.aux + w;
-        fold(c[0], pk_unary<LTL_NEXT>(v, x), x);
-        fold(c[1], pk_unary<LTL_FIN>(v, x), x);
-        fold(c[2], pk_unary<LTL_GLOB>(v, x), x);
+        fold<CHK>(c[0], pk_unary<LTL_NEXT>(v, x), x);
+        fold<CHK>(c[1], pk_unary<LTL_FIN>(v, x), x);
+        fold<CHK>(c[2], pk_unary<LTL_GLOB>(v, x), x);
     };
     constexpr int UB = 8;
     int w = 0;
@@ -218,7 +221,7 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
         const uint64_t id = A.id_base + (uint64_t)j * (uint64_t)n + (uint64_t)t.a;
-        if (c[j].mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id);
+        if (CHK && c[j].mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id);
         if (otf) continue;
         const size_t k = (size_t)j * (size_t)slots + ix;
         A.t_slot[k] = tab_insert(A.tab, A.tmask, table_key(c[j].f1), kPending | id, &A.st->probe_fail);
@@ -249,7 +252,7 @@ __global__ __launch_bounds__(256) void k_gen1m(ChunkArgs A) {
         for (int k = 0; k < K; ++k) fold(c0, y[k], aux + k);
     }
     if (!t.valid) return;
-    gen1_tail(A, t, c0, otf);
+    gen1_tail<true>(A, t, c0, otf);
 }
 
 template <int K>
@@ -280,7 +283,7 @@ __global__ __launch_bounds__(256) void k_gen2m(ChunkArgs A) {  // U
         }
     }
     if (!t.valid) return;
-    gen2_tail<LTL_UNTIL>(A, t, c0, c1, otf);
+    gen2_tail<LTL_UNTIL, true>(A, t, c0, c1, otf);
 }
 
 // Atoms: interned with committed values (store index = atom index), so that
@@ -704,7 +707,8 @@ static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles +
 // (B = 0) has kernels of its own.
 #define LTL_DISPATCH_BX(KER, OP, B, RX, GRID, BLOCK, S, ...)                          \
     if ((B) != 1) return hipErrorInvalidValue;                                          \
-    hipLaunchKernelGGL((KER<OP, RX>), GRID, BLOCK, 0, S, __VA_ARGS__);
+    if (a.gen_check) hipLaunchKernelGGL((KER<OP, RX, true>), GRID, BLOCK, 0, S, __VA_ARGS__); \
+    else hipLaunchKernelGGL((KER<OP, RX, false>), GRID, BLOCK, 0, S, __VA_ARGS__);
 
 #define LTL_DISPATCH_B(KER, OP, B, GRID, BLOCK, S, ...)                               \
     if ((B) != 1) return hipErrorInvalidValue;                                          \
@@ -725,8 +729,8 @@ static hipError_t launch_gen_mw(int op, const ChunkArgs &a, hipStream_t s) {
     case LTL_NEXT: LTL_DISPATCH_K(k_gen1m, LTL_NEXT, a.K, grid_tiles(a), s, a); break;
     case LTL_FIN: LTL_DISPATCH_K(k_gen1m, LTL_FIN, a.K, grid_tiles(a), s, a); break;
     case LTL_GLOB: LTL_DISPATCH_K(k_gen1m, LTL_GLOB, a.K, grid_tiles(a), s, a); break;
-    case LTL_AND: hipLaunchKernelGGL((k_gen2<LTL_AND, false>), grid_binary(a), dim3(256), 0, s, a); break;
-    case LTL_OR: hipLaunchKernelGGL((k_gen2<LTL_OR, false>), grid_binary(a), dim3(256), 0, s, a); break;
+    case LTL_AND: hipLaunchKernelGGL((k_gen2<LTL_AND, false, true>), grid_binary(a), dim3(256), 0, s, a); break;
+    case LTL_OR: hipLaunchKernelGGL((k_gen2<LTL_OR, false, true>), grid_binary(a), dim3(256), 0, s, a); break;
     case LTL_UNTIL:
         switch (a.K) {
         case 2: hipLaunchKernelGGL(k_gen2m<2>, grid_binary(a), dim3(256), 0, s, a); break;
@@ -757,7 +761,8 @@ static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s)
 
 hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, hipStream_t s) {
     if (a.kmode) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_gen_xfg, grid_tiles(a), dim3(256), 0, s, a, n, slots);
+    if (a.gen_check) hipLaunchKernelGGL(k_gen_xfg<true>, grid_tiles(a), dim3(256), 0, s, a, n, slots);
+    else hipLaunchKernelGGL(k_gen_xfg<false>, grid_tiles(a), dim3(256), 0, s, a, n, slots);
     return hipGetLastError();
 }
 

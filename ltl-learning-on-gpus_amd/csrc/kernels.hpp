@@ -133,6 +133,10 @@ struct ChunkArgs {
     // each block's end, so that the blocks before an accepted one still
     // store their survivors, as they would one after the other).
     uint64_t id_end = kEmpty;
+    // 0: the level was tested completely by a check pass before it is
+    // generated, so the gen kernels skip the target check
+    int32_t gen_check = 1;
+    int32_t pad3_ = 0;
 };
 
 // Check passes visit words alternately from the positive and the negative

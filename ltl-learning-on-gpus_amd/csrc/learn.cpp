@@ -1221,6 +1221,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         // accepted first), so a check-only pass finds the same result as
         // generation with dedup, without generating the level.
         bool generate = true;
+        bool tested = false;  // every candidate of the level went through a check pass
         const bool la = learned && !run.otf;
         if (la) {
             // Level `cost`: unary blocks (binary ones too if they were not
@@ -1289,6 +1290,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             }
             run.check_only = false;
             run.base.anti = 0;
+            tested = true;
             if (ahead) looked_ahead = cost + 1;
             else generate = false;  // the last level is tested completely; its sets would serve no later level
         } else if (o.precheck || run.otf || learned) {
@@ -1303,6 +1305,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             run.check_only = false;
             if (rc) break;
             if (h.found != kEmpty) { found = true; found_cost = cost; Le[cost] = n_stored; break; }
+            tested = true;
             if (run.otf) generate = false;  // check-only level (overflow)
             else if (!learned) { run.all_ltls = save; run.blocks.resize(nb); }
         }
@@ -1331,7 +1334,11 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             }
         }
         if (generate) {
-            if ((rc = level_blocks(cost))) break;
+            // a tested level has no accepted candidate: generation skips the check
+            run.base.gen_check = tested ? 0 : 1;
+            rc = level_blocks(cost);
+            run.base.gen_check = 1;
+            if (rc) break;
             if ((rc = sync_state())) break;
             n_stored = h.n_stored;
             if (h.found != kEmpty) { found = true; found_cost = cost; Le[cost] = n_stored; break; }
