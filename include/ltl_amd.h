@@ -215,6 +215,13 @@ int32_t ltl_get_kernel_stats(const ltl_ctx *ctx, ltl_kernel_stat *out, int32_t m
 int32_t ltl_store_read(ltl_ctx *ctx, int64_t first, int64_t count, uint64_t *cs,
                        uint8_t *op, int32_t *left, int32_t *right);
 
+/* The per-trace summaries of stored sets [first, first+count) of the last
+ * call, in trace order: vector v (0: position 0 holds, 1: position 1 holds,
+ * 2: some position holds, 3: every position holds) of set k in
+ * sums[(k*4 + v) * ceil(W/64) + i/64], bit i%64 for trace i.  Introspection
+ * for tests (no reference counterpart). */
+int32_t ltl_store_summaries(ltl_ctx *ctx, int64_t first, int64_t count, uint64_t *sums);
+
 /* Level boundaries of the last call: start[c], end[c] for c < max. */
 int32_t ltl_levels(const ltl_ctx *ctx, int64_t *start, int64_t *end, int32_t max);
 

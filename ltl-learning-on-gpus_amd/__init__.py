@@ -98,7 +98,7 @@ class GenInstance(ctypes.Structure):
 
 EXPORTS = ["ltl_default_options", "ltl_create", "ltl_set_options", "ltl_learn", "ltl_learn_batch",
            "ltl_set_batch_params", "ltl_get_batch_params", "ltl_batch_last_stats", "ltl_autotune_batch",
-           "ltl_result_free", "ltl_destroy", "ltl_get_kernel_stats", "ltl_store_read", "ltl_levels",
+           "ltl_result_free", "ltl_destroy", "ltl_get_kernel_stats", "ltl_store_read", "ltl_store_summaries", "ltl_levels",
            "ltl_read_trace_file", "ltl_read_trace_file_words", "ltl_eval_formula", "ltl_mismatches", "ltl_version",
            "ltl_gen_config", "ltl_gen_custom", "ltl_gen_free",
            "ltl_scorer_create", "ltl_scorer_destroy", "ltl_scorer_get_params", "ltl_scorer_set_params",
@@ -137,6 +137,7 @@ def lib():
     L.ltl_get_kernel_stats.argtypes = [ctypes.c_void_p, P(KernelStat), ctypes.c_int32]
     L.ltl_store_read.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, P(ctypes.c_uint64),
                                  P(ctypes.c_uint8), P(ctypes.c_int32), P(ctypes.c_int32)]
+    L.ltl_store_summaries.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, P(ctypes.c_uint64)]
     L.ltl_levels.argtypes = [ctypes.c_void_p, P(ctypes.c_int64), P(ctypes.c_int64), ctypes.c_int32]
     L.ltl_read_trace_file.argtypes = [ctypes.c_char_p, ctypes.c_int32, P(ctypes.c_int32), P(ctypes.c_int32),
                                       P(ctypes.c_int32), P(ctypes.c_uint8), P(ctypes.c_uint64), ctypes.c_char_p]
@@ -423,6 +424,17 @@ class Learner:
         n = lib().ltl_levels(self._ctx, s.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                              e.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), max_levels)
         return s[:n], e[:n]
+
+    def summaries(self, first, count, W):
+        """Per-trace summaries of stored sets [first, first + count): an
+        array (count, 4, ceil(W / 64)) -- position 0, position 1, some
+        position, every position -- in trace order."""
+        nsw = (W + 63) // 64
+        out = np.zeros(count * 4 * nsw, dtype=np.uint64)
+        rc = lib().ltl_store_summaries(self._ctx, first, count, _u64p(out))
+        if rc != 0:
+            raise LtlError(f"ltl_store_summaries -> {rc}")
+        return out.reshape(count, 4, nsw)
 
     def store(self, first, count, W):
         """Stored sets [first, first + count): W words each (traces x words

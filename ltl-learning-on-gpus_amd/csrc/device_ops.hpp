@@ -179,12 +179,16 @@ __device__ __forceinline__ uint64_t pk_unary(uint64_t x, const WordAux *a) {
 // (b1), any position (nz = F at 0) and every position (fl = G at 0), taken
 // at the traces' position-0 bits (chk, wave-uniform) in packing order and
 // OR-ed in from bit `sbit` of the word's summary word (vl / vh: low and high
-// halves of the four summary words).  Bit positions are wave-uniform, so the
-// loop runs on the scalar unit and each bit moves with one 32-bit extract and
-// one shift-or.
+// halves of the four summary words).  VMASK selects the vectors (bit v for
+// vector v).  Bit positions are wave-uniform, so the loop runs on the scalar
+// unit and each bit moves with one 32-bit extract and one shift-or.
+template <int VMASK = 15>
 __device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint32_t (&vl)[4], uint32_t (&vh)[4]) {
     const uint64_t inner = x->inner, lm = x->lm;
-    const uint64_t u[4] = {c, (c << 1) & inner, pk_chain(inner, c), ~pk_chain(inner, ~c & lm) & lm};
+    uint64_t u[4] = {c, 0, 0, 0};
+    if constexpr ((VMASK & 2) != 0) u[1] = (c << 1) & inner;
+    if constexpr ((VMASK & 4) != 0) u[2] = pk_chain(inner, c);
+    if constexpr ((VMASK & 8) != 0) u[3] = ~pk_chain(inner, ~c & lm) & lm;
     uint64_t m = x->chk;
     int k = sinfo_bit(x->sinfo);
     while (m) {
@@ -193,17 +197,21 @@ __device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint3
         uint32_t bit[4];
         if (b >= 32) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bit[j] = __builtin_amdgcn_ubfe((uint32_t)(u[j] >> 32), (uint32_t)(b - 32), 1u);
+            for (int j = 0; j < 4; ++j)
+                if ((VMASK >> j) & 1) bit[j] = __builtin_amdgcn_ubfe((uint32_t)(u[j] >> 32), (uint32_t)(b - 32), 1u);
         } else {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) bit[j] = __builtin_amdgcn_ubfe((uint32_t)u[j], (uint32_t)b, 1u);
+            for (int j = 0; j < 4; ++j)
+                if ((VMASK >> j) & 1) bit[j] = __builtin_amdgcn_ubfe((uint32_t)u[j], (uint32_t)b, 1u);
         }
         if (k >= 32) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) vh[j] |= bit[j] << (k - 32);
+            for (int j = 0; j < 4; ++j)
+                if ((VMASK >> j) & 1) vh[j] |= bit[j] << (k - 32);
         } else {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) vl[j] |= bit[j] << k;
+            for (int j = 0; j < 4; ++j)
+                if ((VMASK >> j) & 1) vl[j] |= bit[j] << k;
         }
         ++k;
     }

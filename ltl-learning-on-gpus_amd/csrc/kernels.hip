@@ -439,11 +439,19 @@ __global__ __launch_bounds__(256) void k_compact(ChunkArgs A) {
     if (m1 & bit) A.t_list[p0 + ((m0 & bit) ? 1 : 0)] = ND * ix + 1;
 }
 
-// `aux` is A.aux as a separate restrict parameter: the per-word records are
-// then known not to be written through the store pointer, and are read with
-// scalar loads instead of vector loads that wait behind the data loads.
+// `aux` and `saux` are A.aux and A.saux as separate restrict parameters: the
+// per-word records are then known not to be written through the store
+// pointer, and are read with scalar loads instead of vector loads that wait
+// behind the data loads.
+//
+// Summary vectors that follow from the operands' summaries are taken from
+// there (one load per vector and summary word instead of a bit per trace and
+// word): for ~, all four (b0 and b1 negated, nz = not fl, fl = not nz); for &,
+// b0, b1 and fl (ANDed); for |, b0, b1 and nz (ORed).  The rest come from the
+// words.
 template <int OP>
-__global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__restrict__ aux) {
+__global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__restrict__ aux,
+                                              const SumAux *__restrict__ saux) {
     if (run_stopped(A) || chunk_otf(A)) return;
     // Waves cover whole store tiles: lane l of wave v writes position
     // 64 * (chunk_base / 64 + v) + l, so every store instruction of a wave
@@ -469,6 +477,8 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
     const uint64_t *pr = A.store + set_base(bin ? sr : sl, A.nw);
     uint64_t *q = A.store + set_base(pos, A.nw);
     uint64_t *z = A.sum + set_base(pos, 4 * A.nsw);
+    const uint64_t *zl = A.sum + set_base(sl, 4 * A.nsw), *zr = A.sum + set_base(bin ? sr : sl, 4 * A.nsw);
+    constexpr int VM = OP == LTL_NOT ? 0 : OP == LTL_AND ? 4 : OP == LTL_OR ? 8 : 15;  // vectors from the words
     uint32_t vl[4] = {0, 0, 0, 0}, vh[4] = {0, 0, 0, 0};
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = aux + w;
@@ -479,14 +489,33 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
         else c = pk_unary<OP>(l, x);
         __builtin_nontemporal_store(c, q + (size_t)w * A.ws);  // streamed: not re-read in this chunk
         // the summary word closes with the last word of its group
-        summarize_pk(c, x, vl, vh);
+        if constexpr (VM != 0) summarize_pk<VM>(c, x, vl, vh);
         const uint64_t si = x->sinfo;
         if (sinfo_last(si)) {
+            const int j = (int)sinfo_word(si);
+            uint64_t out[4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                __builtin_nontemporal_store((uint64_t)vl[j] | ((uint64_t)vh[j] << 32),
-                                            z + (size_t)(j * A.nsw + sinfo_word(si)) * A.ws);
-                vl[j] = vh[j] = 0;
+            for (int v = 0; v < 4; ++v) out[v] = (uint64_t)vl[v] | ((uint64_t)vh[v] << 32);
+            if constexpr (OP == LTL_NOT) {
+                const SumAux sa = saux[j];
+                const uint64_t o0 = zl[(size_t)(0 * A.nsw + j) * A.ws], o1 = zl[(size_t)(1 * A.nsw + j) * A.ws];
+                const uint64_t o2 = zl[(size_t)(2 * A.nsw + j) * A.ws], o3 = zl[(size_t)(3 * A.nsw + j) * A.ws];
+                out[0] = ~o0 & sa.val;
+                out[1] = ~o1 & sa.val2;
+                out[2] = ~o3 & sa.val;
+                out[3] = ~o2 & sa.val;
+            } else if constexpr (OP == LTL_AND || OP == LTL_OR) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if (v == (OP == LTL_AND ? 2 : 3)) continue;  // from the words
+                    const uint64_t a = zl[(size_t)(v * A.nsw + j) * A.ws], b = zr[(size_t)(v * A.nsw + j) * A.ws];
+                    out[v] = OP == LTL_AND ? (a & b) : (a | b);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                __builtin_nontemporal_store(out[v], z + (size_t)(v * A.nsw + j) * A.ws);
+                vl[v] = vh[v] = 0;
             }
         }
     };
@@ -790,13 +819,13 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
         return hipGetLastError();
     }
     switch (op) {
-    case LTL_NOT: LTL_DISPATCH_B(k_emit, LTL_NOT, B, g, b, s, a, a.aux); break;
-    case LTL_NEXT: LTL_DISPATCH_B(k_emit, LTL_NEXT, B, g, b, s, a, a.aux); break;
-    case LTL_FIN: LTL_DISPATCH_B(k_emit, LTL_FIN, B, g, b, s, a, a.aux); break;
-    case LTL_GLOB: LTL_DISPATCH_B(k_emit, LTL_GLOB, B, g, b, s, a, a.aux); break;
-    case LTL_AND: LTL_DISPATCH_B(k_emit, LTL_AND, B, g, b, s, a, a.aux); break;
-    case LTL_OR: LTL_DISPATCH_B(k_emit, LTL_OR, B, g, b, s, a, a.aux); break;
-    case LTL_UNTIL: LTL_DISPATCH_B(k_emit, LTL_UNTIL, B, g, b, s, a, a.aux); break;
+    case LTL_NOT: LTL_DISPATCH_B(k_emit, LTL_NOT, B, g, b, s, a, a.aux, a.saux); break;
+    case LTL_NEXT: LTL_DISPATCH_B(k_emit, LTL_NEXT, B, g, b, s, a, a.aux, a.saux); break;
+    case LTL_FIN: LTL_DISPATCH_B(k_emit, LTL_FIN, B, g, b, s, a, a.aux, a.saux); break;
+    case LTL_GLOB: LTL_DISPATCH_B(k_emit, LTL_GLOB, B, g, b, s, a, a.aux, a.saux); break;
+    case LTL_AND: LTL_DISPATCH_B(k_emit, LTL_AND, B, g, b, s, a, a.aux, a.saux); break;
+    case LTL_OR: LTL_DISPATCH_B(k_emit, LTL_OR, B, g, b, s, a, a.aux, a.saux); break;
+    case LTL_UNTIL: LTL_DISPATCH_B(k_emit, LTL_UNTIL, B, g, b, s, a, a.aux, a.saux); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -62,6 +62,7 @@ __host__ __device__ __forceinline__ bool sinfo_last(uint64_t i) { return (i >> 4
 // Per summary word j (traces 64j..64j+63): positive traces and valid traces.
 struct SumAux {
     uint64_t tgt, val;
+    uint64_t val2;  // traces of at least two positions
 };
 
 // Device-resident run state (one per context).

@@ -83,6 +83,7 @@ struct ltl_ctx {
     // layout of the last call
     int W = 0, P = 0, A = 0, nw = 0, B = 1, K = 1;  // B = 1: packed, B = 0: K words per trace
     std::vector<int> pk_word, pk_off;  // packed layout: word and bit offset of each trace
+    std::vector<int> sum_order;        // summary bit -> trace
     std::vector<uint8_t> lens;         // trace lengths of the last call
     int64_t capacity = 0;
     int64_t ws = 0;  // word stride of the word-major store (capacity rounded up)
@@ -378,6 +379,7 @@ struct Prepared {
     // packed layout (B = 1)
     std::vector<uint64_t> ptrace;      // per packed trace: trace | offset << 32 | length << 40
     std::vector<int> pk_word, pk_off;  // per trace: word and bit offset
+    std::vector<int> order;            // summary bit -> trace (packing order)
 };
 
 int words_per_trace(const ltl_problem *pb) { return pb->words_per_trace > 1 ? pb->words_per_trace : 1; }
@@ -399,7 +401,8 @@ void prepare_packed(const ltl_problem *pb, Prepared &q) {
     q.pk_word.assign(W, 0);
     q.pk_off.assign(W, 0);
     q.ptrace.assign(W, 0);
-    std::vector<int> order;  // packed index -> trace
+    std::vector<int> &order = q.order;  // packed index -> trace
+    order.clear();
     order.reserve(W);
     for (int g = 0; g < q.nsw; ++g) {
         const int t0 = 64 * g, t1 = std::min(W, t0 + 64);
@@ -441,10 +444,11 @@ void prepare_packed(const ltl_problem *pb, Prepared &q) {
         }
     }
     q.nw = (int)q.aux.size();
-    q.saux.assign(q.nsw, SumAux{0, 0});
+    q.saux.assign(q.nsw, SumAux{0, 0, 0});
     for (int t = 0; t < W; ++t) {
         q.saux[t / 64].val |= 1ull << (t % 64);
         if (order[t] < P) q.saux[t / 64].tgt |= 1ull << (t % 64);
+        if (pb->lens[order[t]] >= 2) q.saux[t / 64].val2 |= 1ull << (t % 64);
     }
     q.atom_words.assign((size_t)q.nw * A, 0);
     const int nsw = q.nsw;
@@ -485,10 +489,11 @@ void prepare_multiword(const ltl_problem *pb, Prepared &q) {
     q.atom_words.assign((size_t)q.nw * A, 0);
     const int nsw = q.nsw;
     q.atom_sums.assign((size_t)4 * nsw * A, 0);
-    q.saux.assign(nsw, SumAux{0, 0});
+    q.saux.assign(nsw, SumAux{0, 0, 0});
     for (int i = 0; i < W; ++i) {
         q.saux[i / 64].val |= 1ull << (i % 64);
         if (i < P) q.saux[i / 64].tgt |= 1ull << (i % 64);
+        if (pb->lens[i] >= 2) q.saux[i / 64].val2 |= 1ull << (i % 64);
     }
     for (int a = 0; a < A; ++a)
         for (int i = 0; i < W; ++i) {
@@ -917,6 +922,11 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     c->nw = prep.nw;
     c->nsw = prep.nsw;
     c->pk_word.swap(prep.pk_word);
+    if (B == 1) c->sum_order.swap(prep.order);
+    else {
+        c->sum_order.resize(W);
+        for (int i = 0; i < W; ++i) c->sum_order[i] = i;
+    }
     c->pk_off.swap(prep.pk_off);
     c->lens.assign(pb->lens, pb->lens + W);
     if (int32_t e = ensure_pool(c)) return res->status = e;
@@ -1706,6 +1716,30 @@ extern "C" int32_t ltl_store_read(ltl_ctx *c, int64_t first, int64_t count, uint
     if (op) HIP_TRY(hipMemcpy(op, c->d_op + first, (size_t)count, hipMemcpyDeviceToHost));
     if (left) HIP_TRY(hipMemcpy(left, c->d_left + first, (size_t)count * 4, hipMemcpyDeviceToHost));
     if (right) HIP_TRY(hipMemcpy(right, c->d_right + first, (size_t)count * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+extern "C" int32_t ltl_store_summaries(ltl_ctx *c, int64_t first, int64_t count, uint64_t *sums) {
+    if (!c || !sums || first < 0 || count < 0 || !c->W) return LTL_E_ARG;
+    if (count == 0) return 0;
+    if (first + count > c->capacity) return LTL_E_ARG;
+    HIP_TRY(hipSetDevice(c->opt.device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const int nsw = c->nsw, W = c->W;
+    const int64_t t0 = first / 64, t1 = (first + count - 1) / 64 + 1;
+    const size_t tile_words = (size_t)4 * nsw * 64;
+    std::vector<uint64_t> buf((size_t)(t1 - t0) * tile_words);
+    HIP_TRY(hipMemcpy(buf.data(), c->d_sum + (size_t)t0 * tile_words, buf.size() * 8, hipMemcpyDeviceToHost));
+    memset(sums, 0, (size_t)count * 4 * nsw * 8);
+    for (int64_t k = 0; k < count; ++k) {
+        const uint64_t *z = buf.data() + set_base(first + k, 4 * nsw) - (size_t)t0 * tile_words;
+        for (int v = 0; v < 4; ++v)
+            for (int t = 0; t < W; ++t)
+                if ((z[(size_t)(v * nsw + t / 64) * 64] >> (t % 64)) & 1ull) {
+                    const int i = c->sum_order[t];
+                    sums[((size_t)k * 4 + v) * nsw + i / 64] |= 1ull << (i % 64);
+                }
+    }
     return 0;
 }
 
