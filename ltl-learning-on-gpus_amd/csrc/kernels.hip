@@ -447,7 +447,9 @@ __global__ __launch_bounds__(256) void k_compact(ChunkArgs A) {
 // Summary vectors that follow from the operands' summaries are taken from
 // there (one load per vector and summary word instead of a bit per trace and
 // word): for ~, all four (b0 and b1 negated, nz = not fl, fl = not nz); for &,
-// b0, b1 and fl (ANDed); for |, b0, b1 and nz (ORed).  The rest come from the
+// b0, b1 and fl (ANDed); for |, b0, b1 and nz (ORed); for X, b0 (the
+// operand's b1); for F, b0 and nz (the operand's nz); for G, b0 and fl (the
+// operand's fl); for U, nz (the right operand's).  The rest come from the
 // words.
 template <int OP>
 __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__restrict__ aux,
@@ -478,7 +480,9 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
     uint64_t *q = A.store + set_base(pos, A.nw);
     uint64_t *z = A.sum + set_base(pos, 4 * A.nsw);
     const uint64_t *zl = A.sum + set_base(sl, 4 * A.nsw), *zr = A.sum + set_base(bin ? sr : sl, 4 * A.nsw);
-    constexpr int VM = OP == LTL_NOT ? 0 : OP == LTL_AND ? 4 : OP == LTL_OR ? 8 : 15;  // vectors from the words
+    // vectors computed from the words (bit v for vector v); the others follow from the operands
+    constexpr int VM = OP == LTL_NOT ? 0 : OP == LTL_AND ? 4 : OP == LTL_OR ? 8 : OP == LTL_NEXT ? 14
+                     : OP == LTL_FIN ? 10 : OP == LTL_GLOB ? 6 : 11;
     uint32_t vl[4] = {0, 0, 0, 0}, vh[4] = {0, 0, 0, 0};
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = aux + w;
@@ -504,6 +508,15 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
                 out[1] = ~o1 & sa.val2;
                 out[2] = ~o3 & sa.val;
                 out[3] = ~o2 & sa.val;
+            } else if constexpr (OP == LTL_NEXT) {
+                out[0] = zl[(size_t)(1 * A.nsw + j) * A.ws];                   // X f at 0 = f at 1
+            } else if constexpr (OP == LTL_FIN) {
+                out[0] = out[2] = zl[(size_t)(2 * A.nsw + j) * A.ws];          // F f at 0 / anywhere = f anywhere
+            } else if constexpr (OP == LTL_GLOB) {
+                out[0] = out[3] = zl[(size_t)(3 * A.nsw + j) * A.ws];          // G f at 0 / everywhere = f everywhere
+            } else if constexpr (OP == LTL_UNTIL) {
+                // l U r holds somewhere iff r does (the right operand: sl when swapped)
+                out[2] = (swap ? zl : zr)[(size_t)(2 * A.nsw + j) * A.ws];
             } else if constexpr (OP == LTL_AND || OP == LTL_OR) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
