@@ -879,7 +879,7 @@ hipError_t launch_finish(const ChunkArgs &a, hipStream_t s) {
 
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s) {
     if (n_atoms <= 0) return hipSuccess;
-    (void)B;
+    (void)B;  // one kernel for both layouts: the fingerprints are per word
     hipLaunchKernelGGL(k_atoms, dim3((unsigned)n_atoms), dim3(64), 0, s, a, n_atoms);
     return hipGetLastError();
 }

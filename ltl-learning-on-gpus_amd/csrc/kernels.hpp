@@ -9,9 +9,9 @@
 // contiguous 512-byte segment, and the words of one tile follow each other
 // (a wave streams through 512-byte segments of one region, whatever the
 // capacity).  Summaries use the same layout with 4 * nsw words per set.
-// Traces longer than 64 positions (slot code B = 0): K words per trace, word
+// Traces longer than 64 positions (layout code B = 0): K words per trace, word
 // k (positions 64k..64k+63) of trace i is word i * K + k of the set.
-// Packed layout (slot code B = 1, every instance of traces of at most 64
+// Packed layout (layout code B = 1, every instance of traces of at most 64
 // positions): whole traces packed into words (first fit by decreasing length,
 // within groups of 64 consecutive traces), a trace of length L at bits
 // [o, o + L) with position p at bit o + L - 1 - p, so that "the next
@@ -38,8 +38,8 @@ constexpr uint64_t kPending = 1ull << 62;   // table value tag: candidate id
 struct WordAux {
     uint64_t k1, k2;  // NH keys of the two fingerprints
     uint64_t lm;      // bits inside the traces (length mask)
-    uint64_t chk;     // position 0 of every valid slot
-    uint64_t tgt;     // position 0 of every positive slot
+    uint64_t chk;     // position 0 of every trace in the word
+    uint64_t tgt;     // position 0 of every positive trace in the word
     // packed layout only: bits whose next position is in the same trace
     // (every trace bit but the last position's); and, in one word so that it
     // is read with one scalar load, the summary word of the word's traces
@@ -152,7 +152,8 @@ __host__ __device__ __forceinline__ int interleave(int j, int np, int n) {
     return np > n - np ? m + j : np + m + j;
 }
 
-// Launchers (stream-ordered, asynchronous).
+// Launchers (stream-ordered, asynchronous).  B is the layout code: 1 packed
+// words, 0 K words per trace (a.K).
 hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s);
 // X, F and G of one operand block in one pass (packed layout, exact keys):
 // block j in {X, F, G} has ids id_base + j * n and its temporaries at offset
