@@ -177,19 +177,19 @@ __device__ void emit_one(const Inst &X, int64_t pos, int64_t sl, int64_t sr, boo
     constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
     const uint64_t *pl = X.store + sl, *pr = X.store + sr;
     uint64_t *q = X.store + pos, *z = X.sum + pos;
-    uint32_t vl[4] = {0, 0, 0, 0}, vh[4] = {0, 0, 0, 0};
+    uint64_t acc[4] = {0, 0, 0, 0};
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = X.aux + w;
         const uint64_t c = combine<OP>(l, r, x, swap);
         q[(size_t)w * X.ws] = c;
         // the summary word closes with the last word of its group
-        summarize_pk(c, x, vl, vh);
+        summarize_pk(c, x, acc);
         const uint64_t si = x->sinfo;
         if (sinfo_last(si)) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                z[(size_t)(j * X.nsw + sinfo_word(si)) * X.ws] = (uint64_t)vl[j] | ((uint64_t)vh[j] << 32);
-                vl[j] = vh[j] = 0;
+                z[(size_t)(j * X.nsw + sinfo_word(si)) * X.ws] = acc[j];
+                acc[j] = 0;
             }
         }
     };

@@ -177,43 +177,33 @@ __device__ __forceinline__ uint64_t pk_unary(uint64_t x, const WordAux *a) {
 
 // Summary bits of the traces of one packed word: positions 0 (b0) and 1
 // (b1), any position (nz = F at 0) and every position (fl = G at 0), taken
-// at the traces' position-0 bits (chk, wave-uniform) in packing order and
-// OR-ed in from bit `sbit` of the word's summary word (vl / vh: low and high
-// halves of the four summary words).  VMASK selects the vectors (bit v for
-// vector v).  Bit positions are wave-uniform, so the loop runs on the scalar
-// unit and each bit moves with one 32-bit extract and one shift-or.
+// at the traces' position-0 bits (chk, wave-uniform) from the word's top bit
+// down and shifted into the four accumulators (acc = acc << 1 | bit), so that
+// the t-th of the n traces of a group ends at bit n - 1 - t of its summary
+// word.  VMASK selects the vectors (bit v for vector v).  Bit positions are
+// wave-uniform: the loops run on the scalar unit, one per 32-bit half, and
+// each bit costs one extract and one shift-or per vector.
 template <int VMASK = 15>
-__device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint32_t (&vl)[4], uint32_t (&vh)[4]) {
+__device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint64_t (&acc)[4]) {
     const uint64_t inner = x->inner, lm = x->lm;
     uint64_t u[4] = {c, 0, 0, 0};
     if constexpr ((VMASK & 2) != 0) u[1] = (c << 1) & inner;
     if constexpr ((VMASK & 4) != 0) u[2] = pk_chain(inner, c);
     if constexpr ((VMASK & 8) != 0) u[3] = ~pk_chain(inner, ~c & lm) & lm;
-    uint64_t m = x->chk;
-    int k = sinfo_bit(x->sinfo);
-    while (m) {
-        const int b = 63 - __builtin_clzll(m);  // traces from the top bit down
-        m ^= 1ull << b;
-        uint32_t bit[4];
-        if (b >= 32) {
+    const uint64_t chk = x->chk;
+    for (uint32_t m = (uint32_t)(chk >> 32); m;) {
+        const int b = 31 - __builtin_clz(m);
+        m ^= 1u << b;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if ((VMASK >> j) & 1) bit[j] = __builtin_amdgcn_ubfe((uint32_t)(u[j] >> 32), (uint32_t)(b - 32), 1u);
-        } else {
+        for (int j = 0; j < 4; ++j)
+            if ((VMASK >> j) & 1) acc[j] = (acc[j] << 1) | __builtin_amdgcn_ubfe((uint32_t)(u[j] >> 32), (uint32_t)b, 1u);
+    }
+    for (uint32_t m = (uint32_t)chk; m;) {
+        const int b = 31 - __builtin_clz(m);
+        m ^= 1u << b;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if ((VMASK >> j) & 1) bit[j] = __builtin_amdgcn_ubfe((uint32_t)u[j], (uint32_t)b, 1u);
-        }
-        if (k >= 32) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if ((VMASK >> j) & 1) vh[j] |= bit[j] << (k - 32);
-        } else {
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if ((VMASK >> j) & 1) vl[j] |= bit[j] << k;
-        }
-        ++k;
+        for (int j = 0; j < 4; ++j)
+            if ((VMASK >> j) & 1) acc[j] = (acc[j] << 1) | __builtin_amdgcn_ubfe((uint32_t)u[j], (uint32_t)b, 1u);
     }
 }
 

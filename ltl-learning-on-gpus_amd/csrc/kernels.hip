@@ -483,7 +483,7 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
     // vectors computed from the words (bit v for vector v); the others follow from the operands
     constexpr int VM = OP == LTL_NOT ? 0 : OP == LTL_AND ? 4 : OP == LTL_OR ? 8 : OP == LTL_NEXT ? 14
                      : OP == LTL_FIN ? 10 : OP == LTL_GLOB ? 6 : 11;
-    uint32_t vl[4] = {0, 0, 0, 0}, vh[4] = {0, 0, 0, 0};
+    uint64_t acc[4] = {0, 0, 0, 0};
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = aux + w;
         uint64_t c;
@@ -493,13 +493,13 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
         else c = pk_unary<OP>(l, x);
         __builtin_nontemporal_store(c, q + (size_t)w * A.ws);  // streamed: not re-read in this chunk
         // the summary word closes with the last word of its group
-        if constexpr (VM != 0) summarize_pk<VM>(c, x, vl, vh);
+        if constexpr (VM != 0) summarize_pk<VM>(c, x, acc);
         const uint64_t si = x->sinfo;
         if (sinfo_last(si)) {
             const int j = (int)sinfo_word(si);
             uint64_t out[4];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) out[v] = (uint64_t)vl[v] | ((uint64_t)vh[v] << 32);
+            for (int v = 0; v < 4; ++v) out[v] = acc[v];
             if constexpr (OP == LTL_NOT) {
                 const SumAux sa = saux[j];
                 const uint64_t o0 = zl[(size_t)(0 * A.nsw + j) * A.ws], o1 = zl[(size_t)(1 * A.nsw + j) * A.ws];
@@ -528,7 +528,7 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 __builtin_nontemporal_store(out[v], z + (size_t)(v * A.nsw + j) * A.ws);
-                vl[v] = vh[v] = 0;
+                acc[v] = 0;
             }
         }
     };

@@ -444,6 +444,14 @@ void prepare_packed(const ltl_problem *pb, Prepared &q) {
         }
     }
     q.nw = (int)q.aux.size();
+    // The emit kernels shift each trace's summary bits in, in packing order:
+    // the t-th of the n traces of group g sits at bit n - 1 - t of summary
+    // word g.  order[] becomes summary bit -> trace.
+    std::vector<int> packed(order);
+    for (int g = 0; g < q.nsw; ++g) {
+        const int t0 = 64 * g, n = std::min(W, t0 + 64) - t0;
+        for (int p = 0; p < n; ++p) order[t0 + n - 1 - p] = packed[t0 + p];
+    }
     q.saux.assign(q.nsw, SumAux{0, 0, 0});
     for (int t = 0; t < W; ++t) {
         q.saux[t / 64].val |= 1ull << (t % 64);
