@@ -178,20 +178,18 @@ __device__ void emit_one(const Inst &X, int64_t pos, int64_t sl, int64_t sr, boo
     const uint64_t *pl = X.store + sl, *pr = X.store + sr;
     uint64_t *q = X.store + pos, *z = X.sum + pos;
     uint64_t acc[4] = {0, 0, 0, 0};
+    auto flush = [&](int j) {  // summary word j is complete
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            z[(size_t)(v * X.nsw + j) * X.ws] = acc[v];
+            acc[v] = 0;
+        }
+    };
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = X.aux + w;
         const uint64_t c = combine<OP>(l, r, x, swap);
         q[(size_t)w * X.ws] = c;
-        // the summary word closes with the last word of its group
-        summarize_pk(c, x, acc);
-        const uint64_t si = x->sinfo;
-        if (sinfo_last(si)) {
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                z[(size_t)(j * X.nsw + sinfo_word(si)) * X.ws] = acc[j];
-                acc[j] = 0;
-            }
-        }
+        summarize_pk<15>(c, x, acc, X.W, flush);
     };
     int w = 0;
     for (; w + UB <= X.nw; w += UB) {

@@ -178,32 +178,38 @@ __device__ __forceinline__ uint64_t pk_unary(uint64_t x, const WordAux *a) {
 // Summary bits of the traces of one packed word: positions 0 (b0) and 1
 // (b1), any position (nz = F at 0) and every position (fl = G at 0), taken
 // at the traces' position-0 bits (chk, wave-uniform) from the word's top bit
-// down and shifted into the four accumulators (acc = acc << 1 | bit), so that
-// the t-th of the n traces of a group ends at bit n - 1 - t of its summary
-// word.  VMASK selects the vectors (bit v for vector v).  Bit positions are
-// wave-uniform: the loops run on the scalar unit, one per 32-bit half, and
-// each bit costs one extract and one shift-or per vector.
-template <int VMASK = 15>
-__device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint64_t (&acc)[4]) {
+// down and shifted into the four accumulators (acc = acc << 1 | bit).  Trace
+// t in packing order belongs to group t / 64 (summary word t / 64); after
+// the group's last trace, flush(t / 64) stores and clears the accumulators,
+// so the i-th of the n traces of a group ends at bit n - 1 - i.  VMASK
+// selects the vectors (bit v for vector v; 0: only the flushes).  Bit
+// positions are wave-uniform: the loops run on the scalar unit, one per
+// 32-bit half, and each bit costs one extract and one shift-or per vector.
+template <int VMASK, class Flush>
+__device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint64_t (&acc)[4], int W,
+                                             Flush &&flush) {
     const uint64_t inner = x->inner, lm = x->lm;
     uint64_t u[4] = {c, 0, 0, 0};
     if constexpr ((VMASK & 2) != 0) u[1] = (c << 1) & inner;
     if constexpr ((VMASK & 4) != 0) u[2] = pk_chain(inner, c);
     if constexpr ((VMASK & 8) != 0) u[3] = ~pk_chain(inner, ~c & lm) & lm;
-    const uint64_t chk = x->chk;
-    for (uint32_t m = (uint32_t)(chk >> 32); m;) {
+    const uint64_t chk = x->chk, si = x->sinfo;
+    int t = (int)sinfo_word(si) * 64 + sinfo_bit(si);  // packed index of the word's first trace
+    for (uint32_t m = (uint32_t)(chk >> 32); m; ++t) {
         const int b = 31 - __builtin_clz(m);
         m ^= 1u << b;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             if ((VMASK >> j) & 1) acc[j] = (acc[j] << 1) | __builtin_amdgcn_ubfe((uint32_t)(u[j] >> 32), (uint32_t)b, 1u);
+        if ((t & 63) == 63 || t == W - 1) flush(t >> 6);
     }
-    for (uint32_t m = (uint32_t)chk; m;) {
+    for (uint32_t m = (uint32_t)chk; m; ++t) {
         const int b = 31 - __builtin_clz(m);
         m ^= 1u << b;
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             if ((VMASK >> j) & 1) acc[j] = (acc[j] << 1) | __builtin_amdgcn_ubfe((uint32_t)u[j], (uint32_t)b, 1u);
+        if ((t & 63) == 63 || t == W - 1) flush(t >> 6);
     }
 }
 

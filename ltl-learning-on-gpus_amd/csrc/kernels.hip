@@ -484,6 +484,43 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
     constexpr int VM = OP == LTL_NOT ? 0 : OP == LTL_AND ? 4 : OP == LTL_OR ? 8 : OP == LTL_NEXT ? 14
                      : OP == LTL_FIN ? 10 : OP == LTL_GLOB ? 6 : 11;
     uint64_t acc[4] = {0, 0, 0, 0};
+    // Summary word j is complete: the vectors that follow from the operands'
+    // summaries, the accumulated ones, stored; the accumulators cleared.
+    auto flush = [&](int j) {
+        uint64_t out[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[v] = acc[v];
+        if constexpr (OP == LTL_NOT) {
+            const SumAux sa = saux[j];
+            const uint64_t o0 = zl[(size_t)(0 * A.nsw + j) * A.ws], o1 = zl[(size_t)(1 * A.nsw + j) * A.ws];
+            const uint64_t o2 = zl[(size_t)(2 * A.nsw + j) * A.ws], o3 = zl[(size_t)(3 * A.nsw + j) * A.ws];
+            out[0] = ~o0 & sa.val;
+            out[1] = ~o1 & sa.val2;
+            out[2] = ~o3 & sa.val;
+            out[3] = ~o2 & sa.val;
+        } else if constexpr (OP == LTL_NEXT) {
+            out[0] = zl[(size_t)(1 * A.nsw + j) * A.ws];                   // X f at 0 = f at 1
+        } else if constexpr (OP == LTL_FIN) {
+            out[0] = out[2] = zl[(size_t)(2 * A.nsw + j) * A.ws];          // F f at 0 / anywhere = f anywhere
+        } else if constexpr (OP == LTL_GLOB) {
+            out[0] = out[3] = zl[(size_t)(3 * A.nsw + j) * A.ws];          // G f at 0 / everywhere = f everywhere
+        } else if constexpr (OP == LTL_UNTIL) {
+            // l U r holds somewhere iff r does (the right operand: sl when swapped)
+            out[2] = (swap ? zl : zr)[(size_t)(2 * A.nsw + j) * A.ws];
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                if (v == (OP == LTL_AND ? 2 : 3)) continue;  // from the words
+                const uint64_t a = zl[(size_t)(v * A.nsw + j) * A.ws], b = zr[(size_t)(v * A.nsw + j) * A.ws];
+                out[v] = OP == LTL_AND ? (a & b) : (a | b);
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            __builtin_nontemporal_store(out[v], z + (size_t)(v * A.nsw + j) * A.ws);
+            acc[v] = 0;
+        }
+    };
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = aux + w;
         uint64_t c;
@@ -492,45 +529,7 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
         else if constexpr (OP == LTL_UNTIL) c = swap ? pk_until(r, l, x->inner) : pk_until(l, r, x->inner);
         else c = pk_unary<OP>(l, x);
         __builtin_nontemporal_store(c, q + (size_t)w * A.ws);  // streamed: not re-read in this chunk
-        // the summary word closes with the last word of its group
-        if constexpr (VM != 0) summarize_pk<VM>(c, x, acc);
-        const uint64_t si = x->sinfo;
-        if (sinfo_last(si)) {
-            const int j = (int)sinfo_word(si);
-            uint64_t out[4];
-#pragma unroll
-            for (int v = 0; v < 4; ++v) out[v] = acc[v];
-            if constexpr (OP == LTL_NOT) {
-                const SumAux sa = saux[j];
-                const uint64_t o0 = zl[(size_t)(0 * A.nsw + j) * A.ws], o1 = zl[(size_t)(1 * A.nsw + j) * A.ws];
-                const uint64_t o2 = zl[(size_t)(2 * A.nsw + j) * A.ws], o3 = zl[(size_t)(3 * A.nsw + j) * A.ws];
-                out[0] = ~o0 & sa.val;
-                out[1] = ~o1 & sa.val2;
-                out[2] = ~o3 & sa.val;
-                out[3] = ~o2 & sa.val;
-            } else if constexpr (OP == LTL_NEXT) {
-                out[0] = zl[(size_t)(1 * A.nsw + j) * A.ws];                   // X f at 0 = f at 1
-            } else if constexpr (OP == LTL_FIN) {
-                out[0] = out[2] = zl[(size_t)(2 * A.nsw + j) * A.ws];          // F f at 0 / anywhere = f anywhere
-            } else if constexpr (OP == LTL_GLOB) {
-                out[0] = out[3] = zl[(size_t)(3 * A.nsw + j) * A.ws];          // G f at 0 / everywhere = f everywhere
-            } else if constexpr (OP == LTL_UNTIL) {
-                // l U r holds somewhere iff r does (the right operand: sl when swapped)
-                out[2] = (swap ? zl : zr)[(size_t)(2 * A.nsw + j) * A.ws];
-            } else if constexpr (OP == LTL_AND || OP == LTL_OR) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    if (v == (OP == LTL_AND ? 2 : 3)) continue;  // from the words
-                    const uint64_t a = zl[(size_t)(v * A.nsw + j) * A.ws], b = zr[(size_t)(v * A.nsw + j) * A.ws];
-                    out[v] = OP == LTL_AND ? (a & b) : (a | b);
-                }
-            }
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                __builtin_nontemporal_store(out[v], z + (size_t)(v * A.nsw + j) * A.ws);
-                acc[v] = 0;
-            }
-        }
+        summarize_pk<VM>(c, x, acc, A.W, flush);
     };
     constexpr int UB = bin ? 8 : 16;
     int w = 0;

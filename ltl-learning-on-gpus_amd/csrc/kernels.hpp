@@ -12,12 +12,12 @@
 // Traces longer than 64 positions (layout code B = 0): K words per trace, word
 // k (positions 64k..64k+63) of trace i is word i * K + k of the set.
 // Packed layout (layout code B = 1, every instance of traces of at most 64
-// positions): whole traces packed into words (first fit by decreasing length,
-// within groups of 64 consecutive traces), a trace of length L at bits
+// positions): whole traces packed into words (best fit by decreasing
+// length), a trace of length L at bits
 // [o, o + L) with position p at bit o + L - 1 - p, so that "the next
 // position" is the next lower bit and the until recurrence runs upward like a
-// carry.  Summary bit t of a set belongs to the t-th trace in packing order
-// (a permutation of the traces within each group of 64).
+// carry.  Summary word g covers the traces 64g .. 64g + 63 of the packing
+// order (a group may start or end inside a word).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -42,22 +42,20 @@ struct WordAux {
     uint64_t tgt;     // position 0 of every positive trace in the word
     // packed layout only: bits whose next position is in the same trace
     // (every trace bit but the last position's); and, in one word so that it
-    // is read with one scalar load, the summary word of the word's traces
-    // (bits 0-31), the summary bit of its first trace (32-39), its trace
-    // count (40-47) and whether it is the last word of its group of 64
-    // traces (bit 48)
+    // is read with one scalar load, the packing index of the word's first
+    // trace as group (bits 0-31) and index within the group (32-39), and the
+    // word's trace count (40-47)
     uint64_t inner;
     uint64_t sinfo;
     uint64_t pad_;
 };
 
-__host__ __device__ __forceinline__ uint64_t make_sinfo(uint32_t sword, int sbit, int ntr, bool last) {
-    return (uint64_t)sword | ((uint64_t)sbit << 32) | ((uint64_t)ntr << 40) | ((uint64_t)last << 48);
+__host__ __device__ __forceinline__ uint64_t make_sinfo(uint32_t sword, int sbit, int ntr) {
+    return (uint64_t)sword | ((uint64_t)sbit << 32) | ((uint64_t)ntr << 40);
 }
 __host__ __device__ __forceinline__ uint32_t sinfo_word(uint64_t i) { return (uint32_t)i; }
 __host__ __device__ __forceinline__ int sinfo_bit(uint64_t i) { return (int)((i >> 32) & 0xff); }
 __host__ __device__ __forceinline__ int sinfo_ntr(uint64_t i) { return (int)((i >> 40) & 0xff); }
-__host__ __device__ __forceinline__ bool sinfo_last(uint64_t i) { return (i >> 48) & 1; }
 
 // Per summary word j (traces 64j..64j+63): positive traces and valid traces.
 struct SumAux {

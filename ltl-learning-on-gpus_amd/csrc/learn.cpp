@@ -387,11 +387,11 @@ int words_per_trace(const ltl_problem *pb) { return pb->words_per_trace > 1 ? pb
 // Layout code: 1 = traces packed whole into words, 0 = K words per trace.
 int slot_width(const ltl_problem *pb) { return words_per_trace(pb) > 1 ? 0 : 1; }
 
-// Packed layout (B = 1): the traces of each group of 64 consecutive traces
-// go into words by first fit in decreasing length order; the traces of a
-// word take its bits from the top down (the first at [64 - L1, 64)), each
-// with its positions in descending bit order.  The packing order (group by
-// group, word by word, top down) numbers the summary bits.
+// Packed layout (B = 1): the traces go into words by best fit in decreasing
+// length order; the traces of a word take its bits from the top down (the
+// first at [64 - L1, 64)), each with its positions in descending bit order.
+// The packing order (word by word, top down) numbers the summary bits: group
+// g = packed traces 64g .. 64g + 63, which may start or end inside a word.
 void prepare_packed(const ltl_problem *pb, Prepared &q) {
     const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms;
     const uint64_t seed = 0x5eed1e55c0ffee11ull;
@@ -404,44 +404,54 @@ void prepare_packed(const ltl_problem *pb, Prepared &q) {
     std::vector<int> &order = q.order;  // packed index -> trace
     order.clear();
     order.reserve(W);
-    for (int g = 0; g < q.nsw; ++g) {
-        const int t0 = 64 * g, t1 = std::min(W, t0 + 64);
-        std::vector<int> idx;
-        for (int i = t0; i < t1; ++i) idx.push_back(i);
-        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return pb->lens[a] > pb->lens[b]; });
-        std::vector<std::vector<int>> bins;
-        std::vector<int> used;
-        for (int i : idx) {
-            size_t b = 0;
-            while (b < bins.size() && used[b] + pb->lens[i] > 64) ++b;
-            if (b == bins.size()) { bins.emplace_back(); used.push_back(0); }
-            bins[b].push_back(i);
-            used[b] += pb->lens[i];
+    // Best fit in decreasing length order: each trace goes to the fullest
+    // word it still fits in (word lists by free bits, 0..64).
+    std::vector<int> idx(W);
+    for (int i = 0; i < W; ++i) idx[i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return pb->lens[a] > pb->lens[b]; });
+    std::vector<std::vector<int>> bins;
+    std::vector<int> used;
+    std::vector<std::vector<int>> by_free(65);
+    for (int i : idx) {
+        const int L = pb->lens[i];
+        int r = L;
+        while (r <= 64 && by_free[r].empty()) ++r;
+        int b;
+        if (r > 64) {
+            b = (int)bins.size();
+            bins.emplace_back();
+            used.push_back(0);
+        } else {
+            b = by_free[r].back();
+            by_free[r].pop_back();
         }
-        for (size_t b = 0; b < bins.size(); ++b) {
-            const int w = (int)q.aux.size();
-            WordAux x;
-            memset(&x, 0, sizeof(x));
-            x.k1 = splitmix(seed ^ (2ull * w));
-            x.k2 = splitmix(~seed ^ (2ull * w + 1));
-            x.sinfo = make_sinfo((uint32_t)g, (int)(order.size() - (size_t)t0), (int)bins[b].size(),
-                                 b + 1 == bins.size());
-            int top = 64;
-            for (int i : bins[b]) {
-                const int L = pb->lens[i], o = top - L;
-                top = o;
-                const uint64_t m = len_mask(L) << o;
-                x.lm |= m;
-                x.inner |= m & ~(1ull << o);
-                x.chk |= 1ull << (o + L - 1);
-                if (i < P) x.tgt |= 1ull << (o + L - 1);
-                q.pk_word[i] = w;
-                q.pk_off[i] = o;
-                q.ptrace[order.size()] = (uint64_t)(uint32_t)i | ((uint64_t)o << 32) | ((uint64_t)L << 40);
-                order.push_back(i);
-            }
-            q.aux.push_back(x);
+        bins[b].push_back(i);
+        used[b] += L;
+        if (used[b] < 64) by_free[64 - used[b]].push_back(b);
+    }
+    for (size_t b = 0; b < bins.size(); ++b) {
+        const int w = (int)q.aux.size();
+        const int t0 = (int)order.size();
+        WordAux x;
+        memset(&x, 0, sizeof(x));
+        x.k1 = splitmix(seed ^ (2ull * w));
+        x.k2 = splitmix(~seed ^ (2ull * w + 1));
+        x.sinfo = make_sinfo((uint32_t)(t0 / 64), t0 % 64, (int)bins[b].size());
+        int top = 64;
+        for (int i : bins[b]) {
+            const int L = pb->lens[i], o = top - L;
+            top = o;
+            const uint64_t m = len_mask(L) << o;
+            x.lm |= m;
+            x.inner |= m & ~(1ull << o);
+            x.chk |= 1ull << (o + L - 1);
+            if (i < P) x.tgt |= 1ull << (o + L - 1);
+            q.pk_word[i] = w;
+            q.pk_off[i] = o;
+            q.ptrace[order.size()] = (uint64_t)(uint32_t)i | ((uint64_t)o << 32) | ((uint64_t)L << 40);
+            order.push_back(i);
         }
+        q.aux.push_back(x);
     }
     q.nw = (int)q.aux.size();
     // The emit kernels shift each trace's summary bits in, in packing order:
@@ -979,12 +989,16 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     ba.t_list = c->t_list;
     ba.m_op = c->d_op; ba.m_left = c->d_left; ba.m_right = c->d_right; ba.m_fp2 = c->d_fp2; ba.m_key = c->d_key;
     ba.sum = c->d_sum; ba.saux = c->d_saux; ba.nsw = nsw;
-    ba.nps = std::min(nsw, (P + 63) / 64);
+    // Check passes alternate between positive and negative traces (early
+    // exit, interleave()): with K words per trace both are in trace order,
+    // positives first; packed words and their summary groups mix positive
+    // and negative traces already, so they are visited in order.
     if (K > 1) {
+        ba.nps = std::min(nsw, (P + 63) / 64);
         ba.npw = P;  // traces
     } else {
-        ba.npw = 0;  // words of the groups of positive traces
-        while (ba.npw < c->nw && (int)sinfo_word(aux[ba.npw].sinfo) < ba.nps) ++ba.npw;
+        ba.nps = 0;
+        ba.npw = 0;
     }
     ba.K = K;
     ba.ptrace = c->d_ptrace;
