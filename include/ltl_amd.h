@@ -86,7 +86,8 @@ typedef struct {
     int64_t  capacity;         /* max stored sets; 0 = sized from budget */
     uint64_t hbm_budget_bytes; /* 0 = 75% of free device memory */
     int32_t  device;           /* HIP device ordinal */
-    int32_t  profile;          /* 1 = time every kernel with HIP events */
+    int32_t  profile;          /* 1 = time every kernel with HIP events, 2 = the generation
+                                  and emit kernels only (fewer dispatch barriers) */
     int32_t  reserved[6];
 } ltl_options;
 
