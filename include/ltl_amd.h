@@ -116,6 +116,10 @@ typedef struct {
     uint64_t hbm_bytes;   /* algorithmic bytes of the enumeration kernels */
     uint64_t candidates;  /* candidates generated (full sets computed) */
     uint64_t checked;     /* candidates tested on summaries only */
+    int64_t  words_deferred; /* sets of the last generated level stored with
+                              * summaries and parents only (plain order: no
+                              * later pass reads their words); the next
+                              * ltl_store_read recomputes their words */
 } ltl_result;
 
 typedef struct ltl_ctx ltl_ctx;

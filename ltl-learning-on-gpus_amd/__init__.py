@@ -61,7 +61,8 @@ class Result(ctypes.Structure):
                 ("n_nodes", ctypes.c_int32), ("nodes", ctypes.POINTER(Node)),
                 ("formula", ctypes.c_char_p), ("seconds", ctypes.c_double),
                 ("kernel_seconds", ctypes.c_double), ("hbm_bytes", ctypes.c_uint64),
-                ("candidates", ctypes.c_uint64), ("checked", ctypes.c_uint64)]
+                ("candidates", ctypes.c_uint64), ("checked", ctypes.c_uint64),
+                ("words_deferred", ctypes.c_int64)]
 
 
 class KernelStat(ctypes.Structure):
@@ -315,7 +316,7 @@ class Learner:
                    n_stored=res.n_stored, on_the_fly=res.on_the_fly,
                    formula=(res.formula or b"").decode(), seconds=res.seconds,
                    kernel_seconds=res.kernel_seconds, hbm_bytes=res.hbm_bytes,
-                   candidates=res.candidates, checked=res.checked,
+                   candidates=res.candidates, checked=res.checked, words_deferred=res.words_deferred,
                    nodes=[(res.nodes[i].op, res.nodes[i].atom, res.nodes[i].left, res.nodes[i].right)
                           for i in range(res.n_nodes)])
         lib().ltl_result_free(ctypes.byref(res))

@@ -185,6 +185,8 @@ __device__ __forceinline__ uint64_t pk_unary(uint64_t x, const WordAux *a) {
 // selects the vectors (bit v for vector v; 0: only the flushes).  Bit
 // positions are wave-uniform: the loops run on the scalar unit, one per
 // 32-bit half, and each bit costs one extract and one shift-or per vector.
+// The traces of a half are taken in runs that end at a group's end, so the
+// end-of-group test runs once per run, not once per trace.
 template <int VMASK, class Flush>
 __device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint64_t (&acc)[4], int W,
                                              Flush &&flush) {
@@ -194,22 +196,40 @@ __device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint6
     if constexpr ((VMASK & 4) != 0) u[2] = pk_chain(inner, c);
     if constexpr ((VMASK & 8) != 0) u[3] = ~pk_chain(inner, ~c & lm) & lm;
     const uint64_t chk = x->chk, si = x->sinfo;
-    int t = (int)sinfo_word(si) * 64 + sinfo_bit(si);  // packed index of the word's first trace
-    for (uint32_t m = (uint32_t)(chk >> 32); m; ++t) {
-        const int b = 31 - __builtin_clz(m);
-        m ^= 1u << b;
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if ((VMASK >> j) & 1) acc[j] = (acc[j] << 1) | __builtin_amdgcn_ubfe((uint32_t)(u[j] >> 32), (uint32_t)b, 1u);
-        if ((t & 63) == 63 || t == W - 1) flush(t >> 6);
+    int g = (int)sinfo_word(si);                       // group of the word's first trace
+    const int t = g * 64 + sinfo_bit(si);              // its packed index
+    int left = (W - 1 < (t | 63) ? W - 1 : (t | 63)) - t + 1;  // traces until the group is complete
+    if constexpr (VMASK == 0) {  // only the flushes
+        for (int n = __popcll(chk); left > 0 && n >= left;) {
+            n -= left;
+            flush(g);
+            ++g;
+            left = W - 64 * g < 64 ? W - 64 * g : 64;
+        }
+        return;
     }
-    for (uint32_t m = (uint32_t)chk; m; ++t) {
-        const int b = 31 - __builtin_clz(m);
-        m ^= 1u << b;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if ((VMASK >> j) & 1) acc[j] = (acc[j] << 1) | __builtin_amdgcn_ubfe((uint32_t)u[j], (uint32_t)b, 1u);
-        if ((t & 63) == 63 || t == W - 1) flush(t >> 6);
+    for (int h = 1; h >= 0; --h) {
+        uint32_t m = (uint32_t)(chk >> (32 * h));
+        uint32_t uh[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) uh[j] = (uint32_t)(u[j] >> (32 * h));
+        while (m && left > 0) {  // left <= 0: every trace taken
+            const int pc = __popc(m), n = pc < left ? pc : left;
+            for (int k = 0; k < n; ++k) {
+                const int b = 31 - __builtin_clz(m);
+                m ^= 1u << b;
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if ((VMASK >> j) & 1) acc[j] = (acc[j] << 1) | __builtin_amdgcn_ubfe(uh[j], (uint32_t)b, 1u);
+            }
+            left -= n;
+            if (left == 0) {
+                flush(g);
+                ++g;
+                left = W - 64 * g < 64 ? W - 64 * g : 64;
+            }
+        }
     }
 }
 

@@ -142,11 +142,11 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     for (; w + UB <= A.nw; w += UB) {
         uint64_t l[UB], r[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) * A.ws]; r[u] = pr[(size_t)(w + u) * A.ws]; }
+        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) * kTileWS]; r[u] = pr[(size_t)(w + u) * kTileWS]; }
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
     }
-    for (; w < A.nw; ++w) step(w, pl[(size_t)w * A.ws], pr[(size_t)w * A.ws]);
+    for (; w < A.nw; ++w) step(w, pl[(size_t)w * kTileWS], pr[(size_t)w * kTileWS]);
     if (!t.valid) return;
     if constexpr (RX) {
         key_finish(c0, k0);
@@ -176,11 +176,11 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     for (; w + UB <= A.nw; w += UB) {
         uint64_t v[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) v[u] = p[(size_t)(w + u) * A.ws];
+        for (int u = 0; u < UB; ++u) v[u] = p[(size_t)(w + u) * kTileWS];
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, v[u]);
     }
-    for (; w < A.nw; ++w) step(w, p[(size_t)w * A.ws]);
+    for (; w < A.nw; ++w) step(w, p[(size_t)w * kTileWS]);
     if (!t.valid) return;
     if constexpr (RX) key_finish(c0, k0);
     gen1_tail<CHK>(A, t, c0, otf);
@@ -211,11 +211,11 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
     for (; w + UB <= A.nw; w += UB) {
         uint64_t v[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) v[u] = p[(size_t)(w + u) * A.ws];
+        for (int u = 0; u < UB; ++u) v[u] = p[(size_t)(w + u) * kTileWS];
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, v[u]);
     }
-    for (; w < A.nw; ++w) step(w, p[(size_t)w * A.ws]);
+    for (; w < A.nw; ++w) step(w, p[(size_t)w * kTileWS]);
     if (!t.valid) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
 #pragma unroll
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_gen1m(ChunkArgs A) {
     for (int i = 0; i < T; ++i) {
         uint64_t x[K], y[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) x[k] = p[(size_t)(i * K + k) * A.ws];
+        for (int k = 0; k < K; ++k) x[k] = p[(size_t)(i * K + k) * kTileWS];
         const WordAux *aux = A.aux + i * K;
         trace_unary<OP, K>(x, aux, y);
 #pragma unroll
@@ -270,8 +270,8 @@ __global__ __launch_bounds__(256) void k_gen2m(ChunkArgs A) {  // U
         uint64_t l[K], r[K], u0[K], u1[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            l[k] = pl[(size_t)(i * K + k) * A.ws];
-            r[k] = pr[(size_t)(i * K + k) * A.ws];
+            l[k] = pl[(size_t)(i * K + k) * kTileWS];
+            r[k] = pr[(size_t)(i * K + k) * kTileWS];
         }
         trace_until<K>(l, r, u0);
         trace_until<K>(r, l, u1);
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms) {
     Acc c0;
     KeyAcc k0;
     for (int w = l; w < A.nw; w += 64) {
-        const uint64_t v = p[(size_t)w * A.ws];
+        const uint64_t v = p[(size_t)w * kTileWS];
         fold(c0, v, A.aux + w);
         if (A.kmode) key_fold_pk(k0, v, w, A.aux + w, A);  // relaxed keys: packed layout only
     }
@@ -492,35 +492,43 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
         for (int v = 0; v < 4; ++v) out[v] = acc[v];
         if constexpr (OP == LTL_NOT) {
             const SumAux sa = saux[j];
-            const uint64_t o0 = zl[(size_t)(0 * A.nsw + j) * A.ws], o1 = zl[(size_t)(1 * A.nsw + j) * A.ws];
-            const uint64_t o2 = zl[(size_t)(2 * A.nsw + j) * A.ws], o3 = zl[(size_t)(3 * A.nsw + j) * A.ws];
+            const uint64_t o0 = zl[(size_t)(0 * A.nsw + j) * kTileWS], o1 = zl[(size_t)(1 * A.nsw + j) * kTileWS];
+            const uint64_t o2 = zl[(size_t)(2 * A.nsw + j) * kTileWS], o3 = zl[(size_t)(3 * A.nsw + j) * kTileWS];
             out[0] = ~o0 & sa.val;
             out[1] = ~o1 & sa.val2;
             out[2] = ~o3 & sa.val;
             out[3] = ~o2 & sa.val;
         } else if constexpr (OP == LTL_NEXT) {
-            out[0] = zl[(size_t)(1 * A.nsw + j) * A.ws];                   // X f at 0 = f at 1
+            out[0] = zl[(size_t)(1 * A.nsw + j) * kTileWS];                   // X f at 0 = f at 1
         } else if constexpr (OP == LTL_FIN) {
-            out[0] = out[2] = zl[(size_t)(2 * A.nsw + j) * A.ws];          // F f at 0 / anywhere = f anywhere
+            out[0] = out[2] = zl[(size_t)(2 * A.nsw + j) * kTileWS];          // F f at 0 / anywhere = f anywhere
         } else if constexpr (OP == LTL_GLOB) {
-            out[0] = out[3] = zl[(size_t)(3 * A.nsw + j) * A.ws];          // G f at 0 / everywhere = f everywhere
+            out[0] = out[3] = zl[(size_t)(3 * A.nsw + j) * kTileWS];          // G f at 0 / everywhere = f everywhere
         } else if constexpr (OP == LTL_UNTIL) {
             // l U r holds somewhere iff r does (the right operand: sl when swapped)
-            out[2] = (swap ? zl : zr)[(size_t)(2 * A.nsw + j) * A.ws];
+            out[2] = (swap ? zl : zr)[(size_t)(2 * A.nsw + j) * kTileWS];
         } else {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 if (v == (OP == LTL_AND ? 2 : 3)) continue;  // from the words
-                const uint64_t a = zl[(size_t)(v * A.nsw + j) * A.ws], b = zr[(size_t)(v * A.nsw + j) * A.ws];
+                const uint64_t a = zl[(size_t)(v * A.nsw + j) * kTileWS], b = zr[(size_t)(v * A.nsw + j) * kTileWS];
                 out[v] = OP == LTL_AND ? (a & b) : (a | b);
             }
         }
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            __builtin_nontemporal_store(out[v], z + (size_t)(v * A.nsw + j) * A.ws);
+            __builtin_nontemporal_store(out[v], z + (size_t)(v * A.nsw + j) * kTileWS);
             acc[v] = 0;
         }
     };
+    const bool sw = A.store_words != 0;
+    if constexpr (VM == 0) {
+        if (!sw) {  // summaries only, and all of them follow from the operand's
+            for (int j = 0; j < A.nsw; ++j) flush(j);
+            commit(A, pos, k, OP, (int32_t)sl, -1);
+            return;
+        }
+    }
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = aux + w;
         uint64_t c;
@@ -528,7 +536,7 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
         else if constexpr (OP == LTL_OR) c = l | r;
         else if constexpr (OP == LTL_UNTIL) c = swap ? pk_until(r, l, x->inner) : pk_until(l, r, x->inner);
         else c = pk_unary<OP>(l, x);
-        __builtin_nontemporal_store(c, q + (size_t)w * A.ws);  // streamed: not re-read in this chunk
+        if (sw) __builtin_nontemporal_store(c, q + (size_t)w * kTileWS);  // streamed: not re-read in this chunk
         summarize_pk<VM>(c, x, acc, A.W, flush);
     };
     constexpr int UB = bin ? 8 : 16;
@@ -537,15 +545,44 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
         uint64_t l[UB], r[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
-            l[u] = pl[(size_t)(w + u) * A.ws];
-            r[u] = bin ? pr[(size_t)(w + u) * A.ws] : 0;
+            l[u] = pl[(size_t)(w + u) * kTileWS];
+            r[u] = bin ? pr[(size_t)(w + u) * kTileWS] : 0;
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
     }
-    for (; w < A.nw; ++w) step(w, pl[(size_t)w * A.ws], bin ? pr[(size_t)w * A.ws] : 0);
+    for (; w < A.nw; ++w) step(w, pl[(size_t)w * kTileWS], bin ? pr[(size_t)w * kTileWS] : 0);
     if (bin) commit(A, pos, k, OP, (int32_t)(swap ? sr : sl), (int32_t)(swap ? sl : sr));
     else commit(A, pos, k, OP, (int32_t)sl, -1);
+}
+
+// Words of the stored sets [s0, s1), recomputed from their parents as the
+// metadata records them (m_op, m_left, m_right; until with the operands in
+// formula order): the sets of a level that was emitted with summaries only.
+// One lane per set; packed layout.
+__global__ __launch_bounds__(256) void k_materialize(ChunkArgs A, int64_t s0, int64_t s1) {
+    const int64_t s = (s0 & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s < s0 || s >= s1) return;
+    const int op = A.m_op[s];
+    const int64_t l = A.m_left[s], r = A.m_right[s] >= 0 ? A.m_right[s] : l;
+    const uint64_t *pl = A.store + set_base(l, A.nw), *pr = A.store + set_base(r, A.nw);
+    uint64_t *q = A.store + set_base(s, A.nw);
+    for (int w = 0; w < A.nw; ++w) {
+        const WordAux *x = A.aux + w;
+        const uint64_t a = pl[(size_t)w * kTileWS], b = pr[(size_t)w * kTileWS];
+        uint64_t v;
+        switch (op) {
+        case LTL_NOT: v = pk_unary<LTL_NOT>(a, x); break;
+        case LTL_NEXT: v = pk_unary<LTL_NEXT>(a, x); break;
+        case LTL_FIN: v = pk_unary<LTL_FIN>(a, x); break;
+        case LTL_GLOB: v = pk_unary<LTL_GLOB>(a, x); break;
+        case LTL_AND: v = a & b; break;
+        case LTL_OR: v = a | b; break;
+        case LTL_UNTIL: v = pk_until(a, b, x->inner); break;
+        default: v = a; break;  // atoms are stored with their words
+        }
+        q[(size_t)w * kTileWS] = v;
+    }
 }
 
 // Multi-word traces (B = 0): one trace (K words) per step; its summary bits
@@ -580,8 +617,8 @@ __global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
         uint64_t x[K], r[K], y[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-            x[kk] = pl[(size_t)(i * K + kk) * A.ws];
-            r[kk] = bin ? pr[(size_t)(i * K + kk) * A.ws] : 0ull;
+            x[kk] = pl[(size_t)(i * K + kk) * kTileWS];
+            r[kk] = bin ? pr[(size_t)(i * K + kk) * kTileWS] : 0ull;
         }
         const WordAux *aux = A.aux + i * K;
         if constexpr (OP == LTL_AND) {
@@ -596,9 +633,9 @@ __global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
             trace_unary<OP, K>(x, aux, y);
         }
 #pragma unroll
-        for (int kk = 0; kk < K; ++kk) __builtin_nontemporal_store(y[kk], q + (size_t)(i * K + kk) * A.ws);
+        for (int kk = 0; kk < K; ++kk) __builtin_nontemporal_store(y[kk], q + (size_t)(i * K + kk) * kTileWS);
         summarize_trace<K>(y, aux, i & 63, v);
-        flush_summary(z, i, T, A.nsw, A.ws, v);
+        flush_summary(z, i, T, A.nsw, kTileWS, v);
     }
     if (bin) commit(A, pos, k, OP, (int32_t)(swap ? sr : sl), (int32_t)(swap ? sl : sr));
     else commit(A, pos, k, OP, (int32_t)sl, -1);
@@ -614,14 +651,14 @@ __global__ __launch_bounds__(256) void k_chk1(ChunkArgs A) {
     Tile t;
     if (!tile_unary(A, t)) return;
     constexpr int VEC = OP == LTL_NOT ? 0 : OP == LTL_NEXT ? 1 : OP == LTL_FIN ? 2 : 3;
-    const uint64_t *p = A.sum + set_base(A.ls + (t.valid ? t.a : A.a0), 4 * A.nsw) + (size_t)(VEC * A.nsw) * A.ws;
+    const uint64_t *p = A.sum + set_base(A.ls + (t.valid ? t.a : A.a0), 4 * A.nsw) + (size_t)(VEC * A.nsw) * kTileWS;
     const uint32_t k = (uint32_t)A.hamming_k;
     const bool anti = A.anti && t.valid;
     uint32_t m = t.valid ? 0u : 0x7fffffffu;  // invalid lanes never accept
     uint32_t seen = 0;                         // traces tested so far
     for (int i = 0; i < A.nsw; ++i) {
         const int j = interleave(i, A.nps, A.nsw);
-        uint64_t v = p[(size_t)j * A.ws];
+        uint64_t v = p[(size_t)j * kTileWS];
         if (OP == LTL_NOT) v = ~v;
         m += (uint32_t)__popcll((v ^ A.saux[j].tgt) & A.saux[j].val);
         seen += (uint32_t)__popcll(A.saux[j].val);
@@ -646,7 +683,7 @@ __global__ __launch_bounds__(256) void k_chk2(ChunkArgs A) {
     uint32_t m = t.valid ? 0u : 0x7fffffffu;  // invalid lanes never accept
     for (int i = 0; i < A.nsw; ++i) {
         const int j = interleave(i, A.nps, A.nsw);
-        const uint64_t l = pl[(size_t)j * A.ws], r = pr[(size_t)j * A.ws];
+        const uint64_t l = pl[(size_t)j * kTileWS], r = pr[(size_t)j * kTileWS];
         const uint64_t c = OP == LTL_AND ? (l & r) : (l | r);
         m += (uint32_t)__popcll((c ^ A.saux[j].tgt) & A.saux[j].val);
         if (!__any(m <= (uint32_t)A.hamming_k)) return;  // every lane of the wave already fails
@@ -678,8 +715,8 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
             wu[u] = interleave(i + u, A.npw, A.nw);
-            l[u] = pl[(size_t)wu[u] * A.ws];
-            r[u] = pr[(size_t)wu[u] * A.ws];
+            l[u] = pl[(size_t)wu[u] * kTileWS];
+            r[u] = pr[(size_t)wu[u] * kTileWS];
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(wu[u], l[u], r[u]);
@@ -688,7 +725,7 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     }
     for (; i < A.nw; ++i) {
         const int w = interleave(i, A.npw, A.nw);
-        step(w, pl[(size_t)w * A.ws], pr[(size_t)w * A.ws]);
+        step(w, pl[(size_t)w * kTileWS], pr[(size_t)w * kTileWS]);
     }
     if (!t.valid) return;
     const uint64_t id0 = A.id_base + 2 * ((uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b);
@@ -711,7 +748,7 @@ __global__ __launch_bounds__(256) void k_chkUm(ChunkArgs A) {
     for (int j = 0; j < T; ++j) {
         const int i = interleave(j, A.npw, T);
         uint32_t h0, h1;
-        until_at0_trace(pl + (size_t)i * K * A.ws, pr + (size_t)i * K * A.ws, A.ws, K, h0, h1);
+        until_at0_trace(pl + (size_t)i * K * kTileWS, pr + (size_t)i * K * kTileWS, kTileWS, K, h0, h1);
         const uint32_t tg = (uint32_t)(A.aux[i * K].tgt & 1ull);
         m0 += h0 ^ tg;
         m1 += h1 ^ tg;
@@ -801,18 +838,20 @@ static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s)
 }
 
 hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, hipStream_t s) {
-    if (a.kmode) return hipErrorInvalidValue;
+    if (a.kmode || a.ws != kTileWS) return hipErrorInvalidValue;
     if (a.gen_check) hipLaunchKernelGGL(k_gen_xfg<true>, grid_tiles(a), dim3(256), 0, s, a, n, slots);
     else hipLaunchKernelGGL(k_gen_xfg<false>, grid_tiles(a), dim3(256), 0, s, a, n, slots);
     return hipGetLastError();
 }
 
 hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s) {
+    if (a.ws != kTileWS) return hipErrorInvalidValue;
     if (B == 0) return a.kmode ? hipErrorInvalidValue : launch_gen_mw(op, a, s);
     return a.kmode ? launch_gen_x<true>(op, B, a, s) : launch_gen_x<false>(op, B, a, s);
 }
 
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
+    if (a.ws != kTileWS || (B != 1 && !a.store_words)) return hipErrorInvalidValue;
     // one lane per survivor; the survivor count is on the device, so the
     // grid covers every candidate of the chunk and idle lanes exit
     const int nd = op == LTL_UNTIL ? 2 : 1;
@@ -840,6 +879,14 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
     case LTL_UNTIL: LTL_DISPATCH_B(k_emit, LTL_UNTIL, B, g, b, s, a, a.aux, a.saux); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_materialize(const ChunkArgs &a, int64_t s0, int64_t s1, hipStream_t s) {
+    if (a.ws != kTileWS || s0 < 0 || s1 > a.capacity) return hipErrorInvalidValue;
+    if (s1 <= s0) return hipSuccess;
+    const int64_t n = s1 - (s0 & ~63ll);
+    hipLaunchKernelGGL(k_materialize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, s0, s1);
     return hipGetLastError();
 }
 
@@ -878,6 +925,7 @@ hipError_t launch_finish(const ChunkArgs &a, hipStream_t s) {
 
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s) {
     if (n_atoms <= 0) return hipSuccess;
+    if (a.ws != kTileWS) return hipErrorInvalidValue;
     (void)B;  // one kernel for both layouts: the fingerprints are per word
     hipLaunchKernelGGL(k_atoms, dim3((unsigned)n_atoms), dim3(64), 0, s, a, n_atoms);
     return hipGetLastError();
@@ -892,6 +940,7 @@ hipError_t launch_scatter(uint64_t *dst, int64_t ws, const uint64_t *src, int nc
 }
 
 hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s) {
+    if (a.ws != kTileWS) return hipErrorInvalidValue;
     const dim3 b(256);
     switch (op) {
     case LTL_NOT: hipLaunchKernelGGL(k_chk1<LTL_NOT>, grid_tiles(a), b, 0, s, a); break;

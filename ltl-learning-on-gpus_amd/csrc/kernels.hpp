@@ -32,6 +32,9 @@ constexpr uint64_t kEmpty = ~0ull;          // empty table key / value, no resul
 __host__ __device__ __forceinline__ size_t set_base(int64_t s, int words) {
     return (size_t)(s >> 6) * (size_t)words * 64 + (size_t)(s & 63);
 }
+// Word stride inside a tile (the layout of set_base): word w of set s at
+// set_base(s, words) + w * kTileWS.  ChunkArgs::ws holds the same value.
+constexpr int64_t kTileWS = 64;
 constexpr uint64_t kPending = 1ull << 62;   // table value tag: candidate id
 
 // Per packed word: dedup hash keys and check masks (wave-uniform).
@@ -135,7 +138,9 @@ struct ChunkArgs {
     // 0: the level was tested completely by a check pass before it is
     // generated, so the gen kernels skip the target check
     int32_t gen_check = 1;
-    int32_t pad3_ = 0;
+    // 0: the emit kernels store summaries and metadata but not the words (a
+    // level whose words no later pass reads; packed layout)
+    int32_t store_words = 1;
 };
 
 // Check passes visit words alternately from the positive and the negative
@@ -163,6 +168,10 @@ hipError_t launch_finish(const ChunkArgs &a, hipStream_t s);
 // one lane per output position.
 hipError_t launch_compact(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s);
+// Words of the stored sets [s0, s1) recomputed from their parents (operator
+// and operand indices of the metadata): sets emitted with store_words = 0.
+// Packed layout.
+hipError_t launch_materialize(const ChunkArgs &a, int64_t s0, int64_t s1, hipStream_t s);
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s);
 // dst[r * ws + c] = src[r * ncols + c] for r < nrows, c < ncols.
 hipError_t launch_scatter(uint64_t *dst, int64_t ws, const uint64_t *src, int ncols, int64_t nrows, hipStream_t s);

@@ -105,6 +105,11 @@ struct ltl_ctx {
     void *d_scan = nullptr;
     size_t scan_bytes = 0;
     std::vector<int64_t> lvl_s, lvl_e;
+    // Stored sets [lazy_s, lazy_e) of the last call were emitted with
+    // summaries only (a last level whose words no later pass read); their
+    // words are recomputed from the parents by the first ltl_store_read.
+    int64_t lazy_s = 0, lazy_e = 0;
+    ChunkArgs lazy_args{};
 
     // profiling
     ltl_kernel_stat stats[K_NKIND]{};
@@ -691,9 +696,13 @@ struct Run {
                                    c->stream));
             PendingEvent &pe = c->pending.back();
             pe.count = slot;
+            // operand words read (negation without words: the operand's summaries),
+            // words written, summaries written, metadata
             const uint64_t sv = (uint64_t)c->nsw * 32;
-            pe.alg_per = (bin ? 2 : 1) * cs + cs + sv + 17 + 8 + 4 + 12;
-            pe.min_per = cs + sv + 17 + 8 + 4 + 12;
+            const bool sw = a.store_words != 0;
+            const uint64_t rd = !sw && op == LTL_NOT ? sv : (bin ? 2 : 1) * cs;
+            pe.alg_per = rd + (sw ? cs : 0) + sv + 17 + 8 + 4 + 12;
+            pe.min_per = (sw ? cs : 0) + sv + 17 + 8 + 4 + 12;
         }
         return 0;
     }
@@ -957,6 +966,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     }
     c->pk_off.swap(prep.pk_off);
     c->lens.assign(pb->lens, pb->lens + W);
+    c->lazy_s = c->lazy_e = 0;
     if (int32_t e = ensure_pool(c)) return res->status = e;
     if (int32_t e = carve(c)) return res->status = e;
     if (c->capacity < A) return res->status = LTL_E_OOM;
@@ -1350,6 +1360,50 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             tested = true;
             if (run.otf) generate = false;  // check-only level (overflow)
             else if (!learned) { run.all_ltls = save; run.blocks.resize(nb); }
+            // (a level emitted without words is always followed by a level
+            // that has an accepted candidate; should this one have none,
+            // the words are recomputed before anything reads them)
+            if (c->lazy_e > c->lazy_s) {
+                if (launch_materialize(c->lazy_args, c->lazy_s, c->lazy_e, s) != hipSuccess) { rc = LTL_E_HIP; break; }
+                c->lazy_s = c->lazy_e = 0;
+            }
+        }
+        // Plain order: a level whose words no later pass reads is emitted
+        // with summaries and metadata only.  That is the level of cost maxc
+        // (the last one), and level `cost` when level cost + 1 has an
+        // accepted candidate: the check pass of cost + 1 reads level `cost`
+        // through its summaries only (unary blocks) and combines levels up
+        // to cost - 1 in its binary blocks.  Those binary blocks are tested
+        // here, before the level is generated: an accepted candidate among
+        // them makes cost + 1 the last level (levels up to `cost` have
+        // none).  The pass is repeated in canonical order after the level,
+        // which returns the same candidate as before.
+        bool lazy = false;
+        if (generate && tested && !learned && !run.otf && B == 1 && !run.logging) {
+            if (cost == maxc) {
+                lazy = true;
+            } else {
+                const uint64_t save = run.all_ltls, save_checked = run.checked;
+                const size_t nb = run.blocks.size();
+                run.check_only = true;
+                run.only_binary = true;
+                rc = level_blocks(cost + 1);
+                run.only_binary = false;
+                run.check_only = false;
+                run.all_ltls = save;
+                run.checked = save_checked;
+                run.blocks.resize(nb);
+                if (!rc) rc = sync_state();
+                if (rc) break;
+                if (h.found != kEmpty) {
+                    lazy = true;
+                    if (launch_fill_u64(reinterpret_cast<uint64_t *>(&c->d_state->found), kEmpty, 1, s) != hipSuccess) {
+                        rc = LTL_E_HIP;
+                        break;
+                    }
+                    h.found = kEmpty;
+                }
+            }
         }
         if (generate && !o.hash_slots_log2) {
             // Grow the table so that it stays at most half full with every
@@ -1378,10 +1432,17 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         if (generate) {
             // a tested level has no accepted candidate: generation skips the check
             run.base.gen_check = tested ? 0 : 1;
+            run.base.store_words = lazy ? 0 : 1;
             rc = level_blocks(cost);
             run.base.gen_check = 1;
+            run.base.store_words = 1;
             if (rc) break;
             if ((rc = sync_state())) break;
+            if (lazy && h.n_stored > n_stored) {
+                c->lazy_s = n_stored;
+                c->lazy_e = h.n_stored;
+                c->lazy_args = run.base;
+            }
             n_stored = h.n_stored;
             if (h.found != kEmpty) { found = true; found_cost = cost; Le[cost] = n_stored; break; }
             if (h.overflow_chunk != LLONG_MAX) run.otf = true;
@@ -1402,6 +1463,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     res->on_the_fly = run.otf || h.overflow_chunk != LLONG_MAX;
     res->candidates = run.candidates;
     res->checked = run.checked;
+    res->words_deferred = c->lazy_e - c->lazy_s;
     res->hbm_bytes = run.alg_bytes;
     for (auto &k : c->stats) res->kernel_seconds += k.seconds;
 
@@ -1538,6 +1600,7 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
             return work[a] > work[b];
         });
 
+        c->lazy_s = c->lazy_e = 0;
         if (int32_t e = ensure_pool(c)) return e;
         const ltl_batch_params &bp = c->batch;
         const int tile = bp.tile == 256 || bp.tile == 1024 ? bp.tile : 512;
@@ -1723,6 +1786,10 @@ extern "C" int32_t ltl_store_read(ltl_ctx *c, int64_t first, int64_t count, uint
     if (count == 0) return 0;
     if (first + count > c->capacity) return LTL_E_ARG;
     HIP_TRY(hipSetDevice(c->opt.device));
+    if (c->lazy_e > c->lazy_s) {  // sets emitted with summaries only: their words first
+        HIP_TRY(launch_materialize(c->lazy_args, c->lazy_s, c->lazy_e, c->stream));
+        c->lazy_s = c->lazy_e = 0;
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
     // the whole tiles holding [first, first + count), one contiguous copy
     const int64_t t0 = first / 64, t1 = (first + count - 1) / 64 + 1;
