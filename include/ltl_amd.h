@@ -117,9 +117,14 @@ typedef struct {
     uint64_t candidates;  /* candidates generated (full sets computed) */
     uint64_t checked;     /* candidates tested on summaries only */
     int64_t  words_deferred; /* sets of the last generated level stored with
-                              * summaries and parents only (plain order: no
-                              * later pass reads their words); the next
-                              * ltl_store_read recomputes their words */
+                              * parents only, neither words nor summaries
+                              * (plain order: no later pass reads them); the
+                              * next ltl_store_read or ltl_store_summaries
+                              * recomputes them */
+    int32_t  deferred_succ;  /* bit j: the last level's unary block j (~, X,
+                              * F, G over the deferred level) was tested in
+                              * that level's generation */
+    int32_t  pad_;
 } ltl_result;
 
 typedef struct ltl_ctx ltl_ctx;

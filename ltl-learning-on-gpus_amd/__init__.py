@@ -62,7 +62,8 @@ class Result(ctypes.Structure):
                 ("formula", ctypes.c_char_p), ("seconds", ctypes.c_double),
                 ("kernel_seconds", ctypes.c_double), ("hbm_bytes", ctypes.c_uint64),
                 ("candidates", ctypes.c_uint64), ("checked", ctypes.c_uint64),
-                ("words_deferred", ctypes.c_int64)]
+                ("words_deferred", ctypes.c_int64), ("deferred_succ", ctypes.c_int32),
+                ("pad_", ctypes.c_int32)]
 
 
 class KernelStat(ctypes.Structure):
@@ -317,6 +318,7 @@ class Learner:
                    formula=(res.formula or b"").decode(), seconds=res.seconds,
                    kernel_seconds=res.kernel_seconds, hbm_bytes=res.hbm_bytes,
                    candidates=res.candidates, checked=res.checked, words_deferred=res.words_deferred,
+                   deferred_succ=res.deferred_succ,
                    nodes=[(res.nodes[i].op, res.nodes[i].atom, res.nodes[i].left, res.nodes[i].right)
                           for i in range(res.n_nodes)])
         lib().ltl_result_free(ctypes.byref(res))

@@ -74,12 +74,40 @@ __device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t) {
 // ---------------------------------------------------------------------------
 // gen
 // ---------------------------------------------------------------------------
+// Successors X c, F c, G c of a candidate (bits 1-3 of A.succ_mask) tested
+// at position 0 of every trace, word by word: their mismatch counts.  (~c,
+// bit 0, comes from the level's check pass: the complements it accepts.)
+struct SuccAcc {
+    uint32_t m[3] = {0, 0, 0};
+};
+
+template <bool SUCC>
+__device__ __forceinline__ void succ_fold(SuccAcc &s, uint64_t c, const WordAux *x, int mask) {
+    if constexpr (SUCC) {
+        const uint64_t tgt = x->tgt, chk = x->chk;
+        if (mask & 2) s.m[0] += (uint32_t)__popcll((pk_unary<LTL_NEXT>(c, x) ^ tgt) & chk);
+        if (mask & 4) s.m[1] += (uint32_t)__popcll((pk_unary<LTL_FIN>(c, x) ^ tgt) & chk);
+        if (mask & 8) s.m[2] += (uint32_t)__popcll((pk_unary<LTL_GLOB>(c, x) ^ tgt) & chk);
+    }
+}
+
+// Accepted successors X c, F c, G c of a candidate as bits 1-3.
+__device__ __forceinline__ uint8_t succ_flags(const ChunkArgs &A, const Acc &, const SuccAcc &s) {
+    const uint32_t k = (uint32_t)A.hamming_k;
+    uint8_t f = 0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        if ((A.succ_mask & (2 << j)) && s.m[j] <= k) f |= (uint8_t)(2 << j);
+    return f;
+}
+
 // After the last word: test the candidate(s) of this lane (CHK: the level
 // was not tested by a check pass beforehand), then intern fingerprint 1 with
-// the candidate id as pending value (unless the chunk is check-only).
-template <int OP, bool CHK>
+// the candidate id as pending value (unless the chunk is check-only), and
+// (SUCC) record the candidates' accepted successors.
+template <int OP, bool CHK, bool SUCC>
 __device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, const Acc &c0, const Acc &c1,
-                                          bool otf) {
+                                          const SuccAcc &s0, const SuccAcc &s1, bool otf) {
     const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
     const uint64_t id0 = A.id_base + (OP == LTL_UNTIL ? 2 * pair : pair);
     if (CHK && c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
@@ -92,23 +120,30 @@ __device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, con
         A.t_fp2[2 * ix] = c0.f2;
         A.t_slot[2 * ix + 1] = tab_insert(A.tab, A.tmask, table_key(c1.f1), kPending | (id0 + 1), &A.st->probe_fail);
         A.t_fp2[2 * ix + 1] = c1.f2;
+        if constexpr (SUCC) {
+            A.t_succ[2 * ix] = succ_flags(A, c0, s0);
+            A.t_succ[2 * ix + 1] = succ_flags(A, c1, s1);
+        }
     } else {
         A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
         A.t_fp2[ix] = c0.f2;
+        if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, c0, s0);
     }
 }
 
-template <bool CHK>
-__device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, const Acc &c0, bool otf) {
+template <bool CHK, bool SUCC>
+__device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, const Acc &c0, const SuccAcc &s0,
+                                          bool otf) {
     const uint64_t id0 = A.id_base + (uint64_t)t.a;
     if (CHK && c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
     if (otf) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
     A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
     A.t_fp2[ix] = c0.f2;
+    if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, c0, s0);
 }
 
-template <int OP, bool RX, bool CHK>
+template <int OP, bool RX, bool CHK, bool SUCC>
 __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
@@ -119,12 +154,16 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     const uint64_t *pr = A.store + set_base(sr, A.nw);
     Acc c0, c1;
     KeyAcc k0, k1;
+    SuccAcc s0, s1;
+    const int sm = A.succ_mask;
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = A.aux + w;
         if constexpr (OP == LTL_UNTIL) {
             const uint64_t u0 = pk_until(l, r, x->inner), u1 = pk_until(r, l, x->inner);
             fold<CHK>(c0, u0, x);
             fold<CHK>(c1, u1, x);
+            succ_fold<SUCC>(s0, u0, x, sm);
+            succ_fold<SUCC>(s1, u1, x, sm);
             if constexpr (RX) {
                 key_fold_pk(k0, u0, w, x, A);
                 key_fold_pk(k1, u1, w, x, A);
@@ -132,6 +171,7 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
         } else {
             const uint64_t v = OP == LTL_AND ? (l & r) : (l | r);
             fold<CHK>(c0, v, x);
+            succ_fold<SUCC>(s0, v, x, sm);
             if constexpr (RX) key_fold_pk(k0, v, w, x, A);
         }
     };
@@ -152,10 +192,10 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
         key_finish(c0, k0);
         if constexpr (OP == LTL_UNTIL) key_finish(c1, k1);
     }
-    gen2_tail<OP, CHK>(A, t, c0, c1, otf);
+    gen2_tail<OP, CHK, SUCC>(A, t, c0, c1, s0, s1, otf);
 }
 
-template <int OP, bool RX, bool CHK>
+template <int OP, bool RX, bool CHK, bool SUCC>
 __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
@@ -165,10 +205,13 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     const uint64_t *p = A.store + set_base(s, A.nw);
     Acc c0;
     KeyAcc k0;
+    SuccAcc s0;
+    const int sm = A.succ_mask;
     auto step = [&](int w, uint64_t v) {
         const WordAux *x = A.aux + w;
         const uint64_t y = pk_unary<OP>(v, x);
         fold<CHK>(c0, y, x);
+        succ_fold<SUCC>(s0, y, x, sm);
         if constexpr (RX) key_fold_pk(k0, y, w, x, A);
     };
     constexpr int UB = 8;
@@ -183,7 +226,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     for (; w < A.nw; ++w) step(w, p[(size_t)w * kTileWS]);
     if (!t.valid) return;
     if constexpr (RX) key_finish(c0, k0);
-    gen1_tail<CHK>(A, t, c0, otf);
+    gen1_tail<CHK, SUCC>(A, t, c0, s0, otf);
 }
 
 // X, F and G of one operand block in one pass over the operand words (the
@@ -191,7 +234,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
 // candidates per operand, three fingerprint pairs, three inserts.  The table
 // keeps the smallest id, and X ids < F ids < G ids, so the dedup outcome is
 // the one of three consecutive passes.  Packed layout, exact keys.
-template <bool CHK>
+template <bool CHK, bool SUCC>
 __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t slots) {
     if (run_stopped(A)) return;
     Tile t;
@@ -200,11 +243,16 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
     const int64_t s = A.ls + (t.valid ? t.a : A.a0);
     const uint64_t *p = A.store + set_base(s, A.nw);
     Acc c[3];
+    SuccAcc sc[3];
+    const int sm = A.succ_mask;
     auto step = [&](int w, uint64_t v) {
         const WordAux *x = A.aux + w;
-        fold<CHK>(c[0], pk_unary<LTL_NEXT>(v, x), x);
-        fold<CHK>(c[1], pk_unary<LTL_FIN>(v, x), x);
-        fold<CHK>(c[2], pk_unary<LTL_GLOB>(v, x), x);
+        const uint64_t y[3] = {pk_unary<LTL_NEXT>(v, x), pk_unary<LTL_FIN>(v, x), pk_unary<LTL_GLOB>(v, x)};
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            fold<CHK>(c[j], y[j], x);
+            succ_fold<SUCC>(sc[j], y[j], x, sm);
+        }
     };
     constexpr int UB = 8;
     int w = 0;
@@ -226,6 +274,7 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
         const size_t k = (size_t)j * (size_t)slots + ix;
         A.t_slot[k] = tab_insert(A.tab, A.tmask, table_key(c[j].f1), kPending | id, &A.st->probe_fail);
         A.t_fp2[k] = c[j].f2;
+        if constexpr (SUCC) A.t_succ[k] = succ_flags(A, c[j], sc[j]);
     }
 }
 
@@ -252,7 +301,7 @@ __global__ __launch_bounds__(256) void k_gen1m(ChunkArgs A) {
         for (int k = 0; k < K; ++k) fold(c0, y[k], aux + k);
     }
     if (!t.valid) return;
-    gen1_tail<true>(A, t, c0, otf);
+    gen1_tail<true, false>(A, t, c0, SuccAcc{}, otf);
 }
 
 template <int K>
@@ -283,7 +332,7 @@ __global__ __launch_bounds__(256) void k_gen2m(ChunkArgs A) {  // U
         }
     }
     if (!t.valid) return;
-    gen2_tail<LTL_UNTIL, true>(A, t, c0, c1, otf);
+    gen2_tail<LTL_UNTIL, true, false>(A, t, c0, c1, SuccAcc{}, SuccAcc{}, otf);
 }
 
 // Atoms: interned with committed values (store index = atom index), so that
@@ -521,14 +570,6 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
             acc[v] = 0;
         }
     };
-    const bool sw = A.store_words != 0;
-    if constexpr (VM == 0) {
-        if (!sw) {  // summaries only, and all of them follow from the operand's
-            for (int j = 0; j < A.nsw; ++j) flush(j);
-            commit(A, pos, k, OP, (int32_t)sl, -1);
-            return;
-        }
-    }
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = aux + w;
         uint64_t c;
@@ -536,7 +577,7 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
         else if constexpr (OP == LTL_OR) c = l | r;
         else if constexpr (OP == LTL_UNTIL) c = swap ? pk_until(r, l, x->inner) : pk_until(l, r, x->inner);
         else c = pk_unary<OP>(l, x);
-        if (sw) __builtin_nontemporal_store(c, q + (size_t)w * kTileWS);  // streamed: not re-read in this chunk
+        __builtin_nontemporal_store(c, q + (size_t)w * kTileWS);  // streamed: not re-read in this chunk
         summarize_pk<VM>(c, x, acc, A.W, flush);
     };
     constexpr int UB = bin ? 8 : 16;
@@ -556,10 +597,48 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
     else commit(A, pos, k, OP, (int32_t)sl, -1);
 }
 
-// Words of the stored sets [s0, s1), recomputed from their parents as the
-// metadata records them (m_op, m_left, m_right; until with the operands in
-// formula order): the sets of a level that was emitted with summaries only.
-// One lane per set; packed layout.
+// Deferred level: survivors get their metadata and table commit only (no
+// words, no summaries), and each survivor whose successor j was accepted in
+// generation lowers succ_min[j] to its store index.
+__global__ __launch_bounds__(256) void k_emit_meta(ChunkArgs A, int op) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    const int64_t base = A.st->chunk_base;
+    const int64_t pos = (base & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = pos - base;
+    if (i < 0 || i >= A.st->chunk_total) return;
+    if (pos >= A.capacity) return;
+    const bool bin = op == LTL_AND || op == LTL_OR || op == LTL_UNTIL;
+    const uint32_t k = A.t_list[i];
+    const uint32_t ix = op == LTL_UNTIL ? k >> 1 : k;
+    const bool swap = op == LTL_UNTIL && (k & 1);
+    uint64_t id;  // canonical candidate id
+    if (bin) {
+        const int64_t ti = ix / 64;
+        const int64_t a = A.a0 + ti / A.nbs, b = A.b0 - A.ush + (ti % A.nbs) * 64 + (ix % 64);
+        const uint64_t pair = (uint64_t)a * (uint64_t)A.nR + (uint64_t)b;
+        id = A.id_base + (op == LTL_UNTIL ? 2 * pair + (swap ? 1 : 0) : pair);
+        const int64_t sl = A.ls + a, sr = A.rs + b;
+        commit(A, pos, k, op, (int32_t)(swap ? sr : sl), (int32_t)(swap ? sl : sr));
+    } else {
+        const int64_t a = A.a0 - A.ush + ix;
+        id = A.id_base + (uint64_t)a;
+        commit(A, pos, k, op, (int32_t)(A.ls + a), -1);
+    }
+    // ~c: the smallest candidate id whose complement the check pass of this
+    // level accepted is the first occurrence of its set, so it is stored
+    if (id == A.neg_pick) atomicMin(&A.st->succ_min[0], (unsigned long long)pos);
+    if (A.t_succ) {
+        const uint32_t f = A.t_succ[k];
+#pragma unroll
+        for (int j = 1; j < 4; ++j)
+            if ((f >> j) & 1u) atomicMin(&A.st->succ_min[j], (unsigned long long)pos);
+    }
+}
+
+// Words and summaries of the stored sets [s0, s1), recomputed from their
+// parents as the metadata records them (m_op, m_left, m_right; until with
+// the operands in formula order): the sets of a deferred level.  One lane
+// per set; packed layout.
 __global__ __launch_bounds__(256) void k_materialize(ChunkArgs A, int64_t s0, int64_t s1) {
     const int64_t s = (s0 & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (s < s0 || s >= s1) return;
@@ -567,6 +646,15 @@ __global__ __launch_bounds__(256) void k_materialize(ChunkArgs A, int64_t s0, in
     const int64_t l = A.m_left[s], r = A.m_right[s] >= 0 ? A.m_right[s] : l;
     const uint64_t *pl = A.store + set_base(l, A.nw), *pr = A.store + set_base(r, A.nw);
     uint64_t *q = A.store + set_base(s, A.nw);
+    uint64_t *z = A.sum + set_base(s, 4 * A.nsw);
+    uint64_t acc[4] = {0, 0, 0, 0};
+    auto flush = [&](int j) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            z[(size_t)(v * A.nsw + j) * kTileWS] = acc[v];
+            acc[v] = 0;
+        }
+    };
     for (int w = 0; w < A.nw; ++w) {
         const WordAux *x = A.aux + w;
         const uint64_t a = pl[(size_t)w * kTileWS], b = pr[(size_t)w * kTileWS];
@@ -582,6 +670,7 @@ __global__ __launch_bounds__(256) void k_materialize(ChunkArgs A, int64_t s0, in
         default: v = a; break;  // atoms are stored with their words
         }
         q[(size_t)w * kTileWS] = v;
+        summarize_pk<15>(v, x, acc, A.W, flush);
     }
 }
 
@@ -680,17 +769,22 @@ __global__ __launch_bounds__(256) void k_chk2(ChunkArgs A) {
     const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
     const uint64_t *pl = A.sum + set_base(sl, 4 * A.nsw);
     const uint64_t *pr = A.sum + set_base(sr, 4 * A.nsw);
+    const uint32_t k = (uint32_t)A.hamming_k;
+    const bool anti = A.anti && t.valid;
     uint32_t m = t.valid ? 0u : 0x7fffffffu;  // invalid lanes never accept
+    uint32_t seen = 0;                         // traces tested so far
     for (int i = 0; i < A.nsw; ++i) {
         const int j = interleave(i, A.nps, A.nsw);
         const uint64_t l = pl[(size_t)j * kTileWS], r = pr[(size_t)j * kTileWS];
         const uint64_t c = OP == LTL_AND ? (l & r) : (l | r);
         m += (uint32_t)__popcll((c ^ A.saux[j].tgt) & A.saux[j].val);
-        if (!__any(m <= (uint32_t)A.hamming_k)) return;  // every lane of the wave already fails
+        seen += (uint32_t)__popcll(A.saux[j].val);
+        // every lane of the wave already fails (with anti: its complement too)
+        if (!__any(m <= k || (anti && seen - m <= k))) return;
     }
-    if (m > (uint32_t)A.hamming_k) return;
-    const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
-    atomicMin(&A.st->found, (unsigned long long)(A.id_base + pair));
+    const uint64_t id = A.id_base + (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
+    if (m <= k) atomicMin(&A.st->found, (unsigned long long)id);
+    else if (anti && seen - m <= k) atomicMin(&A.st->neg_found[A.neg_slot], (unsigned long long)id);
 }
 
 __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
@@ -701,11 +795,14 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     const uint64_t *pl = A.store + set_base(sl, A.nw);
     const uint64_t *pr = A.store + set_base(sr, A.nw);
     const uint32_t k = (uint32_t)A.hamming_k;
+    const bool anti = A.anti && t.valid;
     uint32_t m0 = t.valid ? 0u : 0x7fffffffu, m1 = m0;  // invalid lanes never accept
+    uint32_t seen = 0;                                  // traces tested so far
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = A.aux + w;
         m0 += (uint32_t)__popcll((pk_until(l, r, x->inner) ^ x->tgt) & x->chk);
         m1 += (uint32_t)__popcll((pk_until(r, l, x->inner) ^ x->tgt) & x->chk);
+        seen += (uint32_t)__popcll(x->chk);
     };
     constexpr int UB = 4;
     int i = 0;
@@ -720,8 +817,9 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
         }
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(wu[u], l[u], r[u]);
-        // Mismatch counts only grow: stop once every lane of the wave fails.
-        if (!__any(m0 <= k || m1 <= k)) return;
+        // Mismatch and match counts only grow: stop once every lane of the
+        // wave fails (with anti: the complements too).
+        if (!__any(m0 <= k || m1 <= k || (anti && (seen - m0 <= k || seen - m1 <= k)))) return;
     }
     for (; i < A.nw; ++i) {
         const int w = interleave(i, A.npw, A.nw);
@@ -731,6 +829,8 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
     const uint64_t id0 = A.id_base + 2 * ((uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b);
     if (m0 <= k) atomicMin(&A.st->found, (unsigned long long)id0);
     else if (m1 <= k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
+    else if (anti && seen - m0 <= k) atomicMin(&A.st->neg_found[A.neg_slot], (unsigned long long)id0);
+    else if (anti && seen - m1 <= k) atomicMin(&A.st->neg_found[A.neg_slot], (unsigned long long)(id0 + 1));
 }
 
 // Multi-word traces (B = 0): position 0 of U from the first word of each
@@ -783,10 +883,17 @@ static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles +
 
 // The chunk kernels run on the packed layout (B = 1); K words per trace
 // (B = 0) has kernels of its own.
-#define LTL_DISPATCH_BX(KER, OP, B, RX, GRID, BLOCK, S, ...)                          \
-    if ((B) != 1) return hipErrorInvalidValue;                                          \
-    if (a.gen_check) hipLaunchKernelGGL((KER<OP, RX, true>), GRID, BLOCK, 0, S, __VA_ARGS__); \
-    else hipLaunchKernelGGL((KER<OP, RX, false>), GRID, BLOCK, 0, S, __VA_ARGS__);
+// Successor checks (a.t_succ) run with exact keys on a tested level only.
+#define LTL_DISPATCH_BX(KER, OP, B, RX, GRID, BLOCK, S, ...)                                  \
+    if ((B) != 1) return hipErrorInvalidValue;                                                  \
+    if (a.t_succ) {                                                                             \
+        if (RX || a.gen_check) return hipErrorInvalidValue;                                     \
+        hipLaunchKernelGGL((KER<OP, false, false, true>), GRID, BLOCK, 0, S, __VA_ARGS__);      \
+    } else if (a.gen_check) {                                                                   \
+        hipLaunchKernelGGL((KER<OP, RX, true, false>), GRID, BLOCK, 0, S, __VA_ARGS__);         \
+    } else {                                                                                    \
+        hipLaunchKernelGGL((KER<OP, RX, false, false>), GRID, BLOCK, 0, S, __VA_ARGS__);        \
+    }
 
 #define LTL_DISPATCH_B(KER, OP, B, GRID, BLOCK, S, ...)                               \
     if ((B) != 1) return hipErrorInvalidValue;                                          \
@@ -807,8 +914,8 @@ static hipError_t launch_gen_mw(int op, const ChunkArgs &a, hipStream_t s) {
     case LTL_NEXT: LTL_DISPATCH_K(k_gen1m, LTL_NEXT, a.K, grid_tiles(a), s, a); break;
     case LTL_FIN: LTL_DISPATCH_K(k_gen1m, LTL_FIN, a.K, grid_tiles(a), s, a); break;
     case LTL_GLOB: LTL_DISPATCH_K(k_gen1m, LTL_GLOB, a.K, grid_tiles(a), s, a); break;
-    case LTL_AND: hipLaunchKernelGGL((k_gen2<LTL_AND, false, true>), grid_binary(a), dim3(256), 0, s, a); break;
-    case LTL_OR: hipLaunchKernelGGL((k_gen2<LTL_OR, false, true>), grid_binary(a), dim3(256), 0, s, a); break;
+    case LTL_AND: hipLaunchKernelGGL((k_gen2<LTL_AND, false, true, false>), grid_binary(a), dim3(256), 0, s, a); break;
+    case LTL_OR: hipLaunchKernelGGL((k_gen2<LTL_OR, false, true, false>), grid_binary(a), dim3(256), 0, s, a); break;
     case LTL_UNTIL:
         switch (a.K) {
         case 2: hipLaunchKernelGGL(k_gen2m<2>, grid_binary(a), dim3(256), 0, s, a); break;
@@ -839,19 +946,25 @@ static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s)
 
 hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, hipStream_t s) {
     if (a.kmode || a.ws != kTileWS) return hipErrorInvalidValue;
-    if (a.gen_check) hipLaunchKernelGGL(k_gen_xfg<true>, grid_tiles(a), dim3(256), 0, s, a, n, slots);
-    else hipLaunchKernelGGL(k_gen_xfg<false>, grid_tiles(a), dim3(256), 0, s, a, n, slots);
+    if (a.t_succ) {
+        if (a.gen_check) return hipErrorInvalidValue;
+        hipLaunchKernelGGL((k_gen_xfg<false, true>), grid_tiles(a), dim3(256), 0, s, a, n, slots);
+    } else if (a.gen_check) {
+        hipLaunchKernelGGL((k_gen_xfg<true, false>), grid_tiles(a), dim3(256), 0, s, a, n, slots);
+    } else {
+        hipLaunchKernelGGL((k_gen_xfg<false, false>), grid_tiles(a), dim3(256), 0, s, a, n, slots);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s) {
     if (a.ws != kTileWS) return hipErrorInvalidValue;
-    if (B == 0) return a.kmode ? hipErrorInvalidValue : launch_gen_mw(op, a, s);
+    if (B == 0) return a.kmode || a.t_succ ? hipErrorInvalidValue : launch_gen_mw(op, a, s);
     return a.kmode ? launch_gen_x<true>(op, B, a, s) : launch_gen_x<false>(op, B, a, s);
 }
 
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
-    if (a.ws != kTileWS || (B != 1 && !a.store_words)) return hipErrorInvalidValue;
+    if (a.ws != kTileWS || (B != 1 && a.deferred)) return hipErrorInvalidValue;
     // one lane per survivor; the survivor count is on the device, so the
     // grid covers every candidate of the chunk and idle lanes exit
     const int nd = op == LTL_UNTIL ? 2 : 1;
@@ -867,6 +980,11 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
         case LTL_UNTIL: LTL_DISPATCH_K(k_emitm, LTL_UNTIL, a.K, g, s, a); break;
         default: return hipErrorInvalidValue;
         }
+        return hipGetLastError();
+    }
+    if (a.deferred) {
+        if (op < LTL_NOT || op > LTL_UNTIL) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_emit_meta, g, b, 0, s, a, op);
         return hipGetLastError();
     }
     switch (op) {

@@ -78,6 +78,9 @@ struct DevState {
     int probe_fail;               // table full
     unsigned long long neg_found[2];  // per level parity: smallest id of a candidate whose
                                       // complement is accepted
+    // Deferred level: per successor operator (~, X, F, G), the smallest store
+    // index of a set whose successor is accepted, kEmpty = none
+    unsigned long long succ_min[4];
 };
 
 // One chunk of one (operator, split) block, in canonical order.  Binary:
@@ -138,9 +141,18 @@ struct ChunkArgs {
     // 0: the level was tested completely by a check pass before it is
     // generated, so the gen kernels skip the target check
     int32_t gen_check = 1;
-    // 0: the emit kernels store summaries and metadata but not the words (a
-    // level whose words no later pass reads; packed layout)
-    int32_t store_words = 1;
+    // Deferred level (packed layout): 1 = the emit stores the metadata only,
+    // neither words nor summaries (no later pass reads them), and checks the
+    // next level's unary blocks over this level: successor j of a survivor
+    // (~c, X c, F c, G c) accepted lowers succ_min[j] to its store index.
+    // ~c: the survivor whose candidate id is neg_pick (the smallest id whose
+    // complement the level's check pass accepted).  X c, F c, G c (bits 1-3
+    // of succ_mask): tested in generation, which writes the accepted ones as
+    // bits of t_succ[candidate temp index].
+    int32_t deferred = 0;
+    int32_t succ_mask = 0;
+    uint8_t *t_succ = nullptr;
+    uint64_t neg_pick = kEmpty;
 };
 
 // Check passes visit words alternately from the positive and the negative
@@ -168,9 +180,9 @@ hipError_t launch_finish(const ChunkArgs &a, hipStream_t s);
 // one lane per output position.
 hipError_t launch_compact(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s);
-// Words of the stored sets [s0, s1) recomputed from their parents (operator
-// and operand indices of the metadata): sets emitted with store_words = 0.
-// Packed layout.
+// Words and summaries of the stored sets [s0, s1) recomputed from their
+// parents (operator and operand indices of the metadata): sets emitted with
+// deferred = 1.  Packed layout.
 hipError_t launch_materialize(const ChunkArgs &a, int64_t s0, int64_t s1, hipStream_t s);
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s);
 // dst[r * ws + c] = src[r * ncols + c] for r < nrows, c < ncols.

@@ -99,6 +99,7 @@ struct ltl_ctx {
     int32_t *d_left = nullptr, *d_right = nullptr;
     uint32_t *t_slot = nullptr, *t_cnt = nullptr, *t_off = nullptr, *t_list = nullptr;
     uint64_t *t_fp2 = nullptr, *t_mask = nullptr;
+    uint8_t *t_succ = nullptr;    // per candidate: accepted successors (deferred level)
     uint64_t *d_stage = nullptr;  // atom words and summaries, contiguous, before the scatter
     uint32_t *d_kinfo = nullptr;  // per trace: relaxed-key bit range (dedup modes 1-3)
     uint64_t *d_ptrace = nullptr; // packed layout: per packed trace, trace | offset << 32 | length << 40
@@ -106,8 +107,9 @@ struct ltl_ctx {
     size_t scan_bytes = 0;
     std::vector<int64_t> lvl_s, lvl_e;
     // Stored sets [lazy_s, lazy_e) of the last call were emitted with
-    // summaries only (a last level whose words no later pass read); their
-    // words are recomputed from the parents by the first ltl_store_read.
+    // metadata only (a deferred level); their words and summaries are
+    // recomputed from the parents by the first ltl_store_read or
+    // ltl_store_summaries.
     int64_t lazy_s = 0, lazy_e = 0;
     ChunkArgs lazy_args{};
 
@@ -250,9 +252,9 @@ static int32_t ensure_pool(ltl_ctx *c) {
 static int32_t carve(ltl_ctx *c) {
     const size_t budget = c->pool_bytes;
     const size_t per_cs = (size_t)c->nw * 8 + (size_t)c->nsw * 32 + 25;
-    // chunk: per-candidate temporaries (18 B each) within 1/32 of the pool
+    // chunk: per-candidate temporaries (19 B each) within 1/32 of the pool
     int64_t ch = 1 << 26;
-    while (ch > 4096 && (size_t)ch * 18 > budget / 32) ch >>= 1;
+    while (ch > 4096 && (size_t)ch * 19 > budget / 32) ch >>= 1;
     // A fixed table keeps room for the stored sets: a chunk takes at most a
     // quarter of its slots.
     if (c->opt.hash_slots_log2)
@@ -262,7 +264,8 @@ static int32_t carve(ltl_ctx *c) {
     c->scan_bytes = scan_temp_bytes((int)tiles);
     size_t fixed = align_up((size_t)c->nw * sizeof(WordAux), 256) + align_up(sizeof(DevState), 256) +
                    align_up((size_t)c->nsw * sizeof(SumAux), 256) +
-                   2 * align_up((size_t)ch * 4, 256) + align_up((size_t)ch * 8, 256) + align_up(tiles * 16, 256) +
+                   2 * align_up((size_t)ch * 4, 256) + align_up((size_t)ch * 8, 256) + align_up((size_t)ch, 256) +
+                   align_up(tiles * 16, 256) +
                    2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) +
                    align_up((size_t)(c->nw + 4 * c->nsw) * c->A * 8, 256) + align_up((size_t)c->W * 4, 256) +
                    align_up((size_t)c->W * 8, 256) + 11 * 256;
@@ -302,6 +305,7 @@ static int32_t carve(ltl_ctx *c) {
     c->t_slot = (uint32_t *)take((size_t)ch * 4);
     c->t_list = (uint32_t *)take((size_t)ch * 4);
     c->t_fp2 = (uint64_t *)take((size_t)ch * 8);
+    c->t_succ = (uint8_t *)take((size_t)ch);
     c->t_mask = (uint64_t *)take(tiles * 16);
     c->t_cnt = (uint32_t *)take(tiles * 4);
     c->t_off = (uint32_t *)take(tiles * 4);
@@ -611,6 +615,13 @@ struct Run {
     std::vector<LBlock> lblk;  // binary blocks of the current check pass
     int64_t n_ent = 0;         // their tile pairs
     int cur_level = 0;         // level of the blocks being registered
+    // Deferred level [defer_s, defer_e) (stored without words and
+    // summaries): check passes do not launch the unary blocks over it; the
+    // successors were tested in its generation (succ_mask), and the blocks
+    // are listed here as (first candidate id, successor index).
+    int64_t defer_s = -1, defer_e = -1;
+    int succ_mask = 0;
+    std::vector<std::pair<uint64_t, int>> succ_blocks;
 
     // Check-only pass over one block: no temporaries, so no chunking beyond
     // keeping the grid within limits.
@@ -696,13 +707,15 @@ struct Run {
                                    c->stream));
             PendingEvent &pe = c->pending.back();
             pe.count = slot;
-            // operand words read (negation without words: the operand's summaries),
-            // words written, summaries written, metadata
+            // operand words read, words and summaries written, metadata (a
+            // deferred level: metadata and the successor flag only)
             const uint64_t sv = (uint64_t)c->nsw * 32;
-            const bool sw = a.store_words != 0;
-            const uint64_t rd = !sw && op == LTL_NOT ? sv : (bin ? 2 : 1) * cs;
-            pe.alg_per = rd + (sw ? cs : 0) + sv + 17 + 8 + 4 + 12;
-            pe.min_per = (sw ? cs : 0) + sv + 17 + 8 + 4 + 12;
+            if (a.deferred) {
+                pe.alg_per = pe.min_per = 17 + 8 + 4 + 12 + (a.t_succ ? 1 : 0);
+            } else {
+                pe.alg_per = (bin ? 2 : 1) * cs + cs + sv + 17 + 8 + 4 + 12;
+                pe.min_per = cs + sv + 17 + 8 + 4 + 12;
+            }
         }
         return 0;
     }
@@ -750,6 +763,7 @@ struct Run {
                 b.t_mask = c->t_mask + j * tiles;
                 b.t_cnt = c->t_cnt + j * tiles;
                 b.t_off = c->t_off + j * tiles;
+                if (a.t_succ) b.t_succ = c->t_succ + j * slots;
                 if (int32_t r = store_chunk(ops[j], b, (uint64_t)n)) return r;
             }
         all_ltls += 3 * (uint64_t)n;
@@ -764,6 +778,13 @@ struct Run {
             return 0;
         }
         blocks.push_back({all_ltls, (uint64_t)n, op, s, n, 0, 1, cur_level});
+        if (check_only && s == defer_s && e == defer_e) {
+            const int j = op == LTL_NOT ? 0 : op == LTL_NEXT ? 1 : op == LTL_FIN ? 2 : 3;
+            succ_blocks.push_back({all_ltls, j});
+            checked += (uint64_t)n;
+            all_ltls += (uint64_t)n;
+            return 0;
+        }
         // (a chunk's temporaries hold CH candidates, the tile shift up to 63)
         const int64_t step = check_only ? ((int64_t)1 << 36) : c->CH - 64;
         for (int64_t a0 = 0; a0 < n; a0 += step) {
@@ -977,6 +998,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     DevState st0{};
     st0.found = kEmpty;
     st0.neg_found[0] = st0.neg_found[1] = kEmpty;
+    for (auto &m : st0.succ_min) m = kEmpty;
     st0.n_stored = A;
     st0.overflow_chunk = LLONG_MAX;
     hipStream_t s = c->stream;
@@ -1257,6 +1279,10 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     // completely without a hit, the negation of such a level-c candidate is
     // an accepted formula of cost c + 1, and level c is never generated.
     const bool anti_ok = learned && o.neg_type == 1 && c2 == 1;
+    // plain order: deferred levels and the check of their negation successors
+    const bool defer_ok = !learned && B == 1 && !o.dedup_mode;
+    const bool anti_plain = defer_ok && o.neg_type == 1 && c2 == 1;
+    uint64_t neg_pick = kEmpty;  // smallest id of the level whose complement is accepted
     int looked_ahead = -1;  // level whose binary blocks were tested early
     int found_cost = -1;
     bool neg_answer = false;
@@ -1348,38 +1374,76 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         } else if (o.precheck || run.otf || learned) {
             const uint64_t save = run.all_ltls;
             const size_t nb = run.blocks.size();
-            run.check_only = true;
-            run.lblk.clear();
-            run.n_ent = 0;
-            rc = level_blocks(cost);
-            if (!rc && !run.lblk.empty()) rc = ranked_level(cost, cost - 1);
-            if (!rc) rc = sync_state();
-            run.check_only = false;
+            // (plain order: the pass also records the smallest candidate id
+            // whose complement is accepted, the negation successor of a
+            // deferred level)
+            const bool anti = anti_plain && !run.otf;
+            for (int pass = 0; pass < 2; ++pass) {
+                run.check_only = true;
+                run.lblk.clear();
+                run.n_ent = 0;
+                run.succ_blocks.clear();
+                if (anti && launch_fill_u64(reinterpret_cast<uint64_t *>(&c->d_state->neg_found[0]), kEmpty, 1, s) !=
+                                hipSuccess) {
+                    rc = LTL_E_HIP;
+                    break;
+                }
+                run.base.anti = anti ? 1 : 0;
+                run.base.neg_slot = 0;
+                rc = level_blocks(cost);
+                run.base.anti = 0;
+                if (!rc && !run.lblk.empty()) rc = ranked_level(cost, cost - 1);
+                if (!rc) rc = sync_state();
+                run.check_only = false;
+                if (rc) break;
+                neg_pick = anti ? h.neg_found[0] : kEmpty;
+                // unary blocks over a deferred level: their successors were
+                // tested in its generation (smallest store index per operator)
+                for (const auto &sb : run.succ_blocks) {
+                    const uint64_t m = h.succ_min[sb.second];
+                    if (((run.succ_mask >> sb.second) & 1) && m != kEmpty)
+                        h.found = std::min<uint64_t>(h.found, sb.first + (m - (uint64_t)run.defer_s));
+                }
+                if (h.found != kEmpty) break;
+                // (a deferred level is always followed by a level that has an
+                // accepted candidate; should this one have none, its words and
+                // summaries are recomputed, and the pass repeated if it skipped
+                // blocks over it)
+                if (c->lazy_e > c->lazy_s) {
+                    if (launch_materialize(c->lazy_args, c->lazy_s, c->lazy_e, s) != hipSuccess) {
+                        rc = LTL_E_HIP;
+                        break;
+                    }
+                    c->lazy_s = c->lazy_e = 0;
+                }
+                run.defer_s = run.defer_e = -1;
+                if (run.succ_blocks.empty()) break;
+                run.all_ltls = save;
+                run.blocks.resize(nb);
+            }
             if (rc) break;
             if (h.found != kEmpty) { found = true; found_cost = cost; Le[cost] = n_stored; break; }
             tested = true;
             if (run.otf) generate = false;  // check-only level (overflow)
             else if (!learned) { run.all_ltls = save; run.blocks.resize(nb); }
-            // (a level emitted without words is always followed by a level
-            // that has an accepted candidate; should this one have none,
-            // the words are recomputed before anything reads them)
-            if (c->lazy_e > c->lazy_s) {
-                if (launch_materialize(c->lazy_args, c->lazy_s, c->lazy_e, s) != hipSuccess) { rc = LTL_E_HIP; break; }
-                c->lazy_s = c->lazy_e = 0;
-            }
         }
-        // Plain order: a level whose words no later pass reads is emitted
-        // with summaries and metadata only.  That is the level of cost maxc
-        // (the last one), and level `cost` when level cost + 1 has an
-        // accepted candidate: the check pass of cost + 1 reads level `cost`
-        // through its summaries only (unary blocks) and combines levels up
-        // to cost - 1 in its binary blocks.  Those binary blocks are tested
-        // here, before the level is generated: an accepted candidate among
-        // them makes cost + 1 the last level (levels up to `cost` have
-        // none).  The pass is repeated in canonical order after the level,
-        // which returns the same candidate as before.
+        // Plain order: a level whose sets no later pass reads in full is
+        // deferred: stored with metadata only, no words, no summaries.  That
+        // is the level of cost maxc (the last one), and level `cost` when
+        // level cost + 1 has an accepted candidate.  The binary blocks of
+        // cost + 1 combine levels up to cost - 1, so they are tested here,
+        // before the level is generated; an accepted candidate among them
+        // makes cost + 1 the last level (levels up to `cost` have none), and
+        // the unary blocks of cost + 1 over level `cost` are then tested in
+        // its generation instead (the successors ~c, X c, F c, G c of every
+        // candidate, at position 0).  Only the unary blocks that come before
+        // the hit in canonical order matter: the negation block when the hit
+        // is an & or | candidate, all four when it is a U candidate.  The
+        // check pass of cost + 1 then takes the smallest id over its binary
+        // blocks and these.
         bool lazy = false;
-        if (generate && tested && !learned && !run.otf && B == 1 && !run.logging) {
+        int succ_mask = 0;
+        if (generate && tested && defer_ok && !run.otf && !run.logging) {
             if (cost == maxc) {
                 lazy = true;
             } else {
@@ -1390,13 +1454,21 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
                 rc = level_blocks(cost + 1);
                 run.only_binary = false;
                 run.check_only = false;
+                if (!rc) rc = sync_state();
+                int hit_op = -1;
+                if (!rc && h.found != kEmpty)
+                    for (size_t b = nb; b < run.blocks.size(); ++b)
+                        if (h.found >= run.blocks[b].id_base && h.found < run.blocks[b].id_base + run.blocks[b].size)
+                            hit_op = run.blocks[b].op;
                 run.all_ltls = save;
                 run.checked = save_checked;
                 run.blocks.resize(nb);
-                if (!rc) rc = sync_state();
                 if (rc) break;
                 if (h.found != kEmpty) {
                     lazy = true;
+                    const int avail = (o.neg_type == 1 && c2 == 1 ? 1 : 0) | (c5 == 1 ? 2 : 0) |
+                                      (c6 == 1 ? 4 : 0) | (c7 == 1 ? 8 : 0);
+                    succ_mask = (hit_op == LTL_AND || hit_op == LTL_OR ? 1 : 15) & avail;
                     if (launch_fill_u64(reinterpret_cast<uint64_t *>(&c->d_state->found), kEmpty, 1, s) != hipSuccess) {
                         rc = LTL_E_HIP;
                         break;
@@ -1432,16 +1504,32 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         if (generate) {
             // a tested level has no accepted candidate: generation skips the check
             run.base.gen_check = tested ? 0 : 1;
-            run.base.store_words = lazy ? 0 : 1;
+            if (lazy) {
+                run.base.deferred = 1;
+                run.base.succ_mask = succ_mask;
+                run.base.t_succ = (succ_mask & 14) ? c->t_succ : nullptr;
+                run.base.neg_pick = (succ_mask & 1) ? neg_pick : kEmpty;
+                if (succ_mask &&
+                    launch_fill_u64(reinterpret_cast<uint64_t *>(c->d_state->succ_min), kEmpty, 4, s) != hipSuccess) {
+                    rc = LTL_E_HIP;
+                    break;
+                }
+            }
             rc = level_blocks(cost);
             run.base.gen_check = 1;
-            run.base.store_words = 1;
+            run.base.deferred = 0;
+            run.base.succ_mask = 0;
+            run.base.t_succ = nullptr;
+            run.base.neg_pick = kEmpty;
             if (rc) break;
             if ((rc = sync_state())) break;
             if (lazy && h.n_stored > n_stored) {
                 c->lazy_s = n_stored;
                 c->lazy_e = h.n_stored;
                 c->lazy_args = run.base;
+                run.defer_s = n_stored;
+                run.defer_e = h.n_stored;
+                run.succ_mask = succ_mask;
             }
             n_stored = h.n_stored;
             if (h.found != kEmpty) { found = true; found_cost = cost; Le[cost] = n_stored; break; }
@@ -1464,6 +1552,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     res->candidates = run.candidates;
     res->checked = run.checked;
     res->words_deferred = c->lazy_e - c->lazy_s;
+    res->deferred_succ = run.defer_s >= 0 ? run.succ_mask : 0;
     res->hbm_bytes = run.alg_bytes;
     for (auto &k : c->stats) res->kernel_seconds += k.seconds;
 
@@ -1786,7 +1875,7 @@ extern "C" int32_t ltl_store_read(ltl_ctx *c, int64_t first, int64_t count, uint
     if (count == 0) return 0;
     if (first + count > c->capacity) return LTL_E_ARG;
     HIP_TRY(hipSetDevice(c->opt.device));
-    if (c->lazy_e > c->lazy_s) {  // sets emitted with summaries only: their words first
+    if (c->lazy_e > c->lazy_s) {  // a deferred level: its words and summaries first
         HIP_TRY(launch_materialize(c->lazy_args, c->lazy_s, c->lazy_e, c->stream));
         c->lazy_s = c->lazy_e = 0;
     }
@@ -1823,6 +1912,10 @@ extern "C" int32_t ltl_store_summaries(ltl_ctx *c, int64_t first, int64_t count,
     if (count == 0) return 0;
     if (first + count > c->capacity) return LTL_E_ARG;
     HIP_TRY(hipSetDevice(c->opt.device));
+    if (c->lazy_e > c->lazy_s) {  // a deferred level: its words and summaries first
+        HIP_TRY(launch_materialize(c->lazy_args, c->lazy_s, c->lazy_e, c->stream));
+        c->lazy_s = c->lazy_e = 0;
+    }
     HIP_TRY(hipStreamSynchronize(c->stream));
     const int nsw = c->nsw, W = c->W;
     const int64_t t0 = first / 64, t1 = (first + count - 1) / 64 + 1;

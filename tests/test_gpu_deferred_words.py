@@ -1,15 +1,23 @@
-"""Levels emitted with summaries only (plain order, packed layout): the level
-of cost max_cost, and level c when level c + 1 has an accepted candidate among
-its binary blocks (which combine levels up to c - 1, so they are tested before
-level c is generated).  No later pass of the call reads such a level's words;
-the next ltl_store_read recomputes them from the parents.
+"""Deferred levels (plain order, packed layout), stored with metadata only:
+the level of cost max_cost, and level c when level c + 1 has an accepted
+candidate among its binary blocks (which combine levels up to c - 1, so they
+are tested before level c is generated).  No later pass of the call reads
+such a level's words or summaries; the next ltl_store_read or
+ltl_store_summaries recomputes them from the parents.
+
+The unary blocks of the last level over a deferred level are checked without
+its summaries: ~c through the complements accepted by the deferred level's
+own check pass (the smallest such candidate id is the first occurrence of its
+set), X c, F c, G c in its generation, when the first binary hit is a U
+candidate (X, F and G blocks come before U blocks in canonical order).
 
 Pinned here: results, counts, level boundaries and every stored set (the
 deferred level included, read back through the recomputation) equal the
-oracle's; the deferred level's summaries agree with its recomputed words; the
-deferral happens where it should (result field words_deferred) and not
-without the precheck or with K words per trace (the learned order keeps its
-own lookahead and never defers)."""
+oracle's; the deferred level's summaries agree with its recomputed words;
+answers in the negation and X / F / G blocks next to binary twins; the
+deferral happens where it should (result fields words_deferred and
+deferred_succ) and not without the precheck or with K words per trace (the
+learned order keeps its own lookahead and never defers)."""
 import numpy as np
 import pytest
 
@@ -84,6 +92,25 @@ def test_negated_answer_with_binary_twin(ltl, oracle, learner, planted):
     inst = ltl.generate_custom(7, 20, 20, 3, 3, 9, 0.5, planted=planted)
     gpu, ref = check_parity(ltl, oracle, learner, inst, max_cost=9)
     assert gpu["status"] == 1
+
+
+# (planted, seed, P, N, len_min, len_max, hamming_k): the first hit among the
+# binary blocks of the last level is a U candidate (no & or | candidate is
+# accepted there), so the ~, X, F, G blocks over the deferred level are
+# tested in its generation; in the first three the answer is in those blocks.
+U_HIT = [("(F(a))U(b)", 1584, 17, 17, 5, 16, 0), ("(F(a))U(b)", 1591, 24, 17, 5, 16, 0),
+         ("X((a)U(b))", 1388, 15, 15, 3, 10, 2), ("(a)U(b)", 1000, 15, 15, 3, 6, 0)]
+
+
+@pytest.mark.parametrize("case", U_HIT, ids=lambda c: f"{c[0]}-{c[1]}-k{c[6]}")
+def test_successors_tested_in_generation(ltl, oracle, learner, case):
+    planted, seed, P, N, lo, hi, k = case
+    inst = ltl.generate_custom(seed, P, N, 3, lo, hi, 0.5, planted=planted)
+    gpu, ref = check_parity(ltl, oracle, learner, inst, max_cost=9, hamming_k=k)
+    assert gpu["status"] == 1
+    assert gpu["deferred_succ"] & 14 and gpu["words_deferred"] > 0
+    if planted != "(a)U(b)":
+        assert gpu["formula"][0] in "XFG"
 
 
 def test_store_read_twice_and_next_call(ltl, oracle, learner):
