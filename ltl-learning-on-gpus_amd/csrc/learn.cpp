@@ -610,6 +610,18 @@ struct Run {
     bool logging = false;     // check pass: binary blocks are also logged
     bool skip_binary = false; // check pass: binary blocks were tested ahead of the level
     bool only_binary = false; // check pass: test the binary blocks only (lookahead)
+    // Plain order: the binary blocks of this check pass were tested by the
+    // lookahead ahead of the level; they are registered without a launch,
+    // and the lookahead's first hit (index among the pass's binary blocks,
+    // offset in the block; -1: none) becomes a candidate id of this pass.
+    // Likewise for the lookahead's smallest candidate whose complement is
+    // accepted (la_neg_block, la_neg_off -> la_neg).
+    bool reuse_binary = false;
+    int64_t la_block = -1, la_neg_block = -1;
+    uint64_t la_off = 0, la_neg_off = 0;
+    int64_t bin_seen = 0;             // binary blocks registered in the pass
+    uint64_t la_found = ~0ull;        // the hit's id in this pass
+    uint64_t la_neg = ~0ull;
     uint64_t checked = 0;
     std::vector<Block> blocks;
     std::vector<LBlock> lblk;  // binary blocks of the current check pass
@@ -811,6 +823,13 @@ struct Run {
             return 0;
         }
         blocks.push_back({all_ltls, size, op, ls, nL, rs, nR, cur_level});
+        if (check_only && reuse_binary) {
+            if (bin_seen == la_block) la_found = all_ltls + la_off;
+            if (bin_seen == la_neg_block) la_neg = all_ltls + la_neg_off;
+            ++bin_seen;
+            all_ltls += size;
+            return 0;
+        }
         if (check_only && (ranked || logging)) {
             const int64_t nTR = (nR + 63) / 64;
             lblk.push_back({op, li, ri, 0, ls, nL, rs, nR, all_ltls, nTR, n_ent});
@@ -1283,6 +1302,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     const bool defer_ok = !learned && B == 1 && !o.dedup_mode;
     const bool anti_plain = defer_ok && o.neg_type == 1 && c2 == 1;
     uint64_t neg_pick = kEmpty;  // smallest id of the level whose complement is accepted
+    int plain_ahead = -1;        // level whose binary blocks the lookahead tested
     int looked_ahead = -1;  // level whose binary blocks were tested early
     int found_cost = -1;
     bool neg_answer = false;
@@ -1390,13 +1410,18 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
                 }
                 run.base.anti = anti ? 1 : 0;
                 run.base.neg_slot = 0;
+                run.reuse_binary = pass == 0 && plain_ahead == cost;
+                run.bin_seen = 0;
+                run.la_found = run.la_neg = kEmpty;
                 rc = level_blocks(cost);
+                run.reuse_binary = false;
                 run.base.anti = 0;
                 if (!rc && !run.lblk.empty()) rc = ranked_level(cost, cost - 1);
                 if (!rc) rc = sync_state();
                 run.check_only = false;
                 if (rc) break;
-                neg_pick = anti ? h.neg_found[0] : kEmpty;
+                neg_pick = anti ? std::min<uint64_t>(h.neg_found[0], run.la_neg) : kEmpty;
+                h.found = std::min<uint64_t>(h.found, run.la_found);
                 // unary blocks over a deferred level: their successors were
                 // tested in its generation (smallest store index per operator)
                 for (const auto &sb : run.succ_blocks) {
@@ -1418,7 +1443,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
                 }
                 run.defer_s = run.defer_e = -1;
                 if (run.succ_blocks.empty()) break;
-                run.all_ltls = save;
+                run.all_ltls = save;  // again, launching every block
                 run.blocks.resize(nb);
             }
             if (rc) break;
@@ -1440,28 +1465,50 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         // the hit in canonical order matter: the negation block when the hit
         // is an & or | candidate, all four when it is a U candidate.  The
         // check pass of cost + 1 then takes the smallest id over its binary
-        // blocks and these.
+        // blocks and these.  Either way its binary blocks are not launched
+        // again: their first hit, if any, is the lookahead's (same blocks in
+        // the same order; ids shifted by the unary blocks before them).
         bool lazy = false;
         int succ_mask = 0;
         if (generate && tested && defer_ok && !run.otf && !run.logging) {
             if (cost == maxc) {
                 lazy = true;
             } else {
-                const uint64_t save = run.all_ltls, save_checked = run.checked;
+                const uint64_t save = run.all_ltls;
                 const size_t nb = run.blocks.size();
+                if (anti_plain &&
+                    launch_fill_u64(reinterpret_cast<uint64_t *>(&c->d_state->neg_found[1]), kEmpty, 1, s) != hipSuccess) {
+                    rc = LTL_E_HIP;
+                    break;
+                }
                 run.check_only = true;
                 run.only_binary = true;
+                run.base.anti = anti_plain ? 1 : 0;
+                run.base.neg_slot = 1;
                 rc = level_blocks(cost + 1);
+                run.base.anti = 0;
+                run.base.neg_slot = 0;
                 run.only_binary = false;
                 run.check_only = false;
                 if (!rc) rc = sync_state();
                 int hit_op = -1;
-                if (!rc && h.found != kEmpty)
-                    for (size_t b = nb; b < run.blocks.size(); ++b)
-                        if (h.found >= run.blocks[b].id_base && h.found < run.blocks[b].id_base + run.blocks[b].size)
-                            hit_op = run.blocks[b].op;
+                run.la_block = run.la_neg_block = -1;
+                const uint64_t la_neg_id = anti_plain ? h.neg_found[1] : kEmpty;
+                if (!rc)
+                    for (size_t b = nb; b < run.blocks.size(); ++b) {
+                        const Block &bk = run.blocks[b];
+                        if (h.found >= bk.id_base && h.found - bk.id_base < bk.size) {
+                            hit_op = bk.op;
+                            run.la_block = (int64_t)(b - nb);
+                            run.la_off = h.found - bk.id_base;
+                        }
+                        if (la_neg_id >= bk.id_base && la_neg_id - bk.id_base < bk.size) {
+                            run.la_neg_block = (int64_t)(b - nb);
+                            run.la_neg_off = la_neg_id - bk.id_base;
+                        }
+                    }
+                if (!rc) plain_ahead = cost + 1;
                 run.all_ltls = save;
-                run.checked = save_checked;
                 run.blocks.resize(nb);
                 if (rc) break;
                 if (h.found != kEmpty) {
