@@ -233,23 +233,32 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
 // three blocks read the same level when their costs are equal): three
 // candidates per operand, three fingerprint pairs, three inserts.  The table
 // keeps the smallest id, and X ids < F ids < G ids, so the dedup outcome is
-// the one of three consecutive passes.  Packed layout, exact keys.
-template <bool CHK, bool SUCC>
-__global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t slots) {
+// the one of three consecutive passes.  NEG: the negation block of the same
+// operand level too (four candidates per operand; the negation block's ids
+// start at A.id_base, the X block's at id_xfg, after the & and | blocks that
+// come between them in canonical order).  Temporaries of candidate j at
+// j * slots.  Packed layout, exact keys.
+template <bool CHK, bool SUCC, bool NEG>
+__global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t slots, uint64_t id_xfg) {
     if (run_stopped(A)) return;
     Tile t;
     if (!tile_unary(A, t)) return;
     const bool otf = chunk_otf(A);
     const int64_t s = A.ls + (t.valid ? t.a : A.a0);
     const uint64_t *p = A.store + set_base(s, A.nw);
-    Acc c[3];
-    SuccAcc sc[3];
+    constexpr int NC = NEG ? 4 : 3;
+    Acc c[NC];
+    SuccAcc sc[NC];
     const int sm = A.succ_mask;
     auto step = [&](int w, uint64_t v) {
         const WordAux *x = A.aux + w;
-        const uint64_t y[3] = {pk_unary<LTL_NEXT>(v, x), pk_unary<LTL_FIN>(v, x), pk_unary<LTL_GLOB>(v, x)};
+        uint64_t y[NC];
+        if constexpr (NEG) y[0] = pk_unary<LTL_NOT>(v, x);
+        y[NC - 3] = pk_unary<LTL_NEXT>(v, x);
+        y[NC - 2] = pk_unary<LTL_FIN>(v, x);
+        y[NC - 1] = pk_unary<LTL_GLOB>(v, x);
 #pragma unroll
-        for (int j = 0; j < 3; ++j) {
+        for (int j = 0; j < NC; ++j) {
             fold<CHK>(c[j], y[j], x);
             succ_fold<SUCC>(sc[j], y[j], x, sm);
         }
@@ -267,8 +276,9 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
     if (!t.valid) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        const uint64_t id = A.id_base + (uint64_t)j * (uint64_t)n + (uint64_t)t.a;
+    for (int j = 0; j < NC; ++j) {
+        const uint64_t id = NEG ? (j == 0 ? A.id_base : id_xfg + (uint64_t)(j - 1) * (uint64_t)n) + (uint64_t)t.a
+                                : A.id_base + (uint64_t)j * (uint64_t)n + (uint64_t)t.a;
         if (CHK && c[j].mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id);
         if (otf) continue;
         const size_t k = (size_t)j * (size_t)slots + ix;
@@ -944,15 +954,20 @@ static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, hipStream_t s) {
+hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, bool neg, uint64_t id_xfg, hipStream_t s) {
     if (a.kmode || a.ws != kTileWS) return hipErrorInvalidValue;
-    if (a.t_succ) {
+    const dim3 g = grid_tiles(a), b(256);
+    if (neg) {
+        if (a.gen_check) return hipErrorInvalidValue;  // tested levels only
+        if (a.t_succ) hipLaunchKernelGGL((k_gen_xfg<false, true, true>), g, b, 0, s, a, n, slots, id_xfg);
+        else hipLaunchKernelGGL((k_gen_xfg<false, false, true>), g, b, 0, s, a, n, slots, id_xfg);
+    } else if (a.t_succ) {
         if (a.gen_check) return hipErrorInvalidValue;
-        hipLaunchKernelGGL((k_gen_xfg<false, true>), grid_tiles(a), dim3(256), 0, s, a, n, slots);
+        hipLaunchKernelGGL((k_gen_xfg<false, true, false>), g, b, 0, s, a, n, slots, a.id_base);
     } else if (a.gen_check) {
-        hipLaunchKernelGGL((k_gen_xfg<true, false>), grid_tiles(a), dim3(256), 0, s, a, n, slots);
+        hipLaunchKernelGGL((k_gen_xfg<true, false, false>), g, b, 0, s, a, n, slots, a.id_base);
     } else {
-        hipLaunchKernelGGL((k_gen_xfg<false, false>), grid_tiles(a), dim3(256), 0, s, a, n, slots);
+        hipLaunchKernelGGL((k_gen_xfg<false, false, false>), g, b, 0, s, a, n, slots, a.id_base);
     }
     return hipGetLastError();
 }

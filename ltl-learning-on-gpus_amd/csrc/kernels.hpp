@@ -172,8 +172,10 @@ __host__ __device__ __forceinline__ int interleave(int j, int np, int n) {
 hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s);
 // X, F and G of one operand block in one pass (packed layout, exact keys):
 // block j in {X, F, G} has ids id_base + j * n and its temporaries at offset
-// j * slots (t_slot, t_fp2).
-hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, hipStream_t s);
+// j * slots (t_slot, t_fp2).  neg (a tested level): the negation block too,
+// ids from id_base, temporaries at offset 0; then X, F, G have ids
+// id_xfg + j * n and temporaries at (j + 1) * slots.
+hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, bool neg, uint64_t id_xfg, hipStream_t s);
 hipError_t launch_decide(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_finish(const ChunkArgs &a, hipStream_t s);
 // Survivor list (candidate temp index per output position), then emit with

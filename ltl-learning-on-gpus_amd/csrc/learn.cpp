@@ -762,7 +762,8 @@ struct Run {
         const uint64_t alg = (uint64_t)n * (cs + 60);
         candidates += 3 * (uint64_t)n;
         alg_bytes += alg;
-        HIP_TRY(timed(c, K_GEN1, alg, alg, 3 * (uint64_t)n, [&] { return launch_gen_xfg(a, n, slots, c->stream); }));
+        HIP_TRY(timed(c, K_GEN1, alg, alg, 3 * (uint64_t)n,
+                      [&] { return launch_gen_xfg(a, n, slots, false, 0, c->stream); }));
         if (!otf)
             for (int j = 0; j < 3; ++j) {
                 ChunkArgs b = a;
@@ -778,6 +779,88 @@ struct Run {
                 if (a.t_succ) b.t_succ = c->t_succ + j * slots;
                 if (int32_t r = store_chunk(ops[j], b, (uint64_t)n)) return r;
             }
+        all_ltls += 3 * (uint64_t)n;
+        return 0;
+    }
+
+    // Chunk temporaries from offset `off` on (candidates; tiles = off / 64):
+    // the regions below it hold the X, F and G candidates of a fused pass
+    // while the & and | blocks run.
+    int64_t tmp_off = 0;
+    void set_tmp_off(int64_t off) {
+        tmp_off = off;
+        base.t_slot = c->t_slot + off;
+        base.t_fp2 = c->t_fp2 + off;
+        base.t_list = c->t_list + off;
+        base.t_mask = c->t_mask + off / 64;
+        base.t_cnt = c->t_cnt + off / 64;
+        base.t_off = c->t_off + off / 64;
+        if (base.t_succ) base.t_succ = c->t_succ + off;
+    }
+    int64_t ch_cap() const { return c->CH - tmp_off; }
+
+    // ~, X, F and G of one operand level in one generation pass (a tested
+    // level, equal costs): the negation block, first in the level, is
+    // decided and emitted at once; the X, F and G candidates wait in their
+    // temporaries (regions 1-3) until finish_xfg, after the & and | blocks
+    // that come between them in canonical order.  Their ids (from id_xfg)
+    // are larger than every & and | id, so the table's smallest-id rule
+    // gives the outcome of the canonical sequence.
+    struct PendXfg {
+        bool on = false;
+        int64_t n = 0, tiles = 0, slots = 0;
+        uint64_t id_xfg = 0;
+        ChunkArgs a{};
+    } pend;
+    int32_t unary_nxfg(int64_t s, int64_t e, uint64_t id_xfg) {
+        const int64_t n = e - s;
+        const int64_t tiles = (n + 63 + 63) / 64, slots = tiles * 64;
+        blocks.push_back({all_ltls, (uint64_t)n, LTL_NOT, s, n, 0, 1, cur_level});
+        ChunkArgs a = base;
+        a.id_base = all_ltls;
+        a.ls = s; a.rs = 0; a.nR = 1;
+        a.a0 = 0; a.a1 = n;
+        a.b0 = a.b1 = 0;
+        a.nbs = 1;
+        a.ush = (int32_t)((a.ls + a.a0) & 63);
+        a.n_tiles = (int32_t)((n + a.ush + 63) / 64);
+        a.chunk_idx = ++c->chunk_counter;
+        const uint64_t cs = (uint64_t)c->nw * 8;
+        const uint64_t alg = (uint64_t)n * (cs + 80);
+        candidates += 4 * (uint64_t)n;
+        alg_bytes += alg;
+        HIP_TRY(timed(c, K_GEN1, alg, alg, 4 * (uint64_t)n,
+                      [&] { return launch_gen_xfg(a, n, slots, true, id_xfg, c->stream); }));
+        ChunkArgs b = a;
+        b.id_end = b.id_base + (uint64_t)n;
+        if (int32_t r = store_chunk(LTL_NOT, b, (uint64_t)n)) return r;
+        all_ltls += (uint64_t)n;
+        pend = PendXfg{true, n, tiles, slots, id_xfg, a};
+        set_tmp_off(4 * slots);
+        return 0;
+    }
+    int32_t finish_xfg() {
+        set_tmp_off(0);
+        if (!pend.on) return 0;
+        pend.on = false;
+        const int64_t n = pend.n, s = pend.a.ls;
+        if (all_ltls != pend.id_xfg) return LTL_E_HIP;  // the & and | blocks did not add up
+        const int ops[3] = {LTL_NEXT, LTL_FIN, LTL_GLOB};
+        for (int j = 0; j < 3; ++j) blocks.push_back({all_ltls + (uint64_t)j * n, (uint64_t)n, ops[j], s, n, 0, 1, cur_level});
+        for (int j = 0; j < 3; ++j) {
+            ChunkArgs b = pend.a;
+            b.id_base = all_ltls + (uint64_t)j * n;
+            b.id_end = b.id_base + (uint64_t)n;
+            b.chunk_idx = ++c->chunk_counter;
+            b.t_slot = c->t_slot + (j + 1) * pend.slots;
+            b.t_fp2 = c->t_fp2 + (j + 1) * pend.slots;
+            b.t_list = c->t_list + (j + 1) * pend.slots;
+            b.t_mask = c->t_mask + (j + 1) * pend.tiles;
+            b.t_cnt = c->t_cnt + (j + 1) * pend.tiles;
+            b.t_off = c->t_off + (j + 1) * pend.tiles;
+            if (pend.a.t_succ) b.t_succ = c->t_succ + (j + 1) * pend.slots;
+            if (int32_t r = store_chunk(ops[j], b, (uint64_t)n)) return r;
+        }
         all_ltls += 3 * (uint64_t)n;
         return 0;
     }
@@ -798,7 +881,7 @@ struct Run {
             return 0;
         }
         // (a chunk's temporaries hold CH candidates, the tile shift up to 63)
-        const int64_t step = check_only ? ((int64_t)1 << 36) : c->CH - 64;
+        const int64_t step = check_only ? ((int64_t)1 << 36) : ch_cap() - 64;
         for (int64_t a0 = 0; a0 < n; a0 += step) {
             ChunkArgs a = base;
             a.id_base = all_ltls;
@@ -854,8 +937,8 @@ struct Run {
             return 0;
         }
         const int64_t row_slots = (nR + 63 + 63) / 64 * 64 * nd;  // stripes of a row, with the lane shift
-        if (row_slots <= c->CH) {
-            const int64_t rows = std::max<int64_t>(1, c->CH / row_slots);
+        if (row_slots <= ch_cap()) {
+            const int64_t rows = std::max<int64_t>(1, ch_cap() / row_slots);
             for (int64_t a0 = 0; a0 < nL; a0 += rows) {
                 ChunkArgs a = base;
                 a.id_base = all_ltls;
@@ -865,7 +948,7 @@ struct Run {
                 if (int32_t r = run_chunk(op, a)) return r;
             }
         } else {
-            const int64_t cols = std::max<int64_t>(64, (c->CH / nd - 64) / 64 * 64);
+            const int64_t cols = std::max<int64_t>(64, (ch_cap() / nd - 64) / 64 * 64);
             for (int64_t a0 = 0; a0 < nL; ++a0)
                 for (int64_t b0 = 0; b0 < nR; b0 += cols) {
                     ChunkArgs a = base;
@@ -1127,6 +1210,32 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     auto level_blocks = [&](int cost) -> int32_t {
         int32_t r = 0;
         run.cur_level = cost;
+        // Generation of a tested level with equal unary costs: ~, X, F and G
+        // of the operand level in one pass (regions of 4 * slots, at least
+        // 2^22 candidates left for the & and | chunks in between).
+        const int lvn = cost - c2;
+        const int64_t nn = lvn >= c1 ? Le[lvn] - Ls[lvn] : 0;
+        const bool fuse4 = !run.check_only && !run.count_only && !run.otf && run.base.gen_check == 0 &&
+                           !run.base.kmode && B == 1 && o.neg_type == 1 && lvn >= c1 && nn > 0 && c2 == c5 &&
+                           c5 == c6 && c6 == c7 && 4 * ((nn + 127) / 64 * 64) + ((int64_t)1 << 22) <= c->CH;
+        if (fuse4) {
+            uint64_t between = 0;  // candidates of the & and | blocks
+            for (int i = c1; 2 * i <= cost - c3; ++i)
+                between += (uint64_t)std::max<int64_t>(0, Le[i] - Ls[i]) *
+                           (uint64_t)std::max<int64_t>(0, Le[cost - i - c3] - Ls[cost - i - c3]);
+            for (int i = c1; 2 * i <= cost - c4; ++i)
+                between += (uint64_t)std::max<int64_t>(0, Le[i] - Ls[i]) *
+                           (uint64_t)std::max<int64_t>(0, Le[cost - i - c4] - Ls[cost - i - c4]);
+            if ((r = run.unary_nxfg(Ls[lvn], Le[lvn], run.all_ltls + (uint64_t)nn + between))) return r;
+            for (int i = c1; 2 * i <= cost - c3; ++i)
+                if ((r = run.binary(LTL_AND, i, cost - i - c3))) return r;
+            for (int i = c1; 2 * i <= cost - c4; ++i)
+                if ((r = run.binary(LTL_OR, i, cost - i - c4))) return r;
+            if ((r = run.finish_xfg())) return r;
+            for (int i = c1; 2 * i <= cost - c8; ++i)
+                if ((r = run.binary(LTL_UNTIL, i, cost - i - c8))) return r;
+            return 0;
+        }
         if ((o.neg_type == 1 && cost - c2 >= c1) || (o.neg_type == 2 && cost - c2 == c1))
             if ((r = run.unary(LTL_NOT, Ls[cost - c2], Le[cost - c2]))) return r;
         for (int i = c1; 2 * i <= cost - c3; ++i)
