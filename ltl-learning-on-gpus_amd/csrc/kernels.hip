@@ -684,6 +684,46 @@ __global__ __launch_bounds__(256) void k_materialize(ChunkArgs A, int64_t s0, in
     }
 }
 
+// Multi-word traces (B = 0): k_materialize for K words per trace, one trace
+// per step, its summary bits from all K words.
+template <int K>
+__global__ __launch_bounds__(256) void k_materialize_m(ChunkArgs A, int64_t s0, int64_t s1) {
+    const int64_t s = (s0 & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (s < s0 || s >= s1) return;
+    const int op = A.m_op[s];
+    const int64_t l = A.m_left[s], r = A.m_right[s] >= 0 ? A.m_right[s] : l;
+    const uint64_t *pl = A.store + set_base(l, A.nw), *pr = A.store + set_base(r, A.nw);
+    uint64_t *q = A.store + set_base(s, A.nw);
+    uint64_t *z = A.sum + set_base(s, 4 * A.nsw);
+    const int T = A.nw / K;
+    uint64_t v[4] = {0, 0, 0, 0};
+    for (int i = 0; i < T; ++i) {
+        uint64_t x[K], y[K], w[K];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) {
+            x[kk] = pl[(size_t)(i * K + kk) * kTileWS];
+            y[kk] = pr[(size_t)(i * K + kk) * kTileWS];
+        }
+        const WordAux *aux = A.aux + i * K;
+        switch (op) {
+        case LTL_NOT: trace_unary<LTL_NOT, K>(x, aux, w); break;
+        case LTL_NEXT: trace_unary<LTL_NEXT, K>(x, aux, w); break;
+        case LTL_FIN: trace_unary<LTL_FIN, K>(x, aux, w); break;
+        case LTL_GLOB: trace_unary<LTL_GLOB, K>(x, aux, w); break;
+        case LTL_UNTIL: trace_until<K>(x, y, w); break;
+        default:
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk)
+                w[kk] = op == LTL_AND ? (x[kk] & y[kk]) : op == LTL_OR ? (x[kk] | y[kk]) : x[kk];
+            break;
+        }
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) q[(size_t)(i * K + kk) * kTileWS] = w[kk];
+        summarize_trace<K>(w, aux, i & 63, v);
+        flush_summary(z, i, T, A.nsw, kTileWS, v);
+    }
+}
+
 // Multi-word traces (B = 0): one trace (K words) per step; its summary bits
 // come from all K words.
 template <int OP, int K>
@@ -853,6 +893,7 @@ __global__ __launch_bounds__(256) void k_chkUm(ChunkArgs A) {
     const uint64_t *pl = A.store + set_base(sl, A.nw);
     const uint64_t *pr = A.store + set_base(sr, A.nw);
     const uint32_t k = (uint32_t)A.hamming_k;
+    const bool anti = A.anti && t.valid;
     const int K = A.K, T = A.nw / K;
     uint32_t m0 = t.valid ? 0u : 0x7fffffffu, m1 = m0;  // invalid lanes never accept
     for (int j = 0; j < T; ++j) {
@@ -862,13 +903,18 @@ __global__ __launch_bounds__(256) void k_chkUm(ChunkArgs A) {
         const uint32_t tg = (uint32_t)(A.aux[i * K].tgt & 1ull);
         m0 += h0 ^ tg;
         m1 += h1 ^ tg;
-        // Mismatch counts only grow: stop once every lane of the wave fails.
-        if ((j & 3) == 3 && !__any(m0 <= k || m1 <= k)) return;
+        // Mismatch and match counts only grow: stop once every lane of the
+        // wave fails (with anti: the complements too).
+        const uint32_t seen = (uint32_t)j + 1;
+        if ((j & 3) == 3 && !__any(m0 <= k || m1 <= k || (anti && (seen - m0 <= k || seen - m1 <= k)))) return;
     }
     if (!t.valid) return;
+    const uint32_t seen = (uint32_t)T;
     const uint64_t id0 = A.id_base + 2 * ((uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b);
     if (m0 <= k) atomicMin(&A.st->found, (unsigned long long)id0);
     else if (m1 <= k) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
+    else if (anti && seen - m0 <= k) atomicMin(&A.st->neg_found[A.neg_slot], (unsigned long long)id0);
+    else if (anti && seen - m1 <= k) atomicMin(&A.st->neg_found[A.neg_slot], (unsigned long long)(id0 + 1));
 }
 
 __global__ void k_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, int *probe_fail) {
@@ -979,11 +1025,16 @@ hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s) {
 }
 
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
-    if (a.ws != kTileWS || (B != 1 && a.deferred)) return hipErrorInvalidValue;
+    if (a.ws != kTileWS || (B != 1 && a.t_succ)) return hipErrorInvalidValue;
     // one lane per survivor; the survivor count is on the device, so the
     // grid covers every candidate of the chunk and idle lanes exit
     const int nd = op == LTL_UNTIL ? 2 : 1;
     const dim3 g((unsigned)(((int64_t)a.n_tiles * 64 * nd + 63 + 255) / 256)), b(256);
+    if (a.deferred) {
+        if (op < LTL_NOT || op > LTL_UNTIL) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_emit_meta, g, b, 0, s, a, op);
+        return hipGetLastError();
+    }
     if (B == 0) {
         switch (op) {
         case LTL_NOT: LTL_DISPATCH_K(k_emitm, LTL_NOT, a.K, g, s, a); break;
@@ -995,11 +1046,6 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
         case LTL_UNTIL: LTL_DISPATCH_K(k_emitm, LTL_UNTIL, a.K, g, s, a); break;
         default: return hipErrorInvalidValue;
         }
-        return hipGetLastError();
-    }
-    if (a.deferred) {
-        if (op < LTL_NOT || op > LTL_UNTIL) return hipErrorInvalidValue;
-        hipLaunchKernelGGL(k_emit_meta, g, b, 0, s, a, op);
         return hipGetLastError();
     }
     switch (op) {
@@ -1019,7 +1065,14 @@ hipError_t launch_materialize(const ChunkArgs &a, int64_t s0, int64_t s1, hipStr
     if (a.ws != kTileWS || s0 < 0 || s1 > a.capacity) return hipErrorInvalidValue;
     if (s1 <= s0) return hipSuccess;
     const int64_t n = s1 - (s0 & ~63ll);
-    hipLaunchKernelGGL(k_materialize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, s0, s1);
+    const dim3 g((unsigned)((n + 255) / 256)), b(256);
+    switch (a.K) {
+    case 1: hipLaunchKernelGGL(k_materialize, g, b, 0, s, a, s0, s1); break;
+    case 2: hipLaunchKernelGGL(k_materialize_m<2>, g, b, 0, s, a, s0, s1); break;
+    case 3: hipLaunchKernelGGL(k_materialize_m<3>, g, b, 0, s, a, s0, s1); break;
+    case 4: hipLaunchKernelGGL(k_materialize_m<4>, g, b, 0, s, a, s0, s1); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -1408,7 +1408,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     // an accepted formula of cost c + 1, and level c is never generated.
     const bool anti_ok = learned && o.neg_type == 1 && c2 == 1;
     // plain order: deferred levels and the check of their negation successors
-    const bool defer_ok = !learned && B == 1 && !o.dedup_mode;
+    const bool defer_ok = !learned && !o.dedup_mode;
     const bool anti_plain = defer_ok && o.neg_type == 1 && c2 == 1;
     uint64_t neg_pick = kEmpty;  // smallest id of the level whose complement is accepted
     int plain_ahead = -1;        // level whose binary blocks the lookahead tested
@@ -1621,10 +1621,13 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
                 run.blocks.resize(nb);
                 if (rc) break;
                 if (h.found != kEmpty) {
-                    lazy = true;
                     const int avail = (o.neg_type == 1 && c2 == 1 ? 1 : 0) | (c5 == 1 ? 2 : 0) |
                                       (c6 == 1 ? 4 : 0) | (c7 == 1 ? 8 : 0);
                     succ_mask = (hit_op == LTL_AND || hit_op == LTL_OR ? 1 : 15) & avail;
+                    // (K words per trace: X, F, G successors are not tested in
+                    // generation, so such a level is stored in full)
+                    lazy = B == 1 || !(succ_mask & 14);
+                    if (!lazy) succ_mask = 0;
                     if (launch_fill_u64(reinterpret_cast<uint64_t *>(&c->d_state->found), kEmpty, 1, s) != hipSuccess) {
                         rc = LTL_E_HIP;
                         break;

@@ -16,14 +16,16 @@ deferred level included, read back through the recomputation) equal the
 oracle's; the deferred level's summaries agree with its recomputed words;
 answers in the negation and X / F / G blocks next to binary twins; the
 deferral happens where it should (result fields words_deferred and
-deferred_succ) and not without the precheck or with K words per trace (the
-learned order keeps its own lookahead and never defers)."""
+deferred_succ), for K words per trace too (negation successors only), and
+not without the precheck (the learned order keeps its own lookahead and
+never defers)."""
 import numpy as np
 import pytest
 
 from test_gpu_parity import check_parity
 from test_gpu_summaries import expected, unpack
-from test_long_traces import random_instance
+from test_long_traces import check_parity as long_parity
+from test_long_traces import planted_instance, random_instance
 
 pytestmark = pytest.mark.gpu
 
@@ -126,8 +128,33 @@ def test_store_read_twice_and_next_call(ltl, oracle, learner):
     check_parity(ltl, oracle, learner, ltl.generate_custom(42, 9, 8, 2, 2, 8, 0.5, planted_size=9), max_cost=6)
 
 
-def test_not_deferred_elsewhere(ltl, learner):
+def test_not_deferred_without_precheck(ltl, learner):
     inst = ltl.generate_custom(43, 20, 13, 3, 9, 16, 0.5, planted_size=9)
     assert learner.learn(inst, max_cost=5, precheck=0)["words_deferred"] == 0
-    long_inst = random_instance(ltl, 200, 12, 12, 3, 60, 150)
-    assert learner.learn(long_inst, max_cost=5)["words_deferred"] == 0
+
+
+# K words per trace: deferred at max_cost, and by the lookahead when the
+# first binary hit is an & or | candidate (X, F, G successors are not tested
+# in the multi-word generation kernels, so there deferred_succ <= 1)
+@pytest.mark.parametrize("shape", [(12, 12, 3, 60, 150, 5), (20, 20, 2, 65, 255, 6)],
+                         ids=lambda s: "x".join(map(str, s)))
+def test_long_traces_deferred_at_max_cost(ltl, oracle, learner, shape):
+    P, N, A, lo, hi, mc = shape
+    inst = random_instance(ltl, 200 + P, P, N, A, lo, hi)
+    gpu = learner.learn(inst, max_cost=mc)
+    s, e = learner.levels()
+    if gpu["status"] == 0:
+        assert gpu["words_deferred"] == e[mc] - s[mc] > 0
+        lo_, hi_ = int(s[mc]), int(e[mc])
+        words = learner.store(lo_, hi_ - lo_, inst.W * inst.K)[0]
+        got = unpack(learner.summaries(lo_, hi_ - lo_, inst.W), inst.W)
+        assert np.array_equal(got, expected(words, inst.lens, inst.K))
+    long_parity(oracle, learner, inst, max_cost=mc)
+
+
+@pytest.mark.parametrize("planted", ["G((~(a))|(F(b)))", "F((a)&(G(b)))"])
+def test_long_traces_deferred_by_lookahead(ltl, oracle, learner, planted):
+    inst = planted_instance(ltl, oracle, 300 + len(planted), 24, 24, 3, 40, 200, planted)
+    gpu, _ = long_parity(oracle, learner, inst, max_cost=8)
+    assert gpu["status"] == 1
+    assert gpu["deferred_succ"] <= 1
