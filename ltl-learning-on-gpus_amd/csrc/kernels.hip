@@ -314,6 +314,53 @@ __global__ __launch_bounds__(256) void k_gen1m(ChunkArgs A) {
     gen1_tail<true, false>(A, t, c0, SuccAcc{}, otf);
 }
 
+// k_gen_xfg for K words per trace (no successor tests): X, F, G (NEG: and
+// ~) of each operand, trace by trace.
+template <int K, bool CHK, bool NEG>
+__global__ __launch_bounds__(256) void k_gen_xfgm(ChunkArgs A, int64_t n, int64_t slots, uint64_t id_xfg) {
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_unary(A, t)) return;
+    const bool otf = chunk_otf(A);
+    const int64_t s = A.ls + (t.valid ? t.a : A.a0);
+    const uint64_t *p = A.store + set_base(s, A.nw);
+    const int T = A.nw / K;
+    constexpr int NC = NEG ? 4 : 3;
+    Acc c[NC];
+    for (int i = 0; i < T; ++i) {
+        uint64_t x[K], y[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) x[k] = p[(size_t)(i * K + k) * kTileWS];
+        const WordAux *aux = A.aux + i * K;
+        if constexpr (NEG) {
+            trace_unary<LTL_NOT, K>(x, aux, y);
+#pragma unroll
+            for (int k = 0; k < K; ++k) fold<CHK>(c[0], y[k], aux + k);
+        }
+        trace_unary<LTL_NEXT, K>(x, aux, y);
+#pragma unroll
+        for (int k = 0; k < K; ++k) fold<CHK>(c[NC - 3], y[k], aux + k);
+        trace_unary<LTL_FIN, K>(x, aux, y);
+#pragma unroll
+        for (int k = 0; k < K; ++k) fold<CHK>(c[NC - 2], y[k], aux + k);
+        trace_unary<LTL_GLOB, K>(x, aux, y);
+#pragma unroll
+        for (int k = 0; k < K; ++k) fold<CHK>(c[NC - 1], y[k], aux + k);
+    }
+    if (!t.valid) return;
+    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        const uint64_t id = NEG ? (j == 0 ? A.id_base : id_xfg + (uint64_t)(j - 1) * (uint64_t)n) + (uint64_t)t.a
+                                : A.id_base + (uint64_t)j * (uint64_t)n + (uint64_t)t.a;
+        if (CHK && c[j].mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id);
+        if (otf) continue;
+        const size_t k = (size_t)j * (size_t)slots + ix;
+        A.t_slot[k] = tab_insert(A.tab, A.tmask, table_key(c[j].f1), kPending | id, &A.st->probe_fail);
+        A.t_fp2[k] = c[j].f2;
+    }
+}
+
 template <int K>
 __global__ __launch_bounds__(256) void k_gen2m(ChunkArgs A) {  // U
     if (run_stopped(A)) return;
@@ -1000,8 +1047,26 @@ static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s)
     return hipGetLastError();
 }
 
+template <int K>
+static void launch_gen_xfgm(const ChunkArgs &a, int64_t n, int64_t slots, bool neg, uint64_t id_xfg, hipStream_t s) {
+    const dim3 g = grid_tiles(a), b(256);
+    if (neg) hipLaunchKernelGGL((k_gen_xfgm<K, false, true>), g, b, 0, s, a, n, slots, id_xfg);
+    else if (a.gen_check) hipLaunchKernelGGL((k_gen_xfgm<K, true, false>), g, b, 0, s, a, n, slots, a.id_base);
+    else hipLaunchKernelGGL((k_gen_xfgm<K, false, false>), g, b, 0, s, a, n, slots, a.id_base);
+}
+
 hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, bool neg, uint64_t id_xfg, hipStream_t s) {
     if (a.kmode || a.ws != kTileWS) return hipErrorInvalidValue;
+    if (a.K > 1) {  // K words per trace
+        if (a.t_succ || (neg && a.gen_check)) return hipErrorInvalidValue;
+        switch (a.K) {
+        case 2: launch_gen_xfgm<2>(a, n, slots, neg, id_xfg, s); break;
+        case 3: launch_gen_xfgm<3>(a, n, slots, neg, id_xfg, s); break;
+        case 4: launch_gen_xfgm<4>(a, n, slots, neg, id_xfg, s); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     const dim3 g = grid_tiles(a), b(256);
     if (neg) {
         if (a.gen_check) return hipErrorInvalidValue;  // tested levels only

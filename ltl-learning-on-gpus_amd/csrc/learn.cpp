@@ -737,11 +737,11 @@ struct Run {
     // canonical order; each block's kernels stop only for an accepted
     // candidate up to its own end.  Falls back to three passes when the
     // three blocks do not fit one chunk's temporaries, in check-only or
-    // counting passes, and for relaxed keys or multi-word traces.
+    // counting passes, and for relaxed keys.
     int32_t unary_xfg(int64_t s, int64_t e) {
         const int64_t n = e - s;
         const int64_t tiles = (n + 63 + 63) / 64, slots = tiles * 64;
-        if (n <= 0 || check_only || count_only || base.kmode || c->B != 1 || 3 * slots > c->CH) {
+        if (n <= 0 || check_only || count_only || base.kmode || 3 * slots > c->CH) {
             int32_t r = unary(LTL_NEXT, s, e);
             if (!r) r = unary(LTL_FIN, s, e);
             if (!r) r = unary(LTL_GLOB, s, e);
@@ -1216,7 +1216,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         const int lvn = cost - c2;
         const int64_t nn = lvn >= c1 ? Le[lvn] - Ls[lvn] : 0;
         const bool fuse4 = !run.check_only && !run.count_only && !run.otf && run.base.gen_check == 0 &&
-                           !run.base.kmode && B == 1 && o.neg_type == 1 && lvn >= c1 && nn > 0 && c2 == c5 &&
+                           !run.base.kmode && o.neg_type == 1 && lvn >= c1 && nn > 0 && c2 == c5 &&
                            c5 == c6 && c6 == c7 && 4 * ((nn + 127) / 64 * 64) + ((int64_t)1 << 22) <= c->CH;
         if (fuse4) {
             uint64_t between = 0;  // candidates of the & and | blocks
