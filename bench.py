@@ -9,11 +9,14 @@ cost 14, dedup table of 2^28 slots.  The instance is generated (seed 1) and
 uploaded before the timed region; each call re-initialises its device state.
 
 Prints one JSON line.  `value` = unique sets stored per second of whole-call
-wall time; `ms_per_step` = time to the minimal formula.  `roofline` is the
-dominant kernel's algorithmic bytes per launch over its HIP-event duration
-against the 8 TB/s HBM peak; `cpu_baseline` times the CPU oracle (the
-reference enumeration restated in C, OpenMP) on a bounded sample of the same
-instance (levels up to cost 8).  The extras (skipped with --no-extra) time
+wall time (every set deduplicated and stored with its parents; the last
+generated level, which no later pass reads in full, without its words and
+summaries: `words_deferred_per_step`); `ms_per_step` = time to the minimal
+formula.  `roofline` is the dominant kernel's algorithmic bytes per launch
+over its HIP-event duration against the 8 TB/s HBM peak; `cpu_baseline` times
+the CPU oracle (the reference enumeration restated in C, OpenMP, every set
+stored in full) on a bounded sample of the same instance (levels up to cost
+9, about 5 s on 16 threads).  The extras (skipped with --no-extra) time
 the learned order against the plain one (configs 2 and 5), the batched launch
 against one call per instance (config 4), and an instance of traces longer
 than 64 positions (4 words per trace).
@@ -243,7 +246,7 @@ def long_traces(ltl, args):
                         "U[0.3,0.7], planted G(a->F b)&~(c U d), unit costs, max cost 14, seed 1, plain order",
             "words_per_trace": inst.K, "value": round((r["n_stored"] - 5) / dt, 1), "unit": "unique CS/s",
             "seconds": round(dt, 4), "min_cost": r["cost"], "formula": r["formula"],
-            "uniques": r["n_stored"], "store_overflow": bool(r["on_the_fly"]),
+            "uniques": r["n_stored"], "words_deferred": r["words_deferred"], "store_overflow": bool(r["on_the_fly"]),
             "kernels_ms": {k: round(v["seconds"] * 1e3, 3) for k, v in stats.items() if v["launches"]},
             "dominant": {"kernel": dom, "achieved_gbs": round(d["alg_bytes"] / d["seconds"] / 1e9, 1),
                          "frac": round(d["alg_bytes"] / d["seconds"] / HBM_PEAK, 4)}}
@@ -256,7 +259,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--budget-gb", type=float, default=0.0, help="device pool; 0 = 75%% of free memory")
     ap.add_argument("--max-cost", type=int, default=14)
-    ap.add_argument("--cpu-cost", type=int, default=8, help="levels timed for the CPU baseline")
+    ap.add_argument("--cpu-cost", type=int, default=9, help="levels timed for the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the learned-order and batch measurements")
@@ -329,7 +332,7 @@ def main():
                                "2^28 dedup slots, seed 1, plain order",
                    "traces": inst.W, "atoms": int(inst.atoms.shape[0]), "max_cost": args.max_cost},
         "time_to_min_formula_s": round(wall / args.steps, 4),
-        "min_cost": r0["cost"], "formula": r0["formula"], "uniques_per_step": r0["n_stored"],
+        "min_cost": r0["cost"], "formula": r0["formula"], "uniques_per_step": r0["n_stored"], "words_deferred_per_step": r0["words_deferred"],
         "candidates_per_step": r0["candidates"], "checked_per_step": r0["checked"],
         "kernels_ms_per_step": {k: round(v["seconds"] / args.steps * 1e3, 3) for k, v in stats.items()
                                 if v["launches"]},
