@@ -13,11 +13,13 @@ wall time (every set deduplicated and stored with its parents; the last
 generated level, which no later pass reads in full, without its words and
 summaries: `words_deferred_per_step`); `ms_per_step` = time to the minimal
 formula.  `roofline` is the dominant kernel's algorithmic bytes per launch
-over its HIP-event duration against the 8 TB/s HBM peak; `cpu_baseline` times
+(operand sets counted once per candidate and operand, so re-reads from L2
+count too) over its HIP-event duration against the 8 TB/s HBM peak, with its
+physical rate beside it (profiled FETCH + WRITE bytes); `cpu_baseline` times
 the CPU oracle (the reference enumeration restated in C, OpenMP, every set
 stored in full) on a bounded sample of the same instance (levels up to cost
 9, about 5 s on 16 threads).  The extras (skipped with --no-extra) time
-the learned order against the plain one (configs 2 and 5), the batched launch
+the learned order against the plain one (configs 2-5), the batched launch
 against one call per instance (config 4), and an instance of traces longer
 than 64 positions (4 words per trace).
 
@@ -91,7 +93,8 @@ def calibrate_copy(nbytes=1 << 30):
 
 def learned_vs_plain(ltl, L, args):
     """Time to the minimal formula, plain vs learned order, at equal cost:
-    configuration 2 (steps as above) and configuration-5 instances.  The
+    configuration 2 (steps as above), configuration 3, every configuration-4
+    instance (one call each) and configuration-5 instances.  The
     scorer is trained on the device from logs of configurations 1-3 first
     (outside the timed calls)."""
     t = time.perf_counter()
@@ -146,6 +149,22 @@ def learned_vs_plain(ltl, L, args):
             rk = ltl.first_hit_ranks(L, inst, {"trained": trained, "untrained": untrained}, args.config5_cost)
             if rk:
                 ranks.append(rk)
+    # Configuration 4: every instance, one call each, plain vs learned.
+    insts4 = [ltl.generate(4, seed=1, index=i) for i in range(256)]
+    mc4 = max(i.max_cost for i in insts4)
+    t4p, t4l, faster4, same4, slower4 = 0.0, 0.0, 0, 0, []
+    for idx, inst in enumerate(insts4):
+        p = best_of(inst, 2, None, max_cost=mc4)
+        q = best_of(inst, 2, trained, max_cost=mc4)
+        same4 += (q["status"], q["cost"]) == (p["status"], p["cost"])
+        t4p += p["seconds"]
+        t4l += q["seconds"]
+        faster4 += q["seconds"] < p["seconds"]
+        if q["seconds"] >= p["seconds"]:
+            slower4.append([idx, p["cost"], round(p["seconds"] * 1e3, 3), round(q["seconds"] * 1e3, 3)])
+    out["config4"] = {"instances": len(insts4), "max_cost": mc4, "plain_s": round(t4p, 4),
+                      "learned_s": round(t4l, 4), "same_status_and_cost": same4, "learned_faster": faster4,
+                      "not_faster": "[index, cost, plain ms, learned ms]", "not_faster_rows": slower4[:32]}
     out["config5"] = {"instances": args.config5, "max_cost": args.config5_cost,
                       "plain_s": round(tot_p, 4), "learned_s": round(tot_l, 4), "learned_untrained_s": round(tot_u, 4),
                       "learned_faster": faster,
@@ -316,9 +335,15 @@ def main():
     dom = max(stats, key=lambda k: stats[k]["seconds"])
     d = stats[dom]
     achieved = d["alg_bytes"] / d["seconds"] / 1e9
+    traffic = traffic_for(dom)
+    avg_s = d["seconds"] / max(d["launches"], 1)
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
-            "traffic": traffic_for(dom),
+            "traffic": traffic,
+            # physical HBM rate: the profiled bytes per launch over this run's average launch
+            # (below the algorithmic rate where operands are re-read from L2)
+            "physical_gbs": round(traffic / avg_s / 1e9, 1) if traffic else None,
+            "physical_frac": round(traffic / avg_s / HBM_PEAK, 4) if traffic else None,
             "alg_bytes_per_launch": int(d["alg_bytes"] / max(d["launches"], 1)),
             "avg_launch_ms": round(d["seconds"] / max(d["launches"], 1) * 1e3, 4),
             "compulsory_gbs": round(d["min_bytes"] / d["seconds"] / 1e9, 1)}
