@@ -118,6 +118,14 @@ struct ltl_ctx {
     std::vector<PendingEvent> pending;
     std::vector<hipEvent_t> free_events;
     long long *h_counts = nullptr;  // pinned, one slot per profiled emit of a level
+    // pinned staging of small per-level uploads: two halves, each refilled
+    // only after the copy that last read it has completed (its event)
+    char *h_stage = nullptr;
+    size_t h_stage_cap = 0;
+    char *h_init = nullptr;       // pinned staging of a call's initial uploads
+    size_t h_init_cap = 0;
+    int h_stage_i = 0;
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
     int n_counts = 0;
     ltl_batch_params batch{1024, 4, 0, 0, (int64_t)1 << 18};
     ltl_batch_stats batch_stats{};
@@ -143,10 +151,24 @@ struct ltl_ctx {
 };
 
 static int32_t hip_fail(hipError_t e) { return e == hipErrorOutOfMemory ? LTL_E_OOM : LTL_E_HIP; }
+
+// Stream-ordered upload of a small host buffer through pinned staging (an
+// asynchronous copy from pageable memory would hold the host until the
+// stream reaches it).
+static hipError_t upload_pinned(ltl_ctx *c, void *dst, const void *src, size_t bytes, hipStream_t s);
+// With LTL_DEBUG set in the environment, a failing HIP call is reported on
+// stderr with its source line.
+static void report_hip(hipError_t e, const char *what, int line) {
+    static const bool on = getenv("LTL_DEBUG") != nullptr;
+    if (on) fprintf(stderr, "ltl: %s failed at learn.cpp:%d: %s\n", what, line, hipGetErrorString(e));
+}
 #define HIP_TRY(x)                                   \
     do {                                             \
         hipError_t e_ = (x);                         \
-        if (e_ != hipSuccess) return hip_fail(e_);   \
+        if (e_ != hipSuccess) {                      \
+            report_hip(e_, #x, __LINE__);            \
+            return hip_fail(e_);                     \
+        }                                            \
     } while (0)
 
 extern "C" void ltl_default_options(ltl_options *o) {
@@ -209,17 +231,30 @@ extern "C" int32_t ltl_get_options(const ltl_ctx *c, ltl_options *o) {
 
 extern "C" void ltl_destroy(ltl_ctx *c) {
     if (!c) return;
-    hipSetDevice(c->opt.device);
-    if (c->stream) hipStreamSynchronize(c->stream);
-    for (auto &p : c->pending) { hipEventDestroy(p.a); hipEventDestroy(p.b); }
-    for (auto e : c->free_events) hipEventDestroy(e);
-    if (c->pool) hipFree(c->pool);
+    // (errors here are reported with LTL_DEBUG and then cleared: nothing is
+    // left for a later call's launch checks to pick up)
+    auto chk = [](hipError_t e, const char *what) {
+        if (e != hipSuccess) report_hip(e, what, __LINE__);
+    };
+    chk(hipSetDevice(c->opt.device), "hipSetDevice");
+    if (c->stream) chk(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (auto &p : c->pending) {
+        chk(hipEventDestroy(p.a), "hipEventDestroy(pending)");
+        chk(hipEventDestroy(p.b), "hipEventDestroy(pending)");
+    }
+    for (auto e : c->free_events) chk(hipEventDestroy(e), "hipEventDestroy(free)");
+    if (c->pool) chk(hipFree(c->pool), "hipFree(pool)");
     for (void *p : {(void *)c->d_tstat, (void *)c->d_toff, (void *)c->d_blk, (void *)c->d_keys, (void *)c->d_vals,
                     c->d_sort, (void *)c->d_log})
-        if (p) hipFree(p);
-    if (c->h_state) hipHostFree(c->h_state);
-    if (c->h_counts) hipHostFree(c->h_counts);
-    if (c->stream) hipStreamDestroy(c->stream);
+        if (p) chk(hipFree(p), "hipFree");
+    if (c->h_state) chk(hipHostFree(c->h_state), "hipHostFree(state)");
+    if (c->h_counts) chk(hipHostFree(c->h_counts), "hipHostFree(counts)");
+    if (c->h_stage) chk(hipHostFree(c->h_stage), "hipHostFree(stage)");
+    if (c->h_init) chk(hipHostFree(c->h_init), "hipHostFree(init)");
+    for (hipEvent_t e : c->stage_ev)
+        if (e) chk(hipEventDestroy(e), "hipEventDestroy(stage)");
+    if (c->stream) chk(hipStreamDestroy(c->stream), "hipStreamDestroy");
+    (void)hipGetLastError();
     delete c;
 }
 
@@ -325,6 +360,32 @@ static int32_t carve(ltl_ctx *c) {
     c->d_sum = (uint64_t *)take((size_t)capr * c->nsw * 32);
     if ((size_t)(p - c->pool) > budget) return LTL_E_OOM;
     return 0;
+}
+
+static hipError_t upload_pinned(ltl_ctx *c, void *dst, const void *src, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    hipError_t e;
+    if (bytes > c->h_stage_cap) {
+        for (hipEvent_t ev : c->stage_ev)
+            if (ev && (e = hipEventSynchronize(ev)) != hipSuccess) return e;
+        if (c->h_stage) hipHostFree(c->h_stage);
+        c->h_stage = nullptr;
+        c->h_stage_cap = 0;
+        const size_t cap = std::max<size_t>(bytes, (size_t)1 << 16);
+        if ((e = hipHostMalloc((void **)&c->h_stage, 2 * cap, hipHostMallocDefault)) != hipSuccess) return e;
+        c->h_stage_cap = cap;
+    }
+    const int i = c->h_stage_i;
+    c->h_stage_i ^= 1;
+    if (!c->stage_ev[i]) {
+        if ((e = hipEventCreateWithFlags(&c->stage_ev[i], hipEventDisableTiming)) != hipSuccess) return e;
+    } else if ((e = hipEventSynchronize(c->stage_ev[i])) != hipSuccess) {
+        return e;
+    }
+    char *buf = c->h_stage + (size_t)i * c->h_stage_cap;
+    memcpy(buf, src, bytes);
+    if ((e = hipMemcpyAsync(dst, buf, bytes, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    return hipEventRecord(c->stage_ev[i], s);
 }
 
 // ---------------------------------------------------------------------------
@@ -844,7 +905,10 @@ struct Run {
         if (!pend.on) return 0;
         pend.on = false;
         const int64_t n = pend.n, s = pend.a.ls;
-        if (all_ltls != pend.id_xfg) return LTL_E_HIP;  // the & and | blocks did not add up
+        if (all_ltls != pend.id_xfg) {  // the & and | blocks did not add up
+            report_hip(hipErrorUnknown, "finish_xfg", __LINE__);
+            return LTL_E_HIP;
+        }
         const int ops[3] = {LTL_NEXT, LTL_FIN, LTL_GLOB};
         for (int j = 0; j < 3; ++j) blocks.push_back({all_ltls + (uint64_t)j * n, (uint64_t)n, ops[j], s, n, 0, 1, cur_level});
         for (int j = 0; j < 3; ++j) {
@@ -1049,6 +1113,10 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     const double t0 = now_s();
     const ltl_options &o = c->opt;
     if (o.order == LTL_ORDER_LEARNED && !c->scorer) return res->status = LTL_E_ARG;  // no scorer attached
+    // An error left by an earlier HIP call of this thread (kernel launches
+    // are checked with hipGetLastError) is cleared here, reported with
+    // LTL_DEBUG.
+    if (hipError_t stale = hipGetLastError(); stale != hipSuccess) report_hip(stale, "an earlier call", __LINE__);
     std::string names;
     for (auto &s : c->stats) { s.launches = 0; s.seconds = 0; s.alg_bytes = s.min_bytes = s.items = 0; }
     {
@@ -1104,19 +1172,40 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     st0.n_stored = A;
     st0.overflow_chunk = LLONG_MAX;
     hipStream_t s = c->stream;
-    HIP_TRY(hipMemcpyAsync(c->d_aux, aux.data(), aux.size() * sizeof(WordAux), hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(c->d_saux, saux.data(), saux.size() * sizeof(SumAux), hipMemcpyHostToDevice, s));
+    // The call's uploads go through one pinned buffer (asynchronous copies
+    // from pageable memory would each hold the host); the stream is idle
+    // here, so the buffer is free.
+    const size_t init_bytes = align_up(aux.size() * sizeof(WordAux), 256) + align_up(saux.size() * sizeof(SumAux), 256) +
+                              align_up((atom_words.size() + sum0.size()) * 8, 256) + align_up(sizeof(st0), 256) +
+                              align_up((size_t)W * 8, 256);
+    HIP_TRY(hipStreamSynchronize(s));
+    if (init_bytes > c->h_init_cap) {
+        if (c->h_init) hipHostFree(c->h_init);
+        c->h_init = nullptr;
+        c->h_init_cap = 0;
+        HIP_TRY(hipHostMalloc((void **)&c->h_init, init_bytes, hipHostMallocDefault));
+        c->h_init_cap = init_bytes;
+    }
+    size_t init_off = 0;
+    auto upload_init = [&](void *dst, const void *src, size_t n) -> hipError_t {
+        if (n == 0) return hipSuccess;
+        memcpy(c->h_init + init_off, src, n);
+        const hipError_t e = hipMemcpyAsync(dst, c->h_init + init_off, n, hipMemcpyHostToDevice, s);
+        init_off += align_up(n, 256);
+        return e;
+    };
+    HIP_TRY(upload_init(c->d_aux, aux.data(), aux.size() * sizeof(WordAux)));
+    HIP_TRY(upload_init(c->d_saux, saux.data(), saux.size() * sizeof(SumAux)));
     // Atom words and summaries: one contiguous copy, then scattered into the
-    // word-major store (a strided copy from pageable memory goes row by row).
-    // (`stage` and `kinfo` outlive their copies: the first level ends with a
-    // stream sync.)
+    // word-major store.  (`kinfo` outlives its copy: the first level ends
+    // with a stream sync.)
     std::vector<uint32_t> kinfo;
     std::vector<uint64_t> stage(atom_words);
     stage.insert(stage.end(), sum0.begin(), sum0.end());
-    HIP_TRY(hipMemcpyAsync(c->d_stage, stage.data(), stage.size() * 8, hipMemcpyHostToDevice, s));
+    HIP_TRY(upload_init(c->d_stage, stage.data(), stage.size() * 8));
     HIP_TRY(launch_scatter(c->d_store, c->ws, c->d_stage, A, c->nw, s));
     HIP_TRY(launch_scatter(c->d_sum, c->ws, c->d_stage + (size_t)c->nw * A, A, (int64_t)4 * nsw, s));
-    HIP_TRY(hipMemcpyAsync(c->d_state, &st0, sizeof(st0), hipMemcpyHostToDevice, s));
+    HIP_TRY(upload_init(c->d_state, &st0, sizeof(st0)));
     // Without a fixed size the table starts small and grows before a level
     // is generated (clearing 2^31 slots would cost milliseconds per call).
     if (!o.hash_slots_log2) {
@@ -1146,8 +1235,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     }
     ba.K = K;
     ba.ptrace = c->d_ptrace;
-    if (B == 1)
-        HIP_TRY(hipMemcpyAsync(c->d_ptrace, prep.ptrace.data(), (size_t)W * 8, hipMemcpyHostToDevice, s));
+    if (B == 1) HIP_TRY(upload_init(c->d_ptrace, prep.ptrace.data(), (size_t)W * 8));
     // Relaxed keys (code/ltli6463.cu:57-141) only above 126 positions; below,
     // every mode is the exact packing, which the exact dedup reproduces.
     ba.W = W;
@@ -1294,7 +1382,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             tstat_used += (n + 63) / 64;
         }
         tstat_upto = upto;
-        HIP_TRY(hipMemcpyAsync(c->d_toff, toff.data(), (size_t)nlev * sizeof(int64_t), hipMemcpyHostToDevice, s));
+        HIP_TRY(upload_pinned(c, c->d_toff, toff.data(), (size_t)nlev * sizeof(int64_t), s));
         return 0;
     };
     auto upload_blocks = [&]() -> int32_t {
@@ -1307,7 +1395,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             HIP_TRY(hipMalloc((void **)&c->d_blk, (size_t)std::max(nb, 64) * sizeof(LBlock)));
             c->blk_cap = std::max(nb, 64);
         }
-        HIP_TRY(hipMemcpyAsync(c->d_blk, run.lblk.data(), (size_t)nb * sizeof(LBlock), hipMemcpyHostToDevice, s));
+        HIP_TRY(upload_pinned(c, c->d_blk, run.lblk.data(), (size_t)nb * sizeof(LBlock), s));
         return 0;
     };
     auto feat = [&](int cost) { return FeatCtx{c->d_tstat, c->d_toff, (float)cost, (float)P / (float)W, W, P}; };
@@ -1721,7 +1809,10 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         const Block *blk = nullptr;
         for (const auto &b : run.blocks)
             if (id >= b.id_base && id < b.id_base + b.size) { blk = &b; break; }
-        if (!blk) return res->status = LTL_E_HIP;
+        if (!blk) {
+            report_hip(hipErrorUnknown, "block of the found id", __LINE__);
+            return res->status = LTL_E_HIP;
+        }
         res->all_ltls = blk->id_base + blk->size;
         res->found_id = (int64_t)id;
         const uint64_t rel = id - blk->id_base;
