@@ -1532,8 +1532,12 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             run.n_ent = 0;
             rc = level_blocks(cost);
             if (!rc && !run.lblk.empty()) rc = ranked_level(cost, cost - 1);
+            // Only unary blocks launched (no binary blocks, or tested ahead):
+            // their result first, so that a hit there costs no lookahead
+            // (the ranked rounds above end with a sync already).
+            if (!rc && run.lblk.empty()) rc = sync_state();
             run.skip_binary = false;
-            if (!rc && h.found != kEmpty && !run.lblk.empty()) {  // found by the rounds above
+            if (!rc && h.found != kEmpty) {  // found by the pass above
                 found = true;
                 found_cost = cost;
                 Le[cost] = n_stored;
