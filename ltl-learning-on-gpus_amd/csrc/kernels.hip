@@ -92,7 +92,7 @@ __device__ __forceinline__ void succ_fold(SuccAcc &s, uint64_t c, const WordAux 
 }
 
 // Accepted successors X c, F c, G c of a candidate as bits 1-3.
-__device__ __forceinline__ uint8_t succ_flags(const ChunkArgs &A, const Acc &, const SuccAcc &s) {
+__device__ __forceinline__ uint8_t succ_flags(const ChunkArgs &A, const SuccAcc &s) {
     const uint32_t k = (uint32_t)A.hamming_k;
     uint8_t f = 0;
 #pragma unroll
@@ -121,13 +121,13 @@ __device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, con
         A.t_slot[2 * ix + 1] = tab_insert(A.tab, A.tmask, table_key(c1.f1), kPending | (id0 + 1), &A.st->probe_fail);
         A.t_fp2[2 * ix + 1] = c1.f2;
         if constexpr (SUCC) {
-            A.t_succ[2 * ix] = succ_flags(A, c0, s0);
-            A.t_succ[2 * ix + 1] = succ_flags(A, c1, s1);
+            A.t_succ[2 * ix] = succ_flags(A, s0);
+            A.t_succ[2 * ix + 1] = succ_flags(A, s1);
         }
     } else {
         A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
         A.t_fp2[ix] = c0.f2;
-        if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, c0, s0);
+        if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, s0);
     }
 }
 
@@ -140,7 +140,7 @@ __device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, con
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
     A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
     A.t_fp2[ix] = c0.f2;
-    if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, c0, s0);
+    if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, s0);
 }
 
 template <int OP, bool RX, bool CHK, bool SUCC>
@@ -284,7 +284,7 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
         const size_t k = (size_t)j * (size_t)slots + ix;
         A.t_slot[k] = tab_insert(A.tab, A.tmask, table_key(c[j].f1), kPending | id, &A.st->probe_fail);
         A.t_fp2[k] = c[j].f2;
-        if constexpr (SUCC) A.t_succ[k] = succ_flags(A, c[j], sc[j]);
+        if constexpr (SUCC) A.t_succ[k] = succ_flags(A, sc[j]);
     }
 }
 
