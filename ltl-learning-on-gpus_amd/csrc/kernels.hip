@@ -176,8 +176,10 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
         }
     };
     // Batches of UB words: all loads of a batch are issued before use, so
-    // that each lane keeps 2*UB loads in flight.
-    constexpr int UB = 4;
+    // that each lane keeps UB right-operand loads in flight (the left
+    // operand is wave-uniform).  & and |: 16 (6.8 -> 6.5 ms per config-2
+    // call against 4); U, with two carry chains per word: 4 (8 is slower).
+    constexpr int UB = OP == LTL_UNTIL ? 4 : 16;
     int w = 0;
     for (; w + UB <= A.nw; w += UB) {
         uint64_t l[UB], r[UB];
