@@ -35,11 +35,12 @@ using namespace ltl;
 namespace {
 
 enum { K_GEN1, K_GEN2, K_GENU, K_DECIDE, K_SCAN, K_EMIT1, K_EMIT2, K_EMITU, K_CHK1, K_CHK2, K_CHKU,
-       K_TSTAT, K_SCORE, K_SORT, K_CHKR, K_LOG, K_NKIND };
+       K_TSTAT, K_SCORE, K_SORT, K_CHKR, K_LOG, K_EMITM, K_NKIND };
 const char *kKindName[K_NKIND] = {"gen_unary", "gen_and_or", "gen_until", "decide",
                                   "scan_finish", "emit_unary", "emit_and_or", "emit_until",
                                   "check_unary", "check_and_or", "check_until",
-                                  "tile_stats", "score_tiles", "sort_tiles", "check_ranked", "log_rows"};
+                                  "tile_stats", "score_tiles", "sort_tiles", "check_ranked", "log_rows",
+                                  "emit_meta"};
 
 struct Block {
     uint64_t id_base, size;
@@ -428,7 +429,7 @@ static void drain_events(ltl_ctx *c) {
 // profile = 2 times.
 static bool streaming_kind(int kind) {
     return kind == K_GEN1 || kind == K_GEN2 || kind == K_GENU || kind == K_EMIT1 || kind == K_EMIT2 ||
-           kind == K_EMITU;
+           kind == K_EMITU || kind == K_EMITM;
 }
 
 // Launch, bracketed by HIP events on the library's stream when profiling:
@@ -764,7 +765,7 @@ struct Run {
         ltl_ctx *c = this->c;
         const bool bin = op == LTL_AND || op == LTL_OR || op == LTL_UNTIL;
         const uint64_t cs = (uint64_t)c->nw * 8;
-        const int ek = op == LTL_UNTIL ? K_EMITU : bin ? K_EMIT2 : K_EMIT1;
+        const int ek = a.deferred ? K_EMITM : op == LTL_UNTIL ? K_EMITU : bin ? K_EMIT2 : K_EMIT1;
         HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_decide(op, a, c->stream); }));
         HIP_TRY(timed(c, K_SCAN, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles, [&] {
             hipError_t e = launch_scan(a.t_cnt, a.t_off, a.n_tiles, c->d_scan, c->scan_bytes, c->stream);
