@@ -1300,13 +1300,14 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         int32_t r = 0;
         run.cur_level = cost;
         // Generation of a tested level with equal unary costs: ~, X, F and G
-        // of the operand level in one pass (regions of 4 * slots, at least
-        // 2^22 candidates left for the & and | chunks in between).
+        // of the operand level in one pass (regions of 4 * slots within half
+        // of the chunk temporaries; the & and | chunks in between use the
+        // rest).
         const int lvn = cost - c2;
         const int64_t nn = lvn >= c1 ? Le[lvn] - Ls[lvn] : 0;
         const bool fuse4 = !run.check_only && !run.count_only && !run.otf && run.base.gen_check == 0 &&
                            !run.base.kmode && o.neg_type == 1 && lvn >= c1 && nn > 0 && c2 == c5 &&
-                           c5 == c6 && c6 == c7 && 4 * ((nn + 127) / 64 * 64) + ((int64_t)1 << 22) <= c->CH;
+                           c5 == c6 && c6 == c7 && 4 * ((nn + 127) / 64 * 64) <= c->CH / 2;
         if (fuse4) {
             uint64_t between = 0;  // candidates of the & and | blocks
             for (int i = c1; 2 * i <= cost - c3; ++i)

@@ -158,3 +158,22 @@ def test_long_traces_deferred_by_lookahead(ltl, oracle, learner, planted):
     gpu, _ = long_parity(oracle, learner, inst, max_cost=8)
     assert gpu["status"] == 1
     assert gpu["deferred_succ"] <= 1
+
+
+# The negation block joins the fused X/F/G generation pass on tested levels
+# (one pass over the operand level for ~, X, F and G): one gen_unary launch
+# per generated level with the precheck, two without (~ alone, then X/F/G);
+# the results are the same, and equal to the oracle's.
+@pytest.mark.parametrize("shape", NOT_FOUND[:3], ids=lambda s: "x".join(map(str, s)))
+def test_fused_unary_generation(ltl, oracle, learner, shape):
+    seed, P, N, A, lo, hi, mc = shape
+    inst = ltl.generate_custom(seed, P, N, A, lo, hi, 0.5, planted_size=9)
+    fused = learner.learn(inst, max_cost=mc, profile=1)
+    n_fused = learner.kernel_stats()["gen_unary"]["launches"]
+    plain = learner.learn(inst, max_cost=mc, profile=1, precheck=0)
+    n_plain = learner.kernel_stats()["gen_unary"]["launches"]
+    levels = mc - 1  # levels 2..max_cost are generated (no formula up to max_cost)
+    assert n_fused == levels and n_plain == 2 * levels, (n_fused, n_plain)
+    for k in ("status", "cost", "all_ltls", "n_stored"):
+        assert fused[k] == plain[k], k
+    check_parity(ltl, oracle, learner, inst, max_cost=mc)
