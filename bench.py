@@ -378,6 +378,11 @@ def main():
                                "kind": "port", "seconds": round(dt, 3),
                                "sample": f"same instance, levels up to cost {args.cpu_cost} "
                                          f"({u} unique sets), exact dedup, OpenMP"}
+        # one thread, on a smaller sample (levels up to cost 8, about 10 s)
+        v1, dt1, u1 = cpu_baseline(inst, min(args.cpu_cost, 8), 1)
+        out["cpu_baseline"]["single_thread"] = {"value": round(v1, 1), "cores": 1, "seconds": round(dt1, 3),
+                                                "sample": f"levels up to cost {min(args.cpu_cost, 8)} "
+                                                          f"({u1} unique sets)"}
     print(json.dumps(out), flush=True)
 
 
