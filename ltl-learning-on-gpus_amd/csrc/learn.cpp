@@ -15,6 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <chrono>
 #include <climits>
 #include <cstdio>
@@ -1914,10 +1916,26 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
         std::vector<BatchInst> inst(m);
         std::vector<uint64_t> in;
         std::vector<int64_t> work(m);
+        // the instances' host preparation (packing, per-word records) in
+        // parallel; the concatenation below keeps the instance order
+        std::vector<Prepared> preps(m);
+        {
+            const int T = (int)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 16u);
+            std::atomic<int> next{0};
+            auto work_fn = [&] {
+                for (int k; (k = next.fetch_add(1)) < m;) prepare_instance(&pbs[dev[k]], slot_width(&pbs[dev[k]]), preps[k]);
+            };
+            if (T <= 1 || m < 16) {
+                work_fn();
+            } else {
+                std::vector<std::thread> th;
+                for (int t = 0; t < T; ++t) th.emplace_back(work_fn);
+                for (auto &x : th) x.join();
+            }
+        }
         for (int k = 0; k < m; ++k) {
             const ltl_problem *pb = &pbs[dev[k]];
-            Prepared q;
-            prepare_instance(pb, slot_width(pb), q);
+            const Prepared &q = preps[k];
             BatchInst &I = inst[k];
             memset(&I, 0, sizeof(I));
             I.W = pb->num_pos + pb->num_neg; I.P = pb->num_pos; I.A = pb->num_atoms;
