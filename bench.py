@@ -125,9 +125,11 @@ def learned_vs_plain(ltl, L, args):
     out["config2"] = {"plain_s": round(p["seconds"], 4), "learned_s": round(q["seconds"], 4),
                       "speedup": round(p["seconds"] / q["seconds"], 3), "cost": p["cost"],
                       "plain_formula": p["formula"], "learned_formula": q["formula"]}
+    # (configurations 3-5 name no table size: the table starts small and grows
+    # with the levels, instead of clearing 2^28 slots per call)
     inst3 = ltl.generate(3, seed=1)
-    p = best_of(inst3, 2, None, max_cost=inst3.max_cost, hamming_k=inst3.hamming_k)
-    q = best_of(inst3, 2, trained, max_cost=inst3.max_cost, hamming_k=inst3.hamming_k)
+    p = best_of(inst3, 2, None, max_cost=inst3.max_cost, hamming_k=inst3.hamming_k, hash_slots_log2=0)
+    q = best_of(inst3, 2, trained, max_cost=inst3.max_cost, hamming_k=inst3.hamming_k, hash_slots_log2=0)
     assert q["status"] == p["status"] and q["cost"] == p["cost"]
     out["config3"] = {"plain_s": round(p["seconds"], 4), "learned_s": round(q["seconds"], 4),
                       "speedup": round(p["seconds"] / q["seconds"], 3), "cost": p["cost"],
@@ -135,9 +137,9 @@ def learned_vs_plain(ltl, L, args):
     rows5, tot_p, tot_l, tot_u, faster, ranks = [], 0.0, 0.0, 0.0, 0, []
     for idx in range(args.config5):
         inst = ltl.generate(5, seed=1, index=idx)
-        p = best_of(inst, 2, None, max_cost=args.config5_cost)
-        q = best_of(inst, 2, trained, max_cost=args.config5_cost)
-        u = best_of(inst, 2, untrained, max_cost=args.config5_cost)
+        p = best_of(inst, 2, None, max_cost=args.config5_cost, hash_slots_log2=0)
+        q = best_of(inst, 2, trained, max_cost=args.config5_cost, hash_slots_log2=0)
+        u = best_of(inst, 2, untrained, max_cost=args.config5_cost, hash_slots_log2=0)
         assert (q["status"], q["cost"]) == (p["status"], p["cost"]) == (u["status"], u["cost"])
         tot_p += p["seconds"]
         tot_l += q["seconds"]
@@ -154,8 +156,8 @@ def learned_vs_plain(ltl, L, args):
     mc4 = max(i.max_cost for i in insts4)
     t4p, t4l, faster4, same4, slower4 = 0.0, 0.0, 0, 0, []
     for idx, inst in enumerate(insts4):
-        p = best_of(inst, 2, None, max_cost=mc4)
-        q = best_of(inst, 2, trained, max_cost=mc4)
+        p = best_of(inst, 2, None, max_cost=mc4, hash_slots_log2=0)
+        q = best_of(inst, 2, trained, max_cost=mc4, hash_slots_log2=0)
         same4 += (q["status"], q["cost"]) == (p["status"], p["cost"])
         t4p += p["seconds"]
         t4l += q["seconds"]

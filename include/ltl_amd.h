@@ -149,7 +149,11 @@ int32_t ltl_learn(ltl_ctx *ctx, const ltl_problem *pb, ltl_result *res);
 /* Learn n independent instances in one persistent launch (no reference
  * counterpart; the reference runs one process per instance through
  * code/dc.py:23-43).  res[i] receives instance i.  Returns the number of
- * instances solved, or a negative LTL_E_* code. */
+ * instances solved, or a negative LTL_E_* code.  Instances handed to the
+ * whole-device path run up to four at once (plain order, no log): the
+ * context keeps up to three helper contexts for them, each with a quarter
+ * of its HBM budget (an eighth of the free memory, at most 16 GiB, when the
+ * budget is 0), released by ltl_destroy. */
 int32_t ltl_learn_batch(ltl_ctx *ctx, const ltl_problem *pbs, int32_t n, ltl_result *res);
 
 /* Parameters of the batched launch (no reference counterpart).

@@ -452,12 +452,13 @@ __device__ __forceinline__ int64_t temp_index_of(const ChunkArgs &A, int op, uin
     return a - A.a0 + A.ush;
 }
 
+// Survivors of tile ti (64 candidates, two per pair for U): a candidate
+// survives when its table slot still holds its own pending id (it is the
+// smallest id of its fingerprint); a slot taken by a stored set or by another
+// candidate with a different second fingerprint is a collision.  One wave per
+// tile; m[d] is the survivor mask of the tile's d-th candidate of each pair.
 template <int OP>
-__global__ __launch_bounds__(256) void k_decide(ChunkArgs A) {
-    if (run_stopped(A) || chunk_otf(A)) return;
-    const int64_t ti = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (ti >= A.n_tiles) return;
-    const int lane = threadIdx.x & 63;
+__device__ __forceinline__ uint32_t decide_tile(const ChunkArgs &A, int64_t ti, int lane, uint64_t m[2]) {
     constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
     constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
     bool valid;
@@ -474,6 +475,7 @@ __global__ __launch_bounds__(256) void k_decide(ChunkArgs A) {
     }
     const size_t ix = (size_t)ti * 64 + lane;
     uint32_t cnt = 0;
+    m[1] = 0;
 #pragma unroll
     for (int d = 0; d < ND; ++d) {
         bool surv = false;
@@ -492,11 +494,25 @@ __global__ __launch_bounds__(256) void k_decide(ChunkArgs A) {
                 if (o < 0 || A.t_fp2[o] != f2) A.st->collision = 1;
             }
         }
-        const uint64_t m = __ballot(surv);
-        cnt += (uint32_t)__popcll(m);
-        if (lane == 0) A.t_mask[ND * ti + d] = m;
+        m[d] = __ballot(surv);
+        cnt += (uint32_t)__popcll(m[d]);
     }
-    if (lane == 0) A.t_cnt[ti] = cnt;
+    return cnt;
+}
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_decide(ChunkArgs A) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    const int64_t ti = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (ti >= A.n_tiles) return;
+    const int lane = threadIdx.x & 63;
+    constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
+    uint64_t m[2];
+    const uint32_t cnt = decide_tile<OP>(A, ti, lane, m);
+    if (lane == 0) {
+        for (int d = 0; d < ND; ++d) A.t_mask[ND * ti + d] = m[d];
+        A.t_cnt[ti] = cnt;
+    }
 }
 
 __global__ void k_finish(ChunkArgs A) {
@@ -545,6 +561,74 @@ __global__ __launch_bounds__(256) void k_compact(ChunkArgs A) {
     const uint32_t ix = (uint32_t)(ti * 64 + (threadIdx.x & 63));
     if (m0 & bit) A.t_list[p0] = ND * ix;
     if (m1 & bit) A.t_list[p0 + ((m0 & bit) ? 1 : 0)] = ND * ix + 1;
+}
+
+// A chunk of at most kSmallTiles tiles: decide, scan, finish and compact in
+// one workgroup of 16 waves (one launch instead of a decide, a device scan, a
+// finish and a compact).  The tile counts, offsets and survivor masks stay in
+// LDS; t_list, the chunk's base and total and the stored count come out as
+// from the four kernels.
+template <int OP>
+__global__ __launch_bounds__(1024) void k_dedup_small(ChunkArgs A) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
+    __shared__ uint32_t s_cnt[kSmallTiles];  // survivors per tile, then the exclusive offset
+    __shared__ uint64_t s_mask[2 * kSmallTiles];
+    __shared__ uint32_t s_wsum[16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = (int)A.n_tiles;  // <= kSmallTiles (checked at launch)
+    for (int ti = wave; ti < n; ti += 16) {
+        uint64_t m[2];
+        const uint32_t cnt = decide_tile<OP>(A, ti, lane, m);
+        if (lane == 0) {
+            s_mask[2 * ti] = m[0];
+            s_mask[2 * ti + 1] = m[1];
+            s_cnt[ti] = cnt;
+        }
+    }
+    __syncthreads();
+    // exclusive scan over the tile counts: thread t holds tile t
+    const int t = (int)threadIdx.x;
+    const uint32_t v = t < n ? s_cnt[t] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) s_wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+        const uint32_t ws = s_wsum[w];
+        before += w < wave ? ws : 0u;
+        total += ws;
+    }
+    if (t < n) s_cnt[t] = before + x - v;
+    if (t == 0) {
+        const long long base = A.st->n_stored;
+        A.st->chunk_base = base;
+        A.st->chunk_total = (long long)total;
+        const long long room = A.capacity - base;
+        if ((long long)total > room) {
+            if (A.st->overflow_chunk > A.chunk_idx) A.st->overflow_chunk = A.chunk_idx;
+            A.st->dropped += (unsigned long long)((long long)total - room);
+            A.st->n_stored = A.capacity;
+        } else {
+            A.st->n_stored = base + (long long)total;
+        }
+    }
+    __syncthreads();
+    const uint64_t below = lanes_below(), bit = 1ull << lane;
+    for (int ti = wave; ti < n; ti += 16) {
+        const uint64_t m0 = s_mask[2 * ti], m1 = ND == 2 ? s_mask[2 * ti + 1] : 0;
+        if (!(m0 | m1)) continue;
+        const uint32_t p0 = s_cnt[ti] + (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below));
+        const uint32_t ix = (uint32_t)(ti * 64 + lane);
+        if (m0 & bit) A.t_list[p0] = ND * ix;
+        if (m1 & bit) A.t_list[p0 + ((m0 & bit) ? 1 : 0)] = ND * ix + 1;
+    }
 }
 
 // `aux` and `saux` are A.aux and A.saux as separate restrict parameters: the
@@ -1171,6 +1255,22 @@ hipError_t launch_decide(int op, const ChunkArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_dedup_small(int op, const ChunkArgs &a, hipStream_t s) {
+    if (a.n_tiles < 1 || a.n_tiles > kSmallTiles) return hipErrorInvalidValue;
+    const dim3 g(1), b(1024);
+    switch (op) {
+    case LTL_NOT: hipLaunchKernelGGL(k_dedup_small<LTL_NOT>, g, b, 0, s, a); break;
+    case LTL_NEXT: hipLaunchKernelGGL(k_dedup_small<LTL_NEXT>, g, b, 0, s, a); break;
+    case LTL_FIN: hipLaunchKernelGGL(k_dedup_small<LTL_FIN>, g, b, 0, s, a); break;
+    case LTL_GLOB: hipLaunchKernelGGL(k_dedup_small<LTL_GLOB>, g, b, 0, s, a); break;
+    case LTL_AND: hipLaunchKernelGGL(k_dedup_small<LTL_AND>, g, b, 0, s, a); break;
+    case LTL_OR: hipLaunchKernelGGL(k_dedup_small<LTL_OR>, g, b, 0, s, a); break;
+    case LTL_UNTIL: hipLaunchKernelGGL(k_dedup_small<LTL_UNTIL>, g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_finish(const ChunkArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_finish, dim3(1), dim3(1), 0, s, a);
     return hipGetLastError();
@@ -1181,6 +1281,47 @@ hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s) {
     if (a.ws != kTileWS) return hipErrorInvalidValue;
     (void)B;  // one kernel for both layouts: the fingerprints are per word
     hipLaunchKernelGGL(k_atoms, dim3((unsigned)n_atoms), dim3(64), 0, s, a, n_atoms);
+    return hipGetLastError();
+}
+
+// One thread walks the formula from the store's metadata (a formula has a
+// few dozen nodes at most; the walk replaces a host copy per node).  The
+// nodes written so far are the walk's queue: the next node to expand is the
+// first one at or after qi that is not an atom.
+__global__ void k_tree(const uint8_t *op, const int32_t *left, const int32_t *right, int64_t n_stored, int32_t l,
+                       int32_t r, int32_t *out, int32_t cap) {
+    if (threadIdx.x || blockIdx.x) return;
+    int32_t n = 0, qi = -1;
+    for (;;) {
+        int32_t kids[2];
+        if (qi < 0) {
+            kids[0] = l;
+            kids[1] = r;
+            qi = 0;
+        } else {
+            while (qi < n && out[1 + 3 * qi] == LTL_ATOM) ++qi;
+            if (qi >= n) break;
+            kids[0] = out[2 + 3 * qi];
+            kids[1] = out[3 + 3 * qi];
+            ++qi;
+        }
+        for (int s = 0; s < 2; ++s) {
+            const int32_t k = kids[s];
+            if (k < 0) continue;
+            if (k >= n_stored) { out[0] = -2; return; }
+            if (n >= cap) { out[0] = -1; return; }
+            out[1 + 3 * n] = op[k];
+            out[2 + 3 * n] = left[k];
+            out[3 + 3 * n] = right[k];
+            ++n;
+        }
+    }
+    out[0] = n;
+}
+
+hipError_t launch_tree(const uint8_t *op, const int32_t *left, const int32_t *right, int64_t n_stored, int32_t l,
+                       int32_t r, int32_t *out, int32_t cap, hipStream_t s) {
+    hipLaunchKernelGGL(k_tree, dim3(1), dim3(64), 0, s, op, left, right, n_stored, l, r, out, cap);
     return hipGetLastError();
 }
 

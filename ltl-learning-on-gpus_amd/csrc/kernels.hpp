@@ -177,6 +177,12 @@ hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s);
 // id_xfg + j * n and temporaries at (j + 1) * slots.
 hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, bool neg, uint64_t id_xfg, hipStream_t s);
 hipError_t launch_decide(int op, const ChunkArgs &a, hipStream_t s);
+// A chunk of at most kSmallTiles tiles (4096 candidates) is deduplicated by
+// one workgroup: decide, scan, finish and compact in a single launch.  Each of
+// its 16 waves decides up to 4 tiles in turn (a table read per candidate and
+// tile), so larger chunks take the four-kernel path.
+constexpr int64_t kSmallTiles = 64;
+hipError_t launch_dedup_small(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_finish(const ChunkArgs &a, hipStream_t s);
 // Survivor list (candidate temp index per output position), then emit with
 // one lane per output position.
@@ -189,6 +195,11 @@ hipError_t launch_materialize(const ChunkArgs &a, int64_t s0, int64_t s1, hipStr
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s);
 // dst[r * ws + c] = src[r * ncols + c] for r < nrows, c < ncols.
 hipError_t launch_scatter(uint64_t *dst, int64_t ws, const uint64_t *src, int ncols, int64_t nrows, hipStream_t s);
+// The found formula's nodes below its root, as a breadth-first walk from the
+// root's operands (l, r) visits them: out[0] = the count (-1: more than cap,
+// -2: an index beyond the store), then (op, left, right) of each node.
+hipError_t launch_tree(const uint8_t *op, const int32_t *left, const int32_t *right, int64_t n_stored, int32_t l,
+                       int32_t r, int32_t *out, int32_t cap, hipStream_t s);
 // Check-only pass over a whole block (no dedup, no store): atomicMin of the
 // canonical id of every accepted candidate.
 hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s);

@@ -132,6 +132,7 @@ struct ltl_ctx {
     int n_counts = 0;
     ltl_batch_params batch{1024, 4, 0, 0, (int64_t)1 << 18};
     ltl_batch_stats batch_stats{};
+    std::vector<ltl_ctx *> helpers;  // contexts for the whole-device calls of a batch
     int n_cu = 0;
     int64_t chunk_counter = 0;
 
@@ -149,6 +150,7 @@ struct ltl_ctx {
     void *d_sort = nullptr;
     size_t sort_cap = 0;
     float *d_log = nullptr;       // log rows
+    int32_t *d_tree = nullptr;    // the found formula's walk (1 + 3 * kTreeCap)
     int64_t log_cap = 0, log_n = 0;
     int32_t log_per_block = 0;
 };
@@ -239,6 +241,7 @@ extern "C" void ltl_destroy(ltl_ctx *c) {
     auto chk = [](hipError_t e, const char *what) {
         if (e != hipSuccess) report_hip(e, what, __LINE__);
     };
+    for (ltl_ctx *h : c->helpers) ltl_destroy(h);
     chk(hipSetDevice(c->opt.device), "hipSetDevice");
     if (c->stream) chk(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     for (auto &p : c->pending) {
@@ -248,7 +251,7 @@ extern "C" void ltl_destroy(ltl_ctx *c) {
     for (auto e : c->free_events) chk(hipEventDestroy(e), "hipEventDestroy(free)");
     if (c->pool) chk(hipFree(c->pool), "hipFree(pool)");
     for (void *p : {(void *)c->d_tstat, (void *)c->d_toff, (void *)c->d_blk, (void *)c->d_keys, (void *)c->d_vals,
-                    c->d_sort, (void *)c->d_log})
+                    c->d_sort, (void *)c->d_log, (void *)c->d_tree})
         if (p) chk(hipFree(p), "hipFree");
     if (c->h_state) chk(hipHostFree(c->h_state), "hipHostFree(state)");
     if (c->h_counts) chk(hipHostFree(c->h_counts), "hipHostFree(counts)");
@@ -768,13 +771,18 @@ struct Run {
         const bool bin = op == LTL_AND || op == LTL_OR || op == LTL_UNTIL;
         const uint64_t cs = (uint64_t)c->nw * 8;
         const int ek = a.deferred ? K_EMITM : op == LTL_UNTIL ? K_EMITU : bin ? K_EMIT2 : K_EMIT1;
-        HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_decide(op, a, c->stream); }));
-        HIP_TRY(timed(c, K_SCAN, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles, [&] {
-            hipError_t e = launch_scan(a.t_cnt, a.t_off, a.n_tiles, c->d_scan, c->scan_bytes, c->stream);
-            if (e != hipSuccess) return e;
-            return launch_finish(a, c->stream);
-        }));
-        HIP_TRY(timed(c, K_SCAN, 0, 0, 0, [&] { return launch_compact(op, a, c->stream); }));
+        if (a.n_tiles <= kSmallTiles) {
+            // (one workgroup: the launches, not the work, would set the time)
+            HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_dedup_small(op, a, c->stream); }));
+        } else {
+            HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_decide(op, a, c->stream); }));
+            HIP_TRY(timed(c, K_SCAN, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles, [&] {
+                hipError_t e = launch_scan(a.t_cnt, a.t_off, a.n_tiles, c->d_scan, c->scan_bytes, c->stream);
+                if (e != hipSuccess) return e;
+                return launch_finish(a, c->stream);
+            }));
+            HIP_TRY(timed(c, K_SCAN, 0, 0, 0, [&] { return launch_compact(op, a, c->stream); }));
+        }
         HIP_TRY(timed(c, ek, 0, 0, 0, [&] { return launch_emit(op, c->B, a, c->stream); }));
         if (c->opt.profile && c->n_counts < kCountSlots) {
             // survivors of this chunk -> bytes of its emit (read after the level sync)
@@ -1045,17 +1053,36 @@ int32_t read_meta(ltl_ctx *c, int64_t k, Meta &m) {
     return 0;
 }
 
+constexpr int32_t kTreeCap = 1024;
+
+// The nodes below the root as the walk of build_tree visits them, read on the
+// device (one launch and one copy); walk[0] < 0 when the walk did not finish.
+int32_t tree_walk(ltl_ctx *c, int32_t l, int32_t r, int64_t n_stored, std::vector<int32_t> &walk) {
+    if (!c->d_tree) HIP_TRY(hipMalloc((void **)&c->d_tree, (1 + 3 * (size_t)kTreeCap) * 4));
+    HIP_TRY(launch_tree(c->d_op, c->d_left, c->d_right, n_stored, l, r, c->d_tree, kTreeCap, c->stream));
+    walk.assign(1 + 3 * (size_t)kTreeCap, 0);
+    HIP_TRY(hipMemcpyAsync(walk.data(), c->d_tree, walk.size() * 4, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 // Build the node array (breadth first) from the root triple and the store's
 // parent indices; store indices below the atom count are atoms.
 int32_t build_tree(ltl_ctx *c, int op, int32_t l, int32_t r, int atom, std::vector<ltl_node> &nodes,
                    std::vector<int64_t> &src, int64_t n_stored) {
     nodes.clear();
     src.clear();
-    // Small stores: the parent arrays in one copy each instead of three
+    // The nodes' metadata from a walk on the device; should it not finish,
+    // small stores: the parent arrays in one copy each, else three
     // synchronous copies per node.
+    std::vector<int32_t> walk;
+    if (op != LTL_ATOM && n_stored > 0)
+        if (int32_t e = tree_walk(c, l, r, n_stored, walk)) return e;
+    const bool walked = !walk.empty() && walk[0] >= 0;
+    size_t wi = 0;
     std::vector<uint8_t> ops;
     std::vector<int32_t> lefts, rights;
-    if (n_stored > 0 && n_stored <= ((int64_t)1 << 20)) {
+    if (!walked && n_stored > 0 && n_stored <= ((int64_t)1 << 20)) {
         ops.resize(n_stored);
         lefts.resize(n_stored);
         rights.resize(n_stored);
@@ -1073,7 +1100,13 @@ int32_t build_tree(ltl_ctx *c, int op, int32_t l, int32_t r, int atom, std::vect
         for (int s = 0; s < 2; ++s) {
             if (kids[s] < 0) continue;
             Meta m;
-            if (kids[s] < (int64_t)ops.size()) {
+            if (walked) {
+                if ((int64_t)wi >= walk[0]) return LTL_E_HIP;  // (the walks differ: not expected)
+                m.op = walk[1 + 3 * wi];
+                m.l = walk[2 + 3 * wi];
+                m.r = walk[3 + 3 * wi];
+                ++wi;
+            } else if (kids[s] < (int64_t)ops.size()) {
                 m.op = ops[kids[s]];
                 m.l = lefts[kids[s]];
                 m.r = rights[kids[s]];
@@ -1881,6 +1914,69 @@ static int32_t batch_fill_result(const BatchOut &bo, const std::string &names, l
     return r->status = LTL_NOT_FOUND;
 }
 
+// The instances a batched call leaves to the whole-device path (traces longer
+// than 64 positions, levels past the persistent kernel's limits) are small
+// next to the device: each call's time is mostly its launches and per-level
+// synchronisations, with the device idle in between.  Up to kHelpers + 1 of
+// them run at once, from host threads, each on a context of its own (stream,
+// pool, pinned staging; the calling context is one of them), so that one
+// call's launches fill another's gaps.  A helper's pool is a quarter of the
+// calling context's budget (an eighth of the free memory when that is 0,
+// at most 16 GiB); an instance that exhausts it (out of memory, or a store
+// that overflowed into check-only levels) is run again on the calling
+// context.  The learned order (a shared scorer) and the enumeration log (one
+// per context) keep the calls on the calling context, one after another.
+static constexpr int kHelpers = 3;
+
+static void run_whole_device(ltl_ctx *c, const ltl_problem *pbs, ltl_result *res, const std::vector<int> &ids) {
+    const ltl_options &o = c->opt;
+    const bool serial = ids.size() < 2 || o.order == LTL_ORDER_LEARNED || (c->log_cap > 0 && c->log_per_block > 0);
+    std::vector<ltl_ctx *> ctxs{c};
+    if (!serial) {
+        ltl_options ho = o;
+        size_t budget = o.hbm_budget_bytes / 4;
+        if (!o.hbm_budget_bytes) {
+            size_t fr = 0, tot = 0;
+            if (hipSetDevice(o.device) == hipSuccess && hipMemGetInfo(&fr, &tot) == hipSuccess)
+                budget = std::min<size_t>(fr / 8, (size_t)16 << 30);
+        }
+        ho.hbm_budget_bytes = budget;
+        const int want = (int)std::min<size_t>(kHelpers, ids.size() - 1);
+        if (budget >= ((size_t)256 << 20)) {
+            while ((int)c->helpers.size() < want) {
+                int32_t st = 0;
+                ltl_ctx *h = ltl_create(&ho, &st);
+                if (!h) break;
+                c->helpers.push_back(h);
+            }
+            for (int k = 0; k < want && k < (int)c->helpers.size(); ++k)
+                if (ltl_set_options(c->helpers[k], &ho) == 0) ctxs.push_back(c->helpers[k]);
+        }
+    }
+    if (ctxs.size() == 1) {
+        for (int i : ids) ltl_learn(c, &pbs[i], &res[i]);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    std::vector<std::vector<int>> again(ctxs.size());
+    auto work = [&](size_t t) {
+        for (size_t k; (k = next.fetch_add(1)) < ids.size();) {
+            const int i = ids[k];
+            ltl_learn(ctxs[t], &pbs[i], &res[i]);
+            if (t > 0 && (res[i].status == LTL_E_OOM || res[i].on_the_fly)) again[t].push_back(i);
+        }
+    };
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < ctxs.size(); ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto &x : th) x.join();
+    for (const auto &v : again)
+        for (int i : v) {
+            ltl_result_free(&res[i]);
+            ltl_learn(c, &pbs[i], &res[i]);
+        }
+}
+
 extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n, ltl_result *res) {
     if (!c || n < 0 || (n > 0 && (!pbs || !res))) return LTL_E_ARG;
     const double t0 = now_s();
@@ -2025,10 +2121,8 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
         }
     }
     c->W = 0;  // the store of the last call is not readable after a batch
-    for (int i : deferred) {
-        ltl_learn(c, &pbs[i], &res[i]);
-        bs.deferred++;
-    }
+    run_whole_device(c, pbs, res, deferred);
+    bs.deferred += (int32_t)deferred.size();
     int32_t solved = 0;
     for (int i = 0; i < n; ++i) solved += res[i].status == LTL_FOUND;
     bs.seconds = now_s() - t0;

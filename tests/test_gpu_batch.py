@@ -71,3 +71,27 @@ def test_batch_vs_oracle(ltl, oracle, learner):
         oi = oracle.Instance(inst.num_pos, inst.num_neg, inst.lens, inst.atoms, inst.names)
         r = oracle.enumerate_(oi, max_cost=6)
         assert {x: g[x] for x in KEYS} == {x: r[x] for x in KEYS}
+
+
+def test_batch_whole_device_calls_in_parallel(ltl, learner):
+    """Instances left to the whole-device path run several at once, each on a
+    context of its own: nearly every instance here is left (a tiny deferral
+    limit), long-trace instances among them; results equal one ltl_learn each,
+    on two calls in a row (the helper contexts are kept and reused), and the
+    calling context's own single calls are unaffected afterwards."""
+    from test_long_traces import random_instance
+    insts = config4(ltl, 24, seed=6)
+    insts += [random_instance(ltl, 500 + k, 10 + k, 9 + k, 3, 60, 150) for k in range(4)]
+    ref = [learner.learn(i, max_cost=7) for i in insts]
+    learner.set_batch_params(defer_limit=50)
+    try:
+        for _ in range(2):
+            got = learner.learn_batch(insts, max_cost=7)
+            st = learner.batch_stats()
+            assert st["deferred"] >= 20 and st["deferred"] + st["on_device"] + st["host"] == len(insts)
+            for k, (a, b) in enumerate(zip(got, ref)):
+                assert {x: a[x] for x in KEYS} == {x: b[x] for x in KEYS}, k
+    finally:
+        learner.set_batch_params()
+    again = learner.learn(insts[0], max_cost=7)
+    assert {x: again[x] for x in KEYS} == {x: ref[0][x] for x in KEYS}
