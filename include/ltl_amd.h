@@ -84,7 +84,7 @@ typedef struct {
                                   before generating it (default) */
     int32_t  group_lanes;      /* lanes per characteristic set; 0 = tuned */
     int64_t  capacity;         /* max stored sets; 0 = sized from budget */
-    uint64_t hbm_budget_bytes; /* 0 = 75% of free device memory */
+    uint64_t hbm_budget_bytes; /* 0 = 75% of free device memory at the first call */
     int32_t  device;           /* HIP device ordinal */
     int32_t  profile;          /* 1 = time every kernel with HIP events, 2 = the generation
                                   and emit kernels only (fewer dispatch barriers) */

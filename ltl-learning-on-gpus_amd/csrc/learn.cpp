@@ -277,6 +277,9 @@ extern "C" void ltl_result_free(ltl_result *r) {
 // ---------------------------------------------------------------------------
 static int32_t ensure_pool(ltl_ctx *c) {
     size_t want = c->opt.hbm_budget_bytes;
+    // (budget 0: the pool sized from the free memory at the first call is
+    // kept, without a memory query per call)
+    if (!want && c->pool) return 0;
     if (!want) {
         size_t fr = 0, tot = 0;
         HIP_TRY(hipMemGetInfo(&fr, &tot));
