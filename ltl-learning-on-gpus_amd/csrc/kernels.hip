@@ -631,6 +631,96 @@ __global__ __launch_bounds__(1024) void k_dedup_small(ChunkArgs A) {
     }
 }
 
+// Chunks of more than kSmallTiles tiles: one launch of 4-tile workgroups
+// (one wave per tile) in place of decide, device scan, finish and compact.
+// A workgroup takes a ticket (its place in tile order), decides its tiles,
+// publishes its survivor count, then adds the counts of the workgroups before
+// it, walking back until one that has published its inclusive prefix
+// (decoupled look-back; every workgroup it waits for took its ticket earlier
+// and is running).  Status words carry the chunk's epoch, so words of earlier
+// chunks read as not yet published.  The waits are bounded: one that runs
+// out marks the status word poisoned and sets lb_fail, the chunk stores
+// nothing, and the call fails with a HIP error.  The last workgroup does the
+// finish and returns the ticket counter to 0.  The status words are
+// relaxed agent-scope atomics (coherent across the XCDs' L2s): a count
+// travels inside its status word, and what else the workgroups write (t_list,
+// the chunk's base and total) is read by later launches, so no release or
+// acquire fence (an L2 write-back or invalidate per wait) is needed.
+constexpr uint32_t kLbSpinMax = 1u << 20;
+
+template <int OP>
+__global__ __launch_bounds__(256) void k_dedup_lb(ChunkArgs A) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
+    __shared__ uint32_t s_cnt[4];
+    __shared__ uint32_t s_blk, s_excl;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t nblk = (uint32_t)((A.n_tiles + 3) / 4);
+    if (threadIdx.x == 0) s_blk = atomicAdd(A.lb_ticket, 1u);
+    __syncthreads();
+    const uint32_t blk = s_blk;
+    const int64_t ti = (int64_t)blk * 4 + wave;
+    uint64_t m[2] = {0, 0};
+    uint32_t cnt = 0;
+    if (blk < nblk && ti < A.n_tiles) cnt = decide_tile<OP>(A, ti, lane, m);
+    if (lane == 0) s_cnt[wave] = cnt;
+    __syncthreads();
+    if (threadIdx.x == 0 && blk < nblk) {
+        const uint32_t agg = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        const uint64_t tag = (uint64_t)A.lb_epoch << 2;
+        uint32_t excl = 0;
+        bool poison = false;
+        if (blk > 0) {
+            __hip_atomic_store(&A.lb_flags[blk], ((tag | 1) << 32) | agg, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t p = (int64_t)blk - 1;
+            uint32_t spins = 0;
+            while (p >= 0) {
+                const uint64_t v = __hip_atomic_load(&A.lb_flags[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t hi = (uint32_t)(v >> 32);
+                if ((hi >> 2) != A.lb_epoch) {
+                    if (++spins > kLbSpinMax) { poison = true; break; }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                if ((hi & 3) == 3) { poison = true; break; }
+                excl += (uint32_t)v;
+                if ((hi & 3) == 2) break;
+                --p;
+            }
+        }
+        __hip_atomic_store(&A.lb_flags[blk], ((tag | (poison ? 3u : 2u)) << 32) | (uint32_t)(excl + agg),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (poison) __hip_atomic_store(&A.st->lb_fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_excl = excl;
+        if (blk == nblk - 1) {
+            // every workgroup has taken its ticket
+            __hip_atomic_store(A.lb_ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const long long total = poison ? 0 : (long long)excl + (long long)agg;
+            const long long base = A.st->n_stored;
+            A.st->chunk_base = base;
+            A.st->chunk_total = total;
+            const long long room = A.capacity - base;
+            if (total > room) {
+                if (A.st->overflow_chunk > A.chunk_idx) A.st->overflow_chunk = A.chunk_idx;
+                A.st->dropped += (unsigned long long)(total - room);
+                A.st->n_stored = A.capacity;
+            } else {
+                A.st->n_stored = base + total;
+            }
+        }
+    }
+    __syncthreads();
+    if (blk >= nblk || cnt == 0) return;
+    uint32_t before = s_excl;
+    for (int w = 0; w < wave; ++w) before += s_cnt[w];
+    const uint64_t below = lanes_below(), bit = 1ull << lane;
+    const uint64_t m0 = m[0], m1 = ND == 2 ? m[1] : 0;
+    const uint32_t p0 = before + (uint32_t)(__popcll(m0 & below) + __popcll(m1 & below));
+    const uint32_t ix = (uint32_t)(ti * 64 + lane);
+    if (m0 & bit) A.t_list[p0] = ND * ix;
+    if (m1 & bit) A.t_list[p0 + ((m0 & bit) ? 1 : 0)] = ND * ix + 1;
+}
+
 // `aux` and `saux` are A.aux and A.saux as separate restrict parameters: the
 // per-word records are then known not to be written through the store
 // pointer, and are read with scalar loads instead of vector loads that wait
@@ -1266,6 +1356,23 @@ hipError_t launch_dedup_small(int op, const ChunkArgs &a, hipStream_t s) {
     case LTL_AND: hipLaunchKernelGGL(k_dedup_small<LTL_AND>, g, b, 0, s, a); break;
     case LTL_OR: hipLaunchKernelGGL(k_dedup_small<LTL_OR>, g, b, 0, s, a); break;
     case LTL_UNTIL: hipLaunchKernelGGL(k_dedup_small<LTL_UNTIL>, g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_dedup_lb(int op, const ChunkArgs &a, hipStream_t s) {
+    if (a.n_tiles < 1 || !a.lb_flags || !a.lb_ticket || a.lb_epoch == 0 || a.lb_epoch >= (1u << 30))
+        return hipErrorInvalidValue;
+    const dim3 g = grid_tiles(a), b(256);
+    switch (op) {
+    case LTL_NOT: hipLaunchKernelGGL(k_dedup_lb<LTL_NOT>, g, b, 0, s, a); break;
+    case LTL_NEXT: hipLaunchKernelGGL(k_dedup_lb<LTL_NEXT>, g, b, 0, s, a); break;
+    case LTL_FIN: hipLaunchKernelGGL(k_dedup_lb<LTL_FIN>, g, b, 0, s, a); break;
+    case LTL_GLOB: hipLaunchKernelGGL(k_dedup_lb<LTL_GLOB>, g, b, 0, s, a); break;
+    case LTL_AND: hipLaunchKernelGGL(k_dedup_lb<LTL_AND>, g, b, 0, s, a); break;
+    case LTL_OR: hipLaunchKernelGGL(k_dedup_lb<LTL_OR>, g, b, 0, s, a); break;
+    case LTL_UNTIL: hipLaunchKernelGGL(k_dedup_lb<LTL_UNTIL>, g, b, 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

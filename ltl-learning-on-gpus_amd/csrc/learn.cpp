@@ -105,6 +105,9 @@ struct ltl_ctx {
     uint8_t *t_succ = nullptr;    // per candidate: accepted successors (deferred level)
     uint64_t *d_stage = nullptr;  // atom words and summaries, contiguous, before the scatter
     uint32_t *d_kinfo = nullptr;  // per trace: relaxed-key bit range (dedup modes 1-3)
+    uint64_t *d_lb = nullptr;     // look-back status words of a chunk's workgroups + ticket
+    int64_t lb_words = 0;
+    uint32_t lb_epoch = 0;        // chunks deduplicated by look-back in this call
     uint64_t *d_ptrace = nullptr; // packed layout: per packed trace, trace | offset << 32 | length << 40
     void *d_scan = nullptr;
     size_t scan_bytes = 0;
@@ -312,7 +315,7 @@ static int32_t carve(ltl_ctx *c) {
                    align_up(tiles * 16, 256) +
                    2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) +
                    align_up((size_t)(c->nw + 4 * c->nsw) * c->A * 8, 256) + align_up((size_t)c->W * 4, 256) +
-                   align_up((size_t)c->W * 8, 256) + 11 * 256;
+                   align_up((size_t)c->W * 8, 256) + align_up((tiles / 4 + 2) * 8, 256) + 12 * 256;
     if (fixed >= budget) return LTL_E_OOM;
     const size_t R = budget - fixed;
     // Table: load stays <= 1/2 with a full chunk pending (cap + ch <= slots/2).
@@ -357,6 +360,8 @@ static int32_t carve(ltl_ctx *c) {
     c->d_stage = (uint64_t *)take((size_t)(c->nw + 4 * c->nsw) * c->A * 8);
     c->d_kinfo = (uint32_t *)take((size_t)c->W * 4);
     c->d_ptrace = (uint64_t *)take((size_t)c->W * 8);
+    c->lb_words = (int64_t)(tiles / 4 + 2);  // a status word per workgroup, the ticket in the last
+    c->d_lb = (uint64_t *)take((size_t)c->lb_words * 8);
     c->d_tab = (uint64_t *)take(slots * 16);
     const int64_t capr = align_up((size_t)cap, 64);
     c->ws = 64;  // word stride inside a 64-set tile (tile-major store)
@@ -774,9 +779,17 @@ struct Run {
         const bool bin = op == LTL_AND || op == LTL_OR || op == LTL_UNTIL;
         const uint64_t cs = (uint64_t)c->nw * 8;
         const int ek = a.deferred ? K_EMITM : op == LTL_UNTIL ? K_EMITU : bin ? K_EMIT2 : K_EMIT1;
-        if (a.n_tiles <= kSmallTiles) {
+        // Small chunks: one workgroup; mid-size chunks: one look-back launch;
+        // large ones: decide, device scan, finish and compact (LTL_DEDUP_SCAN
+        // in the environment: the four-kernel path for every chunk).
+        static const bool scan_path = getenv("LTL_DEDUP_SCAN") != nullptr;
+        if (!scan_path && a.n_tiles <= kSmallTiles) {
             // (one workgroup: the launches, not the work, would set the time)
             HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_dedup_small(op, a, c->stream); }));
+        } else if (!scan_path && a.n_tiles <= kLbMaxTiles && c->lb_epoch + 1 < (1u << 30)) {
+            ChunkArgs b = a;
+            b.lb_epoch = ++c->lb_epoch;
+            HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_dedup_lb(op, b, c->stream); }));
         } else {
             HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_decide(op, a, c->stream); }));
             HIP_TRY(timed(c, K_SCAN, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles, [&] {
@@ -1253,12 +1266,16 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         c->tmask = (uint32_t)(std::min(slots, c->tab_max_slots) - 1);
     }
     HIP_TRY(launch_fill_u64(c->d_tab, kEmpty, 2 * ((size_t)c->tmask + 1), s));
+    HIP_TRY(launch_fill_u64(c->d_lb, 0, (size_t)c->lb_words, s));
+    c->lb_epoch = 0;
 
     Run run{c, pb};
     ChunkArgs &ba = run.base;
     ba.store = c->d_store; ba.aux = c->d_aux; ba.tab = c->d_tab; ba.st = c->d_state;
     ba.t_slot = c->t_slot; ba.t_fp2 = c->t_fp2; ba.t_mask = c->t_mask; ba.t_cnt = c->t_cnt; ba.t_off = c->t_off;
     ba.t_list = c->t_list;
+    ba.lb_flags = c->d_lb;
+    ba.lb_ticket = reinterpret_cast<uint32_t *>(c->d_lb + c->lb_words - 1);
     ba.m_op = c->d_op; ba.m_left = c->d_left; ba.m_right = c->d_right; ba.m_fp2 = c->d_fp2; ba.m_key = c->d_key;
     ba.sum = c->d_sum; ba.saux = c->d_saux; ba.nsw = nsw;
     // Check passes alternate between positive and negative traces (early
@@ -1330,6 +1347,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
         drain_events(c);
         if (h.collision) return LTL_E_COLLISION;
         if (h.probe_fail) return LTL_E_OOM;
+        if (h.lb_fail) return LTL_E_HIP;
         return 0;
     };
     // The blocks of level `cost` in the reference's order
