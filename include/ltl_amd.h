@@ -150,9 +150,9 @@ int32_t ltl_learn(ltl_ctx *ctx, const ltl_problem *pb, ltl_result *res);
  * counterpart; the reference runs one process per instance through
  * code/dc.py:23-43).  res[i] receives instance i.  Returns the number of
  * instances solved, or a negative LTL_E_* code.  Instances handed to the
- * whole-device path run up to four at once (plain order, no log): the
- * context keeps up to three helper contexts for them, each with a quarter
- * of its HBM budget (an eighth of the free memory, at most 16 GiB, when the
+ * whole-device path run up to six at once (plain order, no log): the
+ * context keeps up to five helper contexts for them, each with an eighth
+ * of its HBM budget (a sixteenth of the free memory, at most 8 GiB, when the
  * budget is 0), released by ltl_destroy. */
 int32_t ltl_learn_batch(ltl_ctx *ctx, const ltl_problem *pbs, int32_t n, ltl_result *res);
 
