@@ -105,16 +105,32 @@ __device__ __forceinline__ uint8_t succ_flags(const ChunkArgs &A, const SuccAcc 
 // was not tested by a check pass beforehand), then intern fingerprint 1 with
 // the candidate id as pending value (unless the chunk is check-only), and
 // (SUCC) record the candidates' accepted successors.
-template <int OP, bool CHK, bool SUCC>
+// Relaxed keys (RX): outside check-only chunks a candidate counts only if
+// it survives dedup (the reference tests a candidate only when its key is
+// new, code/ltli6463.cu:231-243), so its outcome goes to t_acc and dedup
+// folds it in; in a check-only chunk every candidate is tested
+// (code/ltli6463.cu:198-208).
+template <int OP, bool CHK, bool SUCC, bool RX = false>
 __device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, const Acc &c0, const Acc &c1,
                                           const SuccAcc &s0, const SuccAcc &s1, bool otf) {
     const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
     const uint64_t id0 = A.id_base + (OP == LTL_UNTIL ? 2 * pair : pair);
-    if (CHK && c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
-    if (CHK && OP == LTL_UNTIL && c1.mism <= (uint32_t)A.hamming_k)
-        atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
+    const bool acc0 = CHK && c0.mism <= (uint32_t)A.hamming_k;
+    const bool acc1 = CHK && OP == LTL_UNTIL && c1.mism <= (uint32_t)A.hamming_k;
+    if (!RX || otf) {
+        if (acc0) atomicMin(&A.st->found, (unsigned long long)id0);
+        if (acc1) atomicMin(&A.st->found, (unsigned long long)(id0 + 1));
+    }
     if (otf) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
+    if constexpr (RX) {
+        if constexpr (OP == LTL_UNTIL) {
+            A.t_acc[2 * ix] = (uint8_t)acc0;
+            A.t_acc[2 * ix + 1] = (uint8_t)acc1;
+        } else {
+            A.t_acc[ix] = (uint8_t)acc0;
+        }
+    }
     if constexpr (OP == LTL_UNTIL) {
         A.t_slot[2 * ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
         A.t_fp2[2 * ix] = c0.f2;
@@ -131,13 +147,15 @@ __device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, con
     }
 }
 
-template <bool CHK, bool SUCC>
+template <bool CHK, bool SUCC, bool RX = false>
 __device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, const Acc &c0, const SuccAcc &s0,
                                           bool otf) {
     const uint64_t id0 = A.id_base + (uint64_t)t.a;
-    if (CHK && c0.mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id0);
+    const bool acc0 = CHK && c0.mism <= (uint32_t)A.hamming_k;
+    if ((!RX || otf) && acc0) atomicMin(&A.st->found, (unsigned long long)id0);
     if (otf) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
+    if constexpr (RX) A.t_acc[ix] = (uint8_t)acc0;
     A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
     A.t_fp2[ix] = c0.f2;
     if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, s0);
@@ -194,7 +212,7 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
         key_finish(c0, k0);
         if constexpr (OP == LTL_UNTIL) key_finish(c1, k1);
     }
-    gen2_tail<OP, CHK, SUCC>(A, t, c0, c1, s0, s1, otf);
+    gen2_tail<OP, CHK, SUCC, RX>(A, t, c0, c1, s0, s1, otf);
 }
 
 template <int OP, bool RX, bool CHK, bool SUCC>
@@ -228,7 +246,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     for (; w < A.nw; ++w) step(w, p[(size_t)w * kTileWS]);
     if (!t.valid) return;
     if constexpr (RX) key_finish(c0, k0);
-    gen1_tail<CHK, SUCC>(A, t, c0, s0, otf);
+    gen1_tail<CHK, SUCC, RX>(A, t, c0, s0, otf);
 }
 
 // X, F and G of one operand block in one pass over the operand words (the
@@ -487,6 +505,8 @@ __device__ __forceinline__ uint32_t decide_tile(const ChunkArgs &A, int64_t ti, 
             const uint64_t id = id0 + d;
             if (v == (kPending | id)) {
                 surv = true;
+                // relaxed keys: the survivor's own test (set in generation)
+                if (A.t_acc && A.t_acc[k]) atomicMin(&A.st->rx_found, (unsigned long long)id);
             } else if (v < kPending) {
                 if (A.m_fp2[v] != f2) A.st->collision = 1;
             } else {
@@ -529,6 +549,48 @@ __global__ void k_finish(ChunkArgs A) {
         A.st->n_stored = A.capacity;
     } else {
         A.st->n_stored = base + total;
+    }
+}
+
+// Canonical id of the candidate at temp index k of a chunk (the inverse of
+// temp_index_of); false for lanes outside the chunk's rows and columns.
+template <int OP>
+__device__ __forceinline__ bool id_of_temp(const ChunkArgs &A, int64_t k, uint64_t &id) {
+    if constexpr (OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL) {
+        constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
+        const int64_t ix = k / ND, ti = ix / 64;
+        const int64_t a = A.a0 + ti / A.nbs, b = A.b0 - A.ush + (ti % A.nbs) * 64 + ix % 64;
+        if (a >= A.a1 || b < A.b0 || b >= A.b1) return false;
+        id = A.id_base + ND * ((uint64_t)a * (uint64_t)A.nR + (uint64_t)b) + (uint64_t)(k % ND);
+        return true;
+    } else {
+        const int64_t a = A.a0 - A.ush + k;
+        if (a < A.a0 || a >= A.a1) return false;
+        id = A.id_base + (uint64_t)a;
+        return true;
+    }
+}
+
+// Relaxed keys, after the chunk's dedup (one launch of at most 1024
+// workgroups): the smallest accepted survivor becomes `found`.  In the chunk
+// that overflowed the store, the first survivor that did not fit switches the
+// reference to testing every candidate (code/ltli6463.cu:198-208, 1021-1027):
+// every accepted candidate after it lowers `found` too.
+template <int OP>
+__global__ __launch_bounds__(256) void k_rx_tail(ChunkArgs A) {
+    if (run_stopped(A)) return;
+    const unsigned long long rx = A.st->rx_found;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && rx != kEmpty) atomicMin(&A.st->found, rx);
+    if (A.st->overflow_chunk != A.chunk_idx) return;
+    const int64_t room = A.capacity - A.st->chunk_base;
+    if (room < 0 || room >= A.st->chunk_total) return;
+    uint64_t over;
+    if (!id_of_temp<OP>(A, (int64_t)A.t_list[room], over)) return;
+    constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
+    const int64_t n = (int64_t)A.n_tiles * 64 * ND;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t id;
+        if (A.t_acc[k] && id_of_temp<OP>(A, k, id) && id > over) atomicMin(&A.st->found, (unsigned long long)id);
     }
 }
 
@@ -1261,6 +1323,8 @@ hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, bool neg
 
 hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s) {
     if (a.ws != kTileWS) return hipErrorInvalidValue;
+    // relaxed keys: generation tests every candidate into t_acc
+    if (a.kmode && (!a.t_acc || !a.gen_check)) return hipErrorInvalidValue;
     if (B == 0) return a.kmode || a.t_succ ? hipErrorInvalidValue : launch_gen_mw(op, a, s);
     return a.kmode ? launch_gen_x<true>(op, B, a, s) : launch_gen_x<false>(op, B, a, s);
 }
@@ -1373,6 +1437,20 @@ hipError_t launch_dedup_lb(int op, const ChunkArgs &a, hipStream_t s) {
     case LTL_AND: hipLaunchKernelGGL(k_dedup_lb<LTL_AND>, g, b, 0, s, a); break;
     case LTL_OR: hipLaunchKernelGGL(k_dedup_lb<LTL_OR>, g, b, 0, s, a); break;
     case LTL_UNTIL: hipLaunchKernelGGL(k_dedup_lb<LTL_UNTIL>, g, b, 0, s, a); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_rx_tail(int op, const ChunkArgs &a, hipStream_t s) {
+    if (!a.t_acc || a.n_tiles < 1) return hipErrorInvalidValue;
+    const int nd = op == LTL_UNTIL ? 2 : 1;
+    const dim3 g((unsigned)std::min<int64_t>(((int64_t)a.n_tiles * 64 * nd + 255) / 256, 1024)), b(256);
+    switch (op) {
+    case LTL_NOT: case LTL_NEXT: case LTL_FIN: case LTL_GLOB:
+        hipLaunchKernelGGL(k_rx_tail<LTL_NOT>, g, b, 0, s, a); break;
+    case LTL_AND: case LTL_OR: hipLaunchKernelGGL(k_rx_tail<LTL_AND>, g, b, 0, s, a); break;
+    case LTL_UNTIL: hipLaunchKernelGGL(k_rx_tail<LTL_UNTIL>, g, b, 0, s, a); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -83,6 +83,11 @@ struct DevState {
     unsigned long long succ_min[4];
     int lb_fail;                  // a look-back wait ran out (the chunk was not stored)
     int pad_;
+    // Relaxed keys (dedup modes 1-3): the smallest id of an accepted
+    // candidate that survived dedup, folded into `found` after the chunk's
+    // dedup launch (a dedup launch that lowered `found` itself would stop
+    // its own later workgroups).
+    unsigned long long rx_found;
 };
 
 // One chunk of one (operator, split) block, in canonical order.  Binary:
@@ -161,6 +166,11 @@ struct ChunkArgs {
     uint64_t *lb_flags = nullptr;
     uint32_t *lb_ticket = nullptr;
     uint32_t lb_epoch = 0;    // 1, 2, ... per chunk within a call (< 2^30)
+    // Relaxed keys (kmode != 0): per candidate temp index, bit 0 = the
+    // candidate is accepted.  The reference tests a candidate only when its
+    // key is new (code/ltli6463.cu:231-243), so generation records the
+    // outcome here and dedup folds it in for the survivors only.
+    uint8_t *t_acc = nullptr;
 };
 
 // Check passes visit words alternately from the positive and the negative
@@ -199,6 +209,12 @@ hipError_t launch_dedup_small(int op, const ChunkArgs &a, hipStream_t s);
 constexpr int64_t kLbMaxTiles = 1024;
 hipError_t launch_dedup_lb(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_finish(const ChunkArgs &a, hipStream_t s);
+// Relaxed keys, after a chunk's dedup: `found` lowered to the smallest
+// accepted survivor (rx_found); in the chunk that overflowed the store, every
+// accepted candidate after the first survivor that did not fit lowers it too
+// (from there on the reference tests every candidate, code/ltli6463.cu:198-208,
+// 1021-1027).
+hipError_t launch_rx_tail(int op, const ChunkArgs &a, hipStream_t s);
 // Survivor list (candidate temp index per output position), then emit with
 // one lane per output position.
 hipError_t launch_compact(int op, const ChunkArgs &a, hipStream_t s);

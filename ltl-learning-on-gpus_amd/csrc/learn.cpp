@@ -644,6 +644,14 @@ int32_t precheck_host(const ltl_options &o, const ltl_problem *pb, ltl_result *r
         if (pb->lens[i] < 1 || pb->lens[i] > 64 * K) return LTL_E_LIMIT;
     // the reference's relaxed keys are defined on one word per trace
     if (K > 1 && o.dedup_mode != LTL_DEDUP_EXACT) return LTL_E_ARG;
+    // The learned order tests every candidate of a level in its check
+    // passes, which is the reference's outcome only with exact dedup
+    // (relaxed keys test key-new candidates only, code/ltli6463.cu:231-243).
+    if (o.order == LTL_ORDER_LEARNED && o.dedup_mode != LTL_DEDUP_EXACT) {
+        int64_t len_sum = 0;
+        for (int i = 0; i < W; ++i) len_sum += pb->lens[i];
+        if (len_sum > 126) return LTL_E_ARG;
+    }
     names = pb->names ? std::string(pb->names, A) : std::string();
     if (names.empty()) for (int a = 0; a < A; ++a) names += (char)('a' + a);
     res->cost = o.cost[0];
@@ -799,6 +807,10 @@ struct Run {
             }));
             HIP_TRY(timed(c, K_SCAN, 0, 0, 0, [&] { return launch_compact(op, a, c->stream); }));
         }
+        // relaxed keys: the accepted survivors (and, in the chunk that
+        // overflowed the store, every accepted candidate after the first
+        // survivor that did not fit) lower the result
+        if (a.kmode) HIP_TRY(timed(c, K_DECIDE, 0, 0, 0, [&] { return launch_rx_tail(op, a, c->stream); }));
         HIP_TRY(timed(c, ek, 0, 0, 0, [&] { return launch_emit(op, c->B, a, c->stream); }));
         if (c->opt.profile && c->n_counts < kCountSlots) {
             // survivors of this chunk -> bytes of its emit (read after the level sync)
@@ -884,6 +896,7 @@ struct Run {
         base.t_cnt = c->t_cnt + off / 64;
         base.t_off = c->t_off + off / 64;
         if (base.t_succ) base.t_succ = c->t_succ + off;
+        if (base.t_acc) base.t_acc = c->t_succ + off;
     }
     int64_t ch_cap() const { return c->CH - tmp_off; }
 
@@ -1219,6 +1232,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     std::vector<uint64_t> &atom_words = prep.atom_words, &sum0 = prep.atom_sums;
     DevState st0{};
     st0.found = kEmpty;
+    st0.rx_found = kEmpty;
     st0.neg_found[0] = st0.neg_found[1] = kEmpty;
     for (auto &m : st0.succ_min) m = kEmpty;
     st0.n_stored = A;
@@ -1314,6 +1328,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
                 j += cnt;
             }
             ba.kmode = o.dedup_mode;
+            ba.t_acc = c->t_succ;  // per candidate: accepted, folded in for dedup survivors only
             ba.kstride = stride;
             ba.kw_end = 0;  // words holding a trace that takes key bits
             for (int i = 0; i < W; ++i)
@@ -1569,10 +1584,15 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
                 last_round = true;
         }
         Ls[cost] = n_stored;
-        // Precheck: the first accepted candidate of a level is always the
-        // first occurrence of its set (an earlier equal set would have been
-        // accepted first), so a check-only pass finds the same result as
-        // generation with dedup, without generating the level.
+        // Precheck: with exact dedup the first accepted candidate of a level
+        // is always the first occurrence of its set (an earlier equal set
+        // would have been accepted first), so a check-only pass finds the
+        // same result as generation with dedup, without generating the
+        // level.  Relaxed keys merge different sets, and the reference tests
+        // only candidates whose key is new (code/ltli6463.cu:231-243): those
+        // levels are tested in generation, through dedup, except in
+        // check-only mode after an overflow, where the reference tests every
+        // candidate (code/ltli6463.cu:198-208).
         bool generate = true;
         bool tested = false;  // every candidate of the level went through a check pass
         const bool la = learned && !run.otf;
@@ -1650,7 +1670,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
             tested = true;
             if (ahead) looked_ahead = cost + 1;
             else generate = false;  // the last level is tested completely; its sets would serve no later level
-        } else if (o.precheck || run.otf || learned) {
+        } else if ((o.precheck && !ba.kmode) || run.otf || learned) {
             const uint64_t save = run.all_ltls;
             const size_t nb = run.blocks.size();
             // (plain order: the pass also records the smallest candidate id

@@ -46,3 +46,12 @@ def test_dc_bad_arguments(ltl, learner):
         learner.learn_dc(inst, window=1)
     with pytest.raises(ltl.LtlError):
         learner.learn_dc(inst, window=8, algo=5)
+
+
+def test_dc_hand_traced(ltl, learner):
+    # the DetSplit trace written out in tests/test_oracle_dc.py
+    from test_oracle_dc import hand_traced_instance
+    o = hand_traced_instance()
+    inst = ltl.Instance(o.num_pos, o.num_neg, o.lens, o.atoms, o.names)
+    got = learner.learn_dc(inst, window=4, algo=0, max_cost=6)
+    assert (got["formula"], got["calls"]) == ("((a)&(~(c)))|(b)", 3)

@@ -179,3 +179,79 @@ def test_relaxed_mode3_merges_mirrored_sets(ltl, oracle, learner, seed):
     exact = oracle.enumerate_(to_oracle(oracle, inst), max_cost=5)
     if gpu["status"] == 0 and exact["status"] == 0:
         assert gpu["n_stored"] < exact["n_stored"]  # the mode really merged sets
+
+
+def _twin4(ltl, seed, n=40):
+    """One positive and three negative traces of n positions over a, b: t1 is
+    t0 with a and b exchanged, t3 is t2 with a and b exchanged, so every set
+    and its a<->b mirror have equal mode-3 keys (t0, t1 fold into one key
+    half, t2, t3 into the other).  At position 1, b holds on t0 only and a
+    holds nowhere: X(b) separates, X(a) (its key twin, one id earlier) does
+    not; a and b both hold at position 0 of t0, so neither atom separates."""
+    rng = np.random.default_rng(seed)
+    a0, b0, a2, b2 = (rng.integers(0, 2, n) for _ in range(4))
+    a0[1], b0[1], a0[0], b0[0], a2[1], b2[1] = 0, 1, 1, 1, 0, 0
+    w = lambda v: int(sum(int(x) << k for k, x in enumerate(v)))
+    atoms = np.array([[w(a0), w(b0), w(a2), w(b2)], [w(b0), w(a0), w(b2), w(a2)]], dtype=np.uint64)
+    return ltl.Instance(1, 3, np.full(4, n, dtype=np.uint8), atoms, "ab")
+
+
+def test_relaxed_mode3_tests_key_new_candidates_only(ltl, oracle, learner):
+    # The reference tests a candidate only when its relaxed key is new
+    # (code/ltli6463.cu:231-243): X(b) is a key duplicate of X(a) and is never
+    # tested, so mode 3 answers at a higher cost than exact dedup.
+    inst = _twin4(ltl, 3)
+    exact = oracle.enumerate_(to_oracle(oracle, inst), max_cost=6)
+    assert (exact["cost"], exact["formula"]) == (2, "X(b)")
+    gpu, ref = check_parity(ltl, oracle, learner, inst, max_cost=6, dedup_mode=3)
+    assert (ref["cost"], ref["formula"]) == (4, "X((a)U(b))")
+    # precheck on: ignored for relaxed keys (it would test key duplicates)
+    check_parity(ltl, oracle, learner, inst, max_cost=6, dedup_mode=3, precheck=1)
+
+
+def _twins_mixed(ltl, seed, planted=None, pairs=8, lo=8, hi=14):
+    """Twin traces (a<->b exchanged) at random positions, labelled at random
+    or by a planted formula, so that key halves and twin pairs mix both
+    classes; more than 126 positions in all."""
+    rng = np.random.default_rng(seed)
+    cols = []
+    for _ in range(pairs):
+        n = int(rng.integers(lo, hi + 1))
+        a, b, c = (int(rng.integers(0, 1 << n)) for _ in range(3))
+        cols += [(n, a, b, c), (n, b, a, c)]
+    cols = [cols[i] for i in rng.permutation(len(cols))]
+    if planted is None:
+        lab = rng.permutation(np.arange(len(cols)) < int(rng.integers(3, len(cols) - 2)))
+    else:
+        every = ltl.Instance(len(cols), 0, np.array([t[0] for t in cols], dtype=np.uint8),
+                             np.array([[t[k] for t in cols] for k in (1, 2, 3)], dtype=np.uint64), "abc")
+        lab = (ltl.eval_formula(every, planted) & np.uint64(1)).astype(bool)
+    cols = [c for c, p in zip(cols, lab) if p] + [c for c, p in zip(cols, lab) if not p]
+    P = int(lab.sum())
+    lens = np.array([t[0] for t in cols], dtype=np.uint8)
+    atoms = np.array([[t[k] for t in cols] for k in (1, 2, 3)], dtype=np.uint64)
+    return ltl.Instance(P, len(cols) - P, lens, atoms, "abc")
+
+
+PLANTED_TWINS = [None, "X((a)U(b))", "F((a)&(X(c)))", "(G(c))|(X(b))"]
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+@pytest.mark.parametrize("planted", PLANTED_TWINS)
+@pytest.mark.parametrize("seed", [1, 2])
+def test_relaxed_modes_mixed_twins(ltl, oracle, learner, mode, planted, seed):
+    inst = _twins_mixed(ltl, seed, planted)
+    assert int(inst.lens.sum()) > 126
+    check_parity(ltl, oracle, learner, inst, max_cost=6, dedup_mode=mode)
+
+
+@pytest.mark.parametrize("mode", [1, 3])
+@pytest.mark.parametrize("capacity", [64, 100, 150])
+@pytest.mark.parametrize("planted", PLANTED_TWINS[1:3] + ["(X(c))U((a)&(b))"])
+def test_relaxed_modes_overflow(ltl, oracle, learner, mode, capacity, planted):
+    # The store overflows: from the first survivor that does not fit on, the
+    # reference tests every candidate (code/ltli6463.cu:198-208, 1021-1027),
+    # key duplicates included (mode 3, capacity 64, third planted formula:
+    # cost 5 where the run without overflow answers at cost 6).
+    inst = _twins_mixed(ltl, 10 + capacity, planted)
+    check_parity(ltl, oracle, learner, inst, max_cost=7, dedup_mode=mode, capacity=capacity)

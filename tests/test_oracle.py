@@ -272,3 +272,100 @@ def test_trace_reader_known_values(oracle):
     # first negative line "1,0;0,1;0,0;0,1;0,1": p at 0, q at 1, 3, 4
     assert int(inst.atoms[0, 10]) == 0b00001 and int(inst.atoms[1, 10]) == 0b11010
     assert list(inst.lens) == [5] * 20
+
+
+# ---- relaxed keys above 126 positions: hand-derived vectors ---------------
+# Key bit j is bit j of l (j < 63) or bit j - 63 of h (code/ltli6463.cu:57-141).
+def _key(oracle, lens, trace, pos, mode):
+    """Key of the set holding exactly position `pos` of trace `trace`."""
+    lens = np.array(lens, dtype=np.uint8)
+    inst = oracle.Instance(len(lens), 0, lens, np.zeros((1, len(lens)), dtype=np.uint64))
+    cs = np.zeros(len(lens), dtype=np.uint64)
+    cs[trace] = np.uint64(1 << pos)
+    return oracle.relaxed_key(inst, cs, mode)
+
+
+L = lambda j: (0, 1 << j)          # key bit j < 63
+H = lambda j: (1 << (j - 63), 0)   # key bit j >= 63
+
+
+def test_relaxed_mode1_vectors(oracle):
+    # 5 traces of 52 positions: lenSum 260, stride 260 / 126 = 2; every
+    # trace gives 26 key bits (positions 0, 2, ..., 50) until 126 are taken,
+    # so trace t position p (p even) is key bit 26 t + p / 2, and trace 4
+    # stops after position 42 (key bit 125).
+    lens = [52] * 5
+    assert _key(oracle, lens, 0, 50, 1) == L(25)
+    assert _key(oracle, lens, 1, 2, 1) == L(27)
+    assert _key(oracle, lens, 1, 3, 1) == (0, 0)       # odd positions are not sampled
+    assert _key(oracle, lens, 2, 0, 1) == L(52)
+    assert _key(oracle, lens, 3, 0, 1) == H(78)
+    assert _key(oracle, lens, 4, 42, 1) == H(125)
+    assert _key(oracle, lens, 4, 44, 1) == (0, 0)      # the 127th sample is dropped
+    # 3 traces of 60 positions: lenSum 180, stride 1: trace 2 contributes
+    # positions 0..5 only (key bits 120..125).
+    lens = [60] * 3
+    assert _key(oracle, lens, 1, 59, 1) == H(119)
+    assert _key(oracle, lens, 2, 5, 1) == H(125)
+    assert _key(oracle, lens, 2, 6, 1) == (0, 0)
+
+
+def test_relaxed_mode2_vectors(oracle):
+    # 3 traces of 60 positions: each takes (126 * 60 + 180) / 180 = 43 key
+    # bits (positions 0..42), the third only the 40 that remain.
+    lens = [60] * 3
+    assert _key(oracle, lens, 0, 42, 2) == L(42)
+    assert _key(oracle, lens, 1, 0, 2) == L(43)
+    assert _key(oracle, lens, 1, 42, 2) == H(85)
+    assert _key(oracle, lens, 1, 43, 2) == (0, 0)
+    assert _key(oracle, lens, 2, 39, 2) == H(125)
+    assert _key(oracle, lens, 2, 40, 2) == (0, 0)
+    # lengths 10, 63, 63 (lenSum 136): takes (1260 + 136) / 136 = 10 and
+    # (7938 + 136) / 136 = 59, the last trace the 57 bits that remain.
+    lens = [10, 63, 63]
+    assert _key(oracle, lens, 0, 9, 2) == L(9)
+    assert _key(oracle, lens, 1, 58, 2) == H(68)
+    assert _key(oracle, lens, 1, 59, 2) == (0, 0)
+    assert _key(oracle, lens, 2, 56, 2) == H(125)
+    assert _key(oracle, lens, 2, 57, 2) == (0, 0)
+
+
+def _key_copy(cs, lens, mode, slip=None):
+    """CPU copy of the relaxed keys of modes 1 and 2, with a planted slip."""
+    lsum = sum(lens)
+    h = l = 0
+    j = 0
+    for i, n in enumerate(lens):
+        if mode == 1:
+            stride = lsum // 126 + (1 if slip == "stride" else 0)
+            ks = range(0, n, stride)
+        else:
+            take = (126 * n + (0 if slip == "take" else lsum)) // lsum
+            ks = range(take)
+        for k in ks:
+            if j >= 126:
+                break
+            if (int(cs[i]) >> k) & 1 if k < 64 else 0:
+                if j < 63:
+                    l |= 1 << j
+                else:
+                    h |= 1 << (j - 63)
+            j += 1
+    return h, l
+
+
+def test_relaxed_key_slips_fail_the_vectors(oracle):
+    """A stride off by one (mode 1) or the take without the + lenSum rounding
+    (mode 2), planted in a CPU copy, fails the hand-derived vectors; the copy
+    without a slip equals the oracle on random sets."""
+    rng = np.random.default_rng(4)
+    for lens, mode in (([52] * 5, 1), ([60] * 3, 1), ([60] * 3, 2), ([10, 63, 63], 2)):
+        inst = oracle.Instance(len(lens), 0, np.array(lens, dtype=np.uint8),
+                               np.zeros((1, len(lens)), dtype=np.uint64))
+        for _ in range(200):
+            cs = np.array([int(rng.integers(0, 1 << n, dtype=np.uint64)) for n in lens], dtype=np.uint64)
+            assert _key_copy(cs, lens, mode) == oracle.relaxed_key(inst, cs, mode)
+    one = lambda lens, t, p: np.array([1 << p if i == t else 0 for i in range(len(lens))], dtype=np.uint64)
+    assert _key_copy(one([52] * 5, 1, 2), [52] * 5, 1, "stride") != L(27)
+    assert _key_copy(one([60] * 3, 1, 42), [60] * 3, 2, "take") != H(85)
+    assert _key_copy(one([10, 63, 63], 0, 9), [10, 63, 63], 2, "take") != L(9)

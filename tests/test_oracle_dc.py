@@ -41,3 +41,32 @@ def test_sampler_draws_distinct():
     d.rng = 5
     s = d.sample(list(range(100)), 30)
     assert len(set(s)) == 30 and all(0 <= x < 100 for x in s)
+
+
+def hand_traced_instance():
+    """Eight one-position traces over a, b, c (values at position 0):
+
+        P0 P1: a     P2 P3: b     N0 N2: none     N1: c     N3: a, c
+
+    DetSplit with window 4 (code/dc.py:188-249), traced by hand:
+      - 8 traces > 4: p1 = P0 P1, p2 = P2 P3, n1 = N0 N1, n2 = N2 N3;
+      - phi11 = learn(P0 P1 | N0 N1) = "a" (the first separating atom);
+        it fails P2 (no a) and accepts N3, so n2AndPhi11 = [N3];
+      - phi12 = learn(P0 P1 | N3): no atom separates; at cost 2 the
+        negations come first, ~(a) fails P0, ~(b) accepts N3, ~(c) separates;
+        ~(c) accepts N0, so it cannot replace phi11: left = (a)&(~(c)), which
+        fails P2: P2MinusLeft = P2 P3;
+      - phi21 = learn(P2 P3 | N0 N1) = "b"; it fails P0, rejects N2 and N3:
+        right = b;
+      - result (left)|(right) after three window calls."""
+    a = [1, 1, 0, 0, 0, 0, 0, 1]
+    b = [0, 0, 1, 1, 0, 0, 0, 0]
+    c = [0, 0, 0, 0, 0, 1, 0, 1]
+    return O.Instance(4, 4, np.ones(8, dtype=np.uint8), np.array([a, b, c], dtype=np.uint64), "abc")
+
+
+def test_detsplit_hand_traced():
+    inst = hand_traced_instance()
+    f, calls = DC.learn_dc(inst, 4, algo=0, max_cost=6)
+    assert (f, calls) == ("((a)&(~(c)))|(b)", 3)
+    assert O.mismatches(inst, O.eval_formula(inst, f)) == 0
