@@ -146,6 +146,22 @@ int32_t ltl_set_options(ltl_ctx *ctx, const ltl_options *opt);
  * LTLI() (code/ltli6463.cu:1043).  Returns res->status. */
 int32_t ltl_learn(ltl_ctx *ctx, const ltl_problem *pb, ltl_result *res);
 
+/* Allocation-free learning call.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same as ltl_learn
+ * on a context whose hbm_budget_bytes is LTL_NOALLOC_SCRATCH_BYTES (same
+ * store capacity, table and chunking, so the same result), except that the
+ * call's device memory is a static scratch region of that size inside the
+ * library, one per GPU, held until the process ends: the call allocates no
+ * device or pinned memory and creates no events.  Plain order only, no log,
+ * no profiling (LTL_E_ARG otherwise).  An instance whose buffers do not fit
+ * the region returns LTL_E_LIMIT.  Every call on a GPU shares that GPU's
+ * region and leaves its contents there (ltl_store_read of the context reads
+ * them until the next call): calls that overlap on one GPU, from any
+ * contexts, race. */
+#define LTL_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int32_t ltl_learn_noalloc(ltl_ctx *ctx, const ltl_problem *pb, ltl_result *res);
+
 /* Learn n independent instances in one persistent launch (no reference
  * counterpart; the reference runs one process per instance through
  * code/dc.py:23-43).  res[i] receives instance i.  Returns the number of

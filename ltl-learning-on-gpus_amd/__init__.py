@@ -25,6 +25,7 @@ E_ARG, E_LIMIT, E_OOM, E_HIP, E_COLLISION, E_NODEVICE = -1, -2, -3, -4, -5, -6
 ATOM, NOT, AND, OR, NEXT, FIN, GLOB, UNTIL = range(8)
 ORDER_PLAIN, ORDER_LEARNED = 0, 1
 DEDUP_EXACT, DEDUP_REF1, DEDUP_REF2, DEDUP_REF3 = 0, 1, 2, 3
+NOALLOC_SCRATCH_BYTES = 128 << 20   # LTL_NOALLOC_SCRATCH_BYTES of include/ltl_amd.h
 
 
 class LtlError(RuntimeError):
@@ -105,7 +106,7 @@ EXPORTS = ["ltl_default_options", "ltl_create", "ltl_set_options", "ltl_learn", 
            "ltl_gen_config", "ltl_gen_custom", "ltl_gen_free",
            "ltl_scorer_create", "ltl_scorer_destroy", "ltl_scorer_get_params", "ltl_scorer_set_params",
            "ltl_scorer_forward", "ltl_scorer_grad", "ltl_scorer_train", "ltl_set_scorer", "ltl_set_log",
-           "ltl_get_log", "ltl_learn_dc", "ltl_get_options"]
+           "ltl_get_log", "ltl_learn_dc", "ltl_get_options", "ltl_learn_noalloc"]
 
 DC_DET, DC_RAND = 0, 1
 
@@ -130,6 +131,7 @@ def lib():
     L.ltl_set_options.argtypes = [ctypes.c_void_p, P(Options)]
     L.ltl_learn.argtypes = [ctypes.c_void_p, P(Problem), P(Result)]
     L.ltl_learn_batch.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int32, P(Result)]
+    L.ltl_learn_noalloc.argtypes = [ctypes.c_void_p, P(Problem), P(Result)]
     L.ltl_result_free.argtypes = [P(Result)]
     L.ltl_set_batch_params.argtypes = [ctypes.c_void_p, P(BatchParams)]
     L.ltl_get_batch_params.argtypes = [ctypes.c_void_p, P(BatchParams)]
@@ -309,10 +311,18 @@ class Learner:
 
     def learn(self, inst, **opts):
         """ltl_learn with the constructor's options overridden by `opts`."""
+        return self._learn(lib().ltl_learn, "ltl_learn", inst, **opts)
+
+    def learn_noalloc(self, inst, **opts):
+        """ltl_learn_noalloc (no device allocation: the library's static
+        scratch region of NOALLOC_SCRATCH_BYTES per GPU)."""
+        return self._learn(lib().ltl_learn_noalloc, "ltl_learn_noalloc", inst, **opts)
+
+    def _learn(self, fn, name, inst, **opts):
         self.set_options(**opts)
         pb = inst.problem()
         res = Result()
-        rc = lib().ltl_learn(self._ctx, ctypes.byref(pb), ctypes.byref(res))
+        rc = fn(self._ctx, ctypes.byref(pb), ctypes.byref(res))
         out = dict(status=res.status, cost=res.cost, all_ltls=res.all_ltls, found_id=res.found_id,
                    n_stored=res.n_stored, on_the_fly=res.on_the_fly,
                    formula=(res.formula or b"").decode(), seconds=res.seconds,
@@ -323,7 +333,7 @@ class Learner:
                           for i in range(res.n_nodes)])
         lib().ltl_result_free(ctypes.byref(res))
         if rc < 0:
-            raise LtlError(f"ltl_learn -> {rc}")
+            raise LtlError(f"{name} -> {rc}")
         return out
 
     def learn_dc(self, inst, window=64, algo=DC_DET, seed=0, **opts):

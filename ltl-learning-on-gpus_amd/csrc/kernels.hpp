@@ -238,6 +238,9 @@ hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s);
 // Insert (keys[s], s) for s < n into an emptied table of tmask + 1 slots.
 hipError_t launch_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, int *probe_fail,
                          hipStream_t s);
+// The device address of the static scratch region of ltl_learn_noalloc on
+// `device` (which must be current): looked up once per device and kept.
+hipError_t noalloc_scratch(int device, char **ptr);
 // Exclusive scan of t_cnt into t_off (n elements); temp sized by scan_temp_bytes.
 size_t scan_temp_bytes(int n_max);
 hipError_t launch_scan(const uint32_t *in, uint32_t *out, int n, void *temp, size_t temp_bytes,

@@ -70,6 +70,9 @@ size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 constexpr int kCountSlots = 1 << 17;
 
+// Nodes of a formula walk read back from the device (k_tree).
+constexpr int32_t kTreeCap = 1024;
+
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -81,6 +84,12 @@ struct ltl_ctx {
     hipStream_t stream = nullptr;
     char *pool = nullptr;
     size_t pool_bytes = 0;
+    // Device memory of the current call: the pool, or the library's static
+    // scratch region of this device (ltl_learn_noalloc; its address from
+    // set-up, noalloc_scratch()).
+    char *arena = nullptr;
+    size_t arena_bytes = 0;
+    char *scratch = nullptr;
     DevState *h_state = nullptr;  // pinned
 
     // layout of the last call
@@ -153,7 +162,7 @@ struct ltl_ctx {
     void *d_sort = nullptr;
     size_t sort_cap = 0;
     float *d_log = nullptr;       // log rows
-    int32_t *d_tree = nullptr;    // the found formula's walk (1 + 3 * kTreeCap)
+    int32_t *d_tree = nullptr;    // the found formula's walk (1 + 3 * kTreeCap), in the call's memory
     int64_t log_cap = 0, log_n = 0;
     int32_t log_per_block = 0;
 };
@@ -219,6 +228,9 @@ extern "C" ltl_ctx *ltl_create(const ltl_options *opt, int32_t *status) {
         return fail(LTL_E_HIP);
     }
     for (int k = 0; k < K_NKIND; ++k) snprintf(c->stats[k].name, sizeof(c->stats[k].name), "%s", kKindName[k]);
+    // The device address of this GPU's scratch region (ltl_learn_noalloc),
+    // taken once per device with the device current; null if unavailable.
+    if (noalloc_scratch(o.device, &c->scratch) != hipSuccess) c->scratch = nullptr;
     if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, o.device) != hipSuccess) c->n_cu = 256;
     if (status) *status = 0;
     return c;
@@ -254,7 +266,7 @@ extern "C" void ltl_destroy(ltl_ctx *c) {
     for (auto e : c->free_events) chk(hipEventDestroy(e), "hipEventDestroy(free)");
     if (c->pool) chk(hipFree(c->pool), "hipFree(pool)");
     for (void *p : {(void *)c->d_tstat, (void *)c->d_toff, (void *)c->d_blk, (void *)c->d_keys, (void *)c->d_vals,
-                    c->d_sort, (void *)c->d_log, (void *)c->d_tree})
+                    c->d_sort, (void *)c->d_log})
         if (p) chk(hipFree(p), "hipFree");
     if (c->h_state) chk(hipHostFree(c->h_state), "hipHostFree(state)");
     if (c->h_counts) chk(hipHostFree(c->h_counts), "hipHostFree(counts)");
@@ -297,7 +309,7 @@ static int32_t ensure_pool(ltl_ctx *c) {
 }
 
 static int32_t carve(ltl_ctx *c) {
-    const size_t budget = c->pool_bytes;
+    const size_t budget = c->arena_bytes;
     const size_t per_cs = (size_t)c->nw * 8 + (size_t)c->nsw * 32 + 25;
     // chunk: per-candidate temporaries (19 B each) within 1/32 of the pool
     int64_t ch = 1 << 26;
@@ -315,7 +327,8 @@ static int32_t carve(ltl_ctx *c) {
                    align_up(tiles * 16, 256) +
                    2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) +
                    align_up((size_t)(c->nw + 4 * c->nsw) * c->A * 8, 256) + align_up((size_t)c->W * 4, 256) +
-                   align_up((size_t)c->W * 8, 256) + align_up((tiles / 4 + 2) * 8, 256) + 12 * 256;
+                   align_up((size_t)c->W * 8, 256) + align_up((tiles / 4 + 2) * 8, 256) +
+                   align_up((1 + 3 * (size_t)kTreeCap) * 4, 256) + 13 * 256;
     if (fixed >= budget) return LTL_E_OOM;
     const size_t R = budget - fixed;
     // Table: load stays <= 1/2 with a full chunk pending (cap + ch <= slots/2).
@@ -344,9 +357,10 @@ static int32_t carve(ltl_ctx *c) {
     c->tmask = (uint32_t)(slots - 1);
     c->tab_max_slots = slots;
 
-    char *p = c->pool;
+    char *p = c->arena;
     auto take = [&](size_t n) { char *q = p; p += align_up(n, 256); return q; };
     c->d_aux = (WordAux *)take((size_t)c->nw * sizeof(WordAux));
+    c->d_tree = (int32_t *)take((1 + 3 * (size_t)kTreeCap) * 4);
     c->d_state = (DevState *)take(sizeof(DevState));
     c->d_saux = (SumAux *)take((size_t)c->nsw * sizeof(SumAux));
     c->t_slot = (uint32_t *)take((size_t)ch * 4);
@@ -372,7 +386,7 @@ static int32_t carve(ltl_ctx *c) {
     c->d_key = (uint64_t *)take((size_t)capr * 8);
     c->d_store = (uint64_t *)take((size_t)capr * c->nw * 8);
     c->d_sum = (uint64_t *)take((size_t)capr * c->nsw * 32);
-    if ((size_t)(p - c->pool) > budget) return LTL_E_OOM;
+    if ((size_t)(p - c->arena) > budget) return LTL_E_OOM;
     return 0;
 }
 
@@ -1082,12 +1096,10 @@ int32_t read_meta(ltl_ctx *c, int64_t k, Meta &m) {
     return 0;
 }
 
-constexpr int32_t kTreeCap = 1024;
 
 // The nodes below the root as the walk of build_tree visits them, read on the
 // device (one launch and one copy); walk[0] < 0 when the walk did not finish.
 int32_t tree_walk(ltl_ctx *c, int32_t l, int32_t r, int64_t n_stored, std::vector<int32_t> &walk) {
-    if (!c->d_tree) HIP_TRY(hipMalloc((void **)&c->d_tree, (1 + 3 * (size_t)kTreeCap) * 4));
     HIP_TRY(launch_tree(c->d_op, c->d_left, c->d_right, n_stored, l, r, c->d_tree, kTreeCap, c->stream));
     walk.assign(1 + 3 * (size_t)kTreeCap, 0);
     HIP_TRY(hipMemcpyAsync(walk.data(), c->d_tree, walk.size() * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1170,7 +1182,10 @@ std::string node_string(const std::vector<ltl_node> &n, int i, const std::string
 
 }  // namespace
 
-extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res) {
+// One learning call.  noalloc: the device memory of the call is this
+// device's static scratch region (no allocation of any kind on the device,
+// no pinned staging, no events); plain order only.
+static int32_t learn_call(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bool noalloc) {
     if (!res) return LTL_E_ARG;
     memset(res, 0, sizeof(*res));
     res->found_id = -1;
@@ -1178,6 +1193,9 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     const double t0 = now_s();
     const ltl_options &o = c->opt;
     if (o.order == LTL_ORDER_LEARNED && !c->scorer) return res->status = LTL_E_ARG;  // no scorer attached
+    // (the learned order, the log and profiling allocate buffers or events)
+    if (noalloc && (o.order != LTL_ORDER_PLAIN || c->log_cap > 0 || o.profile)) return res->status = LTL_E_ARG;
+    if (noalloc && !c->scratch) return res->status = LTL_E_HIP;
     // An error left by an earlier HIP call of this thread (kernel launches
     // are checked with hipGetLastError) is cleared here, reported with
     // LTL_DEBUG.
@@ -1223,9 +1241,22 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     c->pk_off.swap(prep.pk_off);
     c->lens.assign(pb->lens, pb->lens + W);
     c->lazy_s = c->lazy_e = 0;
-    if (int32_t e = ensure_pool(c)) return res->status = e;
-    if (int32_t e = carve(c)) return res->status = e;
-    if (c->capacity < A) return res->status = LTL_E_OOM;
+    if (noalloc) {
+        c->arena = c->scratch;
+        c->arena_bytes = LTL_NOALLOC_SCRATCH_BYTES;
+        // an instance whose buffers do not fit the scratch region is beyond
+        // this entry point's limits
+        if (carve(c) || c->capacity < A) {
+            report_hip(hipErrorOutOfMemory, "ltl_learn_noalloc: instance beyond the scratch region", __LINE__);
+            return res->status = LTL_E_LIMIT;
+        }
+    } else {
+        if (int32_t e = ensure_pool(c)) return res->status = e;
+        c->arena = c->pool;
+        c->arena_bytes = c->pool_bytes;
+        if (int32_t e = carve(c)) return res->status = e;
+        if (c->capacity < A) return res->status = LTL_E_OOM;
+    }
     const int nsw = c->nsw;
     std::vector<WordAux> &aux = prep.aux;
     std::vector<SumAux> &saux = prep.saux;
@@ -1245,7 +1276,7 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
                               align_up((atom_words.size() + sum0.size()) * 8, 256) + align_up(sizeof(st0), 256) +
                               align_up((size_t)W * 8, 256);
     HIP_TRY(hipStreamSynchronize(s));
-    if (init_bytes > c->h_init_cap) {
+    if (!noalloc && init_bytes > c->h_init_cap) {
         if (c->h_init) hipHostFree(c->h_init);
         c->h_init = nullptr;
         c->h_init_cap = 0;
@@ -1255,6 +1286,8 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     size_t init_off = 0;
     auto upload_init = [&](void *dst, const void *src, size_t n) -> hipError_t {
         if (n == 0) return hipSuccess;
+        // (no staging buffer: a copy from pageable memory, waited for below)
+        if (noalloc) return hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s);
         memcpy(c->h_init + init_off, src, n);
         const hipError_t e = hipMemcpyAsync(dst, c->h_init + init_off, n, hipMemcpyHostToDevice, s);
         init_off += align_up(n, 256);
@@ -1933,6 +1966,18 @@ extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res)
     }
     res->seconds = now_s() - t0;
     return res->status;
+}
+
+extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res) {
+    return learn_call(c, pb, res, false);
+}
+
+// The call's device memory is this device's scratch region of
+// LTL_NOALLOC_SCRATCH_BYTES (shared by every context on the device: calls
+// that overlap on one GPU race); every call initialises what it reads (state,
+// table, look-back words) with kernels before use.
+extern "C" int32_t ltl_learn_noalloc(ltl_ctx *c, const ltl_problem *pb, ltl_result *res) {
+    return learn_call(c, pb, res, true);
 }
 
 // ---------------------------------------------------------------------------
