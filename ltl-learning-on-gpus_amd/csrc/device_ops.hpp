@@ -186,24 +186,36 @@ __device__ __forceinline__ uint64_t pk_unary(uint64_t x, const WordAux *a) {
 // positions are wave-uniform: the loops run on the scalar unit, one per
 // 32-bit half, and each bit costs one extract and one shift-or per vector.
 // The traces of a half are taken in runs that end at a group's end, so the
-// end-of-group test runs once per run, not once per trace.
+// end-of-group test runs once per run, not once per trace.  [g_lo, g_hi):
+// only the traces of these groups are taken (a word range of a split emit).
 template <int VMASK, class Flush>
 __device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint64_t (&acc)[4], int W,
-                                             Flush &&flush) {
+                                             Flush &&flush, int g_lo = 0, int g_hi = 0x7fffffff) {
     const uint64_t inner = x->inner, lm = x->lm;
     uint64_t u[4] = {c, 0, 0, 0};
     if constexpr ((VMASK & 2) != 0) u[1] = (c << 1) & inner;
     if constexpr ((VMASK & 4) != 0) u[2] = pk_chain(inner, c);
     if constexpr ((VMASK & 8) != 0) u[3] = ~pk_chain(inner, ~c & lm) & lm;
-    const uint64_t chk = x->chk, si = x->sinfo;
+    uint64_t chk = x->chk;
+    const uint64_t si = x->sinfo;
     int g = (int)sinfo_word(si);                       // group of the word's first trace
-    const int t = g * 64 + sinfo_bit(si);              // its packed index
+    int t = g * 64 + sinfo_bit(si);                    // its packed index
+    // Groups below g_lo belong to an earlier word range: their traces, the
+    // word's first ones, are dropped from the top of chk.
+    if (g < g_lo) {
+        for (int k = g_lo * 64 - t, n = __popcll(chk); k > 0 && n > 0; --k, --n)
+            chk &= ~(1ull << (63 - __builtin_clzll(chk)));
+        if (!chk) return;
+        g = g_lo;
+        t = g_lo * 64;
+    }
+    if (g >= g_hi) return;
     int left = (W - 1 < (t | 63) ? W - 1 : (t | 63)) - t + 1;  // traces until the group is complete
     if constexpr (VMASK == 0) {  // only the flushes
         for (int n = __popcll(chk); left > 0 && n >= left;) {
             n -= left;
             flush(g);
-            ++g;
+            if (++g >= g_hi) return;
             left = W - 64 * g < 64 ? W - 64 * g : 64;
         }
         return;
@@ -226,7 +238,7 @@ __device__ __forceinline__ void summarize_pk(uint64_t c, const WordAux *x, uint6
             left -= n;
             if (left == 0) {
                 flush(g);
-                ++g;
+                if (++g >= g_hi) return;
                 left = W - 64 * g < 64 ? W - 64 * g : 64;
             }
         }

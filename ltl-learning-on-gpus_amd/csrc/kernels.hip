@@ -49,12 +49,16 @@ struct Tile {
     bool valid;
 };
 
+// Binary chunks: a workgroup of 4 waves covers 4 R consecutive left rows of
+// one 64-column right stripe, wave w rows 4 R q + R w .. + R - 1 (t.a is the
+// first, t.ti its tile; row r's tile is t.ti + r * nbs).
+template <int R = 1>
 __device__ __forceinline__ bool tile_binary(const ChunkArgs &A, Tile &t) {
-    const int nA = (int)(A.a1 - A.a0), nAq = (nA + 3) >> 2;
+    const int nA = (int)(A.a1 - A.a0), nAq = (nA + 4 * R - 1) / (4 * R);
     const int lw = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int bs = lw / nAq, aq = lw % nAq;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    t.a = A.a0 + 4 * aq + wave;
+    t.a = A.a0 + (int64_t)(4 * aq + wave) * R;
     if (t.a >= A.a1) return false;
     t.b = A.b0 - A.ush + 64 * (int64_t)bs + (threadIdx.x & 63);
     t.valid = t.b >= A.b0 && t.b < A.b1;
@@ -161,58 +165,86 @@ __device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, con
     if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, s0);
 }
 
-template <int OP, bool RX, bool CHK, bool SUCC>
+// R left rows per wave: each right word is loaded once for R candidates
+// (the left words are wave-uniform scalar loads), so the right operand's
+// reads from L2 drop R-fold; the fingerprint arithmetic per candidate stays.
+template <int OP, bool RX, bool CHK, bool SUCC, int R>
 __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
-    if (!tile_binary(A, t)) return;
+    if (!tile_binary<R>(A, t)) return;
     const bool otf = chunk_otf(A);
-    const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
-    const uint64_t *pl = A.store + set_base(sl, A.nw);
+    const int nrow = (int)(A.a1 - t.a < R ? A.a1 - t.a : R);  // wave-uniform
+    const int64_t sr = A.rs + (t.valid ? t.b : A.b0);
     const uint64_t *pr = A.store + set_base(sr, A.nw);
-    Acc c0, c1;
-    KeyAcc k0, k1;
-    SuccAcc s0, s1;
+    const uint64_t *pl[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) pl[r] = A.store + set_base(A.ls + t.a + (r < nrow ? r : 0), A.nw);
+    Acc c0[R], c1[R];
+    KeyAcc k0[R], k1[R];
+    SuccAcc s0[R], s1[R];
     const int sm = A.succ_mask;
-    auto step = [&](int w, uint64_t l, uint64_t r) {
+    auto step = [&](int w, const uint64_t (&l)[R], uint64_t rv) {
         const WordAux *x = A.aux + w;
-        if constexpr (OP == LTL_UNTIL) {
-            const uint64_t u0 = pk_until(l, r, x->inner), u1 = pk_until(r, l, x->inner);
-            fold<CHK>(c0, u0, x);
-            fold<CHK>(c1, u1, x);
-            succ_fold<SUCC>(s0, u0, x, sm);
-            succ_fold<SUCC>(s1, u1, x, sm);
-            if constexpr (RX) {
-                key_fold_pk(k0, u0, w, x, A);
-                key_fold_pk(k1, u1, w, x, A);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            if constexpr (OP == LTL_UNTIL) {
+                const uint64_t u0 = pk_until(l[r], rv, x->inner), u1 = pk_until(rv, l[r], x->inner);
+                fold<CHK>(c0[r], u0, x);
+                fold<CHK>(c1[r], u1, x);
+                succ_fold<SUCC>(s0[r], u0, x, sm);
+                succ_fold<SUCC>(s1[r], u1, x, sm);
+                if constexpr (RX) {
+                    key_fold_pk(k0[r], u0, w, x, A);
+                    key_fold_pk(k1[r], u1, w, x, A);
+                }
+            } else {
+                const uint64_t v = OP == LTL_AND ? (l[r] & rv) : (l[r] | rv);
+                fold<CHK>(c0[r], v, x);
+                succ_fold<SUCC>(s0[r], v, x, sm);
+                if constexpr (RX) key_fold_pk(k0[r], v, w, x, A);
             }
-        } else {
-            const uint64_t v = OP == LTL_AND ? (l & r) : (l | r);
-            fold<CHK>(c0, v, x);
-            succ_fold<SUCC>(s0, v, x, sm);
-            if constexpr (RX) key_fold_pk(k0, v, w, x, A);
         }
     };
-    // Batches of UB words: all loads of a batch are issued before use, so
-    // that each lane keeps UB right-operand loads in flight (the left
-    // operand is wave-uniform).  & and |: 16 (6.8 -> 6.5 ms per config-2
-    // call against 4); U, with two carry chains per word: 4 (8 is slower).
-    constexpr int UB = OP == LTL_UNTIL ? 4 : 16;
+    // Batches of UB words: all right-operand loads of a batch are issued
+    // before use, so that each lane keeps UB loads in flight.  Measured at
+    // config 2 (gen_and_or per call): R = 1, UB = 16: 6.4 ms; R = 2, UB = 8:
+    // 4.9 ms; R = 4, UB = 4 or 8: 5.5-5.6 ms.  U: R = 1, UB = 4: 5.45 ms;
+    // UB = 8 or R = 2: 5.7-6.0 ms (two carry chains and four fingerprints
+    // per pair keep it VALU-bound).
+    constexpr int UB = R == 1 ? (OP == LTL_UNTIL ? 4 : 16) : 8;
     int w = 0;
     for (; w + UB <= A.nw; w += UB) {
-        uint64_t l[UB], r[UB];
+        uint64_t rv[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) { l[u] = pl[(size_t)(w + u) * kTileWS]; r[u] = pr[(size_t)(w + u) * kTileWS]; }
+        for (int u = 0; u < UB; ++u) rv[u] = pr[(size_t)(w + u) * kTileWS];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
+        for (int u = 0; u < UB; ++u) {
+            uint64_t l[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) l[r] = pl[r][(size_t)(w + u) * kTileWS];
+            step(w + u, l, rv[u]);
+        }
     }
-    for (; w < A.nw; ++w) step(w, pl[(size_t)w * kTileWS], pr[(size_t)w * kTileWS]);
+    for (; w < A.nw; ++w) {
+        uint64_t l[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) l[r] = pl[r][(size_t)w * kTileWS];
+        step(w, l, pr[(size_t)w * kTileWS]);
+    }
     if (!t.valid) return;
-    if constexpr (RX) {
-        key_finish(c0, k0);
-        if constexpr (OP == LTL_UNTIL) key_finish(c1, k1);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (r >= nrow) break;
+        if constexpr (RX) {
+            key_finish(c0[r], k0[r]);
+            if constexpr (OP == LTL_UNTIL) key_finish(c1[r], k1[r]);
+        }
+        Tile tr = t;
+        tr.a = t.a + r;
+        tr.ti = t.ti + (int64_t)r * A.nbs;
+        gen2_tail<OP, CHK, SUCC, RX>(A, tr, c0[r], c1[r], s0[r], s1[r], otf);
     }
-    gen2_tail<OP, CHK, SUCC, RX>(A, t, c0, c1, s0, s1, otf);
 }
 
 template <int OP, bool RX, bool CHK, bool SUCC>
@@ -795,15 +827,26 @@ __global__ __launch_bounds__(256) void k_dedup_lb(ChunkArgs A) {
 // operand's b1); for F, b0 and nz (the operand's nz); for G, b0 and fl (the
 // operand's fl); for U, nz (the right operand's).  The rest come from the
 // words.
+//
+// Word ranges: the A.esplit waves of a store tile each take the summary
+// groups [g0, g1) of an even split of the nsw groups, and the words that hold
+// their traces, [gfirst[g0], glast[g1 - 1]] (a word shared by two groups at a
+// range boundary is written by both ranges, with the same value).  More waves
+// stream each tile's words in parallel: a chunk's survivors alone are too few
+// waves to hide the memory latency.
 template <int OP>
 __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__restrict__ aux,
                                               const SumAux *__restrict__ saux) {
     if (run_stopped(A) || chunk_otf(A)) return;
-    // Waves cover whole store tiles: lane l of wave v writes position
-    // 64 * (chunk_base / 64 + v) + l, so every store instruction of a wave
-    // fills one 512-byte segment of one tile.
+    // Waves cover whole store tiles: lane l of the waves of tile v writes
+    // position 64 * (chunk_base / 64 + v) + l, so every store instruction of a
+    // wave fills one 512-byte segment of one tile.
     const int64_t base = A.st->chunk_base;
-    const int64_t pos = (base & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int K = A.esplit;
+    // (wave-uniform: the range loops below run on the scalar unit)
+    const int64_t q = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int rg = (int)(q % K);
+    const int64_t pos = (base & ~63ll) + (q / K) * 64 + (threadIdx.x & 63);
     const int64_t i = pos - base;
     if (i < 0 || i >= A.st->chunk_total) return;
     if (pos >= A.capacity) return;
@@ -821,13 +864,15 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
     }
     const uint64_t *pl = A.store + set_base(sl, A.nw);
     const uint64_t *pr = A.store + set_base(bin ? sr : sl, A.nw);
-    uint64_t *q = A.store + set_base(pos, A.nw);
+    uint64_t *qs = A.store + set_base(pos, A.nw);
     uint64_t *z = A.sum + set_base(pos, 4 * A.nsw);
     const uint64_t *zl = A.sum + set_base(sl, 4 * A.nsw), *zr = A.sum + set_base(bin ? sr : sl, 4 * A.nsw);
     // vectors computed from the words (bit v for vector v); the others follow from the operands
     constexpr int VM = OP == LTL_NOT ? 0 : OP == LTL_AND ? 4 : OP == LTL_OR ? 8 : OP == LTL_NEXT ? 14
                      : OP == LTL_FIN ? 10 : OP == LTL_GLOB ? 6 : 11;
     uint64_t acc[4] = {0, 0, 0, 0};
+    const int g0 = rg * A.nsw / K, g1 = (rg + 1) * A.nsw / K;
+    const int w0 = K > 1 ? A.gfirst[g0] : 0, w1 = K > 1 ? A.glast[g1 - 1] + 1 : A.nw;
     // Summary word j is complete: the vectors that follow from the operands'
     // summaries, the accumulated ones, stored; the accumulators cleared.
     auto flush = [&](int j) {
@@ -872,12 +917,12 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
         else if constexpr (OP == LTL_OR) c = l | r;
         else if constexpr (OP == LTL_UNTIL) c = swap ? pk_until(r, l, x->inner) : pk_until(l, r, x->inner);
         else c = pk_unary<OP>(l, x);
-        __builtin_nontemporal_store(c, q + (size_t)w * kTileWS);  // streamed: not re-read in this chunk
-        summarize_pk<VM>(c, x, acc, A.W, flush);
+        __builtin_nontemporal_store(c, qs + (size_t)w * kTileWS);  // streamed: not re-read in this chunk
+        summarize_pk<VM>(c, x, acc, A.W, flush, g0, g1);
     };
     constexpr int UB = bin ? 8 : 16;
-    int w = 0;
-    for (; w + UB <= A.nw; w += UB) {
+    int w = w0;
+    for (; w + UB <= w1; w += UB) {
         uint64_t l[UB], r[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) {
@@ -887,7 +932,8 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
     }
-    for (; w < A.nw; ++w) step(w, pl[(size_t)w * kTileWS], bin ? pr[(size_t)w * kTileWS] : 0);
+    for (; w < w1; ++w) step(w, pl[(size_t)w * kTileWS], bin ? pr[(size_t)w * kTileWS] : 0);
+    if (rg != 0) return;
     if (bin) commit(A, pos, k, OP, (int32_t)(swap ? sr : sl), (int32_t)(swap ? sl : sr));
     else commit(A, pos, k, OP, (int32_t)sl, -1);
 }
@@ -1216,10 +1262,17 @@ __global__ void k_fill(uint64_t *p, uint64_t v, size_t n) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-static dim3 grid_binary(const ChunkArgs &a) {
-    const int64_t nAq = (a.a1 - a.a0 + 3) / 4;
+// R: left rows per wave (tile_binary<R>).
+static dim3 grid_binary(const ChunkArgs &a, int R = 1) {
+    const int64_t nAq = (a.a1 - a.a0 + 4 * R - 1) / (4 * R);
     return dim3((unsigned)(nAq * a.nbs));
 }
+// Left rows per wave of the binary generation kernels (& and |; U keeps one
+// row: its two carry chains per word and candidate pair are the larger
+// cost); relaxed keys fold per-row key accumulators and keep one row.
+template <int OP>
+constexpr int gen_rows() { return OP == LTL_UNTIL ? 1 : 2; }
+constexpr int kGenRows = 2;
 static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles + 3) / 4)); }
 
 // The chunk kernels run on the packed layout (B = 1); K words per trace
@@ -1234,6 +1287,21 @@ static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles +
         hipLaunchKernelGGL((KER<OP, RX, true, false>), GRID, BLOCK, 0, S, __VA_ARGS__);         \
     } else {                                                                                    \
         hipLaunchKernelGGL((KER<OP, RX, false, false>), GRID, BLOCK, 0, S, __VA_ARGS__);        \
+    }
+
+// The same for k_gen2 with R left rows per wave (R = 1 with relaxed keys).
+#define LTL_DISPATCH_GEN2(OP, B, RX, S)                                                                  \
+    if ((B) != 1) return hipErrorInvalidValue;                                                           \
+    if (a.t_succ) {                                                                                      \
+        if (RX || a.gen_check) return hipErrorInvalidValue;                                              \
+        hipLaunchKernelGGL((k_gen2<OP, false, false, true, gen_rows<OP>()>), grid_binary(a, gen_rows<OP>()), dim3(256), 0, S, a); \
+    } else if (RX) {                                                                                     \
+        if (a.gen_check) hipLaunchKernelGGL((k_gen2<OP, RX, true, false, 1>), grid_binary(a), dim3(256), 0, S, a); \
+        else hipLaunchKernelGGL((k_gen2<OP, RX, false, false, 1>), grid_binary(a), dim3(256), 0, S, a);  \
+    } else if (a.gen_check) {                                                                            \
+        hipLaunchKernelGGL((k_gen2<OP, false, true, false, gen_rows<OP>()>), grid_binary(a, gen_rows<OP>()), dim3(256), 0, S, a); \
+    } else {                                                                                             \
+        hipLaunchKernelGGL((k_gen2<OP, false, false, false, gen_rows<OP>()>), grid_binary(a, gen_rows<OP>()), dim3(256), 0, S, a); \
     }
 
 #define LTL_DISPATCH_B(KER, OP, B, GRID, BLOCK, S, ...)                               \
@@ -1255,8 +1323,12 @@ static hipError_t launch_gen_mw(int op, const ChunkArgs &a, hipStream_t s) {
     case LTL_NEXT: LTL_DISPATCH_K(k_gen1m, LTL_NEXT, a.K, grid_tiles(a), s, a); break;
     case LTL_FIN: LTL_DISPATCH_K(k_gen1m, LTL_FIN, a.K, grid_tiles(a), s, a); break;
     case LTL_GLOB: LTL_DISPATCH_K(k_gen1m, LTL_GLOB, a.K, grid_tiles(a), s, a); break;
-    case LTL_AND: hipLaunchKernelGGL((k_gen2<LTL_AND, false, true, false>), grid_binary(a), dim3(256), 0, s, a); break;
-    case LTL_OR: hipLaunchKernelGGL((k_gen2<LTL_OR, false, true, false>), grid_binary(a), dim3(256), 0, s, a); break;
+    case LTL_AND:
+        hipLaunchKernelGGL((k_gen2<LTL_AND, false, true, false, kGenRows>), grid_binary(a, kGenRows), dim3(256), 0, s, a);
+        break;
+    case LTL_OR:
+        hipLaunchKernelGGL((k_gen2<LTL_OR, false, true, false, kGenRows>), grid_binary(a, kGenRows), dim3(256), 0, s, a);
+        break;
     case LTL_UNTIL:
         switch (a.K) {
         case 2: hipLaunchKernelGGL(k_gen2m<2>, grid_binary(a), dim3(256), 0, s, a); break;
@@ -1277,9 +1349,9 @@ static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s)
     case LTL_NEXT: LTL_DISPATCH_BX(k_gen1, LTL_NEXT, B, RX, grid_tiles(a), dim3(256), s, a); break;
     case LTL_FIN: LTL_DISPATCH_BX(k_gen1, LTL_FIN, B, RX, grid_tiles(a), dim3(256), s, a); break;
     case LTL_GLOB: LTL_DISPATCH_BX(k_gen1, LTL_GLOB, B, RX, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_AND: LTL_DISPATCH_BX(k_gen2, LTL_AND, B, RX, grid_binary(a), dim3(256), s, a); break;
-    case LTL_OR: LTL_DISPATCH_BX(k_gen2, LTL_OR, B, RX, grid_binary(a), dim3(256), s, a); break;
-    case LTL_UNTIL: LTL_DISPATCH_BX(k_gen2, LTL_UNTIL, B, RX, grid_binary(a), dim3(256), s, a); break;
+    case LTL_AND: LTL_DISPATCH_GEN2(LTL_AND, B, RX, s); break;
+    case LTL_OR: LTL_DISPATCH_GEN2(LTL_OR, B, RX, s); break;
+    case LTL_UNTIL: LTL_DISPATCH_GEN2(LTL_UNTIL, B, RX, s); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1332,9 +1404,13 @@ hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s) {
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
     if (a.ws != kTileWS || (B != 1 && a.t_succ)) return hipErrorInvalidValue;
     // one lane per survivor; the survivor count is on the device, so the
-    // grid covers every candidate of the chunk and idle lanes exit
+    // grid covers every candidate of the chunk and idle lanes exit (the packed
+    // emit: a.esplit waves per store tile)
     const int nd = op == LTL_UNTIL ? 2 : 1;
-    const dim3 g((unsigned)(((int64_t)a.n_tiles * 64 * nd + 63 + 255) / 256)), b(256);
+    const int64_t pos_tiles = ((int64_t)a.n_tiles * 64 * nd + 63 + 63) / 64;
+    const int K = B == 1 && !a.deferred ? a.esplit : 1;
+    if (K < 1 || (K > 1 && (!a.gfirst || !a.glast || K > a.nsw))) return hipErrorInvalidValue;
+    const dim3 g((unsigned)((pos_tiles * K + 3) / 4)), b(256);
     if (a.deferred) {
         if (op < LTL_NOT || op > LTL_UNTIL) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_emit_meta, g, b, 0, s, a, op);

@@ -171,6 +171,11 @@ struct ChunkArgs {
     // key is new (code/ltli6463.cu:231-243), so generation records the
     // outcome here and dedup folds it in for the survivors only.
     uint8_t *t_acc = nullptr;
+    // Packed layout: the emit's waves per store tile (word ranges at summary
+    // group boundaries) and, per summary group, the first and last word that
+    // hold its traces.
+    int32_t esplit = 1;
+    const int32_t *gfirst = nullptr, *glast = nullptr;
 };
 
 // Check passes visit words alternately from the positive and the negative

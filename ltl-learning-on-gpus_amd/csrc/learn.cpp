@@ -118,6 +118,7 @@ struct ltl_ctx {
     int64_t lb_words = 0;
     uint32_t lb_epoch = 0;        // chunks deduplicated by look-back in this call
     uint64_t *d_ptrace = nullptr; // packed layout: per packed trace, trace | offset << 32 | length << 40
+    int32_t *d_gword = nullptr;   // packed layout: per summary group, its first word; then its last
     void *d_scan = nullptr;
     size_t scan_bytes = 0;
     std::vector<int64_t> lvl_s, lvl_e;
@@ -328,7 +329,7 @@ static int32_t carve(ltl_ctx *c) {
                    2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) +
                    align_up((size_t)(c->nw + 4 * c->nsw) * c->A * 8, 256) + align_up((size_t)c->W * 4, 256) +
                    align_up((size_t)c->W * 8, 256) + align_up((tiles / 4 + 2) * 8, 256) +
-                   align_up((1 + 3 * (size_t)kTreeCap) * 4, 256) + 13 * 256;
+                   align_up((1 + 3 * (size_t)kTreeCap) * 4, 256) + align_up((size_t)c->nsw * 8, 256) + 14 * 256;
     if (fixed >= budget) return LTL_E_OOM;
     const size_t R = budget - fixed;
     // Table: load stays <= 1/2 with a full chunk pending (cap + ch <= slots/2).
@@ -374,6 +375,7 @@ static int32_t carve(ltl_ctx *c) {
     c->d_stage = (uint64_t *)take((size_t)(c->nw + 4 * c->nsw) * c->A * 8);
     c->d_kinfo = (uint32_t *)take((size_t)c->W * 4);
     c->d_ptrace = (uint64_t *)take((size_t)c->W * 8);
+    c->d_gword = (int32_t *)take((size_t)c->nsw * 8);
     c->lb_words = (int64_t)(tiles / 4 + 2);  // a status word per workgroup, the ticket in the last
     c->d_lb = (uint64_t *)take((size_t)c->lb_words * 8);
     c->d_tab = (uint64_t *)take(slots * 16);
@@ -488,6 +490,7 @@ struct Prepared {
     std::vector<uint64_t> ptrace;      // per packed trace: trace | offset << 32 | length << 40
     std::vector<int> pk_word, pk_off;  // per trace: word and bit offset
     std::vector<int> order;            // summary bit -> trace (packing order)
+    std::vector<int32_t> gword;        // per summary group: first word, then (at nsw + g) last word
 };
 
 int words_per_trace(const ltl_problem *pb) { return pb->words_per_trace > 1 ? pb->words_per_trace : 1; }
@@ -537,9 +540,12 @@ void prepare_packed(const ltl_problem *pb, Prepared &q) {
         used[b] += L;
         if (used[b] < 64) by_free[64 - used[b]].push_back(b);
     }
+    std::vector<int32_t> tword;  // packed trace -> word
+    tword.reserve(W);
     for (size_t b = 0; b < bins.size(); ++b) {
         const int w = (int)q.aux.size();
         const int t0 = (int)order.size();
+        for (size_t k = 0; k < bins[b].size(); ++k) tword.push_back(w);
         WordAux x;
         memset(&x, 0, sizeof(x));
         x.k1 = splitmix(seed ^ (2ull * w));
@@ -562,6 +568,11 @@ void prepare_packed(const ltl_problem *pb, Prepared &q) {
         q.aux.push_back(x);
     }
     q.nw = (int)q.aux.size();
+    q.gword.assign(2 * (size_t)q.nsw, 0);
+    for (int g = 0; g < q.nsw; ++g) {
+        q.gword[g] = tword[64 * g];
+        q.gword[q.nsw + g] = tword[std::min(W, 64 * g + 64) - 1];
+    }
     // The emit kernels shift each trace's summary bits in, in packing order:
     // the t-th of the n traces of group g sits at bit n - 1 - t of summary
     // word g.  order[] becomes summary bit -> trace.
@@ -1274,7 +1285,7 @@ static int32_t learn_call(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     // here, so the buffer is free.
     const size_t init_bytes = align_up(aux.size() * sizeof(WordAux), 256) + align_up(saux.size() * sizeof(SumAux), 256) +
                               align_up((atom_words.size() + sum0.size()) * 8, 256) + align_up(sizeof(st0), 256) +
-                              align_up((size_t)W * 8, 256);
+                              align_up((size_t)W * 8, 256) + align_up(prep.gword.size() * 4, 256);
     HIP_TRY(hipStreamSynchronize(s));
     if (!noalloc && init_bytes > c->h_init_cap) {
         if (c->h_init) hipHostFree(c->h_init);
@@ -1339,6 +1350,16 @@ static int32_t learn_call(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     ba.K = K;
     ba.ptrace = c->d_ptrace;
     if (B == 1) HIP_TRY(upload_init(c->d_ptrace, prep.ptrace.data(), (size_t)W * 8));
+    // Packed emits: up to eight waves per store tile, each a word range of
+    // whole summary groups of 40 words or more (config 2, per call: one wave
+    // per tile 16.1 ms of emits, four 10.1 ms, eight 8.9 ms, sixteen 8.9 ms).
+    ba.esplit = 1;
+    if (B == 1) {
+        HIP_TRY(upload_init(c->d_gword, prep.gword.data(), (size_t)nsw * 8));
+        ba.gfirst = c->d_gword;
+        ba.glast = c->d_gword + nsw;
+        ba.esplit = std::max(1, std::min({8, nsw, c->nw / 40}));
+    }
     // Relaxed keys (code/ltli6463.cu:57-141) only above 126 positions; below,
     // every mode is the exact packing, which the exact dedup reproduces.
     ba.W = W;
