@@ -339,15 +339,20 @@ def main():
     achieved = d["alg_bytes"] / d["seconds"] / 1e9
     traffic = traffic_for(dom)
     avg_s = d["seconds"] / max(d["launches"], 1)
-    roof = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
+    # achieved / frac: the dominant kernel's HBM traffic per launch (calibrated
+    # rocprofv3 FETCH_SIZE + WRITE_SIZE, profiles/traffic.json) over this run's
+    # average launch, a physical fraction of the 8 TB/s peak.  The algorithmic
+    # byte model (operand sets counted once per candidate and operand, so
+    # re-reads served by L2 count too) is reported beside it.
+    alg_per = d["alg_bytes"] / max(d["launches"], 1)
+    phys = traffic / avg_s if traffic else None
+    roof = {"bound": "hbm", "kernel": dom,
+            "achieved": round(phys / 1e9, 1) if phys else None, "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s", "frac": round(phys / HBM_PEAK, 4) if phys else None,
             "traffic": traffic,
-            # physical HBM rate: the profiled bytes per launch over this run's average launch
-            # (below the algorithmic rate where operands are re-read from L2)
-            "physical_gbs": round(traffic / avg_s / 1e9, 1) if traffic else None,
-            "physical_frac": round(traffic / avg_s / HBM_PEAK, 4) if traffic else None,
-            "alg_bytes_per_launch": int(d["alg_bytes"] / max(d["launches"], 1)),
-            "avg_launch_ms": round(d["seconds"] / max(d["launches"], 1) * 1e3, 4),
+            "avg_launch_ms": round(avg_s * 1e3, 4),
+            "alg_bytes_per_launch": int(alg_per),
+            "alg_gbs": round(achieved, 1),
             "compulsory_gbs": round(d["min_bytes"] / d["seconds"] / 1e9, 1)}
     out = {
         "metric": METRIC, "value": round(uniques / wall, 1), "unit": "unique CS/s",
@@ -360,6 +365,9 @@ def main():
                    "traces": inst.W, "atoms": int(inst.atoms.shape[0]), "max_cost": args.max_cost},
         "time_to_min_formula_s": round(wall / args.steps, 4),
         "min_cost": r0["cost"], "formula": r0["formula"], "uniques_per_step": r0["n_stored"], "words_deferred_per_step": r0["words_deferred"],
+        # sets stored with their words and summaries (the deferred last level,
+        # stored with parents only, left out), per second of call time
+        "stored_full_per_s": round((uniques - r0["words_deferred"] * args.steps) / wall, 1),
         "candidates_per_step": r0["candidates"], "checked_per_step": r0["checked"],
         "kernels_ms_per_step": {k: round(v["seconds"] / args.steps * 1e3, 3) for k, v in stats.items()
                                 if v["launches"]},

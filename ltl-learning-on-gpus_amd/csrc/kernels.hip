@@ -639,7 +639,10 @@ __device__ __forceinline__ void commit(const ChunkArgs &A, int64_t pos, size_t k
     A.m_left[pos] = l;
     A.m_right[pos] = r;
     A.m_fp2[pos] = A.t_fp2[k];
-    A.m_key[pos] = A.tab[2 * (size_t)A.t_slot[k]];
+    // (the key is kept for re-inserting the stored sets into a grown table;
+    // a table of fixed size never grows, and the read would pull a random
+    // table line per survivor)
+    if (A.keep_keys) A.m_key[pos] = A.tab[2 * (size_t)A.t_slot[k]];
     A.tab[2 * (size_t)A.t_slot[k] + 1] = (uint64_t)pos;
 }
 

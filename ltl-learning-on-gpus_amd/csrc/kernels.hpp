@@ -108,6 +108,7 @@ struct ChunkArgs {
     int32_t *m_left, *m_right;
     uint64_t *m_fp2;
     uint64_t *m_key;          // per stored set: its table key (for rehashing a grown table)
+    int32_t keep_keys;        // 1: survivors record m_key (the table may grow)
     uint64_t *sum;            // summaries: 4 vectors of nsw words per set, word-major
     const SumAux *saux;
     int64_t capacity;

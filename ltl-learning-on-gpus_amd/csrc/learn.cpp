@@ -1391,6 +1391,7 @@ static int32_t learn_call(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         }
     }
     ba.capacity = c->capacity;
+    ba.keep_keys = o.hash_slots_log2 ? 0 : 1;
     ba.ws = c->ws;
     ba.nw = c->nw;
     ba.hamming_k = o.hamming_k;
