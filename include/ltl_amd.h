@@ -124,7 +124,9 @@ typedef struct {
     int32_t  deferred_succ;  /* bit j: the last level's unary block j (~, X,
                               * F, G over the deferred level) was tested in
                               * that level's generation */
-    int32_t  pad_;
+    int32_t  retries;     /* repeats of the call with fresh fingerprint keys
+                             after a detected fingerprint collision (the
+                             result does not depend on the keys) */
 } ltl_result;
 
 typedef struct ltl_ctx ltl_ctx;

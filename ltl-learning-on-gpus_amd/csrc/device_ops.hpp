@@ -90,9 +90,9 @@ __device__ __forceinline__ uint64_t mix64(uint64_t x) {
     x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
     return x ^ (x >> 31);
 }
-__device__ __forceinline__ void key_finish(Acc &a, const KeyAcc &k) {
-    a.f1 = mix64(k.h ^ mix64(k.l + 0x9e3779b97f4a7c15ull));
-    a.f2 = mix64(k.l ^ mix64(k.h + 0xd1b54a32d192ed03ull));
+__device__ __forceinline__ void key_finish(Acc &a, const KeyAcc &k, uint64_t seed = 0) {
+    a.f1 = mix64(k.h ^ mix64(k.l + (0x9e3779b97f4a7c15ull ^ seed)));
+    a.f2 = mix64(k.l ^ mix64(k.h + (0xd1b54a32d192ed03ull ^ seed)));
 }
 
 // Flush the summary words of a set when trace i (of n) closes summary word

@@ -237,8 +237,8 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     for (int r = 0; r < R; ++r) {
         if (r >= nrow) break;
         if constexpr (RX) {
-            key_finish(c0[r], k0[r]);
-            if constexpr (OP == LTL_UNTIL) key_finish(c1[r], k1[r]);
+            key_finish(c0[r], k0[r], A.kseed);
+            if constexpr (OP == LTL_UNTIL) key_finish(c1[r], k1[r], A.kseed);
         }
         Tile tr = t;
         tr.a = t.a + r;
@@ -277,7 +277,7 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
     }
     for (; w < A.nw; ++w) step(w, p[(size_t)w * kTileWS]);
     if (!t.valid) return;
-    if constexpr (RX) key_finish(c0, k0);
+    if constexpr (RX) key_finish(c0, k0, A.kseed);
     gen1_tail<CHK, SUCC, RX>(A, t, c0, s0, otf);
 }
 
@@ -464,7 +464,7 @@ __global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms) {
         k0.h ^= __shfl_xor(k0.h, o);  // contributions of different traces do not overlap (modes 1, 2)
         k0.l ^= __shfl_xor(k0.l, o);
     }
-    if (A.kmode) key_finish(c0, k0);
+    if (A.kmode) key_finish(c0, k0, A.kseed);
     if (l != 0) return;
     const uint64_t key = table_key(c0.f1);
     tab_insert(A.tab, A.tmask, key, (uint64_t)a, &A.st->probe_fail);

@@ -135,6 +135,7 @@ struct ChunkArgs {
     const uint32_t *kinfo;
     const uint64_t *ptrace;   // packed layout: per packed trace, trace | bit offset << 32 | length << 40
     int32_t kmode, kstride, kw_end, W;
+    uint64_t kseed;           // mixed into the relaxed keys' fingerprints (0 = default)
     // Lane shift: unary chunks put candidate a at temp index a - a0 + ush
     // with ush = (ls + a0) % 64; binary chunks put column b of a row at lane
     // offset b - b0 + ush within the row's stripes, ush = (rs + b0) % 64.

@@ -503,9 +503,12 @@ int slot_width(const ltl_problem *pb) { return words_per_trace(pb) > 1 ? 0 : 1; 
 // first at [64 - L1, 64)), each with its positions in descending bit order.
 // The packing order (word by word, top down) numbers the summary bits: group
 // g = packed traces 64g .. 64g + 63, which may start or end inside a word.
-void prepare_packed(const ltl_problem *pb, Prepared &q) {
+// Fingerprint keys per word from `seed` (kDefaultSeed unless a call is
+// repeated after a fingerprint collision).
+constexpr uint64_t kDefaultSeed = 0x5eed1e55c0ffee11ull;
+
+void prepare_packed(const ltl_problem *pb, Prepared &q, uint64_t seed = kDefaultSeed) {
     const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms;
-    const uint64_t seed = 0x5eed1e55c0ffee11ull;
     q.B = 1;
     q.nsw = (W + 63) / 64;
     q.aux.clear();
@@ -606,13 +609,12 @@ void prepare_packed(const ltl_problem *pb, Prepared &q) {
 
 // K words per trace (B = 0): word k of trace i is word i * K + k; only word
 // 0 of a trace takes part in the target check.
-void prepare_multiword(const ltl_problem *pb, Prepared &q) {
+void prepare_multiword(const ltl_problem *pb, Prepared &q, uint64_t seed = kDefaultSeed) {
     const int P = pb->num_pos, W = pb->num_pos + pb->num_neg, A = pb->num_atoms, K = words_per_trace(pb);
     q.B = 0;
     q.nw = W * K;
     q.nsw = (W + 63) / 64;
     q.aux.assign(q.nw, WordAux{});
-    const uint64_t seed = 0x5eed1e55c0ffee11ull;
     for (int w = 0; w < q.nw; ++w) {
         WordAux &x = q.aux[w];
         memset(&x, 0, sizeof(x));
@@ -654,9 +656,9 @@ void prepare_multiword(const ltl_problem *pb, Prepared &q) {
 }
 
 // Layout code B: 1 = packed words, 0 = K words per trace.
-void prepare_instance(const ltl_problem *pb, int B, Prepared &q) {
-    if (B == 0) prepare_multiword(pb, q);
-    else prepare_packed(pb, q);
+void prepare_instance(const ltl_problem *pb, int B, Prepared &q, uint64_t seed = kDefaultSeed) {
+    if (B == 0) prepare_multiword(pb, q, seed);
+    else prepare_packed(pb, q, seed);
 }
 
 // Validate an instance; on an accepted atom fill *res (the reference checks
@@ -1195,8 +1197,11 @@ std::string node_string(const std::vector<ltl_node> &n, int i, const std::string
 
 // One learning call.  noalloc: the device memory of the call is this
 // device's static scratch region (no allocation of any kind on the device,
-// no pinned staging, no events); plain order only.
-static int32_t learn_call(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bool noalloc) {
+// no pinned staging, no events); plain order only.  seed: the fingerprint
+// keys; weak_fp1: (tests only) fingerprint 1 keyed with zeros, which makes
+// its collisions between different sets common.
+static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bool noalloc, uint64_t seed,
+                          bool weak_fp1) {
     if (!res) return LTL_E_ARG;
     memset(res, 0, sizeof(*res));
     res->found_id = -1;
@@ -1239,7 +1244,9 @@ static int32_t learn_call(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     const int K = words_per_trace(pb);
     const int B = K > 1 ? 0 : 1;
     Prepared prep;
-    prepare_instance(pb, B, prep);
+    prepare_instance(pb, B, prep, seed);
+    if (weak_fp1)
+        for (auto &x : prep.aux) x.k1 = 0;
     c->B = B; c->K = K; c->W = W; c->P = P; c->A = A;
     c->nw = prep.nw;
     c->nsw = prep.nsw;
@@ -1365,6 +1372,7 @@ static int32_t learn_call(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     ba.W = W;
     ba.kinfo = c->d_kinfo;
     ba.kmode = 0;
+    ba.kseed = seed == kDefaultSeed ? 0 : seed;
     {
         int len_sum = 0;
         for (int i = 0; i < W; ++i) len_sum += pb->lens[i];
@@ -1990,6 +1998,33 @@ static int32_t learn_call(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     return res->status;
 }
 
+// A fingerprint-1 collision between different sets is detected in dedup
+// (their fingerprints 2 differ); the call is then repeated with fresh
+// fingerprint keys (a new seed, so an input built against the default keys
+// does not collide again), up to kMaxAttempts in all.  The result does not
+// depend on the keys: dedup is by set equality.  res->retries counts the
+// repeats.  With LTL_TEST_WEAK_FP1 in the environment the first attempt
+// keys fingerprint 1 with zeros (collisions become common): tests of the
+// repeat path.
+static constexpr int kMaxAttempts = 4;
+
+static int32_t learn_call(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bool noalloc) {
+    static const bool weak = getenv("LTL_TEST_WEAK_FP1") != nullptr;
+    uint64_t seed = kDefaultSeed;
+    int32_t rc = 0;
+    for (int attempt = 0; attempt < kMaxAttempts; ++attempt) {
+        if (attempt) {
+            ltl_result_free(res);
+            seed = splitmix(seed ^ ((uint64_t)std::chrono::steady_clock::now().time_since_epoch().count() +
+                                    (uint64_t)attempt));
+        }
+        rc = learn_once(c, pb, res, noalloc, seed, weak && attempt == 0);
+        res->retries = attempt;
+        if (rc != LTL_E_COLLISION) break;
+    }
+    return rc;
+}
+
 extern "C" int32_t ltl_learn(ltl_ctx *c, const ltl_problem *pb, ltl_result *res) {
     return learn_call(c, pb, res, false);
 }
@@ -2222,9 +2257,7 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
             if (bo.status == BATCH_FOUND || bo.status == BATCH_NOT_FOUND) {
                 batch_fill_result(bo, names[i], &res[i]);
                 bs.on_device++;
-            } else if (bo.status == BATCH_ERROR && bo.error == 1) {
-                res[i].status = LTL_E_COLLISION;
-            } else {
+            } else {  // (a fingerprint collision too: ltl_learn repeats with fresh keys)
                 deferred.push_back(i);
             }
         }

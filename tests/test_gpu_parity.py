@@ -255,3 +255,41 @@ def test_relaxed_modes_overflow(ltl, oracle, learner, mode, capacity, planted):
     # cost 5 where the run without overflow answers at cost 6).
     inst = _twins_mixed(ltl, 10 + capacity, planted)
     check_parity(ltl, oracle, learner, inst, max_cost=7, dedup_mode=mode, capacity=capacity)
+
+
+@pytest.mark.parametrize("seed", [1, 4])
+def test_fingerprint_collisions_are_survived(ltl, oracle, seed):
+    # With fingerprint 1 keyed with zeros (LTL_TEST_WEAK_FP1, read when the
+    # library is first used in a process: run in a child process), different
+    # sets share fingerprint 1 all the time; dedup detects that through
+    # fingerprint 2 and the call is repeated with fresh keys: the oracle's
+    # result, set for set.
+    import json
+    import subprocess
+    import sys
+    code = f"""
+import json, sys
+sys.path.insert(0, {os.path.join(os.path.dirname(__file__))!r})
+import conftest
+ltl = conftest.load_package()
+inst = ltl.generate_custom({seed}, 30, 30, 3, 5, 16, 0.5, planted_size=13)
+L = ltl.Learner(budget=1 << 30)
+r = L.learn(inst, max_cost=6)
+s, e = L.levels()
+n = int(e[6]) if r["status"] == 0 else int(s[r["cost"]])
+w, op, left, right = L.store(0, n, inst.W)
+print(json.dumps(dict(r=dict((k, r[k]) for k in ("status", "cost", "all_ltls", "found_id", "formula", "retries")),
+                      n=n, words=w.tolist(), left=left.tolist())))
+"""
+    env = dict(os.environ, LTL_TEST_WEAK_FP1="1")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=300)
+    assert out.returncode == 0, out.stderr
+    got = json.loads(out.stdout.strip().splitlines()[-1])
+    inst = ltl.generate_custom(seed, 30, 30, 3, 5, 16, 0.5, planted_size=13)
+    ref = oracle.enumerate_(to_oracle(oracle, inst), max_cost=6, keep_store=True)
+    assert got["r"]["retries"] >= 1
+    for k in ("status", "cost", "all_ltls", "found_id", "formula"):
+        assert got["r"][k] == ref[k], k
+    n = got["n"]
+    assert np.array_equal(np.array(got["words"], dtype=np.uint64), ref["words"][:n])
+    assert got["left"] == ref["left"][:n].tolist()
