@@ -67,7 +67,12 @@ typedef struct {
     int32_t words_per_trace;
     const uint8_t  *lens;   /* [num_pos+num_neg], each 1..64*K */
     const uint64_t *atoms;  /* [num_atoms * (num_pos+num_neg) * K], atom-major */
-    const char     *names;  /* num_atoms single-letter names, or NULL */
+    const char     *names;  /* num_atoms single-character names; or, for more
+                               than 26 propositions, a comma-separated list of
+                               num_atoms names, each a lowercase letter or _
+                               followed by lowercase letters, digits or _
+                               ("p0,p1,...,p29"); or NULL (a, b, c, ...).  The
+                               returned formula spells atoms this way. */
 } ltl_problem;
 
 /* Options.  cost/max_cost/neg_type/dedup_mode carry the reference's
@@ -336,6 +341,19 @@ int32_t ltl_read_trace_file(const char *path, int32_t max_traces,
 int32_t ltl_read_trace_file_words(const char *path, int32_t max_traces, int32_t words_per_trace,
                                   int32_t *num_pos, int32_t *num_neg, int32_t *num_atoms,
                                   int32_t *max_len, uint8_t *lens, uint64_t *atoms, char *names);
+
+/* More than 26 propositions (an extension of the reference format, whose
+ * alphabet is the letters a-z, code/ltli6463.cu:1521): when the alphabet line
+ * is anything but single letters a-z, its comma-separated tokens (at most 64,
+ * each a lowercase letter or _ followed by lowercase letters, digits or _,
+ * distinct) name the columns in file order, and names[names_cap] receives
+ * them as a comma-separated list for ltl_problem.names.  Files with a
+ * letters-only alphabet line read as with ltl_read_trace_file_words.  atoms
+ * holds [64 * max_traces * K].  Returns 0 or LTL_E_ARG / LTL_E_LIMIT. */
+int32_t ltl_read_trace_file_names(const char *path, int32_t max_traces, int32_t words_per_trace,
+                                  int32_t *num_pos, int32_t *num_neg, int32_t *num_atoms,
+                                  int32_t *max_len, uint8_t *lens, uint64_t *atoms, char *names,
+                                  int32_t names_cap);
 
 /* Evaluate a formula in the reference output grammar on every trace
  * (bit-parallel; the product counterpart of code/dc.py:136-186).  Fills

@@ -106,7 +106,7 @@ EXPORTS = ["ltl_default_options", "ltl_create", "ltl_set_options", "ltl_learn", 
            "ltl_gen_config", "ltl_gen_custom", "ltl_gen_free",
            "ltl_scorer_create", "ltl_scorer_destroy", "ltl_scorer_get_params", "ltl_scorer_set_params",
            "ltl_scorer_forward", "ltl_scorer_grad", "ltl_scorer_train", "ltl_set_scorer", "ltl_set_log",
-           "ltl_get_log", "ltl_learn_dc", "ltl_get_options", "ltl_learn_noalloc"]
+           "ltl_get_log", "ltl_learn_dc", "ltl_get_options", "ltl_learn_noalloc", "ltl_read_trace_file_names"]
 
 DC_DET, DC_RAND = 0, 1
 
@@ -148,6 +148,9 @@ def lib():
     L.ltl_read_trace_file_words.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, P(ctypes.c_int32),
                                             P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32),
                                             P(ctypes.c_uint8), P(ctypes.c_uint64), ctypes.c_char_p]
+    L.ltl_read_trace_file_names.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, P(ctypes.c_int32),
+                                            P(ctypes.c_int32), P(ctypes.c_int32), P(ctypes.c_int32),
+                                            P(ctypes.c_uint8), P(ctypes.c_uint64), ctypes.c_char_p, ctypes.c_int32]
     L.ltl_eval_formula.argtypes = [ctypes.c_char_p, P(Problem), P(ctypes.c_uint64)]
     L.ltl_mismatches.argtypes = [P(ctypes.c_uint64), ctypes.c_int32, ctypes.c_int32]
     L.ltl_version.restype = ctypes.c_char_p
@@ -191,7 +194,11 @@ class Instance:
         self.num_pos, self.num_neg, self.K = int(num_pos), int(num_neg), int(K)
         self.lens = np.ascontiguousarray(lens, dtype=np.uint8)
         self.atoms = np.ascontiguousarray(atoms, dtype=np.uint64).reshape(-1, self.W * self.K)
-        self.names = names or "".join(chr(ord("a") + i) for i in range(self.atoms.shape[0]))
+        A = self.atoms.shape[0]
+        # (the library's names: letters up to 26 atoms, else p0, p1, ... as a
+        # comma-separated list)
+        self.names = names or ("".join(chr(ord("a") + i) for i in range(A)) if A <= 26
+                               else ",".join(f"p{i}" for i in range(A)))
         self.planted, self.max_cost, self.hamming_k, self.neg_type = planted, max_cost, hamming_k, neg_type
 
     @property
@@ -240,12 +247,15 @@ def read_trace_file(path, max_traces=4096):
     to 255 positions (the reference stops at 63, :1078): one word per trace
     when the longest fits 64 positions, else K = ceil(longest / 64)."""
     lens = np.zeros(max_traces, dtype=np.uint8)
-    atoms = np.zeros(26 * max_traces * 4, dtype=np.uint64)
-    names = ctypes.create_string_buffer(32)
+    atoms = np.zeros(64 * max_traces * 4, dtype=np.uint64)
+    names = ctypes.create_string_buffer(64 * 40)
     P, N, A, M = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-    rc = lib().ltl_read_trace_file_words(path.encode(), max_traces, 4, ctypes.byref(P), ctypes.byref(N),
+    # (an alphabet line of comma-separated names names more than 26
+    # propositions; the names come back as a comma-separated list)
+    rc = lib().ltl_read_trace_file_names(path.encode(), max_traces, 4, ctypes.byref(P), ctypes.byref(N),
                                          ctypes.byref(A), ctypes.byref(M),
-                                         lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _u64p(atoms), names)
+                                         lens.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), _u64p(atoms), names,
+                                         len(names))
     if rc != 0:
         raise LtlError(f"{path}: error {rc}")
     W, K = P.value + N.value, (max(M.value, 1) + 63) // 64

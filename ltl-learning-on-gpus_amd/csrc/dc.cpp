@@ -37,7 +37,8 @@ struct Driver {
     int algo;
     uint64_t seed;
     uint64_t rng = 0;
-    std::string names;
+    AtomNames names;   // parsed
+    const char *raw = nullptr;  // as the caller gave them (passed to the window calls)
     int W, A, K;  // traces, atoms, words per trace
     int32_t err = 0;
     uint64_t all_ltls = 0;
@@ -100,7 +101,7 @@ struct Driver {
                 for (int j = 0; j < K; ++j)
                     atoms[((size_t)a * w + k) * K + j] = pb->atoms[((size_t)a * W + t) * K + j];
         }
-        ltl_problem sub{P, N, A, K, lens.data(), atoms.data(), names.c_str()};
+        ltl_problem sub{P, N, A, K, lens.data(), atoms.data(), raw};
         ltl_result r;
         const int32_t st = ltl_learn(ctx, &sub, &r);
         ++calls;
@@ -258,8 +259,8 @@ extern "C" int32_t ltl_learn_dc(ltl_ctx *ctx, const ltl_problem *pb, int32_t win
     d.W = pb->num_pos + pb->num_neg;
     d.A = pb->num_atoms;
     d.K = pb->words_per_trace > 1 ? pb->words_per_trace : 1;
-    d.names = pb->names ? std::string(pb->names, pb->num_atoms)
-                        : std::string("abcdefghijklmnopqrstuvwxyz", pb->num_atoms);
+    if (!atom_names(pb->names, pb->num_atoms, d.names)) return res->status = LTL_E_ARG;
+    d.raw = pb->names;
     Idx pos(pb->num_pos), neg(pb->num_neg);
     for (int i = 0; i < pb->num_pos; ++i) pos[i] = i;
     for (int i = 0; i < pb->num_neg; ++i) neg[i] = pb->num_pos + i;

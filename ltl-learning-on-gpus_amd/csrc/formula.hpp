@@ -85,9 +85,18 @@ struct Node {
     std::unique_ptr<Node> a, b;
 };
 
-// Parse; `names` maps letters to atom indices.  Returns nullptr on error.
-std::unique_ptr<Node> parse_formula(const std::string &s, const std::string &names);
-std::string print_formula(const Node &n, const std::string &names);
+// Atom names: ltl_problem.names is NULL (a, b, c, ... for up to 26 atoms,
+// else p0, p1, ...), num_atoms single characters, or a comma-separated list
+// of num_atoms names, each a lowercase letter or _ followed by lowercase
+// letters, digits or _ (more than 26 propositions).  Returns false if the
+// list is malformed.
+using AtomNames = std::vector<std::string>;
+bool atom_names(const char *names, int num_atoms, AtomNames &out);
+// Parse in the reference grammar; an atom is one character when every name
+// is one character, else a maximal run of lowercase letters, digits and _.
+// Returns nullptr on error.
+std::unique_ptr<Node> parse_formula(const std::string &s, const AtomNames &names);
+std::string print_formula(const Node &n, const AtomNames &names);
 int formula_size(const Node &n, const uint16_t *cost);
 // cs[t] for every trace t; atoms is atom-major [A][W].
 void eval_formula(const Node &n, int W, const uint8_t *lens, const uint64_t *atoms, uint64_t *cs);

@@ -16,10 +16,44 @@
 
 namespace ltl {
 
+static bool is_name_char(char c) { return (c >= 'a' && c <= 'z') || (c >= '0' && c <= '9') || c == '_'; }
+
+bool atom_names(const char *names, int A, AtomNames &out) {
+    out.clear();
+    if (A < 0) return false;
+    if (!names) {
+        for (int a = 0; a < A; ++a)
+            out.push_back(A <= 26 ? std::string(1, (char)('a' + a)) : "p" + std::to_string(a));
+        return true;
+    }
+    if (!strchr(names, ',')) {
+        if ((int)strlen(names) < A) return false;
+        for (int a = 0; a < A; ++a) out.push_back(std::string(1, names[a]));
+        return true;
+    }
+    std::string cur;
+    for (const char *p = names;; ++p) {
+        if (*p == ',' || *p == 0) {
+            if (cur.empty() || !((cur[0] >= 'a' && cur[0] <= 'z') || cur[0] == '_')) return false;
+            for (char c : cur)
+                if (!is_name_char(c)) return false;
+            for (const auto &o : out)
+                if (o == cur) return false;
+            out.push_back(cur);
+            cur.clear();
+            if (*p == 0) break;
+        } else {
+            cur += *p;
+        }
+    }
+    return (int)out.size() == A;
+}
+
 namespace {
 struct Parser {
     const std::string &s;
-    const std::string &names;
+    const AtomNames &names;
+    bool single;  // every name is one character
     size_t p = 0;
     bool eat(char c) {
         if (p < s.size() && s[p] == c) { ++p; return true; }
@@ -51,9 +85,16 @@ struct Parser {
             n->op = op; n->a = std::move(l); n->b = std::move(r);
             return n;
         }
-        size_t k = names.find(c);
-        if (k == std::string::npos) return nullptr;
-        ++p;
+        std::string tok;
+        if (single) {
+            tok = std::string(1, c);
+        } else {
+            for (size_t q = p; q < s.size() && is_name_char(s[q]); ++q) tok += s[q];
+        }
+        size_t k = 0;
+        while (k < names.size() && names[k] != tok) ++k;
+        if (tok.empty() || k == names.size()) return nullptr;
+        p += tok.size();
         auto n = std::make_unique<Node>();
         n->op = LTL_ATOM; n->atom = (int)k;
         return n;
@@ -61,16 +102,18 @@ struct Parser {
 };
 }  // namespace
 
-std::unique_ptr<Node> parse_formula(const std::string &s, const std::string &names) {
-    Parser ps{s, names};
+std::unique_ptr<Node> parse_formula(const std::string &s, const AtomNames &names) {
+    bool single = true;
+    for (const auto &n : names) single = single && n.size() == 1;
+    Parser ps{s, names, single};
     auto n = ps.f();
     if (!n || ps.p != s.size()) return nullptr;
     return n;
 }
 
-std::string print_formula(const Node &n, const std::string &names) {
+std::string print_formula(const Node &n, const AtomNames &names) {
     switch (n.op) {
-    case LTL_ATOM: return std::string(1, names[n.atom]);
+    case LTL_ATOM: return names[n.atom];
     case LTL_NOT: return "~(" + print_formula(*n.a, names) + ")";
     case LTL_NEXT: return "X(" + print_formula(*n.a, names) + ")";
     case LTL_FIN: return "F(" + print_formula(*n.a, names) + ")";
@@ -226,7 +269,12 @@ void emit(ltl_gen_instance *out, const Draw &d, const std::vector<uint8_t> &lens
     snprintf(out->planted, sizeof(out->planted), "%s", planted.c_str());
 }
 
-const char *kLetters = "abcdefghijklmnopqrstuvwxyz";
+// Letter names a, b, c, ... of n <= 26 atoms.
+AtomNames letters(int n) {
+    AtomNames v;
+    for (int a = 0; a < n; ++a) v.push_back(std::string(1, (char)('a' + a)));
+    return v;
+}
 
 }  // namespace
 }  // namespace ltl
@@ -242,7 +290,7 @@ extern "C" int32_t ltl_gen_custom(uint64_t seed, int32_t num_pos, int32_t num_ne
         return LTL_E_ARG;
     Rng g(seed);
     Draw d{num_pos, num_neg, num_atoms, len_min, len_max, std::vector<double>(num_atoms, bit_rate)};
-    std::string names(kLetters, num_atoms);
+    const AtomNames names = letters(num_atoms);
     std::vector<uint8_t> lens;
     std::vector<uint64_t> atoms;
     for (int attempt = 0; attempt < 1000; ++attempt) {
@@ -300,7 +348,7 @@ extern "C" int32_t ltl_gen_config(int32_t config, uint64_t seed, int32_t index, 
     default:
         return LTL_E_ARG;
     }
-    std::string names(kLetters, d.A);
+    const AtomNames names = letters(d.A);
     std::vector<uint8_t> lens;
     std::vector<uint64_t> atoms;
     const long budget = 200L * (d.P + d.N) + 20000;
@@ -361,11 +409,20 @@ extern "C" void ltl_gen_free(ltl_gen_instance *inst) {
 // last line (letters a-z, columns in file order); the atoms are the sorted
 // letters; a '1' in column j puts letter j into the position's token.
 // ---------------------------------------------------------------------------
-extern "C" int32_t ltl_read_trace_file_words(const char *path, int32_t max_traces, int32_t words_per_trace,
-                                             int32_t *num_pos, int32_t *num_neg, int32_t *num_atoms,
-                                             int32_t *max_len, uint8_t *lens, uint64_t *atoms, char *names) {
-    const int K = words_per_trace;
-    if (!path || max_traces < 0 || K < 1 || K > 4) return LTL_E_ARG;
+// The reference's trace format (code/ltli6463.cu:1500-1573): positive
+// lines, "---", negative lines, "---", operators, "---", alphabet (the last
+// line).  Positions are separated by ';', the bits of one position by ','.
+// ext = false: the reference's alphabet rule, its letters a-z (as a set, so
+// sorted) name the atoms, column j of a position is the j-th letter of the
+// line in file order (code/ltli6463.cu:1507-1524); names gets the letters.
+// ext = true (more than 26 propositions): if the alphabet line is anything
+// but single letters a-z, its comma-separated tokens name the columns in
+// file order (at most 64, each a valid atom name, distinct) and names gets
+// them as a comma-separated list (the form ltl_problem.names accepts).
+static int32_t read_traces(const char *path, int32_t max_traces, int32_t K, bool ext, int32_t *num_pos,
+                           int32_t *num_neg, int32_t *num_atoms, int32_t *max_len, uint8_t *lens, uint64_t *atoms,
+                           char *names, size_t names_cap) {
+    if (!path || max_traces < 0 || K < 1 || K > 4 || !names || names_cap < 1) return LTL_E_ARG;
     const int cap = std::min(64 * K, 255);
     std::ifstream in(path, std::ios::binary);
     if (!in) return LTL_E_ARG;
@@ -388,29 +445,63 @@ extern "C" int32_t ltl_read_trace_file_words(const char *path, int32_t max_trace
         while (!l.empty() && (l.back() == ' ' || l.back() == '\t')) l.pop_back();
     while (!lines.empty() && lines.back().empty()) lines.pop_back();
     if (lines.empty()) return LTL_E_ARG;
-    std::string alpha;
-    bool inset[26] = {false};
-    for (char c : lines.back())
-        if (c >= 'a' && c <= 'z') { alpha += c; inset[c - 'a'] = true; }
-    int A = 0, col_of[26];
-    for (int c = 0; c < 26; ++c)
-        if (inset[c]) { names[A] = (char)('a' + c); col_of[c] = A++; }
-    names[A] = 0;
+    // column j -> atom col_of[j]
+    std::vector<int> col_of;
+    std::vector<std::string> tokens;
+    {
+        std::string cur;
+        for (char c : lines.back() + ",") {
+            if (c == ',') {
+                while (!cur.empty() && cur.front() == ' ') cur.erase(cur.begin());
+                while (!cur.empty() && cur.back() == ' ') cur.pop_back();
+                tokens.push_back(cur);
+                cur.clear();
+            } else {
+                cur += c;
+            }
+        }
+    }
+    bool letters_only = true;
+    for (const auto &t : tokens) letters_only = letters_only && t.size() == 1 && t[0] >= 'a' && t[0] <= 'z';
+    int A = 0;
+    std::string out;
+    if (!ext || letters_only) {
+        std::string alpha;
+        bool inset[26] = {false};
+        for (char c : lines.back())
+            if (c >= 'a' && c <= 'z') { alpha += c; inset[c - 'a'] = true; }
+        int idx[26];
+        for (int c = 0; c < 26; ++c)
+            if (inset[c]) { out += (char)('a' + c); idx[c] = A++; }
+        for (char c : alpha) col_of.push_back(idx[c - 'a']);
+    } else {
+        for (size_t j = 0; j < tokens.size(); ++j) {
+            if (j) out += ',';
+            out += tokens[j];
+            col_of.push_back((int)j);
+        }
+        A = (int)tokens.size();
+        AtomNames check;
+        if (A > 64 || !atom_names(out.c_str(), A, check)) return LTL_E_LIMIT;
+    }
+    if (out.size() + 1 > names_cap) return LTL_E_LIMIT;
+    memcpy(names, out.c_str(), out.size() + 1);
     *num_atoms = A;
     int W = 0, P = -1, section = 0, longest = 0;
     const size_t row = (size_t)max_traces * K;  // atom stride while reading
+    std::vector<uint64_t> bits((size_t)std::max(A, 1) * 4);
     for (const auto &line : lines) {
         if (section >= 2) break;
         if (line == "---") { if (section == 0) P = W; ++section; continue; }
         if (W >= max_traces) return LTL_E_LIMIT;
-        uint64_t bits[26][4] = {{0}};
+        std::fill(bits.begin(), bits.end(), 0ull);
         int pos = 0, j = 0;
         for (char c : line) {
             if (c == ';') { ++pos; j = 0; }
             else if (c == ',') continue;
             else {
-                if (c == '1' && j < (int)alpha.size() && pos < cap)
-                    bits[col_of[alpha[j] - 'a']][pos >> 6] |= 1ull << (pos & 63);
+                if (c == '1' && j < (int)col_of.size() && pos < cap)
+                    bits[(size_t)col_of[j] * 4 + (pos >> 6)] |= 1ull << (pos & 63);
                 ++j;
             }
         }
@@ -418,7 +509,7 @@ extern "C" int32_t ltl_read_trace_file_words(const char *path, int32_t max_trace
         lens[W] = (uint8_t)(pos + 1);
         longest = std::max(longest, pos + 1);
         for (int a = 0; a < A; ++a)
-            for (int k = 0; k < K; ++k) atoms[(size_t)a * row + (size_t)W * K + k] = bits[a][k];
+            for (int k = 0; k < K; ++k) atoms[(size_t)a * row + (size_t)W * K + k] = bits[(size_t)a * 4 + k];
         ++W;
     }
     if (P < 0) P = W;
@@ -427,6 +518,21 @@ extern "C" int32_t ltl_read_trace_file_words(const char *path, int32_t max_trace
     for (int a = 1; a < A; ++a)
         memmove(atoms + (size_t)a * W * K, atoms + (size_t)a * row, (size_t)W * K * 8);
     return 0;
+}
+
+extern "C" int32_t ltl_read_trace_file_words(const char *path, int32_t max_traces, int32_t words_per_trace,
+                                             int32_t *num_pos, int32_t *num_neg, int32_t *num_atoms,
+                                             int32_t *max_len, uint8_t *lens, uint64_t *atoms, char *names) {
+    return read_traces(path, max_traces, words_per_trace, false, num_pos, num_neg, num_atoms, max_len, lens, atoms,
+                       names, 27);
+}
+
+extern "C" int32_t ltl_read_trace_file_names(const char *path, int32_t max_traces, int32_t words_per_trace,
+                                             int32_t *num_pos, int32_t *num_neg, int32_t *num_atoms,
+                                             int32_t *max_len, uint8_t *lens, uint64_t *atoms, char *names,
+                                             int32_t names_cap) {
+    return read_traces(path, max_traces, words_per_trace, true, num_pos, num_neg, num_atoms, max_len, lens, atoms,
+                       names, names_cap > 0 ? (size_t)names_cap : 0);
 }
 
 extern "C" int32_t ltl_read_trace_file(const char *path, int32_t max_traces, int32_t *num_pos,
@@ -438,7 +544,8 @@ extern "C" int32_t ltl_read_trace_file(const char *path, int32_t max_traces, int
 
 extern "C" int32_t ltl_eval_formula(const char *formula, const ltl_problem *pb, uint64_t *out_cs) {
     if (!formula || !pb) return LTL_E_ARG;
-    std::string names = pb->names ? std::string(pb->names, pb->num_atoms) : std::string(kLetters, pb->num_atoms);
+    AtomNames names;
+    if (!atom_names(pb->names, pb->num_atoms, names)) return LTL_E_ARG;
     auto f = parse_formula(formula, names);
     if (!f) return LTL_E_ARG;
     eval_formula(*f, pb->num_pos + pb->num_neg, pb->words_per_trace, pb->lens, pb->atoms, out_cs);

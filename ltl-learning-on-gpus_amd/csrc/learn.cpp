@@ -664,7 +664,7 @@ void prepare_instance(const ltl_problem *pb, int B, Prepared &q, uint64_t seed =
 // Validate an instance; on an accepted atom fill *res (the reference checks
 // atoms on the host before any device work, code/ltli6463.cu:1100-1136).
 // Returns 1 if res is final, 0 to continue, or a negative status.
-int32_t precheck_host(const ltl_options &o, const ltl_problem *pb, ltl_result *res, std::string &names) {
+int32_t precheck_host(const ltl_options &o, const ltl_problem *pb, ltl_result *res, AtomNames &names) {
     const int P = pb->num_pos, N = pb->num_neg, W = P + N, A = pb->num_atoms, K = words_per_trace(pb);
     if (P < 0 || N < 0 || W < 1 || A < 1 || A > 64 || !pb->lens || !pb->atoms || K > 4) return LTL_E_ARG;
     for (int i = 0; i < W; ++i)
@@ -679,8 +679,7 @@ int32_t precheck_host(const ltl_options &o, const ltl_problem *pb, ltl_result *r
         for (int i = 0; i < W; ++i) len_sum += pb->lens[i];
         if (len_sum > 126) return LTL_E_ARG;
     }
-    names = pb->names ? std::string(pb->names, A) : std::string();
-    if (names.empty()) for (int a = 0; a < A; ++a) names += (char)('a' + a);
+    if (!atom_names(pb->names, A, names)) return LTL_E_ARG;
     res->cost = o.cost[0];
     std::vector<uint64_t> cs(W);
     for (int a = 0; a < A; ++a) {
@@ -692,7 +691,7 @@ int32_t precheck_host(const ltl_options &o, const ltl_problem *pb, ltl_result *r
             res->n_nodes = 1;
             res->nodes = (ltl_node *)malloc(sizeof(ltl_node));
             res->nodes[0] = {LTL_ATOM, a, -1, -1};
-            res->formula = strdup(std::string(1, names[a]).c_str());
+            res->formula = strdup(names[a].c_str());
             return 1;
         }
     }
@@ -1178,10 +1177,10 @@ int32_t build_tree(ltl_ctx *c, int op, int32_t l, int32_t r, int atom, std::vect
     return 0;
 }
 
-std::string node_string(const std::vector<ltl_node> &n, int i, const std::string &names) {
+std::string node_string(const std::vector<ltl_node> &n, int i, const AtomNames &names) {
     const ltl_node &x = n[i];
     switch (x.op) {
-    case LTL_ATOM: return std::string(1, names[x.atom]);
+    case LTL_ATOM: return names[x.atom];
     case LTL_NOT: return "~(" + node_string(n, x.left, names) + ")";
     case LTL_NEXT: return "X(" + node_string(n, x.left, names) + ")";
     case LTL_FIN: return "F(" + node_string(n, x.left, names) + ")";
@@ -1216,7 +1215,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     // are checked with hipGetLastError) is cleared here, reported with
     // LTL_DEBUG.
     if (hipError_t stale = hipGetLastError(); stale != hipSuccess) report_hip(stale, "an earlier call", __LINE__);
-    std::string names;
+    AtomNames names;
     for (auto &s : c->stats) { s.launches = 0; s.seconds = 0; s.alg_bytes = s.min_bytes = s.items = 0; }
     {
         const int32_t pre = precheck_host(o, pb, res, names);
@@ -2040,7 +2039,7 @@ extern "C" int32_t ltl_learn_noalloc(ltl_ctx *c, const ltl_problem *pb, ltl_resu
 // ---------------------------------------------------------------------------
 // Batched learning: one persistent launch over many instances.
 // ---------------------------------------------------------------------------
-static int32_t batch_fill_result(const BatchOut &bo, const std::string &names, ltl_result *r) {
+static int32_t batch_fill_result(const BatchOut &bo, const AtomNames &names, ltl_result *r) {
     r->cost = bo.cost;
     r->all_ltls = bo.all_ltls;
     r->found_id = bo.found_id;
@@ -2127,7 +2126,7 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
     const ltl_options &o = c->opt;
     ltl_batch_stats bs{};
     bs.n = n;
-    std::vector<std::string> names(n);
+    std::vector<AtomNames> names(n);
     std::vector<int> dev;
     for (int i = 0; i < n; ++i) {
         memset(&res[i], 0, sizeof(res[i]));

@@ -4,13 +4,18 @@
 // 1575-1681), so that drivers which parse its output (code/dc.py:23-43, 192-194)
 // work unchanged:
 //
-//   ltli <input_file> <c1> .. <c8> <maxCost> <RlxUnqChkTyp> <NegType>
+//   ltli [--reference-keys] <input_file> <c1> .. <c8> <maxCost> <RlxUnqChkTyp> <NegType>
 //
 // stdout: the traces, "Cost Function: ...", "Number of Traces: N", the
 // run statistics, then a last line  LTL: "<formula>"  (or "not_found").
 // Differences: errors exit with a nonzero status (the reference always exits
-// 0), traces may number more than 64, and deduplication is exact for any
-// total trace length (RlxUnqChkTyp is validated and otherwise ignored).
+// 0), traces may number more than 64 and hold up to 255 positions, an
+// alphabet line of comma-separated names instead of letters names up to 64
+// propositions, and deduplication is exact for any total trace length
+// unless --reference-keys is given (or LTLI_REFERENCE_KEYS=1 is set in the
+// environment): then RlxUnqChkTyp selects the reference's relaxed uniqueness
+// key (code/ltli6463.cu:57-141) above 126 trace positions, as the reference
+// always does.
 #include <algorithm>
 #include <chrono>
 #include <climits>
@@ -25,7 +30,8 @@
 static int usage(const char *argv0) {
     printf("Arguments should be in the form of\n");
     printf("-----------------------------------------------------------------\n");
-    printf("%s <input_file_address> <c1> <c2> <c3> <c4> <c5> <c6> <c7> <c8> <maxCost> <RlxUnqChkTyp> <NegType>\n",
+    printf("%s [--reference-keys] <input_file_address> <c1> <c2> <c3> <c4> <c5> <c6> <c7> <c8> <maxCost> "
+           "<RlxUnqChkTyp> <NegType>\n",
            argv0);
     printf("-----------------------------------------------------------------\n");
     printf("\nFor example\n");
@@ -49,7 +55,14 @@ static std::string file_alphabet(const char *path) {
 }
 
 int main(int argc, char **argv) {
-    if (argc != 13) return usage(argv[0]);
+    bool ref_keys = getenv("LTLI_REFERENCE_KEYS") && std::atoi(getenv("LTLI_REFERENCE_KEYS")) != 0;
+    const char *prog = argv[0];
+    if (argc == 14 && std::string(argv[1]) == "--reference-keys") {
+        ref_keys = true;
+        ++argv;
+        --argc;
+    }
+    if (argc != 13) return usage(prog);
     bool bad = false;
     for (int i = 2; i < argc - 2; ++i) {
         const int v = std::atoi(argv[i]);
@@ -72,21 +85,25 @@ int main(int argc, char **argv) {
     // Traces of up to 255 positions (the reference stops at 63,
     // code/ltli6463.cu:1078): read in four words per trace, then repacked to
     // the fewest words that hold the longest trace (one word up to 64).
-    const int max_traces = 1 << 16;
+    int max_traces = 1;
+    {
+        std::ifstream f(argv[1], std::ios::binary);
+        for (std::string line; std::getline(f, line);) ++max_traces;
+    }
     std::vector<uint8_t> lens(max_traces);
-    std::vector<uint64_t> atoms((size_t)26 * max_traces * 4);
-    char names[32];
+    std::vector<uint64_t> atoms((size_t)64 * max_traces * 4);
+    std::vector<char> names(64 * 40);
     int32_t P = 0, N = 0, A = 0, longest = 0;
-    int32_t rc = ltl_read_trace_file_words(argv[1], max_traces, 4, &P, &N, &A, &longest, lens.data(), atoms.data(),
-                                           names);
+    int32_t rc = ltl_read_trace_file_names(argv[1], max_traces, 4, &P, &N, &A, &longest, lens.data(), atoms.data(),
+                                           names.data(), (int32_t)names.size());
     if (rc == LTL_E_ARG) {
         printf("Failed to open the input file.\n");
         return 1;
     }
     if (rc == LTL_E_LIMIT) {
         printf("In this version, The input can have:\n");
-        printf("1) At most %d traces.\n", max_traces);
-        printf("2) Max(len(trace)) = 255.\n");
+        printf("1) Max(len(trace)) = 255.\n");
+        printf("2) At most 64 propositions, named by letters or by comma-separated names.\n");
         return 1;
     }
     const int W = P + N;
@@ -99,29 +116,42 @@ int main(int argc, char **argv) {
     for (int i = 0; i < 8; ++i) opt.cost[i] = (uint16_t)std::atoi(argv[i + 2]);
     opt.max_cost = (uint16_t)std::atoi(argv[10]);
     opt.neg_type = (uint8_t)neg;
+    if (ref_keys) opt.dedup_mode = (uint8_t)rlx;
     int32_t st = 0;
     ltl_ctx *ctx = ltl_create(&opt, &st);
     if (!ctx) {
         printf("No usable GPU (error %d).\n", st);
         return 1;
     }
-    ltl_problem pb{P, N, A, K, lens.data(), atoms.data(), names};
+    ltl_problem pb{P, N, A, K, lens.data(), atoms.data(), names.data()};
     ltl_result res;
     const auto t0 = std::chrono::high_resolution_clock::now();
     rc = ltl_learn(ctx, &pb, &res);
     const auto t1 = std::chrono::high_resolution_clock::now();
 
-    const std::string alpha = file_alphabet(argv[1]);
+    // (name, atom) in the order a position's token lists them: the letters
+    // of the alphabet line in file order, or the names in column order
+    std::vector<std::pair<std::string, int>> show;
+    const std::string nm(names.data());
+    if (nm.find(',') == std::string::npos) {
+        for (char c : file_alphabet(argv[1])) show.push_back({std::string(1, c), (int)nm.find(c)});
+    } else {
+        std::string cur;
+        for (char c : nm + ",") {
+            if (c == ',') { show.push_back({cur, (int)show.size()}); cur.clear(); }
+            else cur += c;
+        }
+    }
     auto print_traces = [&](const char *title, int from, int to) {
         printf("\n%s: \n", title);
         for (int t = from; t < to; ++t) {
             printf("\t");
             for (int k = 0; k < lens[t]; ++k) {
                 std::string s;
-                for (char c : alpha) {
-                    const int a = (int)(std::string(names).find(c));
+                for (const auto &na : show) {
+                    const int a = na.second;
                     if (a >= 0 && ((atoms[((size_t)a * W + t) * K + k / 64] >> (k % 64)) & 1)) {
-                        s += c;
+                        s += na.first;
                         s += ", ";
                     }
                 }

@@ -33,3 +33,56 @@ def test_cli_argument_errors():
     assert out.returncode != 0
     assert "should be a positive short integer" in out.stdout
     assert "RlxUnqChkTyp" in out.stdout
+
+
+def _write_trace_file(path, inst):
+    """An instance in the reference format (positions ';', bits ',')."""
+    lines = []
+    for t in range(inst.W):
+        n = int(inst.lens[t])
+        lines.append(";".join(",".join(str((int(inst.atoms[a, t]) >> k) & 1) for a in range(inst.atoms.shape[0]))
+                              for k in range(n)))
+    P = inst.num_pos
+    with open(path, "w") as f:
+        f.write("\n".join(lines[:P]) + "\n---\n" + "\n".join(lines[P:]) + "\n---\nF,G,X,!,&,|\n---\n"
+                + ",".join(inst.names))
+
+
+def _run(args, env=None):
+    out = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=120,
+                         env=dict(os.environ, **(env or {})))
+    assert out.returncode == 0, out.stdout + out.stderr
+    return out.stdout.split("\n")[-2].split()[-1][1:-1]     # code/dc.py:194
+
+
+def test_cli_reference_keys(ltl, oracle, tmp_path):
+    # RlxUnqChkTyp (argument 11) selects the reference's relaxed key when
+    # asked to: the mirrored 4-trace instance answers X((a)U(b)) in mode 3,
+    # X(b) with exact dedup (the default)
+    from test_gpu_parity import _twin4, to_oracle
+    inst = _twin4(ltl, 3)
+    path = str(tmp_path / "twin.trace")
+    _write_trace_file(path, inst)
+    args = [path] + ["1"] * 8 + ["6", "3", "1"]
+    assert _run(args) == "X(b)"
+    assert _run(["--reference-keys"] + args) == "X((a)U(b))"
+    assert _run(args, {"LTLI_REFERENCE_KEYS": "1"}) == "X((a)U(b))"
+    ref = oracle.enumerate_(to_oracle(oracle, inst), max_cost=6, dedup_mode=3)
+    assert ref["formula"] == "X((a)U(b))"
+
+
+def test_cli_more_than_26_propositions(ltl, oracle, tmp_path):
+    from test_host import write_named_traces
+    path = str(tmp_path / "named.trace")
+    pos, neg, names = write_named_traces(path)
+    f = _run([path] + ["1"] * 8 + ["6", "3", "1"])
+    inst = ltl.read_trace_file(path)
+    # the oracle spells atom i with one character (a-z, then capitals other
+    # than the operators X, F, G, U): its formula, re-spelled with the names
+    chars = [chr(ord("a") + i) for i in range(26)] + [c for c in "ABCDEHIJKLMNOPQRSTVWYZ"][:4]
+    oi = oracle.Instance(inst.num_pos, inst.num_neg, inst.lens, inst.atoms, "".join(chars))
+    ref = oracle.enumerate_(oi, max_cost=6)
+    assert ref["status"] == 1
+    respelled = "".join(names[chars.index(c)] if c in chars else c for c in ref["formula"])
+    assert f == respelled
+    assert ltl.mismatches(inst, ltl.eval_formula(inst, f)) == 0

@@ -167,3 +167,49 @@ def test_trace_reader_tolerates_editor_artifacts(ltl, tmp_path, variant):
     assert (a.num_pos, a.num_neg, a.names) == (b.num_pos, b.num_neg, b.names)
     assert np.array_equal(a.lens, b.lens)
     assert np.array_equal(a.atoms, b.atoms)
+
+
+def write_named_traces(path, n_props=30, n=24, seed=3):
+    """A trace file whose alphabet line names 30 propositions p0..p29 (more
+    than the reference's 26 letters); labels by F(p27 & X p3), evaluated
+    position by position here."""
+    rng = np.random.default_rng(seed)
+    pos, neg = [], []
+    while len(pos) < n // 2 or len(neg) < n // 2:
+        L = int(rng.integers(3, 9))
+        bits = rng.integers(0, 2, size=(L, n_props))
+        holds = any(bits[k, 27] and bits[k + 1, 3] for k in range(L - 1))
+        line = ";".join(",".join(str(int(b)) for b in row) for row in bits)
+        side = pos if holds else neg
+        if len(side) < n // 2:
+            side.append(line)
+    names = [f"p{i}" for i in range(n_props)]
+    with open(path, "w") as f:
+        f.write("\n".join(pos) + "\n---\n" + "\n".join(neg) + "\n---\nF,G,X,!,&,|\n---\n" + ",".join(names))
+    return pos, neg, names
+
+
+def test_reader_more_than_26_propositions(ltl, tmp_path):
+    path = str(tmp_path / "named.trace")
+    pos, neg, names = write_named_traces(path)
+    inst = ltl.read_trace_file(path)
+    assert (inst.num_pos, inst.num_neg) == (len(pos), len(neg))
+    assert inst.names == ",".join(names) and inst.atoms.shape[0] == 30
+    for t, line in enumerate(pos + neg):
+        rows = [list(map(int, tok.split(","))) for tok in line.split(";")]
+        assert inst.lens[t] == len(rows)
+        for a in range(30):
+            assert int(inst.atoms[a, t]) == sum(r[a] << k for k, r in enumerate(rows))
+    # the planted formula, spelled with the names, separates the traces
+    cs = ltl.eval_formula(inst, "F((p27)&(X(p3)))")
+    assert ltl.mismatches(inst, cs) == 0
+    # a name that is not in the list, or malformed, is a parse error
+    for bad in ("F((p30)&(X(p3)))", "F((p27)&(X(p)))", "(p2)&"):
+        with pytest.raises(ltl.LtlError):
+            ltl.eval_formula(inst, bad)
+
+
+def test_letters_only_files_read_as_before(ltl):
+    # reference files keep the letter rule: sorted letters, plain names
+    inst = ltl.read_trace_file(os.path.join(GOLDEN, "traces", "Ab0.trace"))
+    assert inst.names == "pq"
