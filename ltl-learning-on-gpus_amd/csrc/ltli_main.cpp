@@ -117,6 +117,11 @@ int main(int argc, char **argv) {
     opt.max_cost = (uint16_t)std::atoi(argv[10]);
     opt.neg_type = (uint8_t)neg;
     if (ref_keys) opt.dedup_mode = (uint8_t)rlx;
+    // Device pool: 16 GiB unless LTLI_HBM_BUDGET_GB says otherwise (0 = 75% of
+    // the free memory).  Reserving most of a 288 GB device costs seconds per
+    // process, against milliseconds of learning for the reference's inputs.
+    const char *bud = getenv("LTLI_HBM_BUDGET_GB");
+    opt.hbm_budget_bytes = (uint64_t)((bud ? std::atof(bud) : 16.0) * (double)(1ull << 30));
     int32_t st = 0;
     ltl_ctx *ctx = ltl_create(&opt, &st);
     if (!ctx) {

@@ -1,6 +1,11 @@
 """Write tests/golden/expected.json: oracle results for the reference trace
 fixtures in tests/golden/traces (copied from the reference's benchmarks/),
-with the brute-force minimal cost where the search is small enough.
+with the brute-force minimal cost where the search is small enough; and
+tests/golden/envelope.json: oracle results for the 69 reference trace files
+that fit the reference's single-run limits (at most 64 traces of at most 63
+positions; copied to tests/golden/envelope/, named by their path under
+benchmarks/ with / as __), run as the reference's CLI example runs them
+(unit costs, RlxUnqChkTyp 3, NegType 2) up to cost 10.
 
     python tests/golden/make_golden.py
 """
@@ -14,6 +19,8 @@ sys.path.insert(0, os.path.join(HERE, "..", "..", "oracle"))
 import oracle as O  # noqa: E402
 
 MAX_COST = 7
+ENVELOPE_MAX_COST = 10
+ENVELOPE_REF3_MAX_COST = 8
 
 
 def main():
@@ -35,6 +42,22 @@ def main():
             print(cases[-1])
     with open(os.path.join(HERE, "expected.json"), "w") as f:
         json.dump({"generator": "tests/golden/make_golden.py", "cases": cases}, f, indent=1)
+    env = []
+    for path in sorted(glob.glob(os.path.join(HERE, "envelope", "*.trace"))):
+        inst = O.read_trace_file(path)
+        r = O.enumerate_(inst, max_cost=ENVELOPE_MAX_COST, neg_type=2, threads=8)
+        # the reference's relaxed key of mode 3 (its notebook default), which
+        # differs from exact dedup above 126 trace positions
+        r3 = O.enumerate_(inst, max_cost=ENVELOPE_REF3_MAX_COST, neg_type=2, dedup_mode=3)
+        env.append(dict(file=os.path.basename(path), neg_type=2, max_cost=ENVELOPE_MAX_COST, W=inst.W,
+                        len_sum=int(inst.lens.astype(int).sum()), status=r["status"], cost=r["cost"],
+                        all_ltls=r["all_ltls"], found_id=r["found_id"], formula=r["formula"],
+                        n_stored=r["n_stored"],
+                        ref3=dict(max_cost=ENVELOPE_REF3_MAX_COST, status=r3["status"], cost=r3["cost"],
+                                  all_ltls=r3["all_ltls"], formula=r3["formula"], n_stored=r3["n_stored"])))
+        print(env[-1])
+    with open(os.path.join(HERE, "envelope.json"), "w") as f:
+        json.dump({"generator": "tests/golden/make_golden.py", "cases": env}, f, indent=1)
 
 
 if __name__ == "__main__":

@@ -86,3 +86,31 @@ def test_cli_more_than_26_propositions(ltl, oracle, tmp_path):
     respelled = "".join(names[chars.index(c)] if c in chars else c for c in ref["formula"])
     assert f == respelled
     assert ltl.mismatches(inst, ltl.eval_formula(inst, f)) == 0
+
+
+def _envelope():
+    import json
+    with open(os.path.join(GOLDEN, "envelope.json")) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("case", _envelope(), ids=lambda c: c["file"][-40:])
+def test_cli_envelope_files(case):
+    """Every reference benchmark file within the reference's single-run
+    limits (69 of 611), through the CLI as the reference's example runs it:
+    the oracle's formula, cost and candidate count, exact and with
+    --reference-keys (RlxUnqChkTyp 3, the notebook's default)."""
+    path = os.path.join(GOLDEN, "envelope", case["file"])
+    out = subprocess.run([CLI, path] + ["1"] * 8 + [str(case["max_cost"]), "3", "2"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.split("\n")[-2].split()[-1][1:-1] == case["formula"]
+    assert f"Cost of Final LTL: {case['cost']}\n" in out.stdout
+    assert f"Number of All LTLs: {case['all_ltls']}\n" in out.stdout
+    r3 = case["ref3"]
+    out = subprocess.run([CLI, "--reference-keys", path] + ["1"] * 8 + [str(r3["max_cost"]), "3", "2"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.split("\n")[-2].split()[-1][1:-1] == r3["formula"]
+    assert f"Cost of Final LTL: {r3['cost']}\n" in out.stdout
+    assert f"Number of All LTLs: {r3['all_ltls']}\n" in out.stdout

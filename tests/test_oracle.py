@@ -369,3 +369,25 @@ def test_relaxed_key_slips_fail_the_vectors(oracle):
     assert _key_copy(one([52] * 5, 1, 2), [52] * 5, 1, "stride") != L(27)
     assert _key_copy(one([60] * 3, 1, 42), [60] * 3, 2, "take") != H(85)
     assert _key_copy(one([10, 63, 63], 0, 9), [10, 63, 63], 2, "take") != L(9)
+
+
+def envelope_cases():
+    with open(os.path.join(GOLDEN, "envelope.json")) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("case", envelope_cases(), ids=lambda c: c["file"][-40:])
+def test_envelope_fixtures(oracle, case):
+    """The 69 reference trace files within the reference's single-run limits
+    (tests/golden/envelope/): the oracle reproduces its committed results,
+    exact and with the reference's mode-3 key."""
+    inst = oracle.read_trace_file(os.path.join(GOLDEN, "envelope", case["file"]))
+    assert inst.W == case["W"]
+    r = oracle.enumerate_(inst, max_cost=case["max_cost"], neg_type=2, threads=8)
+    for k in ("status", "cost", "all_ltls", "found_id", "formula", "n_stored"):
+        assert r[k] == case[k], k
+    r3 = oracle.enumerate_(inst, max_cost=case["ref3"]["max_cost"], neg_type=2, dedup_mode=3)
+    for k in ("status", "cost", "all_ltls", "formula", "n_stored"):
+        assert r3[k] == case["ref3"][k], k
+    if r["status"] == 1:
+        assert oracle.mismatches(inst, oracle.eval_formula(inst, r["formula"])) == 0
