@@ -74,6 +74,51 @@ def cpu_baseline(inst, max_cost, threads):
     return uniques / dt, dt, uniques
 
 
+def host_cores():
+    """Cores this process may use (affinity, capped by OMP_NUM_THREADS when
+    the environment sets it: the GPU box grants its jobs a share of the
+    host), and what the host reports."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS")
+    use = min(aff, int(omp)) if omp and omp.isdigit() and int(omp) > 0 else aff
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return use, {"nproc": os.cpu_count(), "affinity": aff, "omp_num_threads": omp, "cpu_model": model}
+
+
+def time_to_min_cpu_gpu(ltl, threads, budget_gb):
+    """Configurations 1 and 3 to their minimal formula: the oracle on the host
+    (one thread and `threads`) beside ltl_learn (best of 3 calls)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    out = {}
+    L = ltl.Learner(budget=int(budget_gb * 1e9))
+    try:
+        for cfg in (1, 3):
+            inst = ltl.generate(cfg, seed=1)
+            oi = O.Instance(inst.num_pos, inst.num_neg, inst.lens, inst.atoms, inst.names)
+            row = {}
+            for th in sorted({1, threads}):
+                t = time.perf_counter()
+                r = O.enumerate_(oi, max_cost=inst.max_cost, hamming_k=inst.hamming_k, threads=th)
+                row[f"cpu_s_{th}_threads"] = round(time.perf_counter() - t, 4)
+            g = min((L.learn(inst, max_cost=inst.max_cost, hamming_k=inst.hamming_k, hash_slots_log2=0)
+                     for _ in range(3)), key=lambda x: x["seconds"])
+            assert (g["status"], g["cost"]) == (r["status"], r["cost"])
+            row.update(gpu_s=round(g["seconds"], 4), cost=r["cost"], formula=g["formula"])
+            out[f"config{cfg}"] = row
+    finally:
+        L.close()
+    return out
+
+
 def calibrate_copy(nbytes=1 << 30):
     """One device-to-device copy of nbytes (the runtime's copy kernel): the
     reference reading for rocprofv3's FETCH_SIZE / WRITE_SIZE scale."""
@@ -281,7 +326,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=0.0, help="device pool; 0 = 75%% of free memory")
     ap.add_argument("--max-cost", type=int, default=14)
     ap.add_argument("--cpu-cost", type=int, default=9, help="levels timed for the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="0 = the cores this process may use (affinity, capped by OMP_NUM_THREADS)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the learned-order and batch measurements")
     ap.add_argument("--config5", type=int, default=16, help="configuration-5 instances compared")
@@ -382,17 +428,20 @@ def main():
         out["config4_batch"] = batch_vs_loop(ltl, args)
         out["long_traces"] = long_traces(ltl, args)
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        avail, host = host_cores()
+        threads = args.cpu_threads or avail
         v, dt, u = cpu_baseline(inst, args.cpu_cost, threads)
         out["cpu_baseline"] = {"value": round(v, 1), "unit": "unique CS/s", "cores": threads,
                                "kind": "port", "seconds": round(dt, 3),
                                "sample": f"same instance, levels up to cost {args.cpu_cost} "
-                                         f"({u} unique sets), exact dedup, OpenMP"}
+                                         f"({u} unique sets), exact dedup, OpenMP",
+                               "host": host}
         # one thread, on a smaller sample (levels up to cost 8, about 10 s)
         v1, dt1, u1 = cpu_baseline(inst, min(args.cpu_cost, 8), 1)
         out["cpu_baseline"]["single_thread"] = {"value": round(v1, 1), "cores": 1, "seconds": round(dt1, 3),
                                                 "sample": f"levels up to cost {min(args.cpu_cost, 8)} "
                                                           f"({u1} unique sets)"}
+        out["cpu_baseline"]["time_to_min"] = time_to_min_cpu_gpu(ltl, threads, args.budget_gb or 8)
     print(json.dumps(out), flush=True)
 
 
