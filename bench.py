@@ -136,6 +136,105 @@ def calibrate_copy(nbytes=1 << 30):
     print(json.dumps({"calibrate_copy_bytes": nbytes}), flush=True)
 
 
+def kernel_kind(name):
+    """The library's kernel kind (ltl_get_kernel_stats names) of a kernel
+    symbol in a rocprofv3 trace; other kernels by their own name."""
+    import re
+    if "rocprim" in name:
+        return "scan_finish"
+    if "copyBuffer" in name:
+        return "copy"
+    m = re.search(r"k_(\w+?)(<[^>]*>)?\(", name)
+    if not m:
+        return name[:40]
+    k = m.group(1)
+    op = re.match(r"<(\d+)", m.group(2) or "")
+    op = int(op.group(1)) if op else None
+    unary, binary = {1, 4, 5, 6}, {2, 3}
+    if k in ("gen1", "gen_xfg", "gen1m", "gen_xfgm"):
+        return "gen_unary"
+    if k == "gen2":
+        return "gen_until" if op == 7 else "gen_and_or"
+    if k == "gen2m":
+        return "gen_until"
+    if k in ("emit", "emitm"):
+        return "emit_unary" if op in unary else "emit_and_or" if op in binary else "emit_until"
+    if k == "emit_meta":
+        return "emit_meta"
+    if k in ("decide", "dedup_small", "dedup_lb", "rx_tail"):
+        return "decide"
+    if k in ("finish", "compact"):
+        return "scan_finish"
+    if k == "chk1":
+        return "check_unary"
+    if k == "chk2":
+        return "check_and_or"
+    if k in ("chkU", "chkUm"):
+        return "check_until"
+    return k
+
+
+def pmc_summary(base):
+    """Per kernel kind, per dispatch, from rocprofv3 --pmc runs under `base`
+    (subfolders fetch, write: FETCH_SIZE / WRITE_SIZE of one bench.py call;
+    calf, calw: the same counters for --calibrate-copy; any other subfolder:
+    more counters of the bench call, cal<name>: of the copy): HBM bytes
+    scaled by the copy's calibration, and the raw counters."""
+    import csv
+    import glob as _glob
+
+    def load(sub):
+        rows = {}
+        path = os.path.join(base, sub, "run_counter_collection.csv")
+        if not os.path.exists(path):
+            return rows
+        for r in csv.DictReader(open(path)):
+            k = kernel_kind(r["Kernel_Name"])
+            d = rows.setdefault(k, {"dispatches": set(), "counters": {}})
+            d["dispatches"].add(r["Dispatch_Id"])
+            d["counters"][r["Counter_Name"]] = d["counters"].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        return rows
+
+    subs = sorted(os.path.basename(p) for p in _glob.glob(os.path.join(base, "*")) if os.path.isdir(p))
+    data = {s: load(s) for s in subs}
+    copy_bytes = 1 << 30
+    calf = data.get("calf", {}).get("copy", {}).get("counters", {}).get("FETCH_SIZE")
+    calw = data.get("calw", {}).get("copy", {}).get("counters", {}).get("WRITE_SIZE")
+    fs = copy_bytes / (calf * 1024) if calf else None
+    ws = copy_bytes / (calw * 1024) if calw else None
+    out = {"calibration": {"copy_bytes": copy_bytes, "fetch_kib": calf, "write_kib": calw,
+                           "fetch_scale": round(fs, 4) if fs else None, "write_scale": round(ws, 4) if ws else None},
+           "kernels": {}, "copy_counters": {}}
+    for sub in subs:
+        if sub.startswith("cal") and sub not in ("calf", "calw"):
+            c = data[sub].get("copy")
+            if c:
+                out["copy_counters"].update({k: v / len(c["dispatches"]) for k, v in c["counters"].items()})
+    kinds = set()
+    for sub in subs:
+        if not sub.startswith("cal"):
+            kinds.update(data[sub])
+    for k in sorted(kinds):
+        e = {}
+        for sub in subs:
+            if sub.startswith("cal") or k not in data[sub]:
+                continue
+            d = data[sub][k]
+            n = len(d["dispatches"])
+            e["dispatches"] = n
+            for c, v in d["counters"].items():
+                e.setdefault("counters", {})[c] = round(v / n, 1)
+        cnt = e.get("counters", {})
+        if "FETCH_SIZE" in cnt and fs:
+            e["fetch_bytes_per_dispatch"] = int(cnt["FETCH_SIZE"] * 1024 * fs)
+        if "WRITE_SIZE" in cnt and ws:
+            e["write_bytes_per_dispatch"] = int(cnt["WRITE_SIZE"] * 1024 * ws)
+        if "fetch_bytes_per_dispatch" in e and "write_bytes_per_dispatch" in e:
+            e["bytes_per_dispatch"] = e["fetch_bytes_per_dispatch"] + e["write_bytes_per_dispatch"]
+        out["kernels"][k] = e
+    return out
+
+
 def learned_vs_plain(ltl, L, args):
     """Time to the minimal formula, plain vs learned order, at equal cost:
     configuration 2 (steps as above), configuration 3, every configuration-4
@@ -338,7 +437,12 @@ def main():
                     help="only run the scorer's forward and gradient on this many random rows")
     ap.add_argument("--long-traces", action="store_true",
                     help="only run the instance of traces longer than 64 positions")
+    ap.add_argument("--pmc-summary", metavar="DIR",
+                    help="only summarize rocprofv3 --pmc runs under DIR (see pmc_summary) as JSON")
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(json.dumps(pmc_summary(args.pmc_summary), indent=1))
+        return
     if args.long_traces:
         print(json.dumps(long_traces(load_package(), args)), flush=True)
         return
