@@ -238,11 +238,16 @@ def pmc_summary(base):
 def learned_vs_plain(ltl, L, args):
     """Time to the minimal formula, plain vs learned order, at equal cost:
     configuration 2 (steps as above), configuration 3, every configuration-4
-    instance (one call each) and configuration-5 instances.  The
-    scorer is trained on the device from logs of configurations 1-3 first
-    (outside the timed calls)."""
+    instance (one call each) and the configuration-5 instances of seed 1, up
+    to their planted size.  The scorer is trained on the device first
+    (outside the timed calls) from logs of configurations 1-3 and of the
+    configuration-5 instances of seeds 2 and 3 (held out from the ones
+    timed)."""
+    import numpy as np
     t = time.perf_counter()
     rows = ltl.collect_logs(learner=L, per_block=256, config2_max_cost=8, config3_max_cost=6)
+    rows = np.concatenate([rows, ltl.collect_logs(learner=L, per_block=1024, config1_seeds=(), config2_max_cost=1,
+                                                  config3_max_cost=1, config5_seeds=(2, 3))])
     t_log = time.perf_counter() - t
     t = time.perf_counter()
     trained, loss, n_rows = ltl.train_scorer(rows, seed=2, epochs=20)
@@ -261,7 +266,9 @@ def learned_vs_plain(ltl, L, args):
         return best
 
     out = {"scorer": {"log_rows": int(n_rows), "log_s": round(t_log, 3), "train_s": round(t_train, 3),
-                      "final_loss": round(float(loss), 6), "trained_on": "configs 1-3 (seeded), plain order logs"}}
+                      "final_loss": round(float(loss), 6),
+                      "trained_on": "plain-order logs of configs 1-3 (seeded) and of the config-5 "
+                                    "instances of seeds 2 and 3"}}
     inst2 = ltl.generate(2, seed=1)
     p = best_of(inst2, max(args.steps, 1), None, max_cost=args.max_cost)
     q = best_of(inst2, max(args.steps, 1), trained, max_cost=args.max_cost)
@@ -278,23 +285,35 @@ def learned_vs_plain(ltl, L, args):
     out["config3"] = {"plain_s": round(p["seconds"], 4), "learned_s": round(q["seconds"], 4),
                       "speedup": round(p["seconds"] / q["seconds"], 3), "cost": p["cost"],
                       "hamming_k": inst3.hamming_k, "plain_formula": p["formula"], "learned_formula": q["formula"]}
-    rows5, tot_p, tot_l, tot_u, faster, ranks = [], 0.0, 0.0, 0.0, 0, []
+    rows5, tot_p, tot_l, tot_u, faster, ranks, shares = [], 0.0, 0.0, 0.0, 0, [], []
     for idx in range(args.config5):
         inst = ltl.generate(5, seed=1, index=idx)
-        p = best_of(inst, 2, None, max_cost=args.config5_cost, hash_slots_log2=0)
-        q = best_of(inst, 2, trained, max_cost=args.config5_cost, hash_slots_log2=0)
-        u = best_of(inst, 2, untrained, max_cost=args.config5_cost, hash_slots_log2=0)
+        mc5 = args.config5_cost or inst.max_cost
+        p = best_of(inst, 2, None, max_cost=mc5, hash_slots_log2=0)
+        q = best_of(inst, 2, trained, max_cost=mc5, hash_slots_log2=0)
+        u = best_of(inst, 2, untrained, max_cost=mc5, hash_slots_log2=0)
         assert (q["status"], q["cost"]) == (p["status"], p["cost"]) == (u["status"], u["cost"])
         tot_p += p["seconds"]
         tot_l += q["seconds"]
         tot_u += u["seconds"]
         faster += q["seconds"] < p["seconds"]
-        rows5.append([idx, p["status"], p["cost"], round(p["seconds"] * 1e3, 2), round(q["seconds"] * 1e3, 2),
+        rows5.append([idx, mc5, p["status"], p["cost"], round(p["seconds"] * 1e3, 2), round(q["seconds"] * 1e3, 2),
                       round(u["seconds"] * 1e3, 2)])
+        if p["status"] == 1 and p["cost"] >= 9:
+            # where the ranking can act: the ranked check rounds' share of the
+            # device time of a learned-order call (kernel timing on)
+            for name, sc in (("trained", trained), ("untrained", untrained)):
+                L.set_scorer(sc)
+                r = L.learn(inst, order=ltl.ORDER_LEARNED, max_cost=mc5, hash_slots_log2=0, profile=1)
+                ks = L.kernel_stats()
+                shares.append([idx, name, round(ks["check_ranked"]["seconds"] * 1e3, 2),
+                               round(r["kernel_seconds"] * 1e3, 2), r["checked"]])
+            L.set_scorer(None)
         if p["status"] == 1 and p["cost"] >= 4:
-            rk = ltl.first_hit_ranks(L, inst, {"trained": trained, "untrained": untrained}, args.config5_cost)
+            rk = ltl.first_hit_ranks(L, inst, {"trained": trained, "untrained": untrained}, min(mc5, 11),
+                                     max_rows=1 << 20)
             if rk:
-                ranks.append(rk)
+                ranks.append(dict(index=idx, **rk))
     # Configuration 4: every instance, one call each, plain vs learned.
     insts4 = [ltl.generate(4, seed=1, index=i) for i in range(256)]
     mc4 = max(i.max_cost for i in insts4)
@@ -311,11 +330,13 @@ def learned_vs_plain(ltl, L, args):
     out["config4"] = {"instances": len(insts4), "max_cost": mc4, "plain_s": round(t4p, 4),
                       "learned_s": round(t4l, 4), "same_status_and_cost": same4, "learned_faster": faster4,
                       "not_faster": "[index, cost, plain ms, learned ms]", "not_faster_rows": slower4[:32]}
-    out["config5"] = {"instances": args.config5, "max_cost": args.config5_cost,
+    out["config5"] = {"instances": args.config5, "max_cost": args.config5_cost or "planted size",
                       "plain_s": round(tot_p, 4), "learned_s": round(tot_l, 4), "learned_untrained_s": round(tot_u, 4),
                       "learned_faster": faster,
-                      "per_instance_ms": "[index, status, cost, plain, learned, learned untrained]",
-                      "rows": rows5, "first_hit_rank": ranks}
+                      "per_instance_ms": "[index, max cost, status, cost, plain, learned, learned untrained]",
+                      "rows": rows5, "first_hit_rank": ranks,
+                      "ranked_share": "[index, scorer, check_ranked ms, all kernels ms, candidates checked]",
+                      "ranked_share_rows": shares}
     trained.close()
     untrained.close()
     return out
@@ -430,7 +451,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the learned-order and batch measurements")
     ap.add_argument("--config5", type=int, default=16, help="configuration-5 instances compared")
-    ap.add_argument("--config5-cost", type=int, default=8)
+    ap.add_argument("--config5-cost", type=int, default=0, help="0 = each instance's planted size")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run one 1 GiB device copy (PMC calibration)")
     ap.add_argument("--scorer-rows", type=int, default=0,

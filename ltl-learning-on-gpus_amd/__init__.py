@@ -596,9 +596,11 @@ def LTLI(costFun, maxCost, RlxUnqChkTyp, NegType, alphabet, pos, neg, learner=No
 
 
 def collect_logs(learner=None, per_block=64, config1_seeds=range(1, 21), config2_max_cost=7,
-                 config3_max_cost=6, max_rows=1 << 20):
-    """Enumeration logs of seeded runs of configurations 1-3 (plain order):
-    rows (n, 25) of tile-pair features and labels for Scorer.train."""
+                 config3_max_cost=6, max_rows=1 << 20, config5_seeds=(), config5_max_cost=10):
+    """Enumeration logs of seeded runs of configurations 1-3 and of the
+    configuration-5 instances of config5_seeds (plain order, each up to its
+    planted size or config5_max_cost): rows (n, 25) of tile-pair features and
+    labels for Scorer.train."""
     own = learner is None
     L = learner or Learner(budget=8 << 30)
     out = []
@@ -612,6 +614,11 @@ def collect_logs(learner=None, per_block=64, config1_seeds=range(1, 21), config2
         inst3 = generate(3, seed=1)
         L.learn(inst3, max_cost=config3_max_cost, hamming_k=inst3.hamming_k, order=ORDER_PLAIN)
         out.append(L.get_log())
+        for s in config5_seeds:
+            for idx in range(16):
+                inst = generate(5, seed=s, index=idx)
+                L.learn(inst, max_cost=min(inst.max_cost, config5_max_cost), order=ORDER_PLAIN)
+                out.append(L.get_log())
         L.set_log(0, 0)
     finally:
         if own:

@@ -147,6 +147,10 @@ struct ChunkArgs {
     // each block's end, so that the blocks before an accepted one still
     // store their survivors, as they would one after the other).
     uint64_t id_end = kEmpty;
+    // Ranked check round submitted ahead of a sync: its workgroups do no
+    // work once an accepted candidate with an id below found_below (a lower
+    // level's) is known when they start (0: always work)
+    uint64_t found_below = 0;
     // 0: the level was tested completely by a check pass before it is
     // generated, so the gen kernels skip the target check
     int32_t gen_check = 1;

@@ -1548,9 +1548,17 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         }
         return 0;
     };
+    constexpr int64_t kRankMin = 4096;  // about 16M candidates: more work than a sync costs
+    // Ranked rounds: the top 1/64 of the tile pairs (at least kRankMin),
+    // then three times as many tile pairs as all rounds before (the last
+    // round takes the rest when it would be short): round_end(E, r0) is the
+    // end of the round that starts at r0.
+    auto round_end = [&](int64_t E, int64_t r0) {
+        const int64_t next = r0 == 0 ? std::max(kRankMin, E / 64) : 4 * r0;
+        return next + next / 4 >= E ? E : next;
+    };
     // Tile-pair order of the level in the second half of d_vals: by
     // descending score, or canonical when one round covers the level.
-    constexpr int64_t kRankMin = 1024;
     auto order_level = [&](int cost, int upto) -> int32_t {
         const int64_t E = run.n_ent;
         if (E > ((int64_t)1 << 31)) return LTL_E_LIMIT;
@@ -1569,7 +1577,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         }
         float *k0 = c->d_keys, *k1 = c->d_keys + c->ent_cap;
         uint32_t *v0 = c->d_vals, *v1 = c->d_vals + c->ent_cap;
-        if (E <= kRankMin) {
+        if (round_end(E, 0) >= E) {  // one round
             HIP_TRY(launch_iota(v1, E, s));
             return 0;
         }
@@ -1594,9 +1602,6 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         run.checked += (uint64_t)n * 4096;
         return 0;
     };
-    // First round of a ranked pass: the top 1/16 of the tile pairs (at
-    // least kRankMin), then the rest.
-    auto first_round = [&](int64_t E) { return std::min(E, std::max(kRankMin, E / 16)); };
     // Log and / or test (ranked rounds, a sync after each) the recorded blocks.
     auto ranked_level = [&](int cost, int upto) -> int32_t {
         if (int32_t e = upload_blocks()) return e;
@@ -1604,10 +1609,9 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             if (int32_t e = log_level(cost, upto)) return e;
         if (!run.ranked) return 0;
         if (int32_t e = order_level(cost, upto)) return e;
-        const int64_t E = run.n_ent, r1 = first_round(E);
-        for (const auto &rr : {std::make_pair((int64_t)0, r1), std::make_pair(r1, E)}) {
-            if (rr.second <= rr.first) continue;
-            if (int32_t e = submit_round(rr.first, rr.second - rr.first)) return e;
+        const int64_t E = run.n_ent;
+        for (int64_t r0 = 0; r0 < E; r0 = round_end(E, r0)) {
+            if (int32_t e = submit_round(r0, round_end(E, r0) - r0)) return e;
             HIP_TRY(hipMemcpyAsync(&h, c->d_state, sizeof(DevState), hipMemcpyDeviceToHost, s));
             HIP_TRY(hipStreamSynchronize(s));
             if (h.found != kEmpty) break;
@@ -1692,6 +1696,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
                 run.base.neg_slot = 1 - slot;
                 run.lblk.clear();
                 run.n_ent = 0;
+                const uint64_t next_base = run.all_ltls;  // first id of level cost + 1
                 if (!rc) rc = level_blocks(cost + 1);
                 run.only_binary = false;
                 E = run.n_ent;
@@ -1699,8 +1704,12 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
                     rc = upload_blocks();
                     if (!rc && run.logging) rc = log_level(cost + 1, cost - 1);
                     if (!rc) rc = order_level(cost + 1, cost - 1);
-                    r1 = first_round(E);
+                    r1 = round_end(E, 0);
+                    // (submitted before the level's result is known: no work
+                    // once the level has an accepted candidate)
+                    run.base.found_below = next_base;
                     if (!rc) rc = submit_round(0, r1);
+                    run.base.found_below = 0;
                 }
             }
             if (!rc) rc = sync_state();
@@ -1718,14 +1727,14 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
                 Le[cost] = n_stored;
                 break;
             }
-            if (r1 < E) {
-                if ((rc = submit_round(r1, E - r1)) || (rc = sync_state())) break;
-                if (h.found != kEmpty) {
-                    found = true;
-                    found_cost = cost + 1;
-                    Le[cost] = n_stored;
-                    break;
-                }
+            for (int64_t r0 = r1; r0 < E && !rc && h.found == kEmpty; r0 = round_end(E, r0))
+                if (!(rc = submit_round(r0, round_end(E, r0) - r0))) rc = sync_state();
+            if (rc) break;
+            if (h.found != kEmpty) {
+                found = true;
+                found_cost = cost + 1;
+                Le[cost] = n_stored;
+                break;
             }
             run.check_only = false;
             run.base.anti = 0;

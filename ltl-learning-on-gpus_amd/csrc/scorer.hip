@@ -563,6 +563,7 @@ __device__ __forceinline__ void pair_mism(const ChunkArgs &A, int op, int64_t sl
 template <int B>
 __global__ __launch_bounds__(256) void k_chk_list(ChunkArgs A, const LBlock *blk, int nblk, const uint32_t *ent,
                                                   int64_t r0) {
+    if (A.found_below && *(volatile const unsigned long long *)&A.st->found < A.found_below) return;
     const int64_t e = ent[r0 + (int64_t)(blockIdx.x >> 4)];
     const LBlock &K = blk[find_block(blk, nblk, e)];
     const int64_t loc = e - K.e_off, ta = loc / K.nTR, tb = loc - ta * K.nTR;
