@@ -393,6 +393,35 @@ def batch_vs_loop(ltl, args):
             "trials": len(trials), "on_device": st["on_device"], "deferred": st["deferred"], "host": st["host"]}
 
 
+def dc_runs(ltl, args):
+    """Divide and conquer (the reference's RandSplit and DetSplit drivers,
+    windows of 64 traces) on 16 problems of 300 + 300 traces (4 atoms,
+    lengths U{8..24}, labelled by planted formulas of size 9, max cost 5 per
+    window): one ltl_learn_dc call per problem vs one ltl_learn_dc_batch call
+    for all 16 (windows of all problems learned together, matches on the
+    device).  Wall time of the calls."""
+    insts = [ltl.generate_custom(900 + k, 300, 300, 4, 8, 24, 0.5, planted_size=9) for k in range(16)]
+    L = ltl.Learner(budget=int(args.budget_gb * 1e9) or (16 << 30), max_cost=5, hash_slots_log2=0)
+    out = {"problems": len(insts), "traces": insts[0].W, "window": 64, "max_cost": 5}
+    try:
+        for algo, name in ((ltl.DC_RAND, "rand"), (ltl.DC_DET, "det")):
+            L.learn_dc(insts[0], window=64, algo=algo, seed=1)
+            t = time.perf_counter()
+            one = [L.learn_dc(i, window=64, algo=algo, seed=1) for i in insts]
+            t_one = time.perf_counter() - t
+            t = time.perf_counter()
+            bat = L.learn_dc_batch(insts, window=64, algo=algo, seed=1)
+            t_bat = time.perf_counter() - t
+            same = sum(a["formula"] == b["formula"] and a["calls"] == b["calls"] for a, b in zip(one, bat))
+            out[name] = {"one_by_one_s": round(t_one, 4), "batched_s": round(t_bat, 4),
+                         "speedup": round(t_one / t_bat, 2), "same_formula_and_calls": same,
+                         "window_calls": sum(a["calls"] for a in one),
+                         "mean_formula_cost": round(sum(a["cost"] for a in one) / len(one), 2)}
+    finally:
+        L.close()
+    return out
+
+
 def long_instance(ltl, seed, P, N, A, lo, hi, planted, pool=4000):
     """P + N traces of U{lo..hi} positions (K = ceil(hi / 64) words each),
     per-atom bit rates U[0.3, 0.7], labelled by `planted` (product host
@@ -456,6 +485,7 @@ def main():
                     help="only run one 1 GiB device copy (PMC calibration)")
     ap.add_argument("--scorer-rows", type=int, default=0,
                     help="only run the scorer's forward and gradient on this many random rows")
+    ap.add_argument("--dc", action="store_true", help="only run the divide-and-conquer comparison")
     ap.add_argument("--long-traces", action="store_true",
                     help="only run the instance of traces longer than 64 positions")
     ap.add_argument("--pmc-summary", metavar="DIR",
@@ -466,6 +496,9 @@ def main():
         return
     if args.long_traces:
         print(json.dumps(long_traces(load_package(), args)), flush=True)
+        return
+    if args.dc:
+        print(json.dumps(dc_runs(load_package(), args)), flush=True)
         return
     if args.calibrate_copy:
         calibrate_copy()
@@ -551,6 +584,7 @@ def main():
         out["learned_vs_plain"] = learned_vs_plain(ltl, L, args)
         L.close()
         out["config4_batch"] = batch_vs_loop(ltl, args)
+        out["divide_and_conquer"] = dc_runs(ltl, args)
         out["long_traces"] = long_traces(ltl, args)
     if not args.no_cpu:
         avail, host = host_cores()

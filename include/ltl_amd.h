@@ -275,6 +275,21 @@ int32_t ltl_levels(const ltl_ctx *ctx, int64_t *start, int64_t *end, int32_t max
 enum { LTL_DC_DET = 0, LTL_DC_RAND = 1 };
 int32_t ltl_learn_dc(ltl_ctx *ctx, const ltl_problem *pb, int32_t window, int32_t algo, uint64_t seed,
                      ltl_result *res);
+/* ltl_learn_dc over n problems at once (the reference runs dc.py once per
+ * benchmark file, runDC, code/dc.py:373-387): one driver per problem, in lock-step;
+ * whenever every driver waits for a window, the windows of all of them are
+ * learned by one ltl_learn_batch call, and the formulas returned are matched
+ * on the device in one launch.  res[i] is what ltl_learn_dc returns for
+ * pbs[i] (res[i].seconds: the whole call).  Returns the number of problems
+ * with a formula, or LTL_E_ARG. */
+int32_t ltl_learn_dc_batch(ltl_ctx *ctx, const ltl_problem *pbs, int32_t n, int32_t window, int32_t algo,
+                           uint64_t seed, ltl_result *res);
+/* The reference's LTLmatch (code/dc.py:136-186) on the device: out[i*W + t]
+ * = 1 iff formulas[i] (reference output grammar) holds at position 0 of
+ * trace t.  Returns 0, LTL_E_ARG (a parse error) or LTL_E_LIMIT (a formula
+ * of more than 2^16 nodes in an unbalanced shape, or more than 4 words per
+ * trace). */
+int32_t ltl_match(ltl_ctx *ctx, const ltl_problem *pb, const char *const *formulas, int32_t n, uint8_t *out);
 
 /* ---- Learned operand ordering ------------------------------------------
  * No counterpart in the reference (its order is fixed by the loops of

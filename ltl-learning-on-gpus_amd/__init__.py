@@ -106,7 +106,8 @@ EXPORTS = ["ltl_default_options", "ltl_create", "ltl_set_options", "ltl_learn", 
            "ltl_gen_config", "ltl_gen_custom", "ltl_gen_free",
            "ltl_scorer_create", "ltl_scorer_destroy", "ltl_scorer_get_params", "ltl_scorer_set_params",
            "ltl_scorer_forward", "ltl_scorer_grad", "ltl_scorer_train", "ltl_set_scorer", "ltl_set_log",
-           "ltl_get_log", "ltl_learn_dc", "ltl_get_options", "ltl_learn_noalloc", "ltl_read_trace_file_names"]
+           "ltl_get_log", "ltl_learn_dc", "ltl_get_options", "ltl_learn_noalloc", "ltl_read_trace_file_names",
+           "ltl_learn_dc_batch", "ltl_match"]
 
 DC_DET, DC_RAND = 0, 1
 
@@ -175,6 +176,10 @@ def lib():
     L.ltl_get_log.restype = ctypes.c_int64
     L.ltl_learn_dc.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64,
                                P(Result)]
+    L.ltl_learn_dc_batch.argtypes = [ctypes.c_void_p, P(Problem), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                     ctypes.c_uint64, P(Result)]
+    L.ltl_match.argtypes = [ctypes.c_void_p, P(Problem), P(ctypes.c_char_p), ctypes.c_int32,
+                            P(ctypes.c_uint8)]
     L.ltl_get_options.argtypes = [ctypes.c_void_p, P(Options)]
     _lib = L
     return L
@@ -358,6 +363,40 @@ class Learner:
         lib().ltl_result_free(ctypes.byref(res))
         if rc < 0:
             raise LtlError(f"ltl_learn_dc -> {rc}")
+        return out
+
+    def learn_dc_batch(self, insts, window=64, algo=DC_DET, seed=0, **opts):
+        """Divide and conquer over several instances in lock-step
+        (ltl_learn_dc_batch): the pending windows of all instances learned
+        together by ltl_learn_batch, formula matches on the device.  Returns
+        one dict per instance, as learn_dc; an instance that fails has its
+        LTL_E_* code as status (the others are unaffected)."""
+        self.set_options(**opts)
+        n = len(insts)
+        pbs = (Problem * n)(*[i.problem() for i in insts])
+        res = (Result * n)()
+        rc = lib().ltl_learn_dc_batch(self._ctx, pbs, n, int(window), int(algo), int(seed), res)
+        out = []
+        for i in range(n):
+            out.append(self._result(res[i]))
+            out[-1]["calls"] = res[i].n_stored
+            lib().ltl_result_free(ctypes.byref(res[i]))
+        if rc < 0:
+            raise LtlError(f"ltl_learn_dc_batch -> {rc}")
+        return out
+
+    def match(self, inst, formulas):
+        """Value at position 0 of each formula on every trace, evaluated on
+        the device (ltl_match; the reference's LTLmatch, code/dc.py:185):
+        uint8 array [len(formulas), W]."""
+        n = len(formulas)
+        arr = (ctypes.c_char_p * max(n, 1))(*[f.encode() for f in formulas])
+        out = np.zeros((n, inst.W), dtype=np.uint8)
+        pb = inst.problem()
+        rc = lib().ltl_match(self._ctx, ctypes.byref(pb), arr, n,
+                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+        if rc != 0:
+            raise LtlError(f"ltl_match -> {rc}")
         return out
 
     @staticmethod

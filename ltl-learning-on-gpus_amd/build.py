@@ -25,7 +25,7 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wno-unused-command-line-argument", "-I", os.path.join(ROOT, "include")]
-SOURCES = ["kernels.hip", "batch.hip", "scorer.hip", "noalloc.hip", "learn.cpp", "scorer_host.cpp", "dc.cpp", "host_util.cpp"]
+SOURCES = ["kernels.hip", "batch.hip", "scorer.hip", "noalloc.hip", "dc_eval.hip", "learn.cpp", "scorer_host.cpp", "dc.cpp", "host_util.cpp"]
 
 
 def _compile(src):

@@ -84,6 +84,14 @@ struct ltl_ctx {
     hipStream_t stream = nullptr;
     char *pool = nullptr;
     size_t pool_bytes = 0;
+    // Workgroup regions of the batched launch whose tables are known empty
+    // (each instance clears the slots it used): the first clean_n regions of
+    // clean_region bytes at clean_base, with clean_tslots table slots.  Any
+    // other use of the pool (ltl_learn stores sets there) sets clean_n to 0.
+    char *clean_base = nullptr;
+    uint64_t clean_region = 0;
+    uint32_t clean_tslots = 0;
+    int clean_n = 0;
     // Device memory of the current call: the pool, or the library's static
     // scratch region of this device (ltl_learn_noalloc; its address from
     // set-up, noalloc_scratch()).
@@ -304,6 +312,7 @@ static int32_t ensure_pool(ltl_ctx *c) {
     want = align_up(want, 1 << 21);
     if (c->pool && c->pool_bytes >= want) return 0;
     if (c->pool) { hipFree(c->pool); c->pool = nullptr; c->pool_bytes = 0; }
+    c->clean_n = 0;
     HIP_TRY(hipMalloc((void **)&c->pool, want));
     c->pool_bytes = want;
     return 0;
@@ -1271,6 +1280,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         if (int32_t e = ensure_pool(c)) return res->status = e;
         c->arena = c->pool;
         c->arena_bytes = c->pool_bytes;
+        c->clean_n = 0;  // (the batched launch's regions are overwritten)
         if (int32_t e = carve(c)) return res->status = e;
         if (c->capacity < A) return res->status = LTL_E_OOM;
     }
@@ -2224,17 +2234,33 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
         int32_t *d_order = (int32_t *)take((size_t)m * 4);
         unsigned *d_next = (unsigned *)take(256);
         uint64_t *d_in = (uint64_t *)take(in.size() * 8);
-        const size_t used = (size_t)(p - c->pool);
+        // the regions start at a multiple of 64 MiB past the inputs, so
+        // that calls with inputs of up to 64 MiB share one layout and the
+        // regions a previous call left clean need no fill
+        const size_t used = align_up((size_t)(p - c->pool), (size_t)64 << 20);
+        p = c->pool + std::min(used, c->pool_bytes);
         if (used + (size_t)G * (64ull << 20) > c->pool_bytes)
             G = (int)std::max<size_t>(1, (c->pool_bytes - std::min(used, c->pool_bytes)) / (64ull << 20));
-        const uint64_t region = ((c->pool_bytes - used) / (size_t)G) / 4096 * 4096;
+        const uint64_t region = ((c->pool_bytes - std::min(used, c->pool_bytes)) / (size_t)G) / 4096 * 4096;
         uint32_t tslots = 1u << 20;
         while (tslots > 4096 && (uint64_t)tslots * 16 > region / 4) tslots >>= 1;
+        // workgroups of the largest launch: regions [0, g_used) are used
+        int g_used = 0;
+        for (int g0 = 0; g0 < m;) {
+            int g1 = g0;
+            while (g1 < m && inst[order[g1]].B == inst[order[g0]].B) ++g1;
+            g_used = std::max(g_used, std::min(G, g1 - g0));
+            g0 = g1;
+        }
+        if (c->clean_base != p || c->clean_region != region || c->clean_tslots != tslots) c->clean_n = 0;
+        const int clean_before = c->clean_n;
+        c->clean_n = 0;  // (known again once the launches have finished)
         hipStream_t s = c->stream;
         HIP_TRY(hipMemcpyAsync(d_inst, inst.data(), inst.size() * sizeof(BatchInst), hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(d_order, order.data(), order.size() * 4, hipMemcpyHostToDevice, s));
         HIP_TRY(hipMemcpyAsync(d_in, in.data(), in.size() * 8, hipMemcpyHostToDevice, s));
-        HIP_TRY(launch_fill_regions(p, region, tslots, G, s));
+        if (g_used > clean_before)
+            HIP_TRY(launch_fill_regions(p + (size_t)clean_before * region, region, tslots, g_used - clean_before, s));
         BatchArgs ba{};
         ba.inst = d_inst; ba.out = d_out; ba.in = d_in;
         ba.regions = p; ba.region_bytes = region; ba.tslots = tslots; ba.next = d_next;
@@ -2254,6 +2280,11 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
         std::vector<BatchOut> out(m);
         HIP_TRY(hipMemcpyAsync(out.data(), d_out, (size_t)m * sizeof(BatchOut), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        // (every instance leaves its region's table empty)
+        c->clean_base = p;
+        c->clean_region = region;
+        c->clean_tslots = tslots;
+        c->clean_n = std::max(clean_before, g_used);
         float ms = 0.f;
         hipEventElapsedTime(&ms, e0, e1);
         c->free_events.push_back(e0);
