@@ -262,6 +262,31 @@ int32_t ltl_store_summaries(ltl_ctx *ctx, int64_t first, int64_t count, uint64_t
 /* Level boundaries of the last call: start[c], end[c] for c < max. */
 int32_t ltl_levels(const ltl_ctx *ctx, int64_t *start, int64_t *end, int32_t max);
 
+/* ---- Per-launch log -------------------------------------------------------
+ * The reference prints a line per kernel launch (code/ltli6463.cu:1096,
+ * 1225, 1259, 1293, 1325, 1357, 1389, 1423):
+ *   Cost N | (op) | AllLTLs: a | StoredLTLs: s | ToBeChecked: n
+ * With a callback set, ltl_learn and ltl_learn_noalloc call it on the calling
+ * thread once for the atoms and once per block of candidates (one operator
+ * over one operand level or pair of levels: the reference launches such a
+ * block in chunks of its temporary capacity), in enumeration order, before
+ * the block's kernels are queued.  ltl_learn_batch logs nothing. */
+typedef struct {
+    int32_t  cost;       /* the level's cost */
+    int32_t  op;         /* LTL_* operator; LTL_ATOM for the atoms */
+    int32_t  pass;       /* 1: generation (tested, deduplicated, stored);
+                            0: a check pass (tested only) */
+    int32_t  launched;   /* 0: registered without a launch (tested ahead of
+                            the level, or in the generation of the level below) */
+    uint64_t all_ltls;   /* canonical id of the block's first candidate (AllLTLs) */
+    int64_t  stored;     /* sets stored when the level started (StoredLTLs) */
+    uint64_t to_check;   /* candidates of the block (ToBeChecked) */
+} ltl_launch_record;
+typedef void (*ltl_launch_fn)(const ltl_launch_record *rec, void *user);
+/* Set (fn not NULL) or clear the per-launch callback.  Returns 0 or
+ * LTL_E_ARG. */
+int32_t ltl_set_launch_log(ltl_ctx *ctx, ltl_launch_fn fn, void *user);
+
 /* ---- Divide and conquer --------------------------------------------------
  * Replaces the reference's Python drivers DetSplit / RandSplit
  * (code/dc.py:188-249, :252-321) that call the CLI once per window: the

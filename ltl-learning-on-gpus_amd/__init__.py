@@ -90,6 +90,15 @@ class BatchStats(ctypes.Structure):
                 ("deferred", ctypes.c_int32), ("host", ctypes.c_int32)]
 
 
+class LaunchRecord(ctypes.Structure):
+    _fields_ = [("cost", ctypes.c_int32), ("op", ctypes.c_int32), ("pass_", ctypes.c_int32),
+                ("launched", ctypes.c_int32), ("all_ltls", ctypes.c_uint64), ("stored", ctypes.c_int64),
+                ("to_check", ctypes.c_uint64)]
+
+
+LAUNCH_FN = ctypes.CFUNCTYPE(None, ctypes.POINTER(LaunchRecord), ctypes.c_void_p)
+
+
 class GenInstance(ctypes.Structure):
     _fields_ = [("num_pos", ctypes.c_int32), ("num_neg", ctypes.c_int32),
                 ("num_atoms", ctypes.c_int32), ("max_cost", ctypes.c_int32),
@@ -107,7 +116,7 @@ EXPORTS = ["ltl_default_options", "ltl_create", "ltl_set_options", "ltl_learn", 
            "ltl_scorer_create", "ltl_scorer_destroy", "ltl_scorer_get_params", "ltl_scorer_set_params",
            "ltl_scorer_forward", "ltl_scorer_grad", "ltl_scorer_train", "ltl_set_scorer", "ltl_set_log",
            "ltl_get_log", "ltl_learn_dc", "ltl_get_options", "ltl_learn_noalloc", "ltl_read_trace_file_names",
-           "ltl_learn_dc_batch", "ltl_match"]
+           "ltl_learn_dc_batch", "ltl_match", "ltl_set_launch_log"]
 
 DC_DET, DC_RAND = 0, 1
 
@@ -181,6 +190,7 @@ def lib():
     L.ltl_match.argtypes = [ctypes.c_void_p, P(Problem), P(ctypes.c_char_p), ctypes.c_int32,
                             P(ctypes.c_uint8)]
     L.ltl_get_options.argtypes = [ctypes.c_void_p, P(Options)]
+    L.ltl_set_launch_log.argtypes = [ctypes.c_void_p, LAUNCH_FN, ctypes.c_void_p]
     _lib = L
     return L
 
@@ -463,6 +473,26 @@ class Learner:
         if rc != 0:
             raise LtlError(f"ltl_set_scorer -> {rc}")
         self._scorer = scorer  # keep it alive while attached
+
+    def launch_log(self, enable=True):
+        """Per-launch log (ltl_set_launch_log; the reference's "Cost N | (op)
+        | AllLTLs | StoredLTLs | ToBeChecked" lines): while enabled, the
+        records of every learn call are appended to self.launches (dicts:
+        cost, op, pass (1 generation, 0 test only), launched, all_ltls,
+        stored, to_check)."""
+        self.launches = []
+        if enable:
+            def cb(rec, _user):
+                r = rec.contents
+                self.launches.append(dict(cost=r.cost, op=r.op, pass_=r.pass_, launched=r.launched,
+                                          all_ltls=r.all_ltls, stored=r.stored, to_check=r.to_check))
+            self._launch_cb = LAUNCH_FN(cb)
+            rc = lib().ltl_set_launch_log(self._ctx, self._launch_cb, None)
+        else:
+            rc = lib().ltl_set_launch_log(self._ctx, LAUNCH_FN(), None)
+            self._launch_cb = None
+        if rc != 0:
+            raise LtlError(f"ltl_set_launch_log -> {rc}")
 
     def set_log(self, max_rows, per_block=64):
         """Log up to per_block tile-pair rows per binary block of every level."""

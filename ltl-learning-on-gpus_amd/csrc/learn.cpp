@@ -174,6 +174,9 @@ struct ltl_ctx {
     int32_t *d_tree = nullptr;    // the found formula's walk (1 + 3 * kTreeCap), in the call's memory
     int64_t log_cap = 0, log_n = 0;
     int32_t log_per_block = 0;
+    // per-launch log (ltl_set_launch_log)
+    ltl_launch_fn launch_fn = nullptr;
+    void *launch_user = nullptr;
 };
 
 static int32_t hip_fail(hipError_t e) { return e == hipErrorOutOfMemory ? LTL_E_OOM : LTL_E_HIP; }
@@ -752,6 +755,18 @@ struct Run {
     int64_t defer_s = -1, defer_e = -1;
     int succ_mask = 0;
     std::vector<std::pair<uint64_t, int>> succ_blocks;
+    int64_t level_stored = 0;  // sets stored when the current level started (launch log)
+
+    // A block of candidates, in enumeration order; `launched`: its kernels
+    // are queued in this pass.
+    void reg(const Block &b, bool launched = true) {
+        blocks.push_back(b);
+        if (c->launch_fn) {
+            const ltl_launch_record rec{b.level,     b.op,         check_only ? 0 : 1, launched ? 1 : 0,
+                                        b.id_base, level_stored, b.size};
+            c->launch_fn(&rec, c->launch_user);
+        }
+    }
 
     // Check-only pass over one block: no temporaries, so no chunking beyond
     // keeping the grid within limits.
@@ -883,7 +898,7 @@ struct Run {
             return r;
         }
         const int ops[3] = {LTL_NEXT, LTL_FIN, LTL_GLOB};
-        for (int j = 0; j < 3; ++j) blocks.push_back({all_ltls + (uint64_t)j * n, (uint64_t)n, ops[j], s, n, 0, 1, cur_level});
+        for (int j = 0; j < 3; ++j) reg({all_ltls + (uint64_t)j * n, (uint64_t)n, ops[j], s, n, 0, 1, cur_level});
         ChunkArgs a = base;
         a.id_base = all_ltls;
         a.ls = s; a.rs = 0; a.nR = 1;
@@ -951,7 +966,7 @@ struct Run {
     int32_t unary_nxfg(int64_t s, int64_t e, uint64_t id_xfg) {
         const int64_t n = e - s;
         const int64_t tiles = (n + 63 + 63) / 64, slots = tiles * 64;
-        blocks.push_back({all_ltls, (uint64_t)n, LTL_NOT, s, n, 0, 1, cur_level});
+        reg({all_ltls, (uint64_t)n, LTL_NOT, s, n, 0, 1, cur_level});
         ChunkArgs a = base;
         a.id_base = all_ltls;
         a.ls = s; a.rs = 0; a.nR = 1;
@@ -985,7 +1000,7 @@ struct Run {
             return LTL_E_HIP;
         }
         const int ops[3] = {LTL_NEXT, LTL_FIN, LTL_GLOB};
-        for (int j = 0; j < 3; ++j) blocks.push_back({all_ltls + (uint64_t)j * n, (uint64_t)n, ops[j], s, n, 0, 1, cur_level});
+        for (int j = 0; j < 3; ++j) reg({all_ltls + (uint64_t)j * n, (uint64_t)n, ops[j], s, n, 0, 1, cur_level});
         for (int j = 0; j < 3; ++j) {
             ChunkArgs b = pend.a;
             b.id_base = all_ltls + (uint64_t)j * n;
@@ -1011,8 +1026,9 @@ struct Run {
             counted += (uint64_t)n;
             return 0;
         }
-        blocks.push_back({all_ltls, (uint64_t)n, op, s, n, 0, 1, cur_level});
-        if (check_only && s == defer_s && e == defer_e) {
+        const bool deferred_succ = check_only && s == defer_s && e == defer_e;
+        reg({all_ltls, (uint64_t)n, op, s, n, 0, 1, cur_level}, !deferred_succ);
+        if (deferred_succ) {
             const int j = op == LTL_NOT ? 0 : op == LTL_NEXT ? 1 : op == LTL_FIN ? 2 : 3;
             succ_blocks.push_back({all_ltls, j});
             checked += (uint64_t)n;
@@ -1044,7 +1060,7 @@ struct Run {
             counted += size;
             return 0;
         }
-        blocks.push_back({all_ltls, size, op, ls, nL, rs, nR, cur_level});
+        reg({all_ltls, size, op, ls, nL, rs, nR, cur_level}, !(check_only && reuse_binary));
         if (check_only && reuse_binary) {
             if (bin_seen == la_block) la_found = all_ltls + la_off;
             if (bin_seen == la_neg_block) la_neg = all_ltls + la_neg_off;
@@ -1414,6 +1430,10 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     ba.hamming_k = o.hamming_k;
     ba.tmask = c->tmask;
     HIP_TRY(launch_atoms(B, ba, A, s));
+    if (c->launch_fn) {
+        const ltl_launch_record rec{c1, LTL_ATOM, 1, 1, 0, 0, (uint64_t)A};
+        c->launch_fn(&rec, c->launch_user);
+    }
     run.all_ltls = all_ltls;
 
     const int nlev = std::max(maxc, c1) + 2;
@@ -1654,6 +1674,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     bool neg_answer = false;
     uint64_t neg_id = kEmpty;
     for (cost = c1 + 1; cost <= maxc; ++cost) {
+        run.level_stored = n_stored;
         if (run.otf) {
             const int dif = cost - shortage;  // code/ltli6463.cu:1202-1206
             if (dif == c2 || dif == c1 + c3 || dif == c1 + c4 || dif == c5 || dif == c6 || dif == c7 || dif == c1 + c8)
@@ -2377,6 +2398,13 @@ extern "C" int32_t ltl_set_scorer(ltl_ctx *c, ltl_scorer *sc) {
     if (!c) return LTL_E_ARG;
     if (sc && sc->device != c->opt.device) return LTL_E_ARG;
     c->scorer = sc;
+    return 0;
+}
+
+extern "C" int32_t ltl_set_launch_log(ltl_ctx *c, ltl_launch_fn fn, void *user) {
+    if (!c) return LTL_E_ARG;
+    c->launch_fn = fn;
+    c->launch_user = fn ? user : nullptr;
     return 0;
 }
 

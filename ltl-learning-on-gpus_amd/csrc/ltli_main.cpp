@@ -4,10 +4,16 @@
 // 1575-1681), so that drivers which parse its output (code/dc.py:23-43, 192-194)
 // work unchanged:
 //
-//   ltli [--reference-keys] <input_file> <c1> .. <c8> <maxCost> <RlxUnqChkTyp> <NegType>
+//   ltli [--reference-keys] [--launch-log] <input_file> <c1> .. <c8> <maxCost> <RlxUnqChkTyp> <NegType>
 //
 // stdout: the traces, "Cost Function: ...", "Number of Traces: N", the
-// run statistics, then a last line  LTL: "<formula>"  (or "not_found").
+// run statistics, then a last line  LTL: "<formula>"  (or "not_found"), as
+// the reference built with MEASUREMENT_MODE prints them.  --launch-log (or
+// LTLI_LAUNCH_LOG=1) first prints a line per block of candidates in the
+// format of the reference's default build (code/ltli6463.cu:1096, 1225):
+//   Cost N  | (op) | AllLTLs: a | StoredLTLs: s | ToBeChecked: n
+// with " | test" appended for the blocks of a check pass (tested, not stored)
+// and " | ahead" for blocks tested before their level.
 // Differences: errors exit with a nonzero status (the reference always exits
 // 0), traces may number more than 64 and hold up to 255 positions, an
 // alphabet line of comma-separated names instead of letters names up to 64
@@ -30,7 +36,7 @@
 static int usage(const char *argv0) {
     printf("Arguments should be in the form of\n");
     printf("-----------------------------------------------------------------\n");
-    printf("%s [--reference-keys] <input_file_address> <c1> <c2> <c3> <c4> <c5> <c6> <c7> <c8> <maxCost> "
+    printf("%s [--reference-keys] [--launch-log] <input_file_address> <c1> <c2> <c3> <c4> <c5> <c6> <c7> <c8> <maxCost> "
            "<RlxUnqChkTyp> <NegType>\n",
            argv0);
     printf("-----------------------------------------------------------------\n");
@@ -54,11 +60,25 @@ static std::string file_alphabet(const char *path) {
     return a;
 }
 
+static void print_launch(const ltl_launch_record *r, void *) {
+    static const char ops[] = "A~&|XFGU";
+    printf("Cost %-2d | (%c) | AllLTLs: %-11lu | StoredLTLs: %-10ld | ToBeChecked: %-10lu %s\n", r->cost,
+           r->op >= 0 && r->op < 8 ? ops[r->op] : '?', (unsigned long)r->all_ltls, (long)r->stored,
+           (unsigned long)r->to_check, r->pass ? "" : r->launched ? "| test" : "| ahead");
+}
+
+static bool env_flag(const char *name) { return getenv(name) && std::atoi(getenv(name)) != 0; }
+
 int main(int argc, char **argv) {
-    bool ref_keys = getenv("LTLI_REFERENCE_KEYS") && std::atoi(getenv("LTLI_REFERENCE_KEYS")) != 0;
+    bool ref_keys = env_flag("LTLI_REFERENCE_KEYS");
+    bool launch_log = env_flag("LTLI_LAUNCH_LOG");
     const char *prog = argv[0];
-    if (argc == 14 && std::string(argv[1]) == "--reference-keys") {
-        ref_keys = true;
+    while (argc > 13) {
+        const std::string a = argv[1];
+        if (a == "--reference-keys") ref_keys = true;
+        else if (a == "--launch-log") launch_log = true;
+        else break;
+        argv[1] = argv[0];
         ++argv;
         --argc;
     }
@@ -128,6 +148,7 @@ int main(int argc, char **argv) {
         printf("No usable GPU (error %d).\n", st);
         return 1;
     }
+    if (launch_log) ltl_set_launch_log(ctx, print_launch, nullptr);
     ltl_problem pb{P, N, A, K, lens.data(), atoms.data(), names.data()};
     ltl_result res;
     const auto t0 = std::chrono::high_resolution_clock::now();
