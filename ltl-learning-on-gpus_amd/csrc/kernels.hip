@@ -39,6 +39,11 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
 __device__ __forceinline__ bool run_stopped(const ChunkArgs &A) {
     return __hip_atomic_load(&A.st->found, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < A.id_end;
 }
+// A look-back wait of the dedup ran out: the survivor lists of the call are
+// not trusted (the call fails at its next synchronisation).
+__device__ __forceinline__ bool look_back_failed(const ChunkArgs &A) {
+    return __hip_atomic_load(&A.st->lb_fail, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
+}
 __device__ __forceinline__ bool chunk_otf(const ChunkArgs &A) {
     return __hip_atomic_load(&A.st->overflow_chunk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < A.chunk_idx;
 }
@@ -610,7 +615,7 @@ __device__ __forceinline__ bool id_of_temp(const ChunkArgs &A, int64_t k, uint64
 // every accepted candidate after it lowers `found` too.
 template <int OP>
 __global__ __launch_bounds__(256) void k_rx_tail(ChunkArgs A) {
-    if (run_stopped(A)) return;
+    if (run_stopped(A) || look_back_failed(A)) return;
     const unsigned long long rx = A.st->rx_found;
     if (blockIdx.x == 0 && threadIdx.x == 0 && rx != kEmpty) atomicMin(&A.st->found, rx);
     if (A.st->overflow_chunk != A.chunk_idx) return;
@@ -736,8 +741,9 @@ __global__ __launch_bounds__(1024) void k_dedup_small(ChunkArgs A) {
 // (decoupled look-back; every workgroup it waits for took its ticket earlier
 // and is running).  Status words carry the chunk's epoch, so words of earlier
 // chunks read as not yet published.  The waits are bounded: one that runs
-// out marks the status word poisoned and sets lb_fail, the chunk stores
-// nothing, and the call fails with a HIP error.  The last workgroup does the
+// out marks the status word poisoned and sets lb_fail; a poisoned workgroup
+// writes no survivor list entries, the kernels that read the list do nothing
+// once lb_fail is set, and the call fails with a HIP error.  The last workgroup does the
 // finish and returns the ticket counter to 0.  The status words are
 // relaxed agent-scope atomics (coherent across the XCDs' L2s): a count
 // travels inside its status word, and what else the workgroups write (t_list,
@@ -750,9 +756,10 @@ __global__ __launch_bounds__(256) void k_dedup_lb(ChunkArgs A) {
     if (run_stopped(A) || chunk_otf(A)) return;
     constexpr int ND = OP == LTL_UNTIL ? 2 : 1;
     __shared__ uint32_t s_cnt[4];
-    __shared__ uint32_t s_blk, s_excl;
+    __shared__ uint32_t s_blk, s_excl, s_poison;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const uint32_t nblk = (uint32_t)((A.n_tiles + 3) / 4);
+    if (threadIdx.x == 0) s_poison = 0;
     if (threadIdx.x == 0) s_blk = atomicAdd(A.lb_ticket, 1u);
     __syncthreads();
     const uint32_t blk = s_blk;
@@ -789,6 +796,7 @@ __global__ __launch_bounds__(256) void k_dedup_lb(ChunkArgs A) {
                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (poison) __hip_atomic_store(&A.st->lb_fail, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_excl = excl;
+        s_poison = poison ? 1u : 0u;
         if (blk == nblk - 1) {
             // every workgroup has taken its ticket
             __hip_atomic_store(A.lb_ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -807,7 +815,8 @@ __global__ __launch_bounds__(256) void k_dedup_lb(ChunkArgs A) {
         }
     }
     __syncthreads();
-    if (blk >= nblk || cnt == 0) return;
+    // (a poisoned workgroup's prefix is unknown: it writes no list entries)
+    if (blk >= nblk || cnt == 0 || s_poison) return;
     uint32_t before = s_excl;
     for (int w = 0; w < wave; ++w) before += s_cnt[w];
     const uint64_t below = lanes_below(), bit = 1ull << lane;
@@ -840,7 +849,7 @@ __global__ __launch_bounds__(256) void k_dedup_lb(ChunkArgs A) {
 template <int OP>
 __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__restrict__ aux,
                                               const SumAux *__restrict__ saux) {
-    if (run_stopped(A) || chunk_otf(A)) return;
+    if (run_stopped(A) || chunk_otf(A) || look_back_failed(A)) return;
     // Waves cover whole store tiles: lane l of the waves of tile v writes
     // position 64 * (chunk_base / 64 + v) + l, so every store instruction of a
     // wave fills one 512-byte segment of one tile.
@@ -945,7 +954,7 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
 // words, no summaries), and each survivor whose successor j was accepted in
 // generation lowers succ_min[j] to its store index.
 __global__ __launch_bounds__(256) void k_emit_meta(ChunkArgs A, int op) {
-    if (run_stopped(A) || chunk_otf(A)) return;
+    if (run_stopped(A) || chunk_otf(A) || look_back_failed(A)) return;
     const int64_t base = A.st->chunk_base;
     const int64_t pos = (base & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t i = pos - base;
@@ -1062,7 +1071,7 @@ __global__ __launch_bounds__(256) void k_materialize_m(ChunkArgs A, int64_t s0, 
 // come from all K words.
 template <int OP, int K>
 __global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
-    if (run_stopped(A) || chunk_otf(A)) return;
+    if (run_stopped(A) || chunk_otf(A) || look_back_failed(A)) return;
     const int64_t base = A.st->chunk_base;
     const int64_t pos = (base & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t rel = pos - base;

@@ -36,13 +36,15 @@ using namespace ltl;
 
 namespace {
 
+// (the dedup_path_* entries count the chunks each dedup path took, with or
+// without profiling, and hold no time: that is in decide and scan_finish)
 enum { K_GEN1, K_GEN2, K_GENU, K_DECIDE, K_SCAN, K_EMIT1, K_EMIT2, K_EMITU, K_CHK1, K_CHK2, K_CHKU,
-       K_TSTAT, K_SCORE, K_SORT, K_CHKR, K_LOG, K_EMITM, K_NKIND };
+       K_TSTAT, K_SCORE, K_SORT, K_CHKR, K_LOG, K_EMITM, K_PSMALL, K_PLB, K_PFOUR, K_NKIND };
 const char *kKindName[K_NKIND] = {"gen_unary", "gen_and_or", "gen_until", "decide",
                                   "scan_finish", "emit_unary", "emit_and_or", "emit_until",
                                   "check_unary", "check_and_or", "check_until",
                                   "tile_stats", "score_tiles", "sort_tiles", "check_ranked", "log_rows",
-                                  "emit_meta"};
+                                  "emit_meta", "dedup_path_small", "dedup_path_lookback", "dedup_path_four"};
 
 struct Block {
     uint64_t id_base, size;
@@ -844,11 +846,17 @@ struct Run {
         if (!scan_path && a.n_tiles <= kSmallTiles) {
             // (one workgroup: the launches, not the work, would set the time)
             HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_dedup_small(op, a, c->stream); }));
+            c->stats[K_PSMALL].launches++;
+            c->stats[K_PSMALL].items += cand;
         } else if (!scan_path && a.n_tiles <= kLbMaxTiles && c->lb_epoch + 1 < (1u << 30)) {
             ChunkArgs b = a;
             b.lb_epoch = ++c->lb_epoch;
             HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_dedup_lb(op, b, c->stream); }));
+            c->stats[K_PLB].launches++;
+            c->stats[K_PLB].items += cand;
         } else {
+            c->stats[K_PFOUR].launches++;
+            c->stats[K_PFOUR].items += cand;
             HIP_TRY(timed(c, K_DECIDE, cand * 28, cand * 28, cand, [&] { return launch_decide(op, a, c->stream); }));
             HIP_TRY(timed(c, K_SCAN, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles * 8, (uint64_t)a.n_tiles, [&] {
                 hipError_t e = launch_scan(a.t_cnt, a.t_off, a.n_tiles, c->d_scan, c->scan_bytes, c->stream);
@@ -1454,7 +1462,10 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         drain_events(c);
         if (h.collision) return LTL_E_COLLISION;
         if (h.probe_fail) return LTL_E_OOM;
-        if (h.lb_fail) return LTL_E_HIP;
+        if (h.lb_fail) {
+            report_hip(hipErrorUnknown, "dedup look-back (a wait ran out)", __LINE__);
+            return LTL_E_HIP;
+        }
         return 0;
     };
     // The blocks of level `cost` in the reference's order

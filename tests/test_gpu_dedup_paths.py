@@ -13,30 +13,33 @@ from test_gpu_parity import check_parity
 pytestmark = pytest.mark.gpu
 
 # (seed, P, N, atoms, len_min, len_max, max_cost); the oracle's stores: about
-# 4.8 K, 0.7 K and 1.1 K sets for SMALL, 63 K and 355 K for the two larger ones
-# (no formula up to their max cost; blocks of up to 2^18 candidates)
+# 4.8 K, 0.7 K and 1.1 K sets for SMALL, 63 K for MID, 812 K for LARGE (no
+# formula up to their max cost; LARGE's level 8 has a block of 5 x 17186
+# pairs, 1350 tiles in one chunk of the large pool)
 SMALL = [(31, 20, 13, 3, 9, 16, 6), (18, 7, 9, 2, 2, 8, 7), (43, 30, 30, 3, 17, 32, 5)]
 MID = [(75, 64, 64, 4, 5, 40, 7)]
-LARGE = [(75, 64, 64, 4, 5, 40, 8)]
+LARGE = [(79, 64, 64, 5, 5, 40, 8)]
 
 
-@pytest.mark.parametrize("budget_mib,shapes,four_kernel",
-                         [(4, SMALL, False), (64, SMALL + MID, False), (2048, SMALL + MID + LARGE, True)],
+@pytest.mark.parametrize("budget_mib,shapes,paths",
+                         [(4, SMALL, "small"), (64, SMALL + MID, "small+lookback"),
+                          (2048, SMALL + MID + LARGE, "small+lookback+four")],
                          ids=["pool4M", "pool64M", "pool2G"])
-def test_dedup_paths_equal_oracle(ltl, oracle, budget_mib, shapes, four_kernel):
+def test_dedup_paths_equal_oracle(ltl, oracle, budget_mib, shapes, paths):
     L = ltl.Learner(budget=budget_mib << 20)
     try:
-        scan_launches = 0
+        count = {"small": 0, "lookback": 0, "four": 0}
         for seed, P, N, A, lo, hi, mc in shapes:
             inst = ltl.generate_custom(seed, P, N, A, lo, hi, 0.5, planted_size=9)
             gpu, _ = check_parity(ltl, oracle, L, inst, max_cost=mc, profile=1)
             assert not gpu["on_the_fly"]
-            scan_launches += L.kernel_stats()["scan_finish"]["launches"]
-        # the four-kernel path runs only where a chunk exceeds 1024 tiles:
-        # never with these pools' chunks (with the large pool the blocks here
-        # stay below that; the configuration-sized instances of the other
-        # tests take it)
-        if not four_kernel:
-            assert scan_launches == 0, scan_launches
+            ks = L.kernel_stats()
+            for k in count:
+                count[k] += ks["dedup_path_" + k]["launches"]
+            if count["four"] == 0:
+                assert ks["scan_finish"]["launches"] == 0
+        # every path the pool allows is taken, and no other
+        want = set(paths.split("+"))
+        assert {k for k, v in count.items() if v > 0} == want, count
     finally:
         L.close()
