@@ -92,8 +92,11 @@ typedef struct {
     uint64_t hbm_budget_bytes; /* 0 = 75% of free device memory at the first call */
     int32_t  device;           /* HIP device ordinal */
     int32_t  profile;          /* 1 = time every kernel with HIP events, 2 = the generation
-                                  and emit kernels only (fewer dispatch barriers) */
-    int32_t  reserved[6];
+                                  and emit kernels only (fewer dispatch barriers), 3 = the
+                                  kinds in profile_kinds only; each timed launch costs a few
+                                  microseconds of dispatch barrier */
+    uint32_t profile_kinds;    /* profile 3: bit k = kernel kind k of ltl_get_kernel_stats */
+    int32_t  reserved[5];
 } ltl_options;
 
 /* One node of the returned formula, rebuilt on the host from parent indices

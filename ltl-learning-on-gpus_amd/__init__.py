@@ -47,7 +47,7 @@ class Options(ctypes.Structure):
                 ("hash_slots_log2", ctypes.c_int32), ("precheck", ctypes.c_int32),
                 ("group_lanes", ctypes.c_int32), ("capacity", ctypes.c_int64),
                 ("hbm_budget_bytes", ctypes.c_uint64), ("device", ctypes.c_int32),
-                ("profile", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+                ("profile", ctypes.c_int32), ("profile_kinds", ctypes.c_uint32), ("reserved", ctypes.c_int32 * 5)]
 
 
 class Node(ctypes.Structure):
@@ -293,7 +293,8 @@ def mismatches(inst, cs):
 
 
 def make_options(cost=(1,) * 8, max_cost=10, neg_type=1, hamming_k=0, order=ORDER_PLAIN,
-                 hash_slots_log2=0, precheck=1, capacity=0, budget=0, device=0, profile=0, dedup_mode=0):
+                 hash_slots_log2=0, precheck=1, capacity=0, budget=0, device=0, profile=0, dedup_mode=0,
+                 profile_kinds=0):
     o = Options()
     lib().ltl_default_options(ctypes.byref(o))
     for i, c in enumerate(cost):
@@ -302,6 +303,7 @@ def make_options(cost=(1,) * 8, max_cost=10, neg_type=1, hamming_k=0, order=ORDE
     o.dedup_mode = int(dedup_mode)
     o.hash_slots_log2, o.precheck, o.capacity = int(hash_slots_log2), int(precheck), int(capacity)
     o.hbm_budget_bytes, o.device, o.profile = int(budget), int(device), int(profile)
+    o.profile_kinds = int(profile_kinds)
     return o
 
 

@@ -18,6 +18,9 @@
 namespace ltl {
 namespace {
 
+// (every instance leaves its region's table empty: one table epoch)
+constexpr uint32_t kBatchEpoch = 1;
+
 template <int T>
 struct Shared {
     unsigned long long found;
@@ -208,7 +211,7 @@ __device__ void emit_one(const Inst &X, int64_t pos, int64_t sl, int64_t sr, boo
     X.m_right[pos] = bin ? (int32_t)(swap ? sl : sr) : -1;
     X.m_fp2[pos] = f2;
     X.m_slot[pos] = slot;
-    X.tab[2 * (size_t)slot + 1] = (uint64_t)pos;
+    X.tab[2 * (size_t)slot + 1] = tab_val((uint64_t)pos, kBatchEpoch);
 }
 
 // Generate one block; returns true if the instance must stop (deferred or
@@ -232,11 +235,13 @@ __device__ bool gen_block(const Inst &X, Shared<T> &S, int ls, int le, int rs, i
         if (valid) {
             Acc c0, c1;
             fingerprint<OP, UB>(X, sl, sr, c0, c1);
-            slot0 = tab_insert(X.tab, X.tmask, table_key(c0.f1), kPending | id0, &S.pfail);
+            slot0 = tab_insert(X.tab, X.tmask, tab_key(c0.f1, kBatchEpoch), tab_val(kPending | id0, kBatchEpoch),
+                               kBatchEpoch, &S.pfail);
             f20 = c0.f2;
             S.fp2[2 * threadIdx.x] = f20;
             if constexpr (ND == 2) {
-                slot1 = tab_insert(X.tab, X.tmask, table_key(c1.f1), kPending | (id0 + 1), &S.pfail);
+                slot1 = tab_insert(X.tab, X.tmask, tab_key(c1.f1, kBatchEpoch),
+                                   tab_val(kPending | (id0 + 1), kBatchEpoch), kBatchEpoch, &S.pfail);
                 f21 = c1.f2;
                 S.fp2[2 * threadIdx.x + 1] = f21;
             }
@@ -247,11 +252,11 @@ __device__ bool gen_block(const Inst &X, Shared<T> &S, int ls, int le, int rs, i
             const uint64_t first = base_id + (uint64_t)(ND * b0);
             auto decide = [&](uint32_t slot, uint64_t id, uint64_t f2) -> bool {
                 const uint64_t v = aload(X.tab + 2 * (size_t)slot + 1);
-                if (v == (kPending | id)) return true;
-                if (v < kPending) {
-                    if (X.m_fp2[v] != f2) S.coll = 1;
+                if (v == tab_val(kPending | id, kBatchEpoch)) return true;
+                if (tab_committed(v, kBatchEpoch)) {
+                    if (X.m_fp2[tab_payload(v)] != f2) S.coll = 1;
                 } else {
-                    const uint64_t rel = (v & ~kPending) - first;
+                    const uint64_t rel = tab_payload(v) - first;
                     const uint64_t k = ND == 2 ? rel : 2 * rel;   // index into S.fp2
                     if (rel >= (uint64_t)(ND * T) || S.fp2[k] != f2) S.coll = 1;
                 }
@@ -335,7 +340,8 @@ __device__ void run_instance(const BatchArgs &G, const BatchInst &I, BatchOut &O
         const int a = threadIdx.x;
         Acc c0;
         for (int w = 0; w < X.nw; ++w) fold(c0, X.store[(size_t)w * X.ws + a], X.aux + w);
-        X.m_slot[a] = tab_insert(X.tab, X.tmask, table_key(c0.f1), (uint64_t)a, &S.pfail);
+        X.m_slot[a] = tab_insert(X.tab, X.tmask, tab_key(c0.f1, kBatchEpoch), tab_val((uint64_t)a, kBatchEpoch),
+                                 kBatchEpoch, &S.pfail);
         X.m_fp2[a] = c0.f2;
         X.m_op[a] = LTL_ATOM;
         X.m_left[a] = X.m_right[a] = -1;

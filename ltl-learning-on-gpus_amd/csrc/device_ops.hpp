@@ -114,15 +114,35 @@ __device__ __forceinline__ uint64_t table_key(uint64_t f) {
     return f == kEmpty ? kEmpty - 2 : f;
 }
 
-// Linear probing; key CAS from empty, then atomicMin of the value.
-__device__ inline uint32_t tab_insert(uint64_t *tab, uint32_t tmask, uint64_t key, uint64_t val, int *probe_fail) {
+// Table epochs: a table of fixed size is cleared once and then serves up
+// to kEpochs calls.  A call's epoch e (1..kEpochs) is the top 4 bits of
+// every key it inserts, so a key of another epoch (or kEmpty, whose top bits
+// are 15) marks a free slot.  Values carry 15 - e in their top 4 bits: a
+// value of the current epoch is smaller than any earlier epoch's, so the
+// first insert's atomicMin replaces a stale value; below them, kPending
+// (bit 59) marks a candidate id, else the value is a store index.
+constexpr uint64_t kKeyBody = (1ull << 60) - 1;
+__device__ __forceinline__ uint64_t tab_key(uint64_t f1, uint32_t e) {
+    return (table_key(f1) & kKeyBody) | ((uint64_t)e << 60);
+}
+__device__ __forceinline__ uint64_t tab_val(uint64_t v, uint32_t e) { return ((uint64_t)(15u - e) << 60) | v; }
+// a store index of epoch e (not a candidate id)
+__device__ __forceinline__ bool tab_committed(uint64_t v, uint32_t e) { return (v & ~(kPending - 1)) == tab_val(0, e); }
+__device__ __forceinline__ uint64_t tab_payload(uint64_t v) { return v & (kPending - 1); }
+
+// Linear probing; key CAS over a free slot (another epoch's key), then
+// atomicMin of the value.  key and val carry the epoch e (tab_key, tab_val).
+__device__ inline uint32_t tab_insert(uint64_t *tab, uint32_t tmask, uint64_t key, uint64_t val, uint32_t e,
+                                      int *probe_fail) {
     uint32_t p = (uint32_t)key & tmask;
     for (uint32_t n = 0; n <= tmask; ++n) {
         unsigned long long *kp = reinterpret_cast<unsigned long long *>(tab + 2 * (size_t)p);
         unsigned long long k = __hip_atomic_load(kp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (k == kEmpty) {
-            k = atomicCAS(kp, (unsigned long long)kEmpty, (unsigned long long)key);
-            if (k == kEmpty) k = key;
+        // (within an epoch a slot only goes from free to a key of the epoch:
+        // a failed CAS has seen such a key, so this runs at most twice)
+        while ((uint32_t)(k >> 60) != e) {
+            const unsigned long long seen = atomicCAS(kp, k, (unsigned long long)key);
+            k = seen == k ? (unsigned long long)key : seen;
         }
         if (k == key) {
             // Values only decrease; a read that is already <= val makes the

@@ -141,16 +141,17 @@ __device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, con
         }
     }
     if constexpr (OP == LTL_UNTIL) {
-        A.t_slot[2 * ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
+        A.t_slot[2 * ix] = tab_insert(A.tab, A.tmask, tab_key(c0.f1, A.epoch), tab_val(kPending | id0, A.epoch), A.epoch, &A.st->probe_fail);
         A.t_fp2[2 * ix] = c0.f2;
-        A.t_slot[2 * ix + 1] = tab_insert(A.tab, A.tmask, table_key(c1.f1), kPending | (id0 + 1), &A.st->probe_fail);
+        A.t_slot[2 * ix + 1] = tab_insert(A.tab, A.tmask, tab_key(c1.f1, A.epoch), tab_val(kPending | (id0 + 1), A.epoch), A.epoch,
+                                          &A.st->probe_fail);
         A.t_fp2[2 * ix + 1] = c1.f2;
         if constexpr (SUCC) {
             A.t_succ[2 * ix] = succ_flags(A, s0);
             A.t_succ[2 * ix + 1] = succ_flags(A, s1);
         }
     } else {
-        A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
+        A.t_slot[ix] = tab_insert(A.tab, A.tmask, tab_key(c0.f1, A.epoch), tab_val(kPending | id0, A.epoch), A.epoch, &A.st->probe_fail);
         A.t_fp2[ix] = c0.f2;
         if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, s0);
     }
@@ -165,7 +166,7 @@ __device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, con
     if (otf) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
     if constexpr (RX) A.t_acc[ix] = (uint8_t)acc0;
-    A.t_slot[ix] = tab_insert(A.tab, A.tmask, table_key(c0.f1), kPending | id0, &A.st->probe_fail);
+    A.t_slot[ix] = tab_insert(A.tab, A.tmask, tab_key(c0.f1, A.epoch), tab_val(kPending | id0, A.epoch), A.epoch, &A.st->probe_fail);
     A.t_fp2[ix] = c0.f2;
     if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, s0);
 }
@@ -339,7 +340,8 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
         if (CHK && c[j].mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id);
         if (otf) continue;
         const size_t k = (size_t)j * (size_t)slots + ix;
-        A.t_slot[k] = tab_insert(A.tab, A.tmask, table_key(c[j].f1), kPending | id, &A.st->probe_fail);
+        A.t_slot[k] = tab_insert(A.tab, A.tmask, tab_key(c[j].f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
+                                &A.st->probe_fail);
         A.t_fp2[k] = c[j].f2;
         if constexpr (SUCC) A.t_succ[k] = succ_flags(A, sc[j]);
     }
@@ -413,7 +415,8 @@ __global__ __launch_bounds__(256) void k_gen_xfgm(ChunkArgs A, int64_t n, int64_
         if (CHK && c[j].mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id);
         if (otf) continue;
         const size_t k = (size_t)j * (size_t)slots + ix;
-        A.t_slot[k] = tab_insert(A.tab, A.tmask, table_key(c[j].f1), kPending | id, &A.st->probe_fail);
+        A.t_slot[k] = tab_insert(A.tab, A.tmask, tab_key(c[j].f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
+                                &A.st->probe_fail);
         A.t_fp2[k] = c[j].f2;
     }
 }
@@ -471,8 +474,8 @@ __global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms) {
     }
     if (A.kmode) key_finish(c0, k0, A.kseed);
     if (l != 0) return;
-    const uint64_t key = table_key(c0.f1);
-    tab_insert(A.tab, A.tmask, key, (uint64_t)a, &A.st->probe_fail);
+    const uint64_t key = tab_key(c0.f1, A.epoch);
+    tab_insert(A.tab, A.tmask, key, tab_val((uint64_t)a, A.epoch), A.epoch, &A.st->probe_fail);
     A.m_key[a] = key;
     A.m_fp2[a] = c0.f2;
     A.m_op[a] = LTL_ATOM;
@@ -540,14 +543,14 @@ __device__ __forceinline__ uint32_t decide_tile(const ChunkArgs &A, int64_t ti, 
             const uint64_t f2 = A.t_fp2[k];
             const uint64_t v = A.tab[2 * (size_t)slot + 1];
             const uint64_t id = id0 + d;
-            if (v == (kPending | id)) {
+            if (v == tab_val(kPending | id, A.epoch)) {
                 surv = true;
                 // relaxed keys: the survivor's own test (set in generation)
                 if (A.t_acc && A.t_acc[k]) atomicMin(&A.st->rx_found, (unsigned long long)id);
-            } else if (v < kPending) {
-                if (A.m_fp2[v] != f2) A.st->collision = 1;
+            } else if (tab_committed(v, A.epoch)) {
+                if (A.m_fp2[tab_payload(v)] != f2) A.st->collision = 1;
             } else {
-                const int64_t o = temp_index_of(A, OP, v & ~kPending);
+                const int64_t o = temp_index_of(A, OP, tab_payload(v));
                 if (o < 0 || A.t_fp2[o] != f2) A.st->collision = 1;
             }
         }
@@ -648,7 +651,7 @@ __device__ __forceinline__ void commit(const ChunkArgs &A, int64_t pos, size_t k
     // a table of fixed size never grows, and the read would pull a random
     // table line per survivor)
     if (A.keep_keys) A.m_key[pos] = A.tab[2 * (size_t)A.t_slot[k]];
-    A.tab[2 * (size_t)A.t_slot[k] + 1] = (uint64_t)pos;
+    A.tab[2 * (size_t)A.t_slot[k] + 1] = tab_val((uint64_t)pos, A.epoch);
 }
 
 template <int OP>
@@ -1260,9 +1263,10 @@ __global__ __launch_bounds__(256) void k_chkUm(ChunkArgs A) {
     else if (anti && seen - m1 <= k) atomicMin(&A.st->neg_found[A.neg_slot], (unsigned long long)(id0 + 1));
 }
 
-__global__ void k_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, int *probe_fail) {
+// (the stored keys carry the call's epoch)
+__global__ void k_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, uint32_t e, int *probe_fail) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        tab_insert(tab, tmask, keys[i], (uint64_t)i, probe_fail);
+        tab_insert(tab, tmask, keys[i], tab_val((uint64_t)i, e), e, probe_fail);
 }
 
 __global__ void k_fill(uint64_t *p, uint64_t v, size_t n) {
@@ -1628,11 +1632,11 @@ hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, int *probe_fail,
-                         hipStream_t s) {
+hipError_t launch_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, uint32_t epoch,
+                         int *probe_fail, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 16384);
-    hipLaunchKernelGGL(k_rehash, dim3(blocks), dim3(256), 0, s, tab, tmask, keys, n, probe_fail);
+    hipLaunchKernelGGL(k_rehash, dim3(blocks), dim3(256), 0, s, tab, tmask, keys, n, epoch, probe_fail);
     return hipGetLastError();
 }
 

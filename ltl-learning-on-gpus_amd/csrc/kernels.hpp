@@ -35,7 +35,8 @@ __host__ __device__ __forceinline__ size_t set_base(int64_t s, int words) {
 // Word stride inside a tile (the layout of set_base): word w of set s at
 // set_base(s, words) + w * kTileWS.  ChunkArgs::ws holds the same value.
 constexpr int64_t kTileWS = 64;
-constexpr uint64_t kPending = 1ull << 62;   // table value tag: candidate id
+constexpr uint64_t kPending = 1ull << 59;   // table value: a candidate id (else a store index)
+constexpr uint32_t kEpochs = 14;            // table epochs (1..14) between two clears (tab_key)
 
 // Per packed word: dedup hash keys and check masks (wave-uniform).
 struct WordAux {
@@ -147,6 +148,7 @@ struct ChunkArgs {
     // each block's end, so that the blocks before an accepted one still
     // store their survivors, as they would one after the other).
     uint64_t id_end = kEmpty;
+    uint32_t epoch = 1;       // the table's epoch of this call (tab_key, tab_val)
     // Ranked check round submitted ahead of a sync: its workgroups do no
     // work once an accepted candidate with an id below found_below (a lower
     // level's) is known when they start (0: always work)
@@ -246,9 +248,10 @@ hipError_t launch_tree(const uint8_t *op, const int32_t *left, const int32_t *ri
 // canonical id of every accepted candidate.
 hipError_t launch_check(int op, int B, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_fill_u64(uint64_t *p, uint64_t v, size_t n, hipStream_t s);
-// Insert (keys[s], s) for s < n into an emptied table of tmask + 1 slots.
-hipError_t launch_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, int *probe_fail,
-                         hipStream_t s);
+// Insert (keys[s], s) for s < n into an emptied table of tmask + 1 slots
+// (keys of table epoch `epoch`).
+hipError_t launch_rehash(uint64_t *tab, uint32_t tmask, const uint64_t *keys, int64_t n, uint32_t epoch,
+                         int *probe_fail, hipStream_t s);
 // The device address of the static scratch region of ltl_learn_noalloc on
 // `device` (which must be current): looked up once per device and kept.
 hipError_t noalloc_scratch(int device, char **ptr);

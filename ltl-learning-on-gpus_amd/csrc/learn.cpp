@@ -94,6 +94,13 @@ struct ltl_ctx {
     uint64_t clean_region = 0;
     uint32_t clean_tslots = 0;
     int clean_n = 0;
+    // A fixed-size dedup table left by the last call (tab_ok: the call ended
+    // without error and nothing has written the pool since): its address,
+    // slots - 1 and epoch.  The next call with the same table takes the next
+    // epoch instead of clearing it (kEpochs calls per clear).
+    bool tab_ok = false;
+    uint64_t *tab_ptr = nullptr;
+    uint32_t tab_mask = 0, tab_epoch = 0;
     // Device memory of the current call: the pool, or the library's static
     // scratch region of this device (ltl_learn_noalloc; its address from
     // set-up, noalloc_scratch()).
@@ -221,6 +228,7 @@ static int32_t check_options(const ltl_options *o) {
     if (o->hamming_k < 0) return LTL_E_ARG;
     if (o->hash_slots_log2 < 0 || o->hash_slots_log2 > 31) return LTL_E_ARG;
     if (o->order != LTL_ORDER_PLAIN && o->order != LTL_ORDER_LEARNED) return LTL_E_ARG;
+    if (o->profile < 0 || o->profile > 3) return LTL_E_ARG;
     return 0;
 }
 
@@ -318,6 +326,7 @@ static int32_t ensure_pool(ltl_ctx *c) {
     if (c->pool && c->pool_bytes >= want) return 0;
     if (c->pool) { hipFree(c->pool); c->pool = nullptr; c->pool_bytes = 0; }
     c->clean_n = 0;
+    c->tab_ok = false;
     HIP_TRY(hipMalloc((void **)&c->pool, want));
     c->pool_bytes = want;
     return 0;
@@ -475,12 +484,23 @@ static bool streaming_kind(int kind) {
            kind == K_EMITU || kind == K_EMITM;
 }
 
-// Launch, bracketed by HIP events on the library's stream when profiling:
-// profile = 1 every kernel, 2 the streaming kernels only (each timed launch
-// costs a dispatch barrier: about 8 us per launch at config 2).
+// Kernel kinds timed with events: profile = 1 every kind, 2 the streaming
+// kinds, 3 the kinds of profile_kinds.
+static bool profiled(const ltl_ctx *c, int kind) {
+    switch (c->opt.profile) {
+    case 1: return true;
+    case 2: return streaming_kind(kind);
+    case 3: return kind < 32 && ((c->opt.profile_kinds >> kind) & 1u);
+    default: return false;
+    }
+}
+
+// Launch, bracketed by HIP events on the library's stream when its kind is
+// profiled (each timed launch costs a dispatch barrier: about 5 us at
+// config 2, 1.4 ms per call with every streaming launch timed).
 template <class F>
 static hipError_t timed(ltl_ctx *c, int kind, uint64_t alg, uint64_t mn, uint64_t items, F &&launch) {
-    if (!c->opt.profile || (c->opt.profile == 2 && !streaming_kind(kind))) return launch();
+    if (!profiled(c, kind)) return launch();
     PendingEvent pe{kind, get_event(c), get_event(c), alg, mn, items, nullptr, 0, 0};
     hipEventRecord(pe.a, c->stream);
     hipError_t e = launch();
@@ -870,7 +890,7 @@ struct Run {
         // survivor that did not fit) lower the result
         if (a.kmode) HIP_TRY(timed(c, K_DECIDE, 0, 0, 0, [&] { return launch_rx_tail(op, a, c->stream); }));
         HIP_TRY(timed(c, ek, 0, 0, 0, [&] { return launch_emit(op, c->B, a, c->stream); }));
-        if (c->opt.profile && c->n_counts < kCountSlots) {
+        if (profiled(c, ek) && c->n_counts < kCountSlots) {
             // survivors of this chunk -> bytes of its emit (read after the level sync)
             long long *slot = c->h_counts + c->n_counts++;
             HIP_TRY(hipMemcpyAsync(slot, &c->d_state->chunk_total, sizeof(long long), hipMemcpyDeviceToHost,
@@ -1363,7 +1383,17 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         while (slots < c->tab_max_slots && slots < 4 * (size_t)A) slots <<= 1;
         c->tmask = (uint32_t)(std::min(slots, c->tab_max_slots) - 1);
     }
-    HIP_TRY(launch_fill_u64(c->d_tab, kEmpty, 2 * ((size_t)c->tmask + 1), s));
+    // The table: cleared, or (a fixed-size table in the context's pool, as
+    // the last call left it) the next epoch's.
+    const bool tab_reuse = !noalloc && o.hash_slots_log2 && c->tab_ok && c->tab_ptr == c->d_tab &&
+                           c->tab_mask == c->tmask && c->tab_epoch < kEpochs;
+    c->tab_ok = false;
+    if (tab_reuse) {
+        ++c->tab_epoch;
+    } else {
+        HIP_TRY(launch_fill_u64(c->d_tab, kEmpty, 2 * ((size_t)c->tmask + 1), s));
+        c->tab_epoch = 1;
+    }
     HIP_TRY(launch_fill_u64(c->d_lb, 0, (size_t)c->lb_words, s));
     c->lb_epoch = 0;
 
@@ -1432,6 +1462,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         }
     }
     ba.capacity = c->capacity;
+    ba.epoch = c->tab_epoch;
     ba.keep_keys = o.hash_slots_log2 ? 0 : 1;
     ba.ws = c->ws;
     ba.nw = c->nw;
@@ -1937,7 +1968,8 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
                 c->tmask = (uint32_t)(slots - 1);
                 ba.tmask = c->tmask;
                 if (launch_fill_u64(c->d_tab, kEmpty, 2 * slots, s) != hipSuccess ||
-                    launch_rehash(c->d_tab, c->tmask, c->d_key, n_stored, &c->d_state->probe_fail, s) != hipSuccess) {
+                    launch_rehash(c->d_tab, c->tmask, c->d_key, n_stored, ba.epoch, &c->d_state->probe_fail, s) !=
+                        hipSuccess) {
                     rc = LTL_E_HIP;
                     break;
                 }
@@ -1985,6 +2017,11 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         hipStreamSynchronize(s);
         drain_events(c);
         return res->status = rc;
+    }
+    if (!noalloc && o.hash_slots_log2) {  // (the table is reusable by the next call)
+        c->tab_ok = true;
+        c->tab_ptr = c->d_tab;
+        c->tab_mask = c->tmask;
     }
     if (cost == maxc + 1) cost--;  // code/ltli6463.cu:1445
     if (found || neg_answer) cost = found_cost;
@@ -2255,6 +2292,7 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
 
         c->lazy_s = c->lazy_e = 0;
         if (int32_t e = ensure_pool(c)) return e;
+        c->tab_ok = false;  // (the regions overwrite the pool)
         const ltl_batch_params &bp = c->batch;
         const int tile = bp.tile == 256 || bp.tile == 1024 ? bp.tile : 512;
         const int wb = bp.word_block == 8 ? 8 : 4;
