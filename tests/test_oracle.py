@@ -391,3 +391,34 @@ def test_envelope_fixtures(oracle, case):
         assert r3[k] == case["ref3"][k], k
     if r["status"] == 1:
         assert oracle.mismatches(inst, oracle.eval_formula(inst, r["formula"])) == 0
+
+
+def _twin4_oracle(oracle, seed, n=40):
+    """The mirrored four-trace instance of the GPU parity tests: t1 is t0 with
+    a and b exchanged, t3 is t2 with a and b exchanged (so a set and its a<->b
+    mirror have equal mode-3 keys); at position 1 b holds on t0 only."""
+    rng = np.random.default_rng(seed)
+    a0, b0, a2, b2 = (rng.integers(0, 2, n) for _ in range(4))
+    a0[1], b0[1], a0[0], b0[0], a2[1], b2[1] = 0, 1, 1, 1, 0, 0
+    w = lambda v: int(sum(int(x) << k for k, x in enumerate(v)))
+    atoms = np.array([[w(a0), w(b0), w(a2), w(b2)], [w(b0), w(a0), w(b2), w(a2)]], dtype=np.uint64)
+    return oracle.Instance(1, 3, np.full(4, n, dtype=np.uint8), atoms, "ab")
+
+
+def test_relaxed_acceptance_tests_key_new_candidates_only(oracle):
+    """The reference tests a candidate only when its relaxed key is new
+    (code/ltli6463.cu:231-243).  X(b) separates this instance (the brute-force
+    minimum, cost 2), but its mode-3 key equals that of X(a), an earlier
+    candidate of the same level: in mode 3 it is never tested, and the answer
+    is a separating formula of higher cost."""
+    inst = _twin4_oracle(oracle, 3)
+    assert oracle.bruteforce_min_cost(inst) == 2
+    exact = oracle.enumerate_(inst, max_cost=6)
+    assert (exact["cost"], exact["formula"]) == (2, "X(b)")
+    xa, xb = oracle.eval_formula(inst, "X(a)"), oracle.eval_formula(inst, "X(b)")
+    assert oracle.relaxed_key(inst, xa, 3) == oracle.relaxed_key(inst, xb, 3)
+    assert oracle.mismatches(inst, xa) > 0 and oracle.mismatches(inst, xb) == 0
+    r3 = oracle.enumerate_(inst, max_cost=6, dedup_mode=3)
+    assert (r3["cost"], r3["formula"]) == (4, "X((a)U(b))")
+    assert oracle.mismatches(inst, oracle.eval_formula(inst, r3["formula"])) == 0
+    assert oracle.formula_cost(r3["formula"]) == 4
