@@ -307,9 +307,10 @@ int32_t ltl_learn_dc(ltl_ctx *ctx, const ltl_problem *pb, int32_t window, int32_
  * benchmark file, runDC, code/dc.py:373-387): one driver per problem, in lock-step;
  * whenever every driver waits for a window, the windows of all of them are
  * learned by one ltl_learn_batch call, and the formulas returned are matched
- * on the device in one launch.  res[i] is what ltl_learn_dc returns for
- * pbs[i] (res[i].seconds: the whole call).  Returns the number of problems
- * with a formula, or LTL_E_ARG. */
+ * on the device in one launch (groups of up to 256 problems, one host thread
+ * each).  res[i] is what ltl_learn_dc returns for pbs[i] (res[i].seconds:
+ * its group's time).  Returns the number of problems with a formula, or
+ * LTL_E_ARG. */
 int32_t ltl_learn_dc_batch(ltl_ctx *ctx, const ltl_problem *pbs, int32_t n, int32_t window, int32_t algo,
                            uint64_t seed, ltl_result *res);
 /* The reference's LTLmatch (code/dc.py:136-186) on the device: out[i*W + t]

@@ -81,6 +81,7 @@ struct Coordinator {
         if (live > 0 && waiting == live) run_locked();
     }
     void run_locked();
+    void serve(const std::vector<Req *> &reqs);  // (run_locked's work)
 };
 
 struct Driver {
@@ -303,6 +304,17 @@ void Coordinator::run_locked() {
     std::vector<Req *> reqs;
     reqs.swap(pending);
     waiting -= (int)reqs.size();
+    try {
+        serve(reqs);
+    } catch (...) {  // (host memory): the posted requests fail, the drivers go on
+        for (Req *r : reqs)
+            if (r->status >= 0) r->status = LTL_E_OOM;
+    }
+    for (Req *r : reqs) r->done = true;
+    cv.notify_all();
+}
+
+void Coordinator::serve(const std::vector<Req *> &reqs) {
     hipError_t he = hipSetDevice(device);
     // the posted windows, one batched call
     std::vector<Req *> sv;
@@ -346,8 +358,6 @@ void Coordinator::run_locked() {
         if (e) owner[k]->status = e;
         else owner[k]->d->cache.emplace(owner[k]->formula, std::move(vals[k]));
     }
-    for (Req *r : reqs) r->done = true;
-    cv.notify_all();
 }
 
 void to_nodes(const Node &n, std::vector<ltl_node> &out) {
@@ -368,11 +378,16 @@ void to_nodes(const Node &n, std::vector<ltl_node> &out) {
     }
 }
 
+// (leaves the coordinator exactly once, whatever happens)
 void run_driver(Driver &d) {
-    Idx pos(d.pb->num_pos), neg(d.pb->num_neg);
-    for (int i = 0; i < d.pb->num_pos; ++i) pos[i] = i;
-    for (int i = 0; i < d.pb->num_neg; ++i) neg[i] = d.pb->num_pos + i;
-    d.result = d.algo == LTL_DC_DET ? d.det(pos, neg, 0) : d.rnd(pos, neg, 0);
+    try {
+        Idx pos(d.pb->num_pos), neg(d.pb->num_neg);
+        for (int i = 0; i < d.pb->num_pos; ++i) pos[i] = i;
+        for (int i = 0; i < d.pb->num_neg; ++i) neg[i] = d.pb->num_pos + i;
+        d.result = d.algo == LTL_DC_DET ? d.det(pos, neg, 0) : d.rnd(pos, neg, 0);
+    } catch (...) {
+        d.err = LTL_E_OOM;
+    }
     d.co->leave();
 }
 
@@ -486,8 +501,14 @@ extern "C" int32_t ltl_learn_dc(ltl_ctx *ctx, const ltl_problem *pb, int32_t win
 extern "C" int32_t ltl_learn_dc_batch(ltl_ctx *ctx, const ltl_problem *pbs, int32_t n, int32_t window, int32_t algo,
                                       uint64_t seed, ltl_result *res) {
     if (n < 0 || (n > 0 && (!pbs || !res))) return LTL_E_ARG;
-    if (n == 0) return 0;
-    return run_dc(ctx, pbs, n, window, algo, seed, res);
+    // (groups of at most 256 problems: one driver thread each)
+    int32_t solved = 0;
+    for (int32_t i = 0; i < n; i += 256) {
+        const int32_t r = run_dc(ctx, pbs + i, std::min<int32_t>(256, n - i), window, algo, seed, res + i);
+        if (r < 0) return r;
+        solved += r;
+    }
+    return solved;
 }
 
 extern "C" int32_t ltl_match(ltl_ctx *ctx, const ltl_problem *pb, const char *const *formulas, int32_t n,
