@@ -55,7 +55,6 @@ struct Coordinator {
     std::mutex mu;
     std::condition_variable cv;
     int live = 0, waiting = 0;
-    int64_t rounds = 0;  // ltl_learn_batch calls
     struct Req {
         Driver *d = nullptr;
         bool solve = true;             // a window to learn, else a formula to match
@@ -325,7 +324,6 @@ void Coordinator::serve(const std::vector<Req *> &reqs) {
         for (Req *r : sv) pbs.push_back(r->sub);
         std::vector<ltl_result> rs(sv.size());
         const int32_t st = ltl_learn_batch(ctx, pbs.data(), (int32_t)pbs.size(), rs.data());
-        ++rounds;
         for (size_t k = 0; k < sv.size(); ++k) {
             Req *r = sv[k];
             r->status = st < 0 ? st : rs[k].status;
