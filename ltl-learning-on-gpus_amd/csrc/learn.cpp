@@ -1715,8 +1715,28 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     int found_cost = -1;
     bool neg_answer = false;
     uint64_t neg_id = kEmpty;
+    // The binary blocks of level `next` tested before the level below it is
+    // generated (plain order): submitted only, a sync follows.
+    auto submit_lookahead = [&](int next) -> int32_t {
+        if (anti_plain &&
+            launch_fill_u64(reinterpret_cast<uint64_t *>(&c->d_state->neg_found[1]), kEmpty, 1, s) != hipSuccess)
+            return LTL_E_HIP;
+        run.check_only = true;
+        run.only_binary = true;
+        run.base.anti = anti_plain ? 1 : 0;
+        run.base.neg_slot = 1;
+        const int32_t r = level_blocks(next);
+        run.base.anti = 0;
+        run.base.neg_slot = 0;
+        run.only_binary = false;
+        run.check_only = false;
+        return r;
+    };
+    std::vector<Block> ahead_blocks;  // the lookahead's blocks (ids relative to their first)
     for (cost = c1 + 1; cost <= maxc; ++cost) {
         run.level_stored = n_stored;
+        bool ahead_done = false;          // the lookahead of cost + 1 ran with the check pass
+        uint64_t ahead_hit = kEmpty, ahead_neg = kEmpty;
         if (run.otf) {
             const int dif = cost - shortage;  // code/ltli6463.cu:1202-1206
             if (dif == c2 || dif == c1 + c3 || dif == c1 + c4 || dif == c5 || dif == c6 || dif == c7 || dif == c1 + c8)
@@ -1821,6 +1841,14 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             // whose complement is accepted, the negation successor of a
             // deferred level)
             const bool anti = anti_plain && !run.otf;
+            // Plain order: the lookahead of cost + 1 (below) is submitted with
+            // this pass, one synchronisation for both; its ids follow this
+            // level's, so the smallest accepted id tells which of the two it
+            // belongs to (once this level has one, the lookahead's launches
+            // do no work).  Not while a deferred level is pending: this level
+            // then has an accepted candidate.
+            const bool ahead_ok = !learned && !run.otf && defer_ok && !run.logging && cost < maxc &&
+                                  c->lazy_e == c->lazy_s && run.defer_s < 0;
             for (int pass = 0; pass < 2; ++pass) {
                 run.check_only = true;
                 run.lblk.clear();
@@ -1840,9 +1868,22 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
                 run.reuse_binary = false;
                 run.base.anti = 0;
                 if (!rc && !run.lblk.empty()) rc = ranked_level(cost, cost - 1);
+                uint64_t ahead_base = kEmpty;
+                if (!rc && pass == 0 && ahead_ok) {
+                    ahead_base = run.all_ltls;
+                    const size_t nb2 = run.blocks.size();
+                    rc = submit_lookahead(cost + 1);
+                    ahead_blocks.assign(run.blocks.begin() + (long)nb2, run.blocks.end());
+                }
                 if (!rc) rc = sync_state();
                 run.check_only = false;
                 if (rc) break;
+                if (ahead_base != kEmpty) {
+                    ahead_done = true;
+                    ahead_hit = h.found >= ahead_base ? h.found : kEmpty;
+                    ahead_neg = h.neg_found[1];
+                    if (h.found >= ahead_base) h.found = kEmpty;
+                }
                 neg_pick = anti ? std::min<uint64_t>(h.neg_found[0], run.la_neg) : kEmpty;
                 h.found = std::min<uint64_t>(h.found, run.la_found);
                 // unary blocks over a deferred level: their successors were
@@ -1897,43 +1938,35 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             if (cost == maxc) {
                 lazy = true;
             } else {
-                const uint64_t save = run.all_ltls;
-                const size_t nb = run.blocks.size();
-                if (anti_plain &&
-                    launch_fill_u64(reinterpret_cast<uint64_t *>(&c->d_state->neg_found[1]), kEmpty, 1, s) != hipSuccess) {
-                    rc = LTL_E_HIP;
-                    break;
+                if (!ahead_done) {  // (submitted with the check pass above, or now)
+                    const uint64_t save = run.all_ltls;
+                    const size_t nb = run.blocks.size();
+                    rc = submit_lookahead(cost + 1);
+                    if (!rc) rc = sync_state();
+                    ahead_blocks.assign(run.blocks.begin() + (long)nb, run.blocks.end());
+                    ahead_hit = h.found;
+                    ahead_neg = h.neg_found[1];
+                    run.all_ltls = save;
+                    run.blocks.resize(nb);
+                    if (rc) break;
                 }
-                run.check_only = true;
-                run.only_binary = true;
-                run.base.anti = anti_plain ? 1 : 0;
-                run.base.neg_slot = 1;
-                rc = level_blocks(cost + 1);
-                run.base.anti = 0;
-                run.base.neg_slot = 0;
-                run.only_binary = false;
-                run.check_only = false;
-                if (!rc) rc = sync_state();
                 int hit_op = -1;
                 run.la_block = run.la_neg_block = -1;
-                const uint64_t la_neg_id = anti_plain ? h.neg_found[1] : kEmpty;
-                if (!rc)
-                    for (size_t b = nb; b < run.blocks.size(); ++b) {
-                        const Block &bk = run.blocks[b];
-                        if (h.found >= bk.id_base && h.found - bk.id_base < bk.size) {
-                            hit_op = bk.op;
-                            run.la_block = (int64_t)(b - nb);
-                            run.la_off = h.found - bk.id_base;
-                        }
-                        if (la_neg_id >= bk.id_base && la_neg_id - bk.id_base < bk.size) {
-                            run.la_neg_block = (int64_t)(b - nb);
-                            run.la_neg_off = la_neg_id - bk.id_base;
-                        }
+                const uint64_t la_neg_id = anti_plain ? ahead_neg : kEmpty;
+                for (size_t b = 0; b < ahead_blocks.size(); ++b) {
+                    const Block &bk = ahead_blocks[b];
+                    if (ahead_hit >= bk.id_base && ahead_hit - bk.id_base < bk.size) {
+                        hit_op = bk.op;
+                        run.la_block = (int64_t)b;
+                        run.la_off = ahead_hit - bk.id_base;
                     }
-                if (!rc) plain_ahead = cost + 1;
-                run.all_ltls = save;
-                run.blocks.resize(nb);
-                if (rc) break;
+                    if (la_neg_id >= bk.id_base && la_neg_id - bk.id_base < bk.size) {
+                        run.la_neg_block = (int64_t)b;
+                        run.la_neg_off = la_neg_id - bk.id_base;
+                    }
+                }
+                plain_ahead = cost + 1;
+                h.found = ahead_hit;
                 if (h.found != kEmpty) {
                     const int avail = (o.neg_type == 1 && c2 == 1 ? 1 : 0) | (c5 == 1 ? 2 : 0) |
                                       (c6 == 1 ? 4 : 0) | (c7 == 1 ? 8 : 0);
