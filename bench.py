@@ -446,13 +446,18 @@ def long_traces(ltl, args):
     """Traces longer than 64 positions (K = 4 words per trace): 100 + 100
     traces of U{65..250} positions, 5 atoms, config-2's planted formula."""
     inst = long_instance(ltl, 1, 100, 100, 5, 65, 250, "(G((~(a))|(F(b))))&(~((c)U(d)))")
-    L = ltl.Learner(budget=int(args.budget_gb * 1e9), profile=2, hash_slots_log2=28, max_cost=14, neg_type=1)
+    L = ltl.Learner(budget=int(args.budget_gb * 1e9), hash_slots_log2=28, max_cost=14, neg_type=1)
     try:
         L.learn(inst)
-        t = time.perf_counter()
-        r = L.learn(inst)
-        dt = time.perf_counter() - t
+        L.learn(inst, profile=1)  # every kernel kind timed (a barrier per launch), untimed
         stats = L.kernel_stats()
+        best = None
+        for _ in range(3):
+            t = time.perf_counter()
+            r = L.learn(inst)
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        dt = best
     finally:
         L.close()
     dom = max(stats, key=lambda k: stats[k]["seconds"])
@@ -460,9 +465,10 @@ def long_traces(ltl, args):
     return {"workload": "100+100 traces, 5 atoms, lengths U{65..250} (4 words per trace), per-atom rate "
                         "U[0.3,0.7], planted G(a->F b)&~(c U d), unit costs, max cost 14, seed 1, plain order",
             "words_per_trace": inst.K, "value": round((r["n_stored"] - 5) / dt, 1), "unit": "unique CS/s",
-            "seconds": round(dt, 4), "min_cost": r["cost"], "formula": r["formula"],
+            "seconds": round(dt, 4), "seconds_is": "best of 3 calls, no kernel timing",
+            "min_cost": r["cost"], "formula": r["formula"],
             "uniques": r["n_stored"], "words_deferred": r["words_deferred"], "store_overflow": bool(r["on_the_fly"]),
-            "kernels_ms": {k: round(v["seconds"] * 1e3, 3) for k, v in stats.items() if v["launches"]},
+            "kernels_ms": {k: round(v["seconds"] * 1e3, 3) for k, v in stats.items() if v["seconds"] > 0},
             "dominant": {"kernel": dom, "achieved_gbs": round(d["alg_bytes"] / d["seconds"] / 1e9, 1),
                          "frac": round(d["alg_bytes"] / d["seconds"] / HBM_PEAK, 4)}}
 
