@@ -54,30 +54,99 @@ struct Tile {
     bool valid;
 };
 
-// Binary chunks: a workgroup of 4 waves covers 4 R consecutive left rows of
-// one 64-column right stripe, wave w rows 4 R q + R w .. + R - 1 (t.a is the
-// first, t.ti its tile; row r's tile is t.ti + r * nbs).
+// Word-split generation (A.gsplit = S, 1, 2 or 4): the 4 waves of a
+// workgroup take 4 / S tasks, each folded by S waves over word ranges
+// [w0, w1) of an even split; S = 1 is one wave per task over every word.
+struct Split {
+    int S, r, tw, per;  // waves per task, the wave's range, its task in the workgroup, tasks per workgroup
+    int w0, w1;
+};
+__device__ __forceinline__ Split split_of(const ChunkArgs &A) {
+    Split s;
+    s.S = A.gsplit;
+    const int v = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    s.r = v & (s.S - 1);
+    s.tw = v / s.S;
+    s.per = 4 / s.S;
+    s.w0 = (int)((int64_t)s.r * A.nw / s.S);
+    s.w1 = (int)((int64_t)(s.r + 1) * A.nw / s.S);
+    return s;
+}
+
+// Binary chunks: a workgroup covers per * R consecutive left rows of one
+// 64-column right stripe, task w rows per * R q + R w .. + R - 1 (t.a is the
+// first, t.ti its tile; row r's tile is t.ti + r * nbs).  Without a task
+// (false) t is an invalid lane of the chunk's first row.
 template <int R = 1>
-__device__ __forceinline__ bool tile_binary(const ChunkArgs &A, Tile &t) {
-    const int nA = (int)(A.a1 - A.a0), nAq = (nA + 4 * R - 1) / (4 * R);
+__device__ __forceinline__ bool tile_binary(const ChunkArgs &A, Tile &t, const Split &sp) {
+    const int rows = sp.per * R;
+    const int nA = (int)(A.a1 - A.a0), nAq = (nA + rows - 1) / rows;
     const int lw = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int bs = lw / nAq, aq = lw % nAq;
-    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    t.a = A.a0 + (int64_t)(4 * aq + wave) * R;
-    if (t.a >= A.a1) return false;
+    t.a = A.a0 + (int64_t)(sp.per * aq + sp.tw) * R;
+    if (t.a >= A.a1) {
+        t.a = A.a0;
+        t.b = A.b0;
+        t.ti = 0;
+        t.valid = false;
+        return false;
+    }
     t.b = A.b0 - A.ush + 64 * (int64_t)bs + (threadIdx.x & 63);
     t.valid = t.b >= A.b0 && t.b < A.b1;
     t.ti = (t.a - A.a0) * A.nbs + bs;
     return true;
 }
+template <int R = 1>
+__device__ __forceinline__ bool tile_binary(const ChunkArgs &A, Tile &t) {
+    return tile_binary<R>(A, t, split_of(A));
+}
 
-__device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t) {
-    t.ti = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (t.ti >= A.n_tiles) return false;
+__device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t, const Split &sp) {
+    t.ti = (int64_t)blockIdx.x * sp.per + sp.tw;
+    if (t.ti >= A.n_tiles) {
+        t.ti = 0;
+        t.a = A.a0;
+        t.b = 0;
+        t.valid = false;
+        return false;
+    }
     t.a = A.a0 - A.ush + 64 * t.ti + (threadIdx.x & 63);
     t.b = 0;
     t.valid = t.a >= A.a0 && t.a < A.a1;
     return true;
+}
+__device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t) { return tile_unary(A, t, split_of(A)); }
+
+// The S waves of a task meet before its tail: waves 1 .. S-1 leave their
+// sums (NH fingerprints mod 2^64, mismatch counts) in LDS, wave 0 adds them.
+// Every wave of the workgroup reaches the barrier (S is uniform); waves
+// without a task take part with zeros.  LDS: (4 - 4 / S) * N * 512 bytes.
+template <int N>
+__device__ __forceinline__ void split_add(uint64_t (&v)[N], const Split &sp) {
+    extern __shared__ uint64_t split_lds[];
+    const int lane = (int)(threadIdx.x & 63);
+    uint64_t *b = split_lds + (size_t)sp.tw * (sp.S - 1) * N * 64;
+    if (sp.r > 0) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) b[((size_t)(sp.r - 1) * N + i) * 64 + lane] = v[i];
+    }
+    __syncthreads();
+    if (sp.r == 0) {
+        for (int j = 1; j < sp.S; ++j) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) v[i] += b[((size_t)(j - 1) * N + i) * 64 + lane];
+        }
+    }
+}
+__device__ __forceinline__ void acc_put(uint64_t *v, const Acc &a) {
+    v[0] = a.f1;
+    v[1] = a.f2;
+    v[2] = a.mism;
+}
+__device__ __forceinline__ void acc_get(const uint64_t *v, Acc &a) {
+    a.f1 = v[0];
+    a.f2 = v[1];
+    a.mism = (uint32_t)v[2];
 }
 
 // ---------------------------------------------------------------------------
@@ -176,9 +245,13 @@ __device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, con
 // reads from L2 drop R-fold; the fingerprint arithmetic per candidate stays.
 template <int OP, bool RX, bool CHK, bool SUCC, int R>
 __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
-    if (run_stopped(A)) return;
+    // (split: every wave reaches split_add; the stop test moves to the tail)
+    const Split sp = split_of(A);
+    const bool split = !RX && !SUCC && sp.S > 1;
+    if (!split && run_stopped(A)) return;
     Tile t;
-    if (!tile_binary<R>(A, t)) return;
+    const bool live = tile_binary<R>(A, t, sp);
+    if (!split && !live) return;
     const bool otf = chunk_otf(A);
     const int nrow = (int)(A.a1 - t.a < R ? A.a1 - t.a : R);  // wave-uniform
     const int64_t sr = A.rs + (t.valid ? t.b : A.b0);
@@ -219,8 +292,9 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
     // UB = 8 or R = 2: 5.7-6.0 ms (two carry chains and four fingerprints
     // per pair keep it VALU-bound).
     constexpr int UB = R == 1 ? (OP == LTL_UNTIL ? 4 : 16) : 8;
-    int w = 0;
-    for (; w + UB <= A.nw; w += UB) {
+    const int we = live ? sp.w1 : sp.w0;
+    int w = sp.w0;
+    for (; w + UB <= we; w += UB) {
         uint64_t rv[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) rv[u] = pr[(size_t)(w + u) * kTileWS];
@@ -232,11 +306,29 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
             step(w + u, l, rv[u]);
         }
     }
-    for (; w < A.nw; ++w) {
+    for (; w < we; ++w) {
         uint64_t l[R];
 #pragma unroll
         for (int r = 0; r < R; ++r) l[r] = pl[r][(size_t)w * kTileWS];
         step(w, l, pr[(size_t)w * kTileWS]);
+    }
+    if constexpr (!RX && !SUCC) {
+        if (split) {
+            constexpr int NO = OP == LTL_UNTIL ? 2 : 1;
+            uint64_t v[3 * NO * R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                acc_put(v + 3 * NO * r, c0[r]);
+                if constexpr (NO == 2) acc_put(v + 3 * NO * r + 3, c1[r]);
+            }
+            split_add(v, sp);
+            if (sp.r != 0 || !live || run_stopped(A)) return;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                acc_get(v + 3 * NO * r, c0[r]);
+                if constexpr (NO == 2) acc_get(v + 3 * NO * r + 3, c1[r]);
+            }
+        }
     }
     if (!t.valid) return;
 #pragma unroll
@@ -255,9 +347,12 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
 
 template <int OP, bool RX, bool CHK, bool SUCC>
 __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
-    if (run_stopped(A)) return;
+    const Split sp = split_of(A);
+    const bool split = !RX && !SUCC && sp.S > 1;
+    if (!split && run_stopped(A)) return;
     Tile t;
-    if (!tile_unary(A, t)) return;
+    const bool live = tile_unary(A, t, sp);
+    if (!split && !live) return;
     const bool otf = chunk_otf(A);
     const int64_t s = A.ls + (t.valid ? t.a : A.a0);
     const uint64_t *p = A.store + set_base(s, A.nw);
@@ -273,15 +368,25 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
         if constexpr (RX) key_fold_pk(k0, y, w, x, A);
     };
     constexpr int UB = 8;
-    int w = 0;
-    for (; w + UB <= A.nw; w += UB) {
+    const int we = live ? sp.w1 : sp.w0;
+    int w = sp.w0;
+    for (; w + UB <= we; w += UB) {
         uint64_t v[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) v[u] = p[(size_t)(w + u) * kTileWS];
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, v[u]);
     }
-    for (; w < A.nw; ++w) step(w, p[(size_t)w * kTileWS]);
+    for (; w < we; ++w) step(w, p[(size_t)w * kTileWS]);
+    if constexpr (!RX && !SUCC) {
+        if (split) {
+            uint64_t v[3];
+            acc_put(v, c0);
+            split_add(v, sp);
+            if (sp.r != 0 || !live || run_stopped(A)) return;
+            acc_get(v, c0);
+        }
+    }
     if (!t.valid) return;
     if constexpr (RX) key_finish(c0, k0, A.kseed);
     gen1_tail<CHK, SUCC, RX>(A, t, c0, s0, otf);
@@ -298,9 +403,12 @@ __global__ __launch_bounds__(256) void k_gen1(ChunkArgs A) {
 // j * slots.  Packed layout, exact keys.
 template <bool CHK, bool SUCC, bool NEG>
 __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t slots, uint64_t id_xfg) {
-    if (run_stopped(A)) return;
+    const Split sp = split_of(A);
+    const bool split = !SUCC && sp.S > 1;
+    if (!split && run_stopped(A)) return;
     Tile t;
-    if (!tile_unary(A, t)) return;
+    const bool live = tile_unary(A, t, sp);
+    if (!split && !live) return;
     const bool otf = chunk_otf(A);
     const int64_t s = A.ls + (t.valid ? t.a : A.a0);
     const uint64_t *p = A.store + set_base(s, A.nw);
@@ -322,15 +430,27 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
         }
     };
     constexpr int UB = 8;
-    int w = 0;
-    for (; w + UB <= A.nw; w += UB) {
+    const int we = live ? sp.w1 : sp.w0;
+    int w = sp.w0;
+    for (; w + UB <= we; w += UB) {
         uint64_t v[UB];
 #pragma unroll
         for (int u = 0; u < UB; ++u) v[u] = p[(size_t)(w + u) * kTileWS];
 #pragma unroll
         for (int u = 0; u < UB; ++u) step(w + u, v[u]);
     }
-    for (; w < A.nw; ++w) step(w, p[(size_t)w * kTileWS]);
+    for (; w < we; ++w) step(w, p[(size_t)w * kTileWS]);
+    if constexpr (!SUCC) {
+        if (split) {
+            uint64_t v[3 * NC];
+#pragma unroll
+            for (int j = 0; j < NC; ++j) acc_put(v + 3 * j, c[j]);
+            split_add(v, sp);
+            if (sp.r != 0 || !live || run_stopped(A)) return;
+#pragma unroll
+            for (int j = 0; j < NC; ++j) acc_get(v + 3 * j, c[j]);
+        }
+    }
     if (!t.valid) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
 #pragma unroll
@@ -1278,9 +1398,10 @@ __global__ void k_fill(uint64_t *p, uint64_t v, size_t n) {
 // ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
-// R: left rows per wave (tile_binary<R>).
+// R: left rows per wave (tile_binary<R>); a.gsplit waves per task.
 static dim3 grid_binary(const ChunkArgs &a, int R = 1) {
-    const int64_t nAq = (a.a1 - a.a0 + 4 * R - 1) / (4 * R);
+    const int rows = (4 / a.gsplit) * R;
+    const int64_t nAq = (a.a1 - a.a0 + rows - 1) / rows;
     return dim3((unsigned)(nAq * a.nbs));
 }
 // Left rows per wave of the binary generation kernels (& and |; U keeps one
@@ -1289,20 +1410,42 @@ static dim3 grid_binary(const ChunkArgs &a, int R = 1) {
 template <int OP>
 constexpr int gen_rows() { return OP == LTL_UNTIL ? 1 : 2; }
 constexpr int kGenRows = 2;
-static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles + 3) / 4)); }
+static dim3 grid_tiles(const ChunkArgs &a) {
+    const int per = 4 / a.gsplit;
+    return dim3((unsigned)((a.n_tiles + per - 1) / per));
+}
+
+// Waves per task of a packed generation launch (exact keys, no successor
+// tests): one while the chunk has enough tasks to fill the device; a chunk
+// of few tasks folds each task's words with 2 or 4 waves, since one wave
+// walking every word of its sets is then the whole launch (a fused
+// ~/X/F/G pass over 81 sets of config 2 took 55 us).  tasks: the waves of
+// the launch with one wave per task.
+static int gen_split(int64_t tasks, int nw) {
+    if (nw >= 64 && tasks <= 2048) return 4;
+    if (nw >= 32 && tasks <= 4096) return 2;
+    return 1;
+}
+// LDS of split_add for n sums per lane.
+static size_t split_lds(const ChunkArgs &a, int n) { return a.gsplit > 1 ? (size_t)(4 - 4 / a.gsplit) * n * 512 : 0; }
+static int64_t binary_tasks(const ChunkArgs &a, int R) { return (int64_t)a.nbs * ((a.a1 - a.a0 + R - 1) / R); }
 
 // The chunk kernels run on the packed layout (B = 1); K words per trace
 // (B = 0) has kernels of its own.
 // Successor checks (a.t_succ) run with exact keys on a tested level only.
-#define LTL_DISPATCH_BX(KER, OP, B, RX, GRID, BLOCK, S, ...)                                  \
+// Exact keys without successor checks: split over a.gsplit waves per task.
+#define LTL_DISPATCH_BX(KER, OP, B, RX, S)                                                      \
     if ((B) != 1) return hipErrorInvalidValue;                                                  \
     if (a.t_succ) {                                                                             \
         if (RX || a.gen_check) return hipErrorInvalidValue;                                     \
-        hipLaunchKernelGGL((KER<OP, false, false, true>), GRID, BLOCK, 0, S, __VA_ARGS__);      \
-    } else if (a.gen_check) {                                                                   \
-        hipLaunchKernelGGL((KER<OP, RX, true, false>), GRID, BLOCK, 0, S, __VA_ARGS__);         \
+        hipLaunchKernelGGL((KER<OP, false, false, true>), grid_tiles(a), dim3(256), 0, S, a);   \
     } else {                                                                                    \
-        hipLaunchKernelGGL((KER<OP, RX, false, false>), GRID, BLOCK, 0, S, __VA_ARGS__);        \
+        ChunkArgs b = a;                                                                        \
+        if (!RX) b.gsplit = gen_split(a.n_tiles, a.nw);                                         \
+        if (a.gen_check)                                                                        \
+            hipLaunchKernelGGL((KER<OP, RX, true, false>), grid_tiles(b), dim3(256), split_lds(b, 3), S, b); \
+        else                                                                                    \
+            hipLaunchKernelGGL((KER<OP, RX, false, false>), grid_tiles(b), dim3(256), split_lds(b, 3), S, b); \
     }
 
 // The same for k_gen2 with R left rows per wave (R = 1 with relaxed keys).
@@ -1314,10 +1457,14 @@ static dim3 grid_tiles(const ChunkArgs &a) { return dim3((unsigned)((a.n_tiles +
     } else if (RX) {                                                                                     \
         if (a.gen_check) hipLaunchKernelGGL((k_gen2<OP, RX, true, false, 1>), grid_binary(a), dim3(256), 0, S, a); \
         else hipLaunchKernelGGL((k_gen2<OP, RX, false, false, 1>), grid_binary(a), dim3(256), 0, S, a);  \
-    } else if (a.gen_check) {                                                                            \
-        hipLaunchKernelGGL((k_gen2<OP, false, true, false, gen_rows<OP>()>), grid_binary(a, gen_rows<OP>()), dim3(256), 0, S, a); \
     } else {                                                                                             \
-        hipLaunchKernelGGL((k_gen2<OP, false, false, false, gen_rows<OP>()>), grid_binary(a, gen_rows<OP>()), dim3(256), 0, S, a); \
+        constexpr int R_ = gen_rows<OP>(), N_ = 3 * R_ * (OP == LTL_UNTIL ? 2 : 1);                       \
+        ChunkArgs b = a;                                                                                 \
+        b.gsplit = gen_split(binary_tasks(a, R_), a.nw);                                                 \
+        if (a.gen_check)                                                                                 \
+            hipLaunchKernelGGL((k_gen2<OP, false, true, false, R_>), grid_binary(b, R_), dim3(256), split_lds(b, N_), S, b); \
+        else                                                                                             \
+            hipLaunchKernelGGL((k_gen2<OP, false, false, false, R_>), grid_binary(b, R_), dim3(256), split_lds(b, N_), S, b); \
     }
 
 #define LTL_DISPATCH_B(KER, OP, B, GRID, BLOCK, S, ...)                               \
@@ -1361,10 +1508,10 @@ static hipError_t launch_gen_mw(int op, const ChunkArgs &a, hipStream_t s) {
 template <bool RX>
 static hipError_t launch_gen_x(int op, int B, const ChunkArgs &a, hipStream_t s) {
     switch (op) {
-    case LTL_NOT: LTL_DISPATCH_BX(k_gen1, LTL_NOT, B, RX, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_NEXT: LTL_DISPATCH_BX(k_gen1, LTL_NEXT, B, RX, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_FIN: LTL_DISPATCH_BX(k_gen1, LTL_FIN, B, RX, grid_tiles(a), dim3(256), s, a); break;
-    case LTL_GLOB: LTL_DISPATCH_BX(k_gen1, LTL_GLOB, B, RX, grid_tiles(a), dim3(256), s, a); break;
+    case LTL_NOT: LTL_DISPATCH_BX(k_gen1, LTL_NOT, B, RX, s); break;
+    case LTL_NEXT: LTL_DISPATCH_BX(k_gen1, LTL_NEXT, B, RX, s); break;
+    case LTL_FIN: LTL_DISPATCH_BX(k_gen1, LTL_FIN, B, RX, s); break;
+    case LTL_GLOB: LTL_DISPATCH_BX(k_gen1, LTL_GLOB, B, RX, s); break;
     case LTL_AND: LTL_DISPATCH_GEN2(LTL_AND, B, RX, s); break;
     case LTL_OR: LTL_DISPATCH_GEN2(LTL_OR, B, RX, s); break;
     case LTL_UNTIL: LTL_DISPATCH_GEN2(LTL_UNTIL, B, RX, s); break;
@@ -1394,17 +1541,21 @@ hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, bool neg
         return hipGetLastError();
     }
     const dim3 g = grid_tiles(a), b(256);
+    // without successor tests: split over gsplit waves per tile
+    ChunkArgs c = a;
+    c.gsplit = a.t_succ ? 1 : gen_split(a.n_tiles, a.nw);
+    const dim3 gc = grid_tiles(c);
     if (neg) {
         if (a.gen_check) return hipErrorInvalidValue;  // tested levels only
         if (a.t_succ) hipLaunchKernelGGL((k_gen_xfg<false, true, true>), g, b, 0, s, a, n, slots, id_xfg);
-        else hipLaunchKernelGGL((k_gen_xfg<false, false, true>), g, b, 0, s, a, n, slots, id_xfg);
+        else hipLaunchKernelGGL((k_gen_xfg<false, false, true>), gc, b, split_lds(c, 12), s, c, n, slots, id_xfg);
     } else if (a.t_succ) {
         if (a.gen_check) return hipErrorInvalidValue;
         hipLaunchKernelGGL((k_gen_xfg<false, true, false>), g, b, 0, s, a, n, slots, a.id_base);
     } else if (a.gen_check) {
-        hipLaunchKernelGGL((k_gen_xfg<true, false, false>), g, b, 0, s, a, n, slots, a.id_base);
+        hipLaunchKernelGGL((k_gen_xfg<true, false, false>), gc, b, split_lds(c, 9), s, c, n, slots, a.id_base);
     } else {
-        hipLaunchKernelGGL((k_gen_xfg<false, false, false>), g, b, 0, s, a, n, slots, a.id_base);
+        hipLaunchKernelGGL((k_gen_xfg<false, false, false>), gc, b, split_lds(c, 9), s, c, n, slots, a.id_base);
     }
     return hipGetLastError();
 }
@@ -1424,8 +1575,13 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
     // emit: a.esplit waves per store tile)
     const int nd = op == LTL_UNTIL ? 2 : 1;
     const int64_t pos_tiles = ((int64_t)a.n_tiles * 64 * nd + 63 + 63) / 64;
-    const int K = B == 1 && !a.deferred ? a.esplit : 1;
+    int K = B == 1 && !a.deferred ? a.esplit : 1;
     if (K < 1 || (K > 1 && (!a.gfirst || !a.glast || K > a.nsw))) return hipErrorInvalidValue;
+    // A chunk of few store tiles: up to 16 word ranges per tile (ranges of at
+    // least 16 words), since each wave's walk over its range sets the time.
+    if (K > 1 && pos_tiles <= 256) K = std::max(K, std::min({16, a.nsw, a.nw / 16}));
+    ChunkArgs e = a;
+    e.esplit = K;
     const dim3 g((unsigned)((pos_tiles * K + 3) / 4)), b(256);
     if (a.deferred) {
         if (op < LTL_NOT || op > LTL_UNTIL) return hipErrorInvalidValue;
@@ -1446,13 +1602,13 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
         return hipGetLastError();
     }
     switch (op) {
-    case LTL_NOT: LTL_DISPATCH_B(k_emit, LTL_NOT, B, g, b, s, a, a.aux, a.saux); break;
-    case LTL_NEXT: LTL_DISPATCH_B(k_emit, LTL_NEXT, B, g, b, s, a, a.aux, a.saux); break;
-    case LTL_FIN: LTL_DISPATCH_B(k_emit, LTL_FIN, B, g, b, s, a, a.aux, a.saux); break;
-    case LTL_GLOB: LTL_DISPATCH_B(k_emit, LTL_GLOB, B, g, b, s, a, a.aux, a.saux); break;
-    case LTL_AND: LTL_DISPATCH_B(k_emit, LTL_AND, B, g, b, s, a, a.aux, a.saux); break;
-    case LTL_OR: LTL_DISPATCH_B(k_emit, LTL_OR, B, g, b, s, a, a.aux, a.saux); break;
-    case LTL_UNTIL: LTL_DISPATCH_B(k_emit, LTL_UNTIL, B, g, b, s, a, a.aux, a.saux); break;
+    case LTL_NOT: LTL_DISPATCH_B(k_emit, LTL_NOT, B, g, b, s, e, e.aux, e.saux); break;
+    case LTL_NEXT: LTL_DISPATCH_B(k_emit, LTL_NEXT, B, g, b, s, e, e.aux, e.saux); break;
+    case LTL_FIN: LTL_DISPATCH_B(k_emit, LTL_FIN, B, g, b, s, e, e.aux, e.saux); break;
+    case LTL_GLOB: LTL_DISPATCH_B(k_emit, LTL_GLOB, B, g, b, s, e, e.aux, e.saux); break;
+    case LTL_AND: LTL_DISPATCH_B(k_emit, LTL_AND, B, g, b, s, e, e.aux, e.saux); break;
+    case LTL_OR: LTL_DISPATCH_B(k_emit, LTL_OR, B, g, b, s, e, e.aux, e.saux); break;
+    case LTL_UNTIL: LTL_DISPATCH_B(k_emit, LTL_UNTIL, B, g, b, s, e, e.aux, e.saux); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -184,6 +184,11 @@ struct ChunkArgs {
     // hold its traces.
     int32_t esplit = 1;
     const int32_t *gfirst = nullptr, *glast = nullptr;
+    // Packed generation, exact keys: the waves that fold one task's words (1,
+    // 2 or 4; a task is a tile, or R left rows of a binary stripe).  Set by
+    // the launchers for chunks of few tasks, where one wave walking every
+    // word of a set is the whole kernel's time; the partial sums meet in LDS.
+    int32_t gsplit = 1;
 };
 
 // Check passes visit words alternately from the positive and the negative
