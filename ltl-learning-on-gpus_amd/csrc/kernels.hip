@@ -73,25 +73,27 @@ __device__ __forceinline__ Split split_of(const ChunkArgs &A) {
     return s;
 }
 
-// Binary chunks: a workgroup covers per * R consecutive left rows of one
-// 64-column right stripe, task w rows per * R q + R w .. + R - 1 (t.a is the
-// first, t.ti its tile; row r's tile is t.ti + r * nbs).  Without a task
-// (false) t is an invalid lane of the chunk's first row.
+// Binary chunks: tasks of R consecutive left rows of one 64-column right
+// stripe, numbered stripe by stripe (task q: stripe q / nAg, rows R (q % nAg)
+// .. + R - 1 with nAg = ceil(rows / R)); a workgroup takes per consecutive
+// tasks, so no wave idles when the left operand has few rows (5 atoms: the
+// rows of one stripe, then the next stripe's).  t.a is the task's first row,
+// t.ti its tile; row r's tile is t.ti + r * nbs.  Without a task (false) t is
+// an invalid lane of the chunk's first row.
 template <int R = 1>
 __device__ __forceinline__ bool tile_binary(const ChunkArgs &A, Tile &t, const Split &sp) {
-    const int rows = sp.per * R;
-    const int nA = (int)(A.a1 - A.a0), nAq = (nA + rows - 1) / rows;
-    const int lw = xcd_remap((int)blockIdx.x, (int)gridDim.x);
-    const int bs = lw / nAq, aq = lw % nAq;
-    t.a = A.a0 + (int64_t)(sp.per * aq + sp.tw) * R;
-    if (t.a >= A.a1) {
+    const int64_t nAg = (A.a1 - A.a0 + R - 1) / R;
+    const int64_t q = (int64_t)xcd_remap((int)blockIdx.x, (int)gridDim.x) * sp.per + sp.tw;
+    const int64_t bs = q / nAg;
+    t.a = A.a0 + (q % nAg) * R;
+    if (bs >= A.nbs) {
         t.a = A.a0;
         t.b = A.b0;
         t.ti = 0;
         t.valid = false;
         return false;
     }
-    t.b = A.b0 - A.ush + 64 * (int64_t)bs + (threadIdx.x & 63);
+    t.b = A.b0 - A.ush + 64 * bs + (threadIdx.x & 63);
     t.valid = t.b >= A.b0 && t.b < A.b1;
     t.ti = (t.a - A.a0) * A.nbs + bs;
     return true;
@@ -1400,9 +1402,8 @@ __global__ void k_fill(uint64_t *p, uint64_t v, size_t n) {
 // ---------------------------------------------------------------------------
 // R: left rows per wave (tile_binary<R>); a.gsplit waves per task.
 static dim3 grid_binary(const ChunkArgs &a, int R = 1) {
-    const int rows = (4 / a.gsplit) * R;
-    const int64_t nAq = (a.a1 - a.a0 + rows - 1) / rows;
-    return dim3((unsigned)(nAq * a.nbs));
+    const int64_t per = 4 / a.gsplit, tasks = (a.a1 - a.a0 + R - 1) / R * a.nbs;
+    return dim3((unsigned)((tasks + per - 1) / per));
 }
 // Left rows per wave of the binary generation kernels (& and |; U keeps one
 // row: its two carry chains per word and candidate pair are the larger
@@ -1422,6 +1423,12 @@ static dim3 grid_tiles(const ChunkArgs &a) {
 // ~/X/F/G pass over 81 sets of config 2 took 55 us).  tasks: the waves of
 // the launch with one wave per task.
 static int gen_split(int64_t tasks, int nw) {
+    // (LTL_GEN_SPLIT = 1, 2 or 4 in the environment forces the choice: tests
+    // run every mode against the oracle)
+    if (const char *e = getenv("LTL_GEN_SPLIT")) {
+        const int f = atoi(e);
+        if (f == 1 || f == 2 || f == 4) return f;
+    }
     if (nw >= 64 && tasks <= 2048) return 4;
     if (nw >= 32 && tasks <= 4096) return 2;
     return 1;
@@ -1451,7 +1458,7 @@ static int64_t binary_tasks(const ChunkArgs &a, int R) { return (int64_t)a.nbs *
 // The same for k_gen2 with R left rows per wave (R = 1 with relaxed keys).
 #define LTL_DISPATCH_GEN2(OP, B, RX, S)                                                                  \
     if ((B) != 1) return hipErrorInvalidValue;                                                           \
-    if (a.t_succ) {                                                                                      \
+    if (a.t_succ) {                                                                               \
         if (RX || a.gen_check) return hipErrorInvalidValue;                                              \
         hipLaunchKernelGGL((k_gen2<OP, false, false, true, gen_rows<OP>()>), grid_binary(a, gen_rows<OP>()), dim3(256), 0, S, a); \
     } else if (RX) {                                                                                     \

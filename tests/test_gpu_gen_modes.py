@@ -1,0 +1,67 @@
+"""Every launch shape of the packed generation kernels gives the oracle's
+store.  The launchers pick, per chunk, how many waves fold one task's words
+(1, or 2 or 4 with the partial sums added in LDS, for chunks of few tasks).
+Test sizes take mostly the split shapes, so each is forced here through the
+environment (LTL_GEN_SPLIT = 1 / 2 / 4, read at every launch) and compared
+set for set with the oracle: results, counts,
+level boundaries, every stored set with its parents.  The instances cover
+generation with and without the target check (precheck 0 / 1), the fused
+~/X/F/G pass, non-unit costs, and deferred levels whose X / F / G successors
+are tested in generation (the successor-test kernel variants)."""
+import os
+
+import pytest
+
+from test_gpu_parity import check_parity
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(params=["1", "2", "4"], ids=lambda m: f"split{m}")
+def gen_mode(request):
+    old = os.environ.get("LTL_GEN_SPLIT")
+    os.environ["LTL_GEN_SPLIT"] = request.param
+    yield request.param
+    if old is None:
+        os.environ.pop("LTL_GEN_SPLIT", None)
+    else:
+        os.environ["LTL_GEN_SPLIT"] = old
+
+
+# (seed, P, N, atoms, len_min, len_max, max_cost): no formula up to max_cost,
+# so every level is stored; atoms 2-5 put left operands of 2-5 rows (tasks of
+# one stripe shared by a workgroup with the next stripe's) against levels of
+# hundreds to thousands of sets
+NOT_FOUND = [(31, 20, 13, 3, 9, 16, 5), (43, 30, 30, 3, 17, 32, 5), (75, 64, 64, 4, 5, 40, 4),
+             (18, 7, 9, 2, 2, 8, 5), (79, 64, 64, 5, 5, 40, 5)]
+
+
+@pytest.mark.parametrize("precheck", [0, 1])
+@pytest.mark.parametrize("shape", NOT_FOUND, ids=lambda s: "x".join(map(str, s)))
+def test_modes_full_store(ltl, oracle, learner, gen_mode, shape, precheck):
+    seed, P, N, A, lo, hi, mc = shape
+    inst = ltl.generate_custom(seed, P, N, A, lo, hi, 0.5, planted_size=9)
+    gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=mc, precheck=precheck)
+    assert gpu["status"] == 0
+
+
+def test_modes_found_and_costs(ltl, oracle, learner, gen_mode):
+    for seed in range(8):
+        inst = ltl.generate_custom(500 + seed, 12 + seed % 9, 10 + seed % 7, 3, 3, 10 + seed % 12, 0.5,
+                                   planted_size=5 + seed % 4)
+        check_parity(ltl, oracle, learner, inst, max_cost=9)
+    inst = ltl.generate_custom(77, 20, 20, 3, 4, 20, 0.5, planted_size=9)
+    check_parity(ltl, oracle, learner, inst, max_cost=8, cost=(1, 2, 1, 2, 1, 1, 2, 2))
+
+
+# the first binary hit of the last level is a U candidate: the deferred
+# level's generation tests the X / F / G successors (successor kernels)
+U_HIT = [("(F(a))U(b)", 1584, 17, 17, 5, 16, 0), ("X((a)U(b))", 1388, 15, 15, 3, 10, 2)]
+
+
+@pytest.mark.parametrize("case", U_HIT, ids=lambda c: f"{c[0]}-{c[1]}")
+def test_modes_successors_in_generation(ltl, oracle, learner, gen_mode, case):
+    planted, seed, P, N, lo, hi, k = case
+    inst = ltl.generate_custom(seed, P, N, 3, lo, hi, 0.5, planted=planted)
+    gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=9, hamming_k=k)
+    assert gpu["status"] == 1 and gpu["deferred_succ"] & 14
