@@ -469,6 +469,107 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
     }
 }
 
+// The binary blocks of one split in one pass (tested levels, packed
+// layout, exact keys): per pair (l, r) of rows a of the left level and
+// columns b of the right level, l & r (the & block, ids from A.id_base,
+// temporaries at A's), l | r (the | block, F.id_or, temporaries F.off_or
+// candidate slots further) and, with U, l U r and r U l (the U block,
+// F.id_u, F.off_u).  The operand words are read once for the three blocks;
+// the table keeps the smallest id, so inserting candidates of later blocks
+// early changes no outcome, provided each block is decided in canonical
+// order (with every earlier block committed).  R left rows per wave (& |:
+// 2; with U: 1).
+template <bool U, bool SUCC, int R>
+__global__ __launch_bounds__(256) void k_gen2f(ChunkArgs A, FusedBin F) {
+    constexpr int NC = U ? 4 : 2;
+    const Split sp = split_of(A);
+    const bool split = !SUCC && sp.S > 1;
+    if (!split && run_stopped(A)) return;
+    Tile t;
+    const bool live = tile_binary<R>(A, t, sp);
+    if (!split && !live) return;
+    const bool otf = chunk_otf(A);
+    const int nrow = (int)(A.a1 - t.a < R ? A.a1 - t.a : R);  // wave-uniform
+    const int64_t sr = A.rs + (t.valid ? t.b : A.b0);
+    const uint64_t *pr = A.store + set_base(sr, A.nw);
+    const uint64_t *pl[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) pl[r] = A.store + set_base(A.ls + t.a + (r < nrow ? r : 0), A.nw);
+    Acc c[R][NC];
+    SuccAcc sc[R][NC];
+    const int sm = A.succ_mask;
+    auto step = [&](int w, const uint64_t (&l)[R], uint64_t rv) {
+        const WordAux *x = A.aux + w;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            uint64_t y[NC];
+            y[0] = l[r] & rv;
+            y[1] = l[r] | rv;
+            if constexpr (U) {
+                y[2] = pk_until(l[r], rv, x->inner);
+                y[3] = pk_until(rv, l[r], x->inner);
+            }
+#pragma unroll
+            for (int j = 0; j < NC; ++j) {
+                fold<false>(c[r][j], y[j], x);
+                succ_fold<SUCC>(sc[r][j], y[j], x, sm);
+            }
+        }
+    };
+    constexpr int UB = U ? 4 : 8;
+    const int we = live ? sp.w1 : sp.w0;
+    int w = sp.w0;
+    for (; w + UB <= we; w += UB) {
+        uint64_t rv[UB];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) rv[u] = pr[(size_t)(w + u) * kTileWS];
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            uint64_t l[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) l[r] = pl[r][(size_t)(w + u) * kTileWS];
+            step(w + u, l, rv[u]);
+        }
+    }
+    for (; w < we; ++w) {
+        uint64_t l[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) l[r] = pl[r][(size_t)w * kTileWS];
+        step(w, l, pr[(size_t)w * kTileWS]);
+    }
+    if constexpr (!SUCC) {
+        if (split) {
+            uint64_t v[3 * NC * R];
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int j = 0; j < NC; ++j) acc_put(v + 3 * (NC * r + j), c[r][j]);
+            split_add(v, sp);
+            if (sp.r != 0 || !live || run_stopped(A)) return;
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+#pragma unroll
+                for (int j = 0; j < NC; ++j) acc_get(v + 3 * (NC * r + j), c[r][j]);
+        }
+    }
+    if (!t.valid || otf) return;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (r >= nrow) break;
+        const uint64_t pair = (uint64_t)(t.a + r) * (uint64_t)A.nR + (uint64_t)t.b;
+        const size_t ix = (size_t)(t.ti + (int64_t)r * A.nbs) * 64 + (threadIdx.x & 63);
+#pragma unroll
+        for (int j = 0; j < NC; ++j) {
+            const uint64_t id = j == 0 ? A.id_base + pair : j == 1 ? F.id_or + pair : F.id_u + 2 * pair + (uint64_t)(j - 2);
+            const size_t k = j == 0 ? ix : j == 1 ? (size_t)F.off_or + ix : (size_t)F.off_u + 2 * ix + (size_t)(j - 2);
+            A.t_slot[k] = tab_insert(A.tab, A.tmask, tab_key(c[r][j].f1, A.epoch), tab_val(kPending | id, A.epoch),
+                                     A.epoch, &A.st->probe_fail);
+            A.t_fp2[k] = c[r][j].f2;
+            if constexpr (SUCC) A.t_succ[k] = succ_flags(A, sc[r][j]);
+        }
+    }
+}
+
 // Multi-word traces (B = 0): K words per trace, combined from the top word
 // down.  & and | are word-wise and use k_gen2<OP, false>.
 template <int OP, int K>
@@ -1563,6 +1664,29 @@ hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, bool neg
         hipLaunchKernelGGL((k_gen_xfg<true, false, false>), gc, b, split_lds(c, 9), s, c, n, slots, a.id_base);
     } else {
         hipLaunchKernelGGL((k_gen_xfg<false, false, false>), gc, b, split_lds(c, 9), s, c, n, slots, a.id_base);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_gen_fused(const ChunkArgs &a, const FusedBin &f, bool with_until, hipStream_t s) {
+    if (a.kmode || a.ws != kTileWS || a.K > 1 || a.gen_check || a.t_acc) return hipErrorInvalidValue;
+    ChunkArgs b = a;
+    if (with_until) {
+        constexpr int R = 1, N = 3 * 4 * R;
+        if (a.t_succ) {
+            hipLaunchKernelGGL((k_gen2f<true, true, R>), grid_binary(a, R), dim3(256), 0, s, a, f);
+        } else {
+            b.gsplit = gen_split(binary_tasks(a, R), a.nw);
+            hipLaunchKernelGGL((k_gen2f<true, false, R>), grid_binary(b, R), dim3(256), split_lds(b, N), s, b, f);
+        }
+    } else {
+        constexpr int R = 2, N = 3 * 2 * R;
+        if (a.t_succ) {
+            hipLaunchKernelGGL((k_gen2f<false, true, R>), grid_binary(a, R), dim3(256), 0, s, a, f);
+        } else {
+            b.gsplit = gen_split(binary_tasks(a, R), a.nw);
+            hipLaunchKernelGGL((k_gen2f<false, false, R>), grid_binary(b, R), dim3(256), split_lds(b, N), s, b, f);
+        }
     }
     return hipGetLastError();
 }

@@ -212,6 +212,17 @@ hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s);
 // ids from id_base, temporaries at offset 0; then X, F, G have ids
 // id_xfg + j * n and temporaries at (j + 1) * slots.
 hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, bool neg, uint64_t id_xfg, hipStream_t s);
+// The &, | (and, with_until, U) blocks of one split (same operand levels:
+// equal costs of the three operators) in one generation pass of a tested
+// level, packed layout, exact keys.  The & block is a's (ids from
+// a.id_base, temporaries at a's); the | and U blocks' first ids and their
+// temporaries' offsets (candidate slots from a's) are f's; each block's
+// decide ... emit later runs at its canonical position with its own region.
+struct FusedBin {
+    uint64_t id_or, id_u;
+    int64_t off_or, off_u;
+};
+hipError_t launch_gen_fused(const ChunkArgs &a, const FusedBin &f, bool with_until, hipStream_t s);
 hipError_t launch_decide(int op, const ChunkArgs &a, hipStream_t s);
 // A chunk of at most kSmallTiles tiles (4096 candidates) is deduplicated by
 // one workgroup: decide, scan, finish and compact in a single launch.  Each of

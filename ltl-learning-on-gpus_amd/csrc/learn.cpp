@@ -1047,6 +1047,116 @@ struct Run {
         return 0;
     }
 
+    // Temporaries of a chunk at candidate slot `off` (the region scheme of
+    // set_tmp_off, for one chunk's arguments).
+    void region(ChunkArgs &a, int64_t off) const {
+        a.t_slot = c->t_slot + off;
+        a.t_fp2 = c->t_fp2 + off;
+        a.t_list = c->t_list + off;
+        a.t_mask = c->t_mask + off / 64;
+        a.t_cnt = c->t_cnt + off / 64;
+        a.t_off = c->t_off + off / 64;
+        a.t_succ = base.t_succ ? c->t_succ + off : nullptr;
+    }
+
+    // Binary blocks of a tested level with equal costs of &, | and U
+    // (level_blocks, fused path): per split (li, ri) one generation pass for
+    // its & and | blocks and, with_u, its U block; the & block is decided
+    // and emitted at once (every earlier block is committed), the | and U
+    // blocks wait in their regions for their canonical positions
+    // (finish_or after the last & block, finish_u after X, F and G).  Every
+    // block is one chunk, so the whole level's binary candidates must fit the
+    // temporaries (fused_fits).  id_u: the first U id of the level.
+    struct PendBin {
+        int op;
+        ChunkArgs a;
+        uint64_t size;
+    };
+    std::vector<PendBin> pend_or, pend_u;
+    struct Split2 {
+        int li, ri;
+        int64_t ls, nL, rs, nR, slots;
+    };
+    std::vector<Split2> splits_of(int cost, int c1, int cb) const {
+        std::vector<Split2> v;
+        for (int i = c1; 2 * i <= cost - cb; ++i) {
+            const int j = cost - i - cb;
+            const int64_t ls = c->lvl_s[i], nL = c->lvl_e[i] - ls, rs = c->lvl_s[j], nR = c->lvl_e[j] - rs;
+            if (nL <= 0 || nR <= 0) continue;
+            const int64_t nbs = (nR + (rs & 63) + 63) / 64;
+            v.push_back({i, j, ls, nL, rs, nR, nL * nbs * 64});
+        }
+        return v;
+    }
+    // Candidate slots the fused blocks of the level need (with_u: U too).
+    int64_t fused_slots(const std::vector<Split2> &v, bool with_u) const {
+        int64_t t = 0;
+        for (const auto &sp : v) t += (with_u ? 4 : 2) * sp.slots;
+        return t;
+    }
+    int32_t binary_fused(const std::vector<Split2> &v, bool with_u, uint64_t id_u) {
+        uint64_t and_total = 0;
+        for (const auto &sp : v) and_total += (uint64_t)sp.nL * (uint64_t)sp.nR;
+        uint64_t id_or = all_ltls + and_total;
+        int64_t off = tmp_off;
+        const uint64_t cs = (uint64_t)c->nw * 8;
+        for (const auto &sp : v) {
+            const uint64_t size = (uint64_t)sp.nL * (uint64_t)sp.nR;
+            ChunkArgs a = base;
+            a.id_base = all_ltls;
+            a.ls = sp.ls; a.rs = sp.rs; a.nR = sp.nR;
+            a.a0 = 0; a.a1 = sp.nL;
+            a.b0 = 0; a.b1 = sp.nR;
+            a.ush = (int32_t)(sp.rs & 63);
+            a.nbs = (int32_t)((sp.nR + a.ush + 63) / 64);
+            a.n_tiles = (int32_t)(sp.nL * a.nbs);
+            a.chunk_idx = ++c->chunk_counter;
+            region(a, off);
+            const FusedBin f{id_or, id_u, sp.slots, 2 * sp.slots};
+            const uint64_t cand = (with_u ? 4 : 2) * size;
+            const uint64_t alg = size * 2 * cs + cand * 20;
+            const uint64_t mn = (uint64_t)(sp.nL + sp.nR) * cs + cand * 20;
+            candidates += cand;
+            alg_bytes += alg;
+            HIP_TRY(timed(c, with_u ? K_GENU : K_GEN2, alg, mn, cand,
+                          [&] { return launch_gen_fused(a, f, with_u, c->stream); }));
+            reg({all_ltls, size, LTL_AND, sp.ls, sp.nL, sp.rs, sp.nR, cur_level});
+            if (!otf)
+                if (int32_t r = store_chunk(LTL_AND, a, size)) return r;
+            all_ltls += size;
+            ChunkArgs b = a;
+            b.id_base = id_or;
+            region(b, off + sp.slots);
+            pend_or.push_back({LTL_OR, b, size});
+            if (with_u) {
+                ChunkArgs u = a;
+                u.id_base = id_u;
+                region(u, off + 2 * sp.slots);
+                pend_u.push_back({LTL_UNTIL, u, 2 * size});
+            }
+            off += (with_u ? 4 : 2) * sp.slots;
+            id_or += size;
+            id_u += 2 * size;
+        }
+        return 0;
+    }
+    // Decide ... emit of the pending blocks of one kind, in order.
+    int32_t finish_pending(std::vector<PendBin> &v) {
+        for (auto &p : v) {
+            reg({all_ltls, p.size, p.op, p.a.ls, p.a.a1 - p.a.a0, p.a.rs, p.a.nR, cur_level});
+            if (all_ltls != p.a.id_base) {  // (ids fixed at generation must be the canonical ones)
+                report_hip(hipErrorUnknown, "finish_pending", __LINE__);
+                return LTL_E_HIP;
+            }
+            p.a.chunk_idx = ++c->chunk_counter;
+            if (!otf)
+                if (int32_t r = store_chunk(p.op, p.a, p.size)) return r;
+            all_ltls += p.size;
+        }
+        v.clear();
+        return 0;
+    }
+
     int32_t unary(int op, int64_t s, int64_t e) {
         const int64_t n = e - s;
         if (n <= 0 || (check_only && only_binary)) return 0;
@@ -1521,12 +1631,31 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             for (int i = c1; 2 * i <= cost - c4; ++i)
                 between += (uint64_t)std::max<int64_t>(0, Le[i] - Ls[i]) *
                            (uint64_t)std::max<int64_t>(0, Le[cost - i - c4] - Ls[cost - i - c4]);
-            if ((r = run.unary_nxfg(Ls[lvn], Le[lvn], run.all_ltls + (uint64_t)nn + between))) return r;
-            for (int i = c1; 2 * i <= cost - c3; ++i)
-                if ((r = run.binary(LTL_AND, i, cost - i - c3))) return r;
-            for (int i = c1; 2 * i <= cost - c4; ++i)
-                if ((r = run.binary(LTL_OR, i, cost - i - c4))) return r;
+            const uint64_t id_xfg = run.all_ltls + (uint64_t)nn + between;
+            if ((r = run.unary_nxfg(Ls[lvn], Le[lvn], id_xfg))) return r;
+            // & and | of each split in one pass (and U when the level's
+            // candidates fit the temporaries); else one pass per block
+            // (LTL_GEN_FUSE = 0 or 1 in the environment caps the fusion at
+            // none or & and |: tests run every path against the oracle)
+            int fused = 0, fuse_max = 2;
+            if (const char *e = getenv("LTL_GEN_FUSE")) fuse_max = std::max(0, std::min(2, atoi(e)));
+            std::vector<Run::Split2> sv;
+            if (c3 == c4 && B == 1 && fuse_max > 0) {
+                sv = run.splits_of(cost, c1, c3);
+                if (fuse_max == 2 && c8 == c3 && run.fused_slots(sv, true) <= run.ch_cap()) fused = 2;
+                else if (run.fused_slots(sv, false) <= run.ch_cap()) fused = 1;
+            }
+            if (fused) {
+                if ((r = run.binary_fused(sv, fused == 2, id_xfg + 3 * (uint64_t)nn))) return r;
+                if ((r = run.finish_pending(run.pend_or))) return r;
+            } else {
+                for (int i = c1; 2 * i <= cost - c3; ++i)
+                    if ((r = run.binary(LTL_AND, i, cost - i - c3))) return r;
+                for (int i = c1; 2 * i <= cost - c4; ++i)
+                    if ((r = run.binary(LTL_OR, i, cost - i - c4))) return r;
+            }
             if ((r = run.finish_xfg())) return r;
+            if (fused == 2) return run.finish_pending(run.pend_u);
             for (int i = c1; 2 * i <= cost - c8; ++i)
                 if ((r = run.binary(LTL_UNTIL, i, cost - i - c8))) return r;
             return 0;

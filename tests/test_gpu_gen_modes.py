@@ -1,13 +1,17 @@
 """Every launch shape of the packed generation kernels gives the oracle's
 store.  The launchers pick, per chunk, how many waves fold one task's words
-(1, or 2 or 4 with the partial sums added in LDS, for chunks of few tasks).
-Test sizes take mostly the split shapes, so each is forced here through the
-environment (LTL_GEN_SPLIT = 1 / 2 / 4, read at every launch) and compared
-set for set with the oracle: results, counts,
-level boundaries, every stored set with its parents.  The instances cover
-generation with and without the target check (precheck 0 / 1), the fused
-~/X/F/G pass, non-unit costs, and deferred levels whose X / F / G successors
-are tested in generation (the successor-test kernel variants)."""
+(1, or 2 or 4 with the partial sums added in LDS, for chunks of few tasks),
+and a tested level with equal costs of &, | and U generates each split's &
+and | blocks (and its U block, when the level's candidates fit the chunk
+temporaries) in one pass, deciding each block later at its canonical
+position.  Test sizes take only some of these, so each is forced here
+through the environment (LTL_GEN_SPLIT = 1 / 2 / 4 and LTL_GEN_FUSE = 0 /
+1 / 2, read at every launch or level) and compared set for set with the
+oracle: results, counts, level boundaries, every stored set with its
+parents.  The instances cover generation with and without the target check
+(precheck 0 / 1), the fused ~/X/F/G pass, non-unit costs, and deferred
+levels whose X / F / G successors are tested in generation (the
+successor-test kernel variants)."""
 import os
 
 import pytest
@@ -17,15 +21,21 @@ from test_gpu_parity import check_parity
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(params=["1", "2", "4"], ids=lambda m: f"split{m}")
+MODES = [("1", "2"), ("2", "2"), ("4", "2"), ("1", "1"), ("4", "1"), ("1", "0"), ("4", "0")]
+
+
+@pytest.fixture(params=MODES, ids=lambda m: f"split{m[0]}-fuse{m[1]}")
 def gen_mode(request):
-    old = os.environ.get("LTL_GEN_SPLIT")
-    os.environ["LTL_GEN_SPLIT"] = request.param
+    keys = ("LTL_GEN_SPLIT", "LTL_GEN_FUSE")
+    old = {k: os.environ.get(k) for k in keys}
+    for k, v in zip(keys, request.param):
+        os.environ[k] = v
     yield request.param
-    if old is None:
-        os.environ.pop("LTL_GEN_SPLIT", None)
-    else:
-        os.environ["LTL_GEN_SPLIT"] = old
+    for k, v in old.items():
+        if v is None:
+            os.environ.pop(k, None)
+        else:
+            os.environ[k] = v
 
 
 # (seed, P, N, atoms, len_min, len_max, max_cost): no formula up to max_cost,
