@@ -12,10 +12,12 @@ Prints one JSON line.  `value` = unique sets stored per second of whole-call
 wall time (every set deduplicated and stored with its parents; the last
 generated level, which no later pass reads in full, without its words and
 summaries: `words_deferred_per_step`); `ms_per_step` = time to the minimal
-formula.  `roofline` is the dominant kernel's algorithmic bytes per launch
-(operand sets counted once per candidate and operand, so re-reads from L2
-count too) over its HIP-event duration against the 8 TB/s HBM peak, with its
-physical rate beside it (profiled FETCH + WRITE bytes); `cpu_baseline` times
+formula.  `roofline` is the dominant kernel's HBM traffic per launch
+(rocprofv3 FETCH + WRITE bytes, calibrated on a device copy, from
+profiles/traffic.json) over its HIP-event duration against the 8 TB/s HBM
+peak (`achieved`, `frac`), with its algorithmic rate beside it (`alg_gbs`:
+operand sets counted once per candidate and operand, so re-reads served by
+L2 count too); `cpu_baseline` times
 the CPU oracle (the reference enumeration restated in C, OpenMP, every set
 stored in full) on a bounded sample of the same instance (levels up to cost
 9, about 5 s on 16 threads).  The extras (skipped with --no-extra) time
@@ -157,6 +159,8 @@ def kernel_kind(name):
         return "gen_until" if op == 7 else "gen_and_or"
     if k == "gen2m":
         return "gen_until"
+    if k == "gen2f":
+        return "gen_binary"
     if k in ("emit", "emitm"):
         return "emit_unary" if op in unary else "emit_and_or" if op in binary else "emit_until"
     if k == "emit_meta":
@@ -469,8 +473,11 @@ def long_traces(ltl, args):
             "min_cost": r["cost"], "formula": r["formula"],
             "uniques": r["n_stored"], "words_deferred": r["words_deferred"], "store_overflow": bool(r["on_the_fly"]),
             "kernels_ms": {k: round(v["seconds"] * 1e3, 3) for k, v in stats.items() if v["seconds"] > 0},
-            "dominant": {"kernel": dom, "achieved_gbs": round(d["alg_bytes"] / d["seconds"] / 1e9, 1),
-                         "frac": round(d["alg_bytes"] / d["seconds"] / HBM_PEAK, 4)}}
+            # (the algorithmic byte model: operand sets counted once per
+            # candidate and operand, re-reads served by L2 included, so the
+            # fraction of the 8 TB/s peak can exceed 1; no traffic counters)
+            "dominant": {"kernel": dom, "alg_gbs": round(d["alg_bytes"] / d["seconds"] / 1e9, 1),
+                         "alg_frac": round(d["alg_bytes"] / d["seconds"] / HBM_PEAK, 4)}}
 
 
 def main():

@@ -38,13 +38,15 @@ namespace {
 
 // (the dedup_path_* entries count the chunks each dedup path took, with or
 // without profiling, and hold no time: that is in decide and scan_finish)
+// (gen_binary: the &, | and U blocks of a split generated in one pass)
 enum { K_GEN1, K_GEN2, K_GENU, K_DECIDE, K_SCAN, K_EMIT1, K_EMIT2, K_EMITU, K_CHK1, K_CHK2, K_CHKU,
-       K_TSTAT, K_SCORE, K_SORT, K_CHKR, K_LOG, K_EMITM, K_PSMALL, K_PLB, K_PFOUR, K_NKIND };
+       K_TSTAT, K_SCORE, K_SORT, K_CHKR, K_LOG, K_EMITM, K_PSMALL, K_PLB, K_PFOUR, K_GENB, K_NKIND };
 const char *kKindName[K_NKIND] = {"gen_unary", "gen_and_or", "gen_until", "decide",
                                   "scan_finish", "emit_unary", "emit_and_or", "emit_until",
                                   "check_unary", "check_and_or", "check_until",
                                   "tile_stats", "score_tiles", "sort_tiles", "check_ranked", "log_rows",
-                                  "emit_meta", "dedup_path_small", "dedup_path_lookback", "dedup_path_four"};
+                                  "emit_meta", "dedup_path_small", "dedup_path_lookback", "dedup_path_four",
+                                  "gen_binary"};
 
 struct Block {
     uint64_t id_base, size;
@@ -480,7 +482,7 @@ static void drain_events(ltl_ctx *c) {
 // Kernel kinds that stream the store (generation and emit): the ones
 // profile = 2 times.
 static bool streaming_kind(int kind) {
-    return kind == K_GEN1 || kind == K_GEN2 || kind == K_GENU || kind == K_EMIT1 || kind == K_EMIT2 ||
+    return kind == K_GEN1 || kind == K_GEN2 || kind == K_GENU || kind == K_GENB || kind == K_EMIT1 || kind == K_EMIT2 ||
            kind == K_EMITU || kind == K_EMITM;
 }
 
@@ -1118,7 +1120,7 @@ struct Run {
             const uint64_t mn = (uint64_t)(sp.nL + sp.nR) * cs + cand * 20;
             candidates += cand;
             alg_bytes += alg;
-            HIP_TRY(timed(c, with_u ? K_GENU : K_GEN2, alg, mn, cand,
+            HIP_TRY(timed(c, K_GENB, alg, mn, cand,
                           [&] { return launch_gen_fused(a, f, with_u, c->stream); }));
             reg({all_ltls, size, LTL_AND, sp.ls, sp.nL, sp.rs, sp.nR, cur_level});
             if (!otf)
