@@ -159,7 +159,7 @@ def kernel_kind(name):
         return "gen_until" if op == 7 else "gen_and_or"
     if k == "gen2m":
         return "gen_until"
-    if k == "gen2f":
+    if k in ("gen2f", "gen2fs"):
         return "gen_binary"
     if k in ("emit", "emitm"):
         return "emit_unary" if op in unary else "emit_and_or" if op in binary else "emit_until"

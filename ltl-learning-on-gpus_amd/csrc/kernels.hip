@@ -480,7 +480,7 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
 // order (with every earlier block committed).  R left rows per wave (& |:
 // 2; with U: 1).
 template <bool U, bool SUCC, int R>
-__global__ __launch_bounds__(256) void k_gen2f(ChunkArgs A, FusedBin F) {
+__device__ __forceinline__ void gen2f_body(const ChunkArgs &A, const FusedBin &F) {
     constexpr int NC = U ? 4 : 2;
     const Split sp = split_of(A);
     const bool split = !SUCC && sp.S > 1;
@@ -568,6 +568,20 @@ __global__ __launch_bounds__(256) void k_gen2f(ChunkArgs A, FusedBin F) {
             if constexpr (SUCC) A.t_succ[k] = succ_flags(A, sc[r][j]);
         }
     }
+}
+
+// Without successor tests the pass fits 64 registers: 8 waves per SIMD
+// (6 at the compiler's own choice of 80) hide more of the operand loads'
+// latency (7-10% faster in a standalone microbenchmark of the pass); the
+// successor tests' counters need more registers (k_gen2fs: forced to 64
+// they spilled to scratch).
+template <bool U, int R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_gen2f(ChunkArgs A, FusedBin F) {
+    gen2f_body<U, false, R>(A, F);
+}
+template <bool U, int R>
+__global__ __launch_bounds__(256) void k_gen2fs(ChunkArgs A, FusedBin F) {
+    gen2f_body<U, true, R>(A, F);
 }
 
 // Multi-word traces (B = 0): K words per trace, combined from the top word
@@ -1072,9 +1086,15 @@ __global__ __launch_bounds__(256) void k_dedup_lb(ChunkArgs A) {
 // range boundary is written by both ranges, with the same value).  More waves
 // stream each tile's words in parallel: a chunk's survivors alone are too few
 // waves to hide the memory latency.
-template <int OP>
-__global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__restrict__ aux,
-                                              const SumAux *__restrict__ saux) {
+//
+// NX (ChunkArgs::nx_fp): the next level's ~, X, F and G of each survivor are
+// fingerprinted from the words computed here (each range hashes the words
+// it owns: a word shared with the previous range at a group boundary is that
+// range's); a workgroup is the esplit (<= 8) waves of one store tile, and
+// wave 0 adds the ranges' sums in LDS and stores them.
+template <int OP, bool NX = false>
+__global__ __launch_bounds__(NX ? 512 : 256) void k_emit(ChunkArgs A, const WordAux *__restrict__ aux,
+                                                         const SumAux *__restrict__ saux) {
     if (run_stopped(A) || chunk_otf(A) || look_back_failed(A)) return;
     // Waves cover whole store tiles: lane l of the waves of tile v writes
     // position 64 * (chunk_base / 64 + v) + l, so every store instruction of a
@@ -1082,14 +1102,16 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
     const int64_t base = A.st->chunk_base;
     const int K = A.esplit;
     // (wave-uniform: the range loops below run on the scalar unit)
-    const int64_t q = (int64_t)blockIdx.x * 4 + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t q = NX ? (int64_t)blockIdx.x * K + wv : (int64_t)blockIdx.x * 4 + wv;
     const int rg = (int)(q % K);
     const int64_t pos = (base & ~63ll) + (q / K) * 64 + (threadIdx.x & 63);
     const int64_t i = pos - base;
-    if (i < 0 || i >= A.st->chunk_total) return;
-    if (pos >= A.capacity) return;
+    // (NX: every lane stays until the workgroup's ranges are added)
+    const bool live = i >= 0 && i < A.st->chunk_total && pos < A.capacity;
+    if (!NX && !live) return;
     constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
-    const uint32_t k = A.t_list[i];           // temp index of the candidate
+    const uint32_t k = live ? A.t_list[i] : 0;  // temp index of the candidate
     const uint32_t ix = OP == LTL_UNTIL ? k >> 1 : k;
     const bool swap = OP == LTL_UNTIL && (k & 1);
     int64_t sl, sr = -1;
@@ -1148,6 +1170,9 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
             acc[v] = 0;
         }
     };
+    // NX: the next level's candidates ~c, X c, F c, G c, hashed from word hw0 on
+    Acc nx[4];
+    const int hw0 = NX && K > 1 && rg > 0 && A.glast[g0 - 1] == w0 ? w0 + 1 : w0;
     auto step = [&](int w, uint64_t l, uint64_t r) {
         const WordAux *x = aux + w;
         uint64_t c;
@@ -1157,23 +1182,81 @@ __global__ __launch_bounds__(256) void k_emit(ChunkArgs A, const WordAux *__rest
         else c = pk_unary<OP>(l, x);
         __builtin_nontemporal_store(c, qs + (size_t)w * kTileWS);  // streamed: not re-read in this chunk
         summarize_pk<VM>(c, x, acc, A.W, flush, g0, g1);
+        if constexpr (NX) {
+            if (w >= hw0) {
+                fold<false>(nx[0], pk_unary<LTL_NOT>(c, x), x);
+                fold<false>(nx[1], pk_unary<LTL_NEXT>(c, x), x);
+                fold<false>(nx[2], pk_unary<LTL_FIN>(c, x), x);
+                fold<false>(nx[3], pk_unary<LTL_GLOB>(c, x), x);
+            }
+        }
     };
     constexpr int UB = bin ? 8 : 16;
-    int w = w0;
-    for (; w + UB <= w1; w += UB) {
-        uint64_t l[UB], r[UB];
+    if (live) {
+        int w = w0;
+        for (; w + UB <= w1; w += UB) {
+            uint64_t l[UB], r[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) {
-            l[u] = pl[(size_t)(w + u) * kTileWS];
-            r[u] = bin ? pr[(size_t)(w + u) * kTileWS] : 0;
+            for (int u = 0; u < UB; ++u) {
+                l[u] = pl[(size_t)(w + u) * kTileWS];
+                r[u] = bin ? pr[(size_t)(w + u) * kTileWS] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
         }
-#pragma unroll
-        for (int u = 0; u < UB; ++u) step(w + u, l[u], r[u]);
+        for (; w < w1; ++w) step(w, pl[(size_t)w * kTileWS], bin ? pr[(size_t)w * kTileWS] : 0);
     }
-    for (; w < w1; ++w) step(w, pl[(size_t)w * kTileWS], bin ? pr[(size_t)w * kTileWS] : 0);
-    if (rg != 0) return;
+    if constexpr (NX) {
+        __shared__ uint64_t nx_lds[7][8][64];
+        const int lane = (int)(threadIdx.x & 63);
+        uint64_t v[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            v[2 * j] = nx[j].f1;
+            v[2 * j + 1] = nx[j].f2;
+        }
+        if (rg > 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) nx_lds[rg - 1][j][lane] = v[j];
+        }
+        __syncthreads();
+        if (rg != 0 || !live) return;
+        for (int r = 1; r < K; ++r) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) v[j] += nx_lds[r - 1][j][lane];
+        }
+        const int64_t nxi = pos - A.nx_ls + A.nx_ush;
+        if (nxi < 0 || nxi >= A.nx_stride) {
+            A.st->nx_over = 1;  // (the level is not taken from these fingerprints)
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) A.nx_fp[(size_t)j * (size_t)A.nx_stride + (size_t)nxi] = v[j];
+        }
+    }
+    if (rg != 0 || !live) return;
     if (bin) commit(A, pos, k, OP, (int32_t)(swap ? sr : sl), (int32_t)(swap ? sl : sr));
     else commit(A, pos, k, OP, (int32_t)sl, -1);
+}
+
+// ~, X, F and G blocks of a tested level (k_gen_xfg<false, false, true>'s
+// tail) from the fingerprints an emit of their operand level stored: one
+// lane per operand; ids and temporaries as there.
+__global__ __launch_bounds__(256) void k_insert_xfg(ChunkArgs A, int64_t n, int64_t slots, uint64_t id_xfg,
+                                                    const uint64_t *__restrict__ fp, int64_t stride) {
+    if (run_stopped(A) || chunk_otf(A)) return;
+    const int64_t ix = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (ix >= (int64_t)A.n_tiles * 64) return;
+    const int64_t a = A.a0 - A.ush + ix;
+    if (a < A.a0 || a >= A.a1) return;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const uint64_t id = (j == 0 ? A.id_base : id_xfg + (uint64_t)(j - 1) * (uint64_t)n) + (uint64_t)a;
+        const uint64_t f1 = fp[(size_t)(2 * j) * (size_t)stride + ix], f2 = fp[(size_t)(2 * j + 1) * (size_t)stride + ix];
+        const size_t k = (size_t)j * (size_t)slots + ix;
+        A.t_slot[k] = tab_insert(A.tab, A.tmask, tab_key(f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
+                                 &A.st->probe_fail);
+        A.t_fp2[k] = f2;
+    }
 }
 
 // Deferred level: survivors get their metadata and table commit only (no
@@ -1674,20 +1757,29 @@ hipError_t launch_gen_fused(const ChunkArgs &a, const FusedBin &f, bool with_unt
     if (with_until) {
         constexpr int R = 1, N = 3 * 4 * R;
         if (a.t_succ) {
-            hipLaunchKernelGGL((k_gen2f<true, true, R>), grid_binary(a, R), dim3(256), 0, s, a, f);
+            hipLaunchKernelGGL((k_gen2fs<true, R>), grid_binary(a, R), dim3(256), 0, s, a, f);
         } else {
             b.gsplit = gen_split(binary_tasks(a, R), a.nw);
-            hipLaunchKernelGGL((k_gen2f<true, false, R>), grid_binary(b, R), dim3(256), split_lds(b, N), s, b, f);
+            hipLaunchKernelGGL((k_gen2f<true, R>), grid_binary(b, R), dim3(256), split_lds(b, N), s, b, f);
         }
     } else {
         constexpr int R = 2, N = 3 * 2 * R;
         if (a.t_succ) {
-            hipLaunchKernelGGL((k_gen2f<false, true, R>), grid_binary(a, R), dim3(256), 0, s, a, f);
+            hipLaunchKernelGGL((k_gen2fs<false, R>), grid_binary(a, R), dim3(256), 0, s, a, f);
         } else {
             b.gsplit = gen_split(binary_tasks(a, R), a.nw);
-            hipLaunchKernelGGL((k_gen2f<false, false, R>), grid_binary(b, R), dim3(256), split_lds(b, N), s, b, f);
+            hipLaunchKernelGGL((k_gen2f<false, R>), grid_binary(b, R), dim3(256), split_lds(b, N), s, b, f);
         }
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_insert_xfg(const ChunkArgs &a, int64_t n, int64_t slots, uint64_t id_xfg, const uint64_t *fp,
+                             int64_t stride, hipStream_t s) {
+    if (a.kmode || a.ws != kTileWS || a.K > 1 || a.t_succ || a.gen_check || !fp) return hipErrorInvalidValue;
+    if ((int64_t)a.n_tiles * 64 > stride) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_insert_xfg, dim3((unsigned)(((int64_t)a.n_tiles * 64 + 255) / 256)), dim3(256), 0, s, a, n,
+                       slots, id_xfg, fp, stride);
     return hipGetLastError();
 }
 
@@ -1711,9 +1803,25 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
     // A chunk of few store tiles: up to 16 word ranges per tile (ranges of at
     // least 16 words), since each wave's walk over its range sets the time.
     if (K > 1 && pos_tiles <= 256) K = std::max(K, std::min({16, a.nsw, a.nw / 16}));
+    const bool nx = B == 1 && !a.deferred && a.nx_fp;
+    if (nx) K = std::min(K, 8);  // (one workgroup of K waves per store tile)
     ChunkArgs e = a;
     e.esplit = K;
     const dim3 g((unsigned)((pos_tiles * K + 3) / 4)), b(256);
+    if (nx) {
+        const dim3 gx((unsigned)pos_tiles), bx((unsigned)(64 * K));
+        switch (op) {
+        case LTL_NOT: hipLaunchKernelGGL((k_emit<LTL_NOT, true>), gx, bx, 0, s, e, e.aux, e.saux); break;
+        case LTL_NEXT: hipLaunchKernelGGL((k_emit<LTL_NEXT, true>), gx, bx, 0, s, e, e.aux, e.saux); break;
+        case LTL_FIN: hipLaunchKernelGGL((k_emit<LTL_FIN, true>), gx, bx, 0, s, e, e.aux, e.saux); break;
+        case LTL_GLOB: hipLaunchKernelGGL((k_emit<LTL_GLOB, true>), gx, bx, 0, s, e, e.aux, e.saux); break;
+        case LTL_AND: hipLaunchKernelGGL((k_emit<LTL_AND, true>), gx, bx, 0, s, e, e.aux, e.saux); break;
+        case LTL_OR: hipLaunchKernelGGL((k_emit<LTL_OR, true>), gx, bx, 0, s, e, e.aux, e.saux); break;
+        case LTL_UNTIL: hipLaunchKernelGGL((k_emit<LTL_UNTIL, true>), gx, bx, 0, s, e, e.aux, e.saux); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     if (a.deferred) {
         if (op < LTL_NOT || op > LTL_UNTIL) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_emit_meta, g, b, 0, s, a, op);

@@ -83,7 +83,7 @@ struct DevState {
     // index of a set whose successor is accepted, kEmpty = none
     unsigned long long succ_min[4];
     int lb_fail;                  // a look-back wait ran out (the chunk was not stored)
-    int pad_;
+    int nx_over;                  // an emit's next-level fingerprints did not fit ChunkArgs::nx_stride
     // Relaxed keys (dedup modes 1-3): the smallest id of an accepted
     // candidate that survived dedup, folded into `found` after the chunk's
     // dedup launch (a dedup launch that lowered `found` itself would stop
@@ -184,6 +184,17 @@ struct ChunkArgs {
     // hold its traces.
     int32_t esplit = 1;
     const int32_t *gfirst = nullptr, *glast = nullptr;
+    // Next level's unary candidates from an emit (packed layout, exact
+    // keys): with nx_fp set, the emit also computes ~c, X c, F c and G c of
+    // every survivor c and stores their two fingerprints, value v (f1, f2 of
+    // ~, X, F, G in turn) of the survivor at store position p at
+    // nx_fp[v * nx_stride + p - nx_ls + nx_ush] -- the temp index that the
+    // next level's unary chunk over this level gives it (nx_ls: the level's
+    // first store index, nx_ush = nx_ls % 64).  launch_insert_xfg then puts
+    // them in the table in place of the ~/X/F/G generation pass.
+    uint64_t *nx_fp = nullptr;
+    int64_t nx_ls = 0, nx_stride = 0;
+    int32_t nx_ush = 0;
     // Packed generation, exact keys: the waves that fold one task's words (1,
     // 2 or 4; a task is a tile, or R left rows of a binary stripe).  Set by
     // the launchers for chunks of few tasks, where one wave walking every
@@ -212,6 +223,12 @@ hipError_t launch_gen(int op, int B, const ChunkArgs &a, hipStream_t s);
 // ids from id_base, temporaries at offset 0; then X, F, G have ids
 // id_xfg + j * n and temporaries at (j + 1) * slots.
 hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, bool neg, uint64_t id_xfg, hipStream_t s);
+// The same blocks (~ and X, F, G; neg, a tested level) from fingerprints an
+// emit of the operand level computed (ChunkArgs::nx_fp, `fp` with value
+// stride `stride`): the table inserts and temporaries of k_gen_xfg, without
+// reading the operand level again.
+hipError_t launch_insert_xfg(const ChunkArgs &a, int64_t n, int64_t slots, uint64_t id_xfg, const uint64_t *fp,
+                             int64_t stride, hipStream_t s);
 // The &, | (and, with_until, U) blocks of one split (same operand levels:
 // equal costs of the three operators) in one generation pass of a tested
 // level, packed layout, exact keys.  The & block is a's (ids from

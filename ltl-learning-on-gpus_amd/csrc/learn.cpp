@@ -134,6 +134,10 @@ struct ltl_ctx {
     uint64_t *d_stage = nullptr;  // atom words and summaries, contiguous, before the scatter
     uint32_t *d_kinfo = nullptr;  // per trace: relaxed-key bit range (dedup modes 1-3)
     uint64_t *d_lb = nullptr;     // look-back status words of a chunk's workgroups + ticket
+    // next level's ~/X/F/G fingerprints from a level's emits (8 values of
+    // nx_stride entries; ChunkArgs::nx_fp)
+    uint64_t *d_nxfp = nullptr;
+    int64_t nx_stride = 0;
     int64_t lb_words = 0;
     uint32_t lb_epoch = 0;        // chunks deduplicated by look-back in this call
     uint64_t *d_ptrace = nullptr; // packed layout: per packed trace, trace | offset << 32 | length << 40
@@ -354,7 +358,8 @@ static int32_t carve(ltl_ctx *c) {
                    2 * align_up(tiles * 4, 256) + align_up(c->scan_bytes, 256) +
                    align_up((size_t)(c->nw + 4 * c->nsw) * c->A * 8, 256) + align_up((size_t)c->W * 4, 256) +
                    align_up((size_t)c->W * 8, 256) + align_up((tiles / 4 + 2) * 8, 256) +
-                   align_up((1 + 3 * (size_t)kTreeCap) * 4, 256) + align_up((size_t)c->nsw * 8, 256) + 14 * 256;
+                   align_up((1 + 3 * (size_t)kTreeCap) * 4, 256) + align_up((size_t)c->nsw * 8, 256) +
+                   align_up(8 * 8 * ((size_t)ch / 8 + 128), 256) + 15 * 256;
     if (fixed >= budget) return LTL_E_OOM;
     const size_t R = budget - fixed;
     // Table: load stays <= 1/2 with a full chunk pending (cap + ch <= slots/2).
@@ -403,6 +408,9 @@ static int32_t carve(ltl_ctx *c) {
     c->d_gword = (int32_t *)take((size_t)c->nsw * 8);
     c->lb_words = (int64_t)(tiles / 4 + 2);  // a status word per workgroup, the ticket in the last
     c->d_lb = (uint64_t *)take((size_t)c->lb_words * 8);
+    // (a level of at most CH / 8 sets: what the fused ~/X/F/G pass over it takes)
+    c->nx_stride = ch / 8 + 128;
+    c->d_nxfp = (uint64_t *)take(8 * 8 * (size_t)c->nx_stride);
     c->d_tab = (uint64_t *)take(slots * 16);
     const int64_t capr = align_up((size_t)cap, 64);
     c->ws = 64;  // word stride inside a 64-set tile (tile-major store)
@@ -987,6 +995,10 @@ struct Run {
     // that come between them in canonical order.  Their ids (from id_xfg)
     // are larger than every & and | id, so the table's smallest-id rule
     // gives the outcome of the canonical sequence.
+    // First store index of the level whose emits computed the next level's
+    // ~/X/F/G fingerprints (ChunkArgs::nx_fp), -1: none; unary_nxfg over that
+    // level takes them instead of generating the four blocks.
+    int64_t nx_from = -1;
     struct PendXfg {
         bool on = false;
         int64_t n = 0, tiles = 0, slots = 0;
@@ -1010,8 +1022,14 @@ struct Run {
         const uint64_t alg = (uint64_t)n * (cs + 80);
         candidates += 4 * (uint64_t)n;
         alg_bytes += alg;
-        HIP_TRY(timed(c, K_GEN1, alg, alg, 4 * (uint64_t)n,
-                      [&] { return launch_gen_xfg(a, n, slots, true, id_xfg, c->stream); }));
+        // (the operand level's emits left the four blocks' fingerprints: no
+        // second read of the level, only the inserts)
+        const bool from_fp = nx_from == s && !a.t_succ;
+        nx_from = -1;
+        HIP_TRY(timed(c, K_GEN1, alg, alg, 4 * (uint64_t)n, [&] {
+            return from_fp ? launch_insert_xfg(a, n, slots, id_xfg, c->d_nxfp, c->nx_stride, c->stream)
+                           : launch_gen_xfg(a, n, slots, true, id_xfg, c->stream);
+        }));
         ChunkArgs b = a;
         b.id_end = b.id_base + (uint64_t)n;
         if (int32_t r = store_chunk(LTL_NOT, b, (uint64_t)n)) return r;
@@ -1864,6 +1882,14 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         return r;
     };
     std::vector<Block> ahead_blocks;  // the lookahead's blocks (ids relative to their first)
+    // (an emit's fingerprints overran their buffer: not again this call;
+    // LTL_GEN_NX=0 in the environment: never, for the tests of both paths)
+    bool nx_off = false, nx_force = false;
+    if (const char *e = getenv("LTL_GEN_NX")) {
+        nx_off = atoi(e) == 0;
+        nx_force = atoi(e) == 2;  // (every level, whatever its size)
+    }
+    constexpr int64_t kNxMinOperands = 1 << 18;
     for (cost = c1 + 1; cost <= maxc; ++cost) {
         run.level_stored = n_stored;
         bool ahead_done = false;          // the lookahead of cost + 1 ran with the check pass
@@ -2153,14 +2179,36 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
                     break;
                 }
             }
+            // The next level's ~, X, F and G over this one are fingerprinted
+            // by this level's emits (plain order, unit unary costs: that
+            // level's fused ~/X/F/G pass reads this level), for large levels
+            // only: the extra arithmetic lengthens every emit wave, which a
+            // level of large chunks hides better than the read it saves
+            // costs (config 2, per 1000 sets: the next level's pass 0.76 us,
+            // the inserts that replace it 0.22 us; the emits' extra 0.30 us
+            // on level 9, generated from 0.7 M sets, 0.57 us on level 8,
+            // from 0.11 M).
+            const int64_t n_prev = Le[cost - 1] - Ls[cost - 1];
+            const bool nx = !lazy && B == 1 && !run.base.kmode && !learned && !run.logging && !run.otf && !nx_off &&
+                            o.neg_type == 1 && c2 == 1 && c5 == 1 && c6 == 1 && c7 == 1 && cost < maxc &&
+                            (n_prev >= kNxMinOperands || nx_force);
+            if (nx) {
+                run.base.nx_fp = c->d_nxfp;
+                run.base.nx_ls = n_stored;
+                run.base.nx_ush = (int32_t)(n_stored & 63);
+                run.base.nx_stride = c->nx_stride;
+            }
             rc = level_blocks(cost);
             run.base.gen_check = 1;
             run.base.deferred = 0;
             run.base.succ_mask = 0;
             run.base.t_succ = nullptr;
             run.base.neg_pick = kEmpty;
+            run.base.nx_fp = nullptr;
             if (rc) break;
             if ((rc = sync_state())) break;
+            if (h.nx_over) nx_off = true;
+            run.nx_from = nx && !nx_off && h.overflow_chunk == LLONG_MAX ? n_stored : -1;
             if (lazy && h.n_stored > n_stored) {
                 c->lazy_s = n_stored;
                 c->lazy_e = h.n_stored;

@@ -4,11 +4,13 @@ store.  The launchers pick, per chunk, how many waves fold one task's words
 and a tested level with equal costs of &, | and U generates each split's &
 and | blocks (and its U block, when the level's candidates fit the chunk
 temporaries) in one pass, deciding each block later at its canonical
-position.  Test sizes take only some of these, so each is forced here
-through the environment (LTL_GEN_SPLIT = 1 / 2 / 4 and LTL_GEN_FUSE = 0 /
-1 / 2, read at every launch or level) and compared set for set with the
-oracle: results, counts, level boundaries, every stored set with its
-parents.  The instances cover generation with and without the target check
+position; with unit unary costs the emits of a level also fingerprint the
+next level's ~, X, F and G of every stored set, so that the next level's
+unary blocks need only their table inserts.  Test sizes take only some of
+these, so each is forced here through the environment (LTL_GEN_SPLIT = 1 /
+2 / 4, LTL_GEN_FUSE = 0 / 1 / 2, LTL_GEN_NX = 0 / 2 -- never / on every level,
+not only the large ones -- read at every launch, level or call) and compared set for set with the oracle: results, counts,
+level boundaries, every stored set with its parents.  The instances cover generation with and without the target check
 (precheck 0 / 1), the fused ~/X/F/G pass, non-unit costs, and deferred
 levels whose X / F / G successors are tested in generation (the
 successor-test kernel variants)."""
@@ -21,12 +23,13 @@ from test_gpu_parity import check_parity
 pytestmark = pytest.mark.gpu
 
 
-MODES = [("1", "2"), ("2", "2"), ("4", "2"), ("1", "1"), ("4", "1"), ("1", "0"), ("4", "0")]
+MODES = [("1", "2", "2"), ("2", "2", "2"), ("4", "2", "2"), ("1", "1", "2"), ("4", "1", "2"), ("1", "0", "2"),
+         ("4", "0", "2"), ("1", "2", "0"), ("4", "0", "0")]
 
 
-@pytest.fixture(params=MODES, ids=lambda m: f"split{m[0]}-fuse{m[1]}")
+@pytest.fixture(params=MODES, ids=lambda m: f"split{m[0]}-fuse{m[1]}-nx{m[2]}")
 def gen_mode(request):
-    keys = ("LTL_GEN_SPLIT", "LTL_GEN_FUSE")
+    keys = ("LTL_GEN_SPLIT", "LTL_GEN_FUSE", "LTL_GEN_NX")
     old = {k: os.environ.get(k) for k in keys}
     for k, v in zip(keys, request.param):
         os.environ[k] = v
