@@ -1191,7 +1191,7 @@ __global__ __launch_bounds__(NX ? 512 : 256) void k_emit(ChunkArgs A, const Word
             }
         }
     };
-    constexpr int UB = NX ? (bin ? 4 : 8) : (bin ? 8 : 16);
+    constexpr int UB = bin ? 4 : 8;
     if (live) {
         int w = w0;
         for (; w + UB <= w1; w += UB) {
