@@ -27,15 +27,15 @@ __device__ __forceinline__ uint64_t suffix_or64(uint64_t x) {
 // Strong until, y_p = r_p | (l_p & y_{p+1}), as one carry chain: with the
 // bits reversed (position p -> bit 63 - p) the recurrence runs upward,
 // Y_q = R_q | (L_q & Y_{q-1}), which is exactly the carry out of bit q of
-// (L | R) + R + cin (generate R_q, propagate L_q).  The carries into the bits
-// of a sum are sum ^ a ^ b, so Y = that >> 1 with the carry out of the top
-// bit on top; cin is the result just above the word (multi-word traces).
-// About a third of the operations of the six-step doubling prefix.
+// a + R + cin with a = L | R (generate R_q, propagate L_q); cin is the result
+// just above the word (multi-word traces).  The carry out of bit q of a sum
+// a + b is (a_q & b_q) | ((a_q | b_q) & ~s_q); as R is a subset of a, that is
+// R_q where s_q is set and a_q where it is not -- one bit select, no shift
+// and no carry out of the top bit (its own bit gives it).
 __device__ __forceinline__ uint64_t until_carry64(uint64_t l, uint64_t r, uint64_t cin) {
     const uint64_t L = __builtin_bitreverse64(l), R = __builtin_bitreverse64(r), a = L | R;
-    const unsigned __int128 t = (unsigned __int128)a + R + cin;
-    const uint64_t s = (uint64_t)t, cout = (uint64_t)(t >> 64);
-    return __builtin_bitreverse64(((s ^ a ^ R) >> 1) | (cout << 63));
+    const uint64_t s = a + R + cin;
+    return __builtin_bitreverse64((s & R) | (~s & a));
 }
 
 // NH compression step (Black et al., UMAC, CRYPTO 1999): the two 32-bit
@@ -172,15 +172,13 @@ __device__ inline uint32_t tab_insert(uint64_t *tab, uint32_t tmask, uint64_t ke
 //          carry crosses from one trace into the next
 //   F f = true U f,  G f = ~F ~f
 // ---------------------------------------------------------------------------
-// Y_b = g_b | (p_b & Y_{b-1}): the carries into the bits of (p | g) + g are
-// sum ^ a ^ b; shifted down one they are the carries out of each bit.
+// Y_b = g_b | (p_b & Y_{b-1}) is the carry out of bit b of s = a + g with
+// a = p | g: (a_b & g_b) | ((a_b | g_b) & ~s_b), which, g being a subset of
+// a, selects g_b where s_b is set and a_b where it is not (v_bfi_b32 per
+// half; with the add and the or, 6 instructions per word).
 __device__ __forceinline__ uint64_t pk_chain(uint64_t p, uint64_t g) {
-    const uint64_t a = p | g;
-    unsigned c1, c2;  // carries out of bits 31 and 63
-    const uint32_t lo = __builtin_addc((uint32_t)a, (uint32_t)g, 0u, &c1);
-    const uint32_t hi = __builtin_addc((uint32_t)(a >> 32), (uint32_t)(g >> 32), c1, &c2);
-    const uint64_t s = (uint64_t)lo | ((uint64_t)hi << 32);
-    return ((s ^ a ^ g) >> 1) | ((uint64_t)c2 << 63);
+    const uint64_t a = p | g, s = a + g;
+    return (s & g) | (~s & a);
 }
 
 __device__ __forceinline__ uint64_t pk_until(uint64_t l, uint64_t r, uint64_t inner) {
