@@ -38,12 +38,20 @@ __device__ __forceinline__ uint64_t until_carry64(uint64_t l, uint64_t r, uint64
     return __builtin_bitreverse64((s & R) | (~s & a));
 }
 
-// NH compression step (Black et al., UMAC, CRYPTO 1999): the two 32-bit
-// halves plus key halves, multiplied to 64 bits, summed mod 2^64.
-__device__ __forceinline__ uint64_t nh(uint64_t c, uint64_t k) {
-    const uint32_t lo = (uint32_t)c + (uint32_t)k;
-    const uint32_t hi = (uint32_t)(c >> 32) + (uint32_t)(k >> 32);
-    return (uint64_t)lo * (uint64_t)hi;
+// Fingerprints are multilinear hashes (Carter and Wegman's multilinear
+// family, as in MMH, Halevi and Krawczyk, FSE 1997, without its prime
+// reduction): the sum mod 2^64 of every 32-bit half of every word times its
+// own 32-bit key, each product exact in 64 bits.  Two different sets differ
+// in some half, by d with 0 < |d| < 2^32; for the other keys fixed, two
+// values k != k' of that half's key give products that differ by
+// (k - k') d, nonzero and below 2^64 in magnitude, so at most one key value
+// makes the sums equal: Pr[collision] <= 2^-32 per fingerprint, independent
+// keys for the two.  One multiply-add per half (v_mad_u64_u32), and the hash
+// is linear: since (l & r) + (l | r) = l + r half by half as integers,
+// H(l | r) = H(l) + H(r) - H(l & r) mod 2^64.
+__device__ __forceinline__ void ml_fold(uint64_t &f, uint64_t c, uint64_t k) {
+    f += (uint64_t)(uint32_t)c * (uint64_t)(uint32_t)k;
+    f += (uint64_t)(uint32_t)(c >> 32) * (uint64_t)(uint32_t)(k >> 32);
 }
 
 struct Acc {
@@ -56,8 +64,8 @@ struct Acc {
 // record, so that only what is used occupies scalar registers.
 template <bool CHK = true>
 __device__ __forceinline__ void fold(Acc &a, uint64_t c, const WordAux *x) {
-    a.f1 += nh(c, x->k1);
-    a.f2 += nh(c, x->k2);
+    ml_fold(a.f1, c, x->k1);
+    ml_fold(a.f2, c, x->k2);
     if constexpr (CHK) a.mism += (uint32_t)__popcll((c ^ x->tgt) & x->chk);
 }
 
@@ -105,7 +113,7 @@ __device__ __forceinline__ void flush_summary(uint64_t *z, int i, int n, int nsw
     v[0] = v[1] = v[2] = v[3] = 0;
 }
 
-// Table key from fingerprint 1: a 64-bit finalizer spreads the NH sum over
+// Table key from fingerprint 1: a 64-bit finalizer spreads the hash sum over
 // all bits (slot index = low bits); the empty key is never produced.
 __device__ __forceinline__ uint64_t table_key(uint64_t f) {
     f ^= f >> 33; f *= 0xff51afd7ed558ccdull;

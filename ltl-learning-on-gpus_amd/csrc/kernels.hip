@@ -1,7 +1,7 @@
 // kernels.hip -- characteristic-set enumeration kernels for gfx950 (wave64).
 //
 // Per chunk of candidates the host enqueues, on one stream:
-//   gen     compute every candidate set word by word, fold it into two NH
+//   gen     compute every candidate set word by word, fold it into two multilinear
 //           fingerprints and the target check, test it (atomicMin of the
 //           canonical id of accepted candidates), and intern fingerprint 1 in
 //           the open-addressing table (64-bit atomicCAS on the key, atomicMin
@@ -120,7 +120,7 @@ __device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t, const Sp
 __device__ __forceinline__ bool tile_unary(const ChunkArgs &A, Tile &t) { return tile_unary(A, t, split_of(A)); }
 
 // The S waves of a task meet before its tail: waves 1 .. S-1 leave their
-// sums (NH fingerprints mod 2^64, mismatch counts) in LDS, wave 0 adds them.
+// sums (fingerprints mod 2^64, mismatch counts) in LDS, wave 0 adds them.
 // Every wave of the workgroup reaches the barrier (S is uniform); waves
 // without a task take part with zeros.  LDS: (4 - 4 / S) * N * 512 bytes.
 template <int N>
@@ -691,7 +691,7 @@ __global__ __launch_bounds__(256) void k_gen2m(ChunkArgs A) {  // U
 
 // Atoms: interned with committed values (store index = atom index), so that
 // the first of equal atoms wins (code/ltli6463.cu:144-173).
-// One wave per atom, lanes over the words; the NH sums (mod 2^64) are
+// One wave per atom, lanes over the words; the hash sums (mod 2^64) are
 // reduced across the wave.
 __global__ __launch_bounds__(64) void k_atoms(ChunkArgs A, int n_atoms) {
     const int a = (int)blockIdx.x, l = (int)threadIdx.x;

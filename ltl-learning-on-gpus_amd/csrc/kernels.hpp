@@ -40,7 +40,7 @@ constexpr uint32_t kEpochs = 14;            // table epochs (1..14) between two 
 
 // Per packed word: dedup hash keys and check masks (wave-uniform).
 struct WordAux {
-    uint64_t k1, k2;  // NH keys of the two fingerprints
+    uint64_t k1, k2;  // multilinear keys of the two fingerprints (two 32-bit halves each)
     uint64_t lm;      // bits inside the traces (length mask)
     uint64_t chk;     // position 0 of every trace in the word
     uint64_t tgt;     // position 0 of every positive trace in the word
@@ -130,7 +130,7 @@ struct ChunkArgs {
                               // traces); summary words holding positive traces, ceil(P / 64)
     int32_t neg_slot;         // anti: neg_found entry written
     // Reference relaxed uniqueness keys (dedup modes 1-3 above 126 trace
-    // positions): kmode 0 = exact (NH fingerprints), else the mode; per
+    // positions): kmode 0 = exact (multilinear fingerprints), else the mode; per
     // trace (first key bit << 8) | key bits taken (modes 1, 2); the stride
     // of mode 1; words holding traces that take key bits.
     const uint32_t *kinfo;
