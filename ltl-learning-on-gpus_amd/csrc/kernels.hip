@@ -28,6 +28,10 @@
 namespace ltl {
 namespace {
 
+// Word 3 of a raw buffer resource: 32-bit data format (nonzero, so that the
+// descriptor is valid), no swizzle, no stride.
+constexpr int kBufferWord3 = 0x00020000;
+
 // Workgroups of a binary chunk are laid out so that consecutive workgroups
 // dispatched to one XCD (dispatch is round-robin over the 8 XCDs) walk the
 // left rows of the same right stripe, which then stays in that XCD's L2.
@@ -490,52 +494,92 @@ __device__ __forceinline__ void gen2f_body(const ChunkArgs &A, const FusedBin &F
     if (!split && !live) return;
     const bool otf = chunk_otf(A);
     const int nrow = (int)(A.a1 - t.a < R ? A.a1 - t.a : R);  // wave-uniform
-    const int64_t sr = A.rs + (t.valid ? t.b : A.b0);
-    const uint64_t *pr = A.store + set_base(sr, A.nw);
+    // The right operand is the stripe's tile (lane l: set 64 bs + l - ush of
+    // the level, the first lane at a tile start), read with buffer loads
+    // bounded to that tile: word w of lane l at byte 512 w + 8 l.  Lanes past
+    // the block's columns read their own sets of the tile (unused).  Loads of
+    // the next batch of words are issued before this batch is folded.
+    const int64_t bs = t.ti - (t.a - A.a0) * A.nbs;
+    const int64_t s_first = A.rs + A.b0 - A.ush + 64 * bs;
+    const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(A.store + set_base(s_first, A.nw)), (short)0, A.nw * 512, kBufferWord3);
+    const int lane8 = (int)(threadIdx.x & 63) * 8;
+    auto rload = [&](int w) -> uint64_t {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(rr, lane8, w * 512, 0);
+        return (uint64_t)v[0] | ((uint64_t)v[1] << 32);
+    };
     const uint64_t *pl[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) pl[r] = A.store + set_base(A.ls + t.a + (r < nrow ? r : 0), A.nw);
     Acc c[R][NC];
     SuccAcc sc[R][NC];
     const int sm = A.succ_mask;
-    auto step = [&](int w, const uint64_t (&l)[R], uint64_t rv) {
+    // per word: the left rows' words and the word's keys (wave-uniform,
+    // scalar loads), loaded for a whole batch of words before any is used so
+    // that the batch waits for its scalar loads once
+    struct Word {
+        uint64_t l[R], k1, k2, inner;
+    };
+    auto load = [&](int w, Word &d) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) d.l[r] = pl[r][(size_t)w * kTileWS];
+        d.k1 = A.aux[w].k1;
+        d.k2 = A.aux[w].k2;
+        if constexpr (U) d.inner = A.aux[w].inner;
+    };
+    auto step = [&](int w, const Word &d, uint64_t rv) {
         const WordAux *x = A.aux + w;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             uint64_t y[NC];
-            y[0] = l[r] & rv;
-            y[1] = l[r] | rv;
+            y[0] = d.l[r] & rv;
+            y[1] = d.l[r] | rv;
             if constexpr (U) {
-                y[2] = pk_until(l[r], rv, x->inner);
-                y[3] = pk_until(rv, l[r], x->inner);
+                y[2] = pk_until(d.l[r], rv, d.inner);
+                y[3] = pk_until(rv, d.l[r], d.inner);
             }
 #pragma unroll
             for (int j = 0; j < NC; ++j) {
-                fold<false>(c[r][j], y[j], x);
+                ml_fold(c[r][j].f1, y[j], d.k1);
+                ml_fold(c[r][j].f2, y[j], d.k2);
                 succ_fold<SUCC>(sc[r][j], y[j], x, sm);
             }
         }
     };
-    constexpr int UB = U ? 4 : 8;
+    constexpr int UB = 4;
     const int we = live ? sp.w1 : sp.w0;
     int w = sp.w0;
-    for (; w + UB <= we; w += UB) {
-        uint64_t rv[UB];
+    // Two batches per iteration in alternate registers (no copies, which
+    // would wait for the loads in flight); past the last batch the last word
+    // is loaded again, unused.
+    auto batch = [&](int w0, const uint64_t (&rv)[UB]) {
+        Word d[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) rv[u] = pr[(size_t)(w + u) * kTileWS];
+        for (int u = 0; u < UB; ++u) load(w0 + u, d[u]);
 #pragma unroll
-        for (int u = 0; u < UB; ++u) {
-            uint64_t l[R];
+        for (int u = 0; u < UB; ++u) step(w0 + u, d[u], rv[u]);
+    };
+    uint64_t ra[UB], rb[UB];
+    if (w + UB <= we) {
 #pragma unroll
-            for (int r = 0; r < R; ++r) l[r] = pl[r][(size_t)(w + u) * kTileWS];
-            step(w + u, l, rv[u]);
-        }
+        for (int u = 0; u < UB; ++u) ra[u] = rload(w + u);
+    }
+    for (; w + 2 * UB <= we; w += 2 * UB) {
+#pragma unroll
+        for (int u = 0; u < UB; ++u) rb[u] = rload(w + UB + u);
+        batch(w, ra);
+#pragma unroll
+        for (int u = 0; u < UB; ++u) ra[u] = rload(w + 2 * UB + u < we ? w + 2 * UB + u : we - 1);
+        batch(w + UB, rb);
+    }
+    if (w + UB <= we) {
+        batch(w, ra);
+        w += UB;
     }
     for (; w < we; ++w) {
-        uint64_t l[R];
-#pragma unroll
-        for (int r = 0; r < R; ++r) l[r] = pl[r][(size_t)w * kTileWS];
-        step(w, l, pr[(size_t)w * kTileWS]);
+        Word d;
+        load(w, d);
+        step(w, d, rload(w));
     }
     if constexpr (!SUCC) {
         if (split) {
