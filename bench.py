@@ -11,7 +11,8 @@ uploaded before the timed region; each call re-initialises its device state.
 Prints one JSON line.  `value` = unique sets stored per second of whole-call
 wall time (every set deduplicated and stored with its parents; the last
 generated level, which no later pass reads in full, without its words and
-summaries: `words_deferred_per_step`); `ms_per_step` = time to the minimal
+summaries: `words_deferred_per_step`; the level below it, when large, without
+its words: `words_unwritten_per_step`); `ms_per_step` = time to the minimal
 formula.  `roofline` is the dominant kernel's HBM traffic per launch
 (rocprofv3 FETCH + WRITE bytes, calibrated on a device copy, from
 profiles/traffic.json) over its HIP-event duration against the 8 TB/s HBM
@@ -585,9 +586,11 @@ def main():
                    "traces": inst.W, "atoms": int(inst.atoms.shape[0]), "max_cost": args.max_cost},
         "time_to_min_formula_s": round(wall / args.steps, 4),
         "min_cost": r0["cost"], "formula": r0["formula"], "uniques_per_step": r0["n_stored"], "words_deferred_per_step": r0["words_deferred"],
+        "words_unwritten_per_step": r0["words_unwritten"],
         # sets stored with their words and summaries (the deferred last level,
-        # stored with parents only, left out), per second of call time
-        "stored_full_per_s": round((uniques - r0["words_deferred"] * args.steps) / wall, 1),
+        # stored with parents only, and the level below it when stored without
+        # words, left out), per second of call time
+        "stored_full_per_s": round((uniques - (r0["words_deferred"] + r0["words_unwritten"]) * args.steps) / wall, 1),
         "candidates_per_step": r0["candidates"], "checked_per_step": r0["checked"],
         # (timed steps: the dominant kind only; every kind: one profiled call)
         "kernels_ms_per_step": {dom: round(d["seconds"] / args.steps * 1e3, 3)},

@@ -135,6 +135,11 @@ typedef struct {
     int32_t  retries;     /* repeats of the call with fresh fingerprint keys
                              after a detected fingerprint collision (the
                              result does not depend on the keys) */
+    int64_t  words_unwritten; /* sets of the level below the last generated
+                               * one stored with summaries and parents but
+                               * without words (plain order, large levels:
+                               * no later pass reads them in bulk); the next
+                               * ltl_store_read recomputes them */
 } ltl_result;
 
 typedef struct ltl_ctx ltl_ctx;

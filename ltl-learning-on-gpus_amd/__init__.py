@@ -64,7 +64,7 @@ class Result(ctypes.Structure):
                 ("kernel_seconds", ctypes.c_double), ("hbm_bytes", ctypes.c_uint64),
                 ("candidates", ctypes.c_uint64), ("checked", ctypes.c_uint64),
                 ("words_deferred", ctypes.c_int64), ("deferred_succ", ctypes.c_int32),
-                ("retries", ctypes.c_int32)]
+                ("retries", ctypes.c_int32), ("words_unwritten", ctypes.c_int64)]
 
 
 class KernelStat(ctypes.Structure):
@@ -355,6 +355,7 @@ class Learner:
                    formula=(res.formula or b"").decode(), seconds=res.seconds,
                    kernel_seconds=res.kernel_seconds, hbm_bytes=res.hbm_bytes,
                    candidates=res.candidates, checked=res.checked, words_deferred=res.words_deferred,
+                   words_unwritten=res.words_unwritten,
                    deferred_succ=res.deferred_succ, retries=res.retries,
                    nodes=[(res.nodes[i].op, res.nodes[i].atom, res.nodes[i].left, res.nodes[i].right)
                           for i in range(res.n_nodes)])

@@ -1224,7 +1224,8 @@ __global__ __launch_bounds__(NX ? 512 : 256) void k_emit(ChunkArgs A, const Word
         else if constexpr (OP == LTL_OR) c = l | r;
         else if constexpr (OP == LTL_UNTIL) c = swap ? pk_until(r, l, x->inner) : pk_until(l, r, x->inner);
         else c = pk_unary<OP>(l, x);
-        __builtin_nontemporal_store(c, qs + (size_t)w * kTileWS);  // streamed: not re-read in this chunk
+        // streamed: not re-read in this chunk
+        if (!NX || !A.wskip) __builtin_nontemporal_store(c, qs + (size_t)w * kTileWS);
         summarize_pk<VM>(c, x, acc, A.W, flush, g0, g1);
         if constexpr (NX) {
             if (w >= hw0) {
@@ -1480,6 +1481,42 @@ __global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
 // check-only pass (no dedup, no store)
 // ---------------------------------------------------------------------------
 // ~, X, F, G: one summary vector of the operand (b0 negated, b1, nz, fl).
+// Words of a stored set, which may lie in the range whose words were not
+// written: then word w is recomputed from the set's parents (whose words
+// are written), as k_materialize does.
+struct SetWords {
+    const uint64_t *p, *pa, *pb;
+    int op;  // -1: the words are at p
+};
+__device__ __forceinline__ bool in_lzw(const ChunkArgs &A, int64_t s) { return s >= A.lzw_s && s < A.lzw_e; }
+__device__ __forceinline__ SetWords set_words(const ChunkArgs &A, int64_t s) {
+    SetWords d;
+    d.p = d.pa = d.pb = A.store + set_base(s, A.nw);
+    d.op = -1;
+    if (in_lzw(A, s)) {
+        d.op = A.m_op[s];
+        const int64_t l = A.m_left[s], r = A.m_right[s] >= 0 ? A.m_right[s] : l;
+        d.pa = A.store + set_base(l, A.nw);
+        d.pb = A.store + set_base(r, A.nw);
+    }
+    return d;
+}
+__device__ __forceinline__ uint64_t word_at(const ChunkArgs &A, const SetWords &d, int w) {
+    if (d.op < 0) return d.p[(size_t)w * kTileWS];
+    const WordAux *x = A.aux + w;
+    const uint64_t a = d.pa[(size_t)w * kTileWS], b = d.pb[(size_t)w * kTileWS];
+    switch (d.op) {
+    case LTL_NOT: return pk_unary<LTL_NOT>(a, x);
+    case LTL_NEXT: return pk_unary<LTL_NEXT>(a, x);
+    case LTL_FIN: return pk_unary<LTL_FIN>(a, x);
+    case LTL_GLOB: return pk_unary<LTL_GLOB>(a, x);
+    case LTL_AND: return a & b;
+    case LTL_OR: return a | b;
+    case LTL_UNTIL: return pk_until(a, b, x->inner);
+    default: return a;
+    }
+}
+
 template <int OP>
 __global__ __launch_bounds__(256) void k_chk1(ChunkArgs A) {
     if (run_stopped(A)) return;
@@ -1550,26 +1587,41 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
         m1 += (uint32_t)__popcll((pk_until(r, l, x->inner) ^ x->tgt) & x->chk);
         seen += (uint32_t)__popcll(x->chk);
     };
-    constexpr int UB = 4;
-    int i = 0;
-    for (; i + UB <= A.nw; i += UB) {
-        uint64_t l[UB], r[UB];
-        int wu[UB];
-#pragma unroll
-        for (int u = 0; u < UB; ++u) {
-            wu[u] = interleave(i + u, A.npw, A.nw);
-            l[u] = pl[(size_t)wu[u] * kTileWS];
-            r[u] = pr[(size_t)wu[u] * kTileWS];
+    auto done = [&]() {
+        return !__any(m0 <= k || m1 <= k || (anti && (seen - m0 <= k || seen - m1 <= k)));
+    };
+    // An operand level whose words were not written (a level is in the range
+    // as a whole, so the test is uniform): its words recomputed from the
+    // parents, word by word, until the wave is done.
+    if (in_lzw(A, A.ls) || in_lzw(A, A.rs)) {
+        const SetWords L = set_words(A, sl), Rw = set_words(A, sr);
+        for (int i = 0; i < A.nw; ++i) {
+            const int w = interleave(i, A.npw, A.nw);
+            step(w, word_at(A, L, w), word_at(A, Rw, w));
+            if ((i & 3) == 3 && done()) return;
         }
+    } else {
+        constexpr int UB = 4;
+        int i = 0;
+        for (; i + UB <= A.nw; i += UB) {
+            uint64_t l[UB], r[UB];
+            int wu[UB];
 #pragma unroll
-        for (int u = 0; u < UB; ++u) step(wu[u], l[u], r[u]);
-        // Mismatch and match counts only grow: stop once every lane of the
-        // wave fails (with anti: the complements too).
-        if (!__any(m0 <= k || m1 <= k || (anti && (seen - m0 <= k || seen - m1 <= k)))) return;
-    }
-    for (; i < A.nw; ++i) {
-        const int w = interleave(i, A.npw, A.nw);
-        step(w, pl[(size_t)w * kTileWS], pr[(size_t)w * kTileWS]);
+            for (int u = 0; u < UB; ++u) {
+                wu[u] = interleave(i + u, A.npw, A.nw);
+                l[u] = pl[(size_t)wu[u] * kTileWS];
+                r[u] = pr[(size_t)wu[u] * kTileWS];
+            }
+#pragma unroll
+            for (int u = 0; u < UB; ++u) step(wu[u], l[u], r[u]);
+            // Mismatch and match counts only grow: stop once every lane of the
+            // wave fails (with anti: the complements too).
+            if (done()) return;
+        }
+        for (; i < A.nw; ++i) {
+            const int w = interleave(i, A.npw, A.nw);
+            step(w, pl[(size_t)w * kTileWS], pr[(size_t)w * kTileWS]);
+        }
     }
     if (!t.valid) return;
     const uint64_t id0 = A.id_base + 2 * ((uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b);

@@ -195,6 +195,13 @@ struct ChunkArgs {
     uint64_t *nx_fp = nullptr;
     int64_t nx_ls = 0, nx_stride = 0;
     int32_t nx_ush = 0;
+    // wskip: such an emit writes the survivors' summaries, metadata and
+    // fingerprints but not their words.  The store range [lzw_s, lzw_e) holds
+    // sets emitted so (one level): the check kernels that read words
+    // recompute a word of such a set from its parents, whose words are
+    // written; launch_materialize writes them.
+    int32_t wskip = 0;
+    int64_t lzw_s = 0, lzw_e = 0;
     // Packed generation, exact keys: the waves that fold one task's words (1,
     // 2 or 4; a task is a tile, or R left rows of a binary stripe).  Set by
     // the launchers for chunks of few tasks, where one wave walking every

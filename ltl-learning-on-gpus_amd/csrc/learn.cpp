@@ -151,6 +151,12 @@ struct ltl_ctx {
     // ltl_store_summaries.
     int64_t lazy_s = 0, lazy_e = 0;
     ChunkArgs lazy_args{};
+    // Stored sets [lzw_s, lzw_e) (the level below the last generated one)
+    // were emitted with summaries and metadata but without words; the check
+    // kernels recompute their words from the parents, and launch_materialize
+    // writes them before anything reads them in bulk.
+    int64_t lzw_s = 0, lzw_e = 0;
+    ChunkArgs lzw_args{};
 
     // profiling
     ltl_kernel_stat stats[K_NKIND]{};
@@ -1441,6 +1447,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     c->pk_off.swap(prep.pk_off);
     c->lens.assign(pb->lens, pb->lens + W);
     c->lazy_s = c->lazy_e = 0;
+    c->lzw_s = c->lzw_e = 0;
     if (noalloc) {
         c->arena = c->scratch;
         c->arena_bytes = LTL_NOALLOC_SCRATCH_BYTES;
@@ -1889,6 +1896,17 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
         nx_off = atoi(e) == 0;
         nx_force = atoi(e) == 2;  // (every level, whatever its size)
     }
+    // LTL_GEN_LZW=0 in the environment: levels fingerprinted by their emits
+    // still write their words (for the tests of both paths)
+    bool lzw_off = false;
+    if (const char *e = getenv("LTL_GEN_LZW")) lzw_off = atoi(e) == 0;
+    auto materialize_lzw = [&]() -> int32_t {
+        if (c->lzw_e <= c->lzw_s) return 0;
+        if (launch_materialize(c->lzw_args, c->lzw_s, c->lzw_e, s) != hipSuccess) return LTL_E_HIP;
+        c->lzw_s = c->lzw_e = 0;
+        run.base.lzw_s = run.base.lzw_e = 0;
+        return 0;
+    };
     constexpr int64_t kNxMinOperands = 1 << 18;
     for (cost = c1 + 1; cost <= maxc; ++cost) {
         run.level_stored = n_stored;
@@ -2056,6 +2074,8 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
                 // summaries are recomputed, and the pass repeated if it skipped
                 // blocks over it)
                 if (c->lazy_e > c->lazy_s) {
+                    // (its parents first: the level below may lack its words)
+                    if ((rc = materialize_lzw())) break;
                     if (launch_materialize(c->lazy_args, c->lazy_s, c->lazy_e, s) != hipSuccess) {
                         rc = LTL_E_HIP;
                         break;
@@ -2166,6 +2186,11 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             }
         }
         if (generate) {
+            // The level below, emitted without its words: they are written
+            // now unless this level is deferred without successor tests (its
+            // emits and its ~/X/F/G pass, which then inserts the fingerprints
+            // those emits computed, read nothing of it; nor do later passes).
+            if (c->lzw_e > c->lzw_s && !(lazy && !(succ_mask & 14)) && (rc = materialize_lzw())) break;
             // a tested level has no accepted candidate: generation skips the check
             run.base.gen_check = tested ? 0 : 1;
             if (lazy) {
@@ -2192,11 +2217,19 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             const bool nx = !lazy && B == 1 && !run.base.kmode && !learned && !run.logging && !run.otf && !nx_off &&
                             o.neg_type == 1 && c2 == 1 && c5 == 1 && c6 == 1 && c7 == 1 && cost < maxc &&
                             (n_prev >= kNxMinOperands || nx_force);
+            // Such a level's emits also leave its words unwritten: the next
+            // level reads it only through those fingerprints (unless it is
+            // generated in full or with successor tests: then the words are
+            // written first, above), and the levels after that only through
+            // the summaries and the until check, which recomputes the words
+            // it reads from the parents (k_chkU).
+            const bool wskip = nx && !lzw_off;
             if (nx) {
                 run.base.nx_fp = c->d_nxfp;
                 run.base.nx_ls = n_stored;
                 run.base.nx_ush = (int32_t)(n_stored & 63);
                 run.base.nx_stride = c->nx_stride;
+                run.base.wskip = wskip ? 1 : 0;
             }
             rc = level_blocks(cost);
             run.base.gen_check = 1;
@@ -2205,10 +2238,21 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             run.base.t_succ = nullptr;
             run.base.neg_pick = kEmpty;
             run.base.nx_fp = nullptr;
+            run.base.wskip = 0;
             if (rc) break;
             if ((rc = sync_state())) break;
             if (h.nx_over) nx_off = true;
             run.nx_from = nx && !nx_off && h.overflow_chunk == LLONG_MAX ? n_stored : -1;
+            if (wskip && h.n_stored > n_stored) {
+                c->lzw_s = n_stored;
+                c->lzw_e = h.n_stored;
+                c->lzw_args = run.base;
+                run.base.lzw_s = c->lzw_s;
+                run.base.lzw_e = c->lzw_e;
+                // (no fingerprints, or a store that overflowed: the next
+                // levels read the words as they are)
+                if (run.nx_from < 0 && (rc = materialize_lzw())) break;
+            }
             if (lazy && h.n_stored > n_stored) {
                 c->lazy_s = n_stored;
                 c->lazy_e = h.n_stored;
@@ -2243,6 +2287,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     res->candidates = run.candidates;
     res->checked = run.checked;
     res->words_deferred = c->lazy_e - c->lazy_s;
+    res->words_unwritten = c->lzw_e - c->lzw_s;
     res->deferred_succ = run.defer_s >= 0 ? run.succ_mask : 0;
     res->hbm_bytes = run.alg_bytes;
     for (auto &k : c->stats) res->kernel_seconds += k.seconds;
@@ -2503,6 +2548,7 @@ extern "C" int32_t ltl_learn_batch(ltl_ctx *c, const ltl_problem *pbs, int32_t n
         });
 
         c->lazy_s = c->lazy_e = 0;
+        c->lzw_s = c->lzw_e = 0;
         if (int32_t e = ensure_pool(c)) return e;
         c->tab_ok = false;  // (the regions overwrite the pool)
         const ltl_batch_params &bp = c->batch;
@@ -2707,16 +2753,30 @@ extern "C" int32_t ltl_levels(const ltl_ctx *c, int64_t *start, int64_t *end, in
     return n;
 }
 
+// Words of the sets emitted without them: the level below the last
+// generated one first (the deferred level's parents are there), then the
+// deferred level (words and summaries).
+static hipError_t materialize_pending(ltl_ctx *c) {
+    if (c->lzw_e > c->lzw_s) {
+        const hipError_t e = launch_materialize(c->lzw_args, c->lzw_s, c->lzw_e, c->stream);
+        if (e != hipSuccess) return e;
+        c->lzw_s = c->lzw_e = 0;
+    }
+    if (c->lazy_e > c->lazy_s) {
+        const hipError_t e = launch_materialize(c->lazy_args, c->lazy_s, c->lazy_e, c->stream);
+        if (e != hipSuccess) return e;
+        c->lazy_s = c->lazy_e = 0;
+    }
+    return hipSuccess;
+}
+
 extern "C" int32_t ltl_store_read(ltl_ctx *c, int64_t first, int64_t count, uint64_t *cs, uint8_t *op,
                                   int32_t *left, int32_t *right) {
     if (!c || first < 0 || count < 0 || !c->W) return LTL_E_ARG;
     if (count == 0) return 0;
     if (first + count > c->capacity) return LTL_E_ARG;
     HIP_TRY(hipSetDevice(c->opt.device));
-    if (c->lazy_e > c->lazy_s) {  // a deferred level: its words and summaries first
-        HIP_TRY(launch_materialize(c->lazy_args, c->lazy_s, c->lazy_e, c->stream));
-        c->lazy_s = c->lazy_e = 0;
-    }
+    HIP_TRY(materialize_pending(c));
     HIP_TRY(hipStreamSynchronize(c->stream));
     // the whole tiles holding [first, first + count), one contiguous copy
     const int64_t t0 = first / 64, t1 = (first + count - 1) / 64 + 1;
@@ -2750,10 +2810,7 @@ extern "C" int32_t ltl_store_summaries(ltl_ctx *c, int64_t first, int64_t count,
     if (count == 0) return 0;
     if (first + count > c->capacity) return LTL_E_ARG;
     HIP_TRY(hipSetDevice(c->opt.device));
-    if (c->lazy_e > c->lazy_s) {  // a deferred level: its words and summaries first
-        HIP_TRY(launch_materialize(c->lazy_args, c->lazy_s, c->lazy_e, c->stream));
-        c->lazy_s = c->lazy_e = 0;
-    }
+    HIP_TRY(materialize_pending(c));
     HIP_TRY(hipStreamSynchronize(c->stream));
     const int nsw = c->nsw, W = c->W;
     const int64_t t0 = first / 64, t1 = (first + count - 1) / 64 + 1;

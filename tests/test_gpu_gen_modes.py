@@ -6,11 +6,15 @@ and | blocks (and its U block, when the level's candidates fit the chunk
 temporaries) in one pass, deciding each block later at its canonical
 position; with unit unary costs the emits of a level also fingerprint the
 next level's ~, X, F and G of every stored set, so that the next level's
-unary blocks need only their table inserts.  Test sizes take only some of
-these, so each is forced here through the environment (LTL_GEN_SPLIT = 1 /
-2 / 4, LTL_GEN_FUSE = 0 / 1 / 2, LTL_GEN_NX = 0 / 2 -- never / on every level,
-not only the large ones -- read at every launch, level or call) and compared set for set with the oracle: results, counts,
-level boundaries, every stored set with its parents.  The instances cover generation with and without the target check
+unary blocks need only their table inserts, and those emits leave the words
+unwritten (the check of until recomputes the words it reads from the
+parents; the words are written before the next level is generated in full,
+or by the first store read).  Test sizes take only some of these, so each is
+forced here through the environment (LTL_GEN_SPLIT = 1 / 2 / 4, LTL_GEN_FUSE
+= 0 / 1 / 2, LTL_GEN_NX = 0 / 2 -- never / on every level, not only the large
+ones --, LTL_GEN_LZW = 0 -- the fingerprinting emits write the words; read at
+every launch, level or call) and compared set for set with the oracle:
+results, counts, level boundaries, every stored set with its parents.  The instances cover generation with and without the target check
 (precheck 0 / 1), the fused ~/X/F/G pass, non-unit costs, and deferred
 levels whose X / F / G successors are tested in generation (the
 successor-test kernel variants)."""
@@ -23,13 +27,14 @@ from test_gpu_parity import check_parity
 pytestmark = pytest.mark.gpu
 
 
-MODES = [("1", "2", "2"), ("2", "2", "2"), ("4", "2", "2"), ("1", "1", "2"), ("4", "1", "2"), ("1", "0", "2"),
-         ("4", "0", "2"), ("1", "2", "0"), ("4", "0", "0")]
+MODES = [("1", "2", "2", "1"), ("2", "2", "2", "1"), ("4", "2", "2", "1"), ("1", "1", "2", "1"), ("4", "1", "2", "1"),
+         ("1", "0", "2", "1"), ("4", "0", "2", "1"), ("1", "2", "2", "0"), ("4", "0", "2", "0"), ("1", "2", "0", "1"),
+         ("4", "0", "0", "1")]
 
 
-@pytest.fixture(params=MODES, ids=lambda m: f"split{m[0]}-fuse{m[1]}-nx{m[2]}")
+@pytest.fixture(params=MODES, ids=lambda m: f"split{m[0]}-fuse{m[1]}-nx{m[2]}-lzw{m[3]}")
 def gen_mode(request):
-    keys = ("LTL_GEN_SPLIT", "LTL_GEN_FUSE", "LTL_GEN_NX")
+    keys = ("LTL_GEN_SPLIT", "LTL_GEN_FUSE", "LTL_GEN_NX", "LTL_GEN_LZW")
     old = {k: os.environ.get(k) for k in keys}
     for k, v in zip(keys, request.param):
         os.environ[k] = v
@@ -56,6 +61,13 @@ def test_modes_full_store(ltl, oracle, learner, gen_mode, shape, precheck):
     inst = ltl.generate_custom(seed, P, N, A, lo, hi, 0.5, planted_size=9)
     gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=mc, precheck=precheck)
     assert gpu["status"] == 0
+    # Level mc - 1 is fingerprinted by its emits (forced); with the precheck
+    # level mc is deferred, so mc - 1's words stay unwritten until the store
+    # is read; without it level mc is generated in full and they are written
+    # before that.
+    s, e = learner.levels()
+    want = e[mc - 1] - s[mc - 1] if gen_mode[2] == "2" and gen_mode[3] == "1" and precheck else 0
+    assert gpu["words_unwritten"] == want
 
 
 def test_modes_found_and_costs(ltl, oracle, learner, gen_mode):
