@@ -628,6 +628,59 @@ __global__ __launch_bounds__(256) void k_gen2fs(ChunkArgs A, FusedBin F) {
     gen2f_body<U, true, R>(A, F);
 }
 
+// Multi-word traces (K words per trace): the fused binary pass of a tested
+// level, as k_gen2f -- per pair (l, r) l & r, l | r and, with U, l U r and
+// r U l (the until over a trace's K words from the top word down), each
+// fingerprinted and inserted into its block's region at its canonical id.
+// One left row per wave, one wave per task; no successor tests (multi-word
+// levels are not deferred with them).
+template <bool U, int K>
+__global__ __launch_bounds__(256) void k_gen2fm(ChunkArgs A, FusedBin F) {
+    if (run_stopped(A)) return;
+    Tile t;
+    if (!tile_binary(A, t)) return;
+    const bool otf = chunk_otf(A);
+    const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
+    const uint64_t *pl = A.store + set_base(sl, A.nw);
+    const uint64_t *pr = A.store + set_base(sr, A.nw);
+    constexpr int NC = U ? 4 : 2;
+    Acc c[NC];
+    const int T = A.nw / K;
+    for (int i = 0; i < T; ++i) {
+        uint64_t l[K], r[K], u0[K], u1[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            l[k] = pl[(size_t)(i * K + k) * kTileWS];
+            r[k] = pr[(size_t)(i * K + k) * kTileWS];
+        }
+        if constexpr (U) {
+            trace_until<K>(l, r, u0);
+            trace_until<K>(r, l, u1);
+        }
+        const WordAux *aux = A.aux + i * K;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            fold<false>(c[0], l[k] & r[k], aux + k);
+            fold<false>(c[1], l[k] | r[k], aux + k);
+            if constexpr (U) {
+                fold<false>(c[2], u0[k], aux + k);
+                fold<false>(c[3], u1[k], aux + k);
+            }
+        }
+    }
+    if (!t.valid || otf) return;
+    const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
+    const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
+#pragma unroll
+    for (int j = 0; j < NC; ++j) {
+        const uint64_t id = j == 0 ? A.id_base + pair : j == 1 ? F.id_or + pair : F.id_u + 2 * pair + (uint64_t)(j - 2);
+        const size_t k = j == 0 ? ix : j == 1 ? (size_t)F.off_or + ix : (size_t)F.off_u + 2 * ix + (size_t)(j - 2);
+        A.t_slot[k] = tab_insert(A.tab, A.tmask, tab_key(c[j].f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
+                                 &A.st->probe_fail);
+        A.t_fp2[k] = c[j].f2;
+    }
+}
+
 // Multi-word traces (B = 0): K words per trace, combined from the top word
 // down.  & and | are word-wise and use k_gen2<OP, false>.
 template <int OP, int K>
@@ -1848,7 +1901,21 @@ hipError_t launch_gen_xfg(const ChunkArgs &a, int64_t n, int64_t slots, bool neg
 }
 
 hipError_t launch_gen_fused(const ChunkArgs &a, const FusedBin &f, bool with_until, hipStream_t s) {
-    if (a.kmode || a.ws != kTileWS || a.K > 1 || a.gen_check || a.t_acc) return hipErrorInvalidValue;
+    if (a.kmode || a.ws != kTileWS || a.gen_check || a.t_acc) return hipErrorInvalidValue;
+    if (a.K > 1) {
+        if (a.t_succ || a.gsplit != 1) return hipErrorInvalidValue;
+        const dim3 g = grid_binary(a, 1);
+        switch (a.K * 2 + (with_until ? 1 : 0)) {
+        case 4: hipLaunchKernelGGL((k_gen2fm<false, 2>), g, dim3(256), 0, s, a, f); break;
+        case 5: hipLaunchKernelGGL((k_gen2fm<true, 2>), g, dim3(256), 0, s, a, f); break;
+        case 6: hipLaunchKernelGGL((k_gen2fm<false, 3>), g, dim3(256), 0, s, a, f); break;
+        case 7: hipLaunchKernelGGL((k_gen2fm<true, 3>), g, dim3(256), 0, s, a, f); break;
+        case 8: hipLaunchKernelGGL((k_gen2fm<false, 4>), g, dim3(256), 0, s, a, f); break;
+        case 9: hipLaunchKernelGGL((k_gen2fm<true, 4>), g, dim3(256), 0, s, a, f); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     ChunkArgs b = a;
     if (with_until) {
         constexpr int R = 1, N = 3 * 4 * R;

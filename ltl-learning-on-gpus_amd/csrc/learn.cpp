@@ -1667,7 +1667,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             int fused = 0, fuse_max = 2;
             if (const char *e = getenv("LTL_GEN_FUSE")) fuse_max = std::max(0, std::min(2, atoi(e)));
             std::vector<Run::Split2> sv;
-            if (c3 == c4 && B == 1 && fuse_max > 0) {
+            if (c3 == c4 && fuse_max > 0) {
                 sv = run.splits_of(cost, c1, c3);
                 if (fuse_max == 2 && c8 == c3 && run.fused_slots(sv, true) <= run.ch_cap()) fused = 2;
                 else if (run.fused_slots(sv, false) <= run.ch_cap()) fused = 1;

@@ -14,15 +14,18 @@ forced here through the environment (LTL_GEN_SPLIT = 1 / 2 / 4, LTL_GEN_FUSE
 = 0 / 1 / 2, LTL_GEN_NX = 0 / 2 -- never / on every level, not only the large
 ones --, LTL_GEN_LZW = 0 -- the fingerprinting emits write the words; read at
 every launch, level or call) and compared set for set with the oracle:
-results, counts, level boundaries, every stored set with its parents.  The instances cover generation with and without the target check
-(precheck 0 / 1), the fused ~/X/F/G pass, non-unit costs, and deferred
-levels whose X / F / G successors are tested in generation (the
-successor-test kernel variants)."""
+results, counts, level boundaries, every stored set with its parents.  The
+instances cover generation with and without the target check (precheck 0 /
+1), the fused ~/X/F/G pass, non-unit costs, deferred levels whose X / F / G
+successors are tested in generation (the successor-test kernel variants),
+and traces of 2-4 words each (the multi-word fused binary pass)."""
 import os
 
 import pytest
 
 from test_gpu_parity import check_parity
+from test_long_traces import check_parity as long_parity
+from test_long_traces import random_instance
 
 pytestmark = pytest.mark.gpu
 
@@ -90,3 +93,17 @@ def test_modes_successors_in_generation(ltl, oracle, learner, gen_mode, case):
     inst = ltl.generate_custom(seed, P, N, 3, lo, hi, 0.5, planted=planted)
     gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=9, hamming_k=k)
     assert gpu["status"] == 1 and gpu["deferred_succ"] & 14
+
+
+# K words per trace (2-4): the fused binary pass of multi-word levels
+# (k_gen2fm) against one pass per block (LTL_GEN_FUSE = 0) and the & / | pass
+# without U (1); the other knobs concern one-word levels only
+LONG = [(20, 20, 3, 65, 190, 8), (50, 50, 4, 64, 250, 7), (9, 11, 2, 60, 128, 7)]
+
+
+@pytest.mark.parametrize("shape", LONG, ids=lambda s: "x".join(map(str, s)))
+def test_modes_long_traces(ltl, oracle, learner, gen_mode, shape):
+    P, N, A, lo, hi, mc = shape
+    inst = random_instance(ltl, 3 + P, P, N, A, lo, hi)
+    assert inst.K > 1
+    long_parity(oracle, learner, inst, max_cost=mc)
