@@ -472,7 +472,8 @@ def long_traces(ltl, args):
             "words_per_trace": inst.K, "value": round((r["n_stored"] - 5) / dt, 1), "unit": "unique CS/s",
             "seconds": round(dt, 4), "seconds_is": "best of 3 calls, no kernel timing",
             "min_cost": r["cost"], "formula": r["formula"],
-            "uniques": r["n_stored"], "words_deferred": r["words_deferred"], "store_overflow": bool(r["on_the_fly"]),
+            "uniques": r["n_stored"], "words_deferred": r["words_deferred"], "words_unwritten": r["words_unwritten"],
+            "store_overflow": bool(r["on_the_fly"]),
             "kernels_ms": {k: round(v["seconds"] * 1e3, 3) for k, v in stats.items() if v["seconds"] > 0},
             # (the algorithmic byte model: operand sets counted once per
             # candidate and operand, re-reads served by L2 included, so the

@@ -1476,7 +1476,9 @@ __global__ __launch_bounds__(256) void k_materialize_m(ChunkArgs A, int64_t s0, 
 
 // Multi-word traces (B = 0): one trace (K words) per step; its summary bits
 // come from all K words.
-template <int OP, int K>
+// NX: also the fingerprints of the survivor's ~, X, F and G (the next
+// level's unary candidates, as k_emit<OP, true>), and with A.wskip no words.
+template <int OP, int K, bool NX = false>
 __global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
     if (run_stopped(A) || chunk_otf(A) || look_back_failed(A)) return;
     const int64_t base = A.st->chunk_base;
@@ -1502,6 +1504,7 @@ __global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
     uint64_t *z = A.sum + set_base(pos, 4 * A.nsw);
     const int T = A.nw / K;
     uint64_t v[4] = {0, 0, 0, 0};
+    Acc nx[4];
     for (int i = 0; i < T; ++i) {
         uint64_t x[K], r[K], y[K];
 #pragma unroll
@@ -1521,10 +1524,39 @@ __global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
         } else {
             trace_unary<OP, K>(x, aux, y);
         }
+        if (!NX || !A.wskip) {
 #pragma unroll
-        for (int kk = 0; kk < K; ++kk) __builtin_nontemporal_store(y[kk], q + (size_t)(i * K + kk) * kTileWS);
+            for (int kk = 0; kk < K; ++kk) __builtin_nontemporal_store(y[kk], q + (size_t)(i * K + kk) * kTileWS);
+        }
         summarize_trace<K>(y, aux, i & 63, v);
         flush_summary(z, i, T, A.nsw, kTileWS, v);
+        if constexpr (NX) {
+            uint64_t u[K];
+            trace_unary<LTL_NOT, K>(y, aux, u);
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) fold<false>(nx[0], u[kk], aux + kk);
+            trace_unary<LTL_NEXT, K>(y, aux, u);
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) fold<false>(nx[1], u[kk], aux + kk);
+            trace_unary<LTL_FIN, K>(y, aux, u);
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) fold<false>(nx[2], u[kk], aux + kk);
+            trace_unary<LTL_GLOB, K>(y, aux, u);
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) fold<false>(nx[3], u[kk], aux + kk);
+        }
+    }
+    if constexpr (NX) {
+        const int64_t nxi = pos - A.nx_ls + A.nx_ush;
+        if (nxi < 0 || nxi >= A.nx_stride) {
+            A.st->nx_over = 1;  // (the level is not taken from these fingerprints)
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                A.nx_fp[(size_t)(2 * j) * (size_t)A.nx_stride + (size_t)nxi] = nx[j].f1;
+                A.nx_fp[(size_t)(2 * j + 1) * (size_t)A.nx_stride + (size_t)nxi] = nx[j].f2;
+            }
+        }
     }
     if (bin) commit(A, pos, k, OP, (int32_t)(swap ? sr : sl), (int32_t)(swap ? sl : sr));
     else commit(A, pos, k, OP, (int32_t)sl, -1);
@@ -1686,6 +1718,66 @@ __global__ __launch_bounds__(256) void k_chkU(ChunkArgs A) {
 
 // Multi-word traces (B = 0): position 0 of U from the first word of each
 // trace that decides it, traces in interleaved order.
+// The K words of trace i of a set whose words may be unwritten (SetWords):
+// loaded, or recomputed from the parents' words of that trace.
+template <int K>
+__device__ __forceinline__ void trace_words(const ChunkArgs &A, const SetWords &d, int i, uint64_t (&y)[K]) {
+    if (d.op < 0) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) y[k] = d.p[(size_t)(i * K + k) * kTileWS];
+        return;
+    }
+    uint64_t a[K], b[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        a[k] = d.pa[(size_t)(i * K + k) * kTileWS];
+        b[k] = d.pb[(size_t)(i * K + k) * kTileWS];
+    }
+    const WordAux *aux = A.aux + i * K;
+    switch (d.op) {
+    case LTL_NOT: trace_unary<LTL_NOT, K>(a, aux, y); break;
+    case LTL_NEXT: trace_unary<LTL_NEXT, K>(a, aux, y); break;
+    case LTL_FIN: trace_unary<LTL_FIN, K>(a, aux, y); break;
+    case LTL_GLOB: trace_unary<LTL_GLOB, K>(a, aux, y); break;
+    case LTL_AND:
+#pragma unroll
+        for (int k = 0; k < K; ++k) y[k] = a[k] & b[k];
+        break;
+    case LTL_OR:
+#pragma unroll
+        for (int k = 0; k < K; ++k) y[k] = a[k] | b[k];
+        break;
+    case LTL_UNTIL: trace_until<K>(a, b, y); break;
+    default:
+#pragma unroll
+        for (int k = 0; k < K; ++k) y[k] = a[k];
+    }
+}
+
+// k_chkUm over an operand level stored without words: each trace's words
+// recomputed from the parents.  false: the wave is done (no lane can still
+// be accepted).
+template <int K>
+__device__ bool chkUm_unwritten(const ChunkArgs &A, int64_t sl, int64_t sr, bool anti, uint32_t &m0, uint32_t &m1) {
+    const uint32_t k = (uint32_t)A.hamming_k;
+    const SetWords L = set_words(A, sl), Rw = set_words(A, sr);
+    const int T = A.nw / K;
+    for (int j = 0; j < T; ++j) {
+        const int i = interleave(j, A.npw, T);
+        uint64_t l[K], r[K];
+        trace_words<K>(A, L, i, l);
+        trace_words<K>(A, Rw, i, r);
+        uint32_t h0, h1;
+        until_at0_trace(l, r, 1, K, h0, h1);
+        const uint32_t tg = (uint32_t)(A.aux[i * K].tgt & 1ull);
+        m0 += h0 ^ tg;
+        m1 += h1 ^ tg;
+        const uint32_t seen = (uint32_t)j + 1;
+        if ((j & 3) == 3 && !__any(m0 <= k || m1 <= k || (anti && (seen - m0 <= k || seen - m1 <= k)))) return false;
+    }
+    return true;
+}
+
 __global__ __launch_bounds__(256) void k_chkUm(ChunkArgs A) {
     if (run_stopped(A)) return;
     Tile t;
@@ -1697,17 +1789,29 @@ __global__ __launch_bounds__(256) void k_chkUm(ChunkArgs A) {
     const bool anti = A.anti && t.valid;
     const int K = A.K, T = A.nw / K;
     uint32_t m0 = t.valid ? 0u : 0x7fffffffu, m1 = m0;  // invalid lanes never accept
-    for (int j = 0; j < T; ++j) {
-        const int i = interleave(j, A.npw, T);
-        uint32_t h0, h1;
-        until_at0_trace(pl + (size_t)i * K * kTileWS, pr + (size_t)i * K * kTileWS, kTileWS, K, h0, h1);
-        const uint32_t tg = (uint32_t)(A.aux[i * K].tgt & 1ull);
-        m0 += h0 ^ tg;
-        m1 += h1 ^ tg;
-        // Mismatch and match counts only grow: stop once every lane of the
-        // wave fails (with anti: the complements too).
-        const uint32_t seen = (uint32_t)j + 1;
-        if ((j & 3) == 3 && !__any(m0 <= k || m1 <= k || (anti && (seen - m0 <= k || seen - m1 <= k)))) return;
+    // (an operand level stored without words: in the range as a whole)
+    if (in_lzw(A, A.ls) || in_lzw(A, A.rs)) {
+        bool more = false;
+        switch (K) {
+        case 2: more = chkUm_unwritten<2>(A, sl, sr, anti, m0, m1); break;
+        case 3: more = chkUm_unwritten<3>(A, sl, sr, anti, m0, m1); break;
+        case 4: more = chkUm_unwritten<4>(A, sl, sr, anti, m0, m1); break;
+        default: return;
+        }
+        if (!more) return;
+    } else {
+        for (int j = 0; j < T; ++j) {
+            const int i = interleave(j, A.npw, T);
+            uint32_t h0, h1;
+            until_at0_trace(pl + (size_t)i * K * kTileWS, pr + (size_t)i * K * kTileWS, kTileWS, K, h0, h1);
+            const uint32_t tg = (uint32_t)(A.aux[i * K].tgt & 1ull);
+            m0 += h0 ^ tg;
+            m1 += h1 ^ tg;
+            // Mismatch and match counts only grow: stop once every lane of the
+            // wave fails (with anti: the complements too).
+            const uint32_t seen = (uint32_t)j + 1;
+            if ((j & 3) == 3 && !__any(m0 <= k || m1 <= k || (anti && (seen - m0 <= k || seen - m1 <= k)))) return;
+        }
     }
     if (!t.valid) return;
     const uint32_t seen = (uint32_t)T;
@@ -1817,6 +1921,13 @@ static int64_t binary_tasks(const ChunkArgs &a, int R) { return (int64_t)a.nbs *
     case 3: hipLaunchKernelGGL((KER<OP, 3>), GRID, dim3(256), 0, S, __VA_ARGS__); break;    \
     case 4: hipLaunchKernelGGL((KER<OP, 4>), GRID, dim3(256), 0, S, __VA_ARGS__); break;    \
     default: return hipErrorInvalidValue;                                                   \
+    }
+#define LTL_DISPATCH_KNX(KER, OP, K, GRID, S, ...)                                              \
+    switch (K) {                                                                                \
+    case 2: hipLaunchKernelGGL((KER<OP, 2, true>), GRID, dim3(256), 0, S, __VA_ARGS__); break;  \
+    case 3: hipLaunchKernelGGL((KER<OP, 3, true>), GRID, dim3(256), 0, S, __VA_ARGS__); break;  \
+    case 4: hipLaunchKernelGGL((KER<OP, 4, true>), GRID, dim3(256), 0, S, __VA_ARGS__); break;  \
+    default: return hipErrorInvalidValue;                                                       \
     }
 
 // Multi-word traces (B = 0, a.K words per trace).
@@ -1939,7 +2050,7 @@ hipError_t launch_gen_fused(const ChunkArgs &a, const FusedBin &f, bool with_unt
 
 hipError_t launch_insert_xfg(const ChunkArgs &a, int64_t n, int64_t slots, uint64_t id_xfg, const uint64_t *fp,
                              int64_t stride, hipStream_t s) {
-    if (a.kmode || a.ws != kTileWS || a.K > 1 || a.t_succ || a.gen_check || !fp) return hipErrorInvalidValue;
+    if (a.kmode || a.ws != kTileWS || a.t_succ || a.gen_check || !fp) return hipErrorInvalidValue;
     if ((int64_t)a.n_tiles * 64 > stride) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_insert_xfg, dim3((unsigned)(((int64_t)a.n_tiles * 64 + 255) / 256)), dim3(256), 0, s, a, n,
                        slots, id_xfg, fp, stride);
@@ -1968,6 +2079,20 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
     if (K > 1 && pos_tiles <= 256) K = std::max(K, std::min({16, a.nsw, a.nw / 16}));
     const bool nx = B == 1 && !a.deferred && a.nx_fp;
     if (nx) K = std::min(K, 8);  // (one workgroup of K waves per store tile)
+    if (B == 0 && !a.deferred && a.nx_fp) {
+        const dim3 gm((unsigned)((pos_tiles + 3) / 4));
+        switch (op) {
+        case LTL_NOT: LTL_DISPATCH_KNX(k_emitm, LTL_NOT, a.K, gm, s, a); break;
+        case LTL_NEXT: LTL_DISPATCH_KNX(k_emitm, LTL_NEXT, a.K, gm, s, a); break;
+        case LTL_FIN: LTL_DISPATCH_KNX(k_emitm, LTL_FIN, a.K, gm, s, a); break;
+        case LTL_GLOB: LTL_DISPATCH_KNX(k_emitm, LTL_GLOB, a.K, gm, s, a); break;
+        case LTL_AND: LTL_DISPATCH_KNX(k_emitm, LTL_AND, a.K, gm, s, a); break;
+        case LTL_OR: LTL_DISPATCH_KNX(k_emitm, LTL_OR, a.K, gm, s, a); break;
+        case LTL_UNTIL: LTL_DISPATCH_KNX(k_emitm, LTL_UNTIL, a.K, gm, s, a); break;
+        default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     ChunkArgs e = a;
     e.esplit = K;
     const dim3 g((unsigned)((pos_tiles * K + 3) / 4)), b(256);

@@ -2214,7 +2214,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             // on level 9, generated from 0.7 M sets, 0.57 us on level 8,
             // from 0.11 M).
             const int64_t n_prev = Le[cost - 1] - Ls[cost - 1];
-            const bool nx = !lazy && B == 1 && !run.base.kmode && !learned && !run.logging && !run.otf && !nx_off &&
+            const bool nx = !lazy && !run.base.kmode && !learned && !run.logging && !run.otf && !nx_off &&
                             o.neg_type == 1 && c2 == 1 && c5 == 1 && c6 == 1 && c7 == 1 && cost < maxc &&
                             (n_prev >= kNxMinOperands || nx_force);
             // Such a level's emits also leave its words unwritten: the next
