@@ -1476,18 +1476,27 @@ __global__ __launch_bounds__(256) void k_materialize_m(ChunkArgs A, int64_t s0, 
 
 // Multi-word traces (B = 0): one trace (K words) per step; its summary bits
 // come from all K words.
-// NX: also the fingerprints of the survivor's ~, X, F and G (the next
-// level's unary candidates, as k_emit<OP, true>), and with A.wskip no words.
+// One workgroup of E = A.esplit waves per store tile: wave r writes the
+// traces of summary groups [r nsw / E, (r + 1) nsw / E) of the tile's 64
+// survivors (lane l: position 64 (chunk_base / 64 + tile) + l), so that E
+// waves share each survivor's walk over its nw words.  NX: also the
+// fingerprints of the survivor's ~, X, F and G (the next level's unary
+// candidates, as k_emit<OP, true>), the waves' partial sums added in LDS;
+// with A.wskip no words.
 template <int OP, int K, bool NX = false>
-__global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
+__global__ __launch_bounds__(512) void k_emitm(ChunkArgs A) {
     if (run_stopped(A) || chunk_otf(A) || look_back_failed(A)) return;
     const int64_t base = A.st->chunk_base;
-    const int64_t pos = (base & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int E = A.esplit;
+    const int rg = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));  // (wave-uniform)
+    const int lane = (int)(threadIdx.x & 63);
+    const int64_t pos = (base & ~63ll) + (int64_t)blockIdx.x * 64 + lane;
     const int64_t rel = pos - base;
-    if (rel < 0 || rel >= A.st->chunk_total) return;
-    if (pos >= A.capacity) return;
+    // (NX: every lane stays until the workgroup's ranges are added)
+    const bool live = rel >= 0 && rel < A.st->chunk_total && pos < A.capacity;
+    if (!NX && !live) return;
     constexpr bool bin = OP == LTL_AND || OP == LTL_OR || OP == LTL_UNTIL;
-    const uint32_t k = A.t_list[rel];         // temp index of the candidate
+    const uint32_t k = live ? A.t_list[rel] : 0;  // temp index of the candidate
     const uint32_t ix = OP == LTL_UNTIL ? k >> 1 : k;
     const bool swap = OP == LTL_UNTIL && (k & 1);
     int64_t sl, sr = -1;
@@ -1503,9 +1512,11 @@ __global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
     uint64_t *q = A.store + set_base(pos, A.nw);
     uint64_t *z = A.sum + set_base(pos, 4 * A.nsw);
     const int T = A.nw / K;
+    const int g0 = rg * A.nsw / E, g1 = (rg + 1) * A.nsw / E;
+    const int i0 = 64 * g0, i1 = T < 64 * g1 ? T : 64 * g1;
     uint64_t v[4] = {0, 0, 0, 0};
     Acc nx[4];
-    for (int i = 0; i < T; ++i) {
+    for (int i = live ? i0 : i1; i < i1; ++i) {
         uint64_t x[K], r[K], y[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
@@ -1547,17 +1558,32 @@ __global__ __launch_bounds__(256) void k_emitm(ChunkArgs A) {
         }
     }
     if constexpr (NX) {
+        __shared__ uint64_t nx_lds[7][8][64];
+        uint64_t f[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            f[2 * j] = nx[j].f1;
+            f[2 * j + 1] = nx[j].f2;
+        }
+        if (rg > 0) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) nx_lds[rg - 1][j][lane] = f[j];
+        }
+        __syncthreads();
+        if (rg != 0 || !live) return;
+        for (int w = 1; w < E; ++w) {
+#pragma unroll
+            for (int j = 0; j < 8; ++j) f[j] += nx_lds[w - 1][j][lane];
+        }
         const int64_t nxi = pos - A.nx_ls + A.nx_ush;
         if (nxi < 0 || nxi >= A.nx_stride) {
             A.st->nx_over = 1;  // (the level is not taken from these fingerprints)
         } else {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                A.nx_fp[(size_t)(2 * j) * (size_t)A.nx_stride + (size_t)nxi] = nx[j].f1;
-                A.nx_fp[(size_t)(2 * j + 1) * (size_t)A.nx_stride + (size_t)nxi] = nx[j].f2;
-            }
+            for (int j = 0; j < 8; ++j) A.nx_fp[(size_t)j * (size_t)A.nx_stride + (size_t)nxi] = f[j];
         }
     }
+    if (rg != 0 || !live) return;
     if (bin) commit(A, pos, k, OP, (int32_t)(swap ? sr : sl), (int32_t)(swap ? sl : sr));
     else commit(A, pos, k, OP, (int32_t)sl, -1);
 }
@@ -1922,12 +1948,19 @@ static int64_t binary_tasks(const ChunkArgs &a, int R) { return (int64_t)a.nbs *
     case 4: hipLaunchKernelGGL((KER<OP, 4>), GRID, dim3(256), 0, S, __VA_ARGS__); break;    \
     default: return hipErrorInvalidValue;                                                   \
     }
-#define LTL_DISPATCH_KNX(KER, OP, K, GRID, S, ...)                                              \
-    switch (K) {                                                                                \
-    case 2: hipLaunchKernelGGL((KER<OP, 2, true>), GRID, dim3(256), 0, S, __VA_ARGS__); break;  \
-    case 3: hipLaunchKernelGGL((KER<OP, 3, true>), GRID, dim3(256), 0, S, __VA_ARGS__); break;  \
-    case 4: hipLaunchKernelGGL((KER<OP, 4, true>), GRID, dim3(256), 0, S, __VA_ARGS__); break;  \
-    default: return hipErrorInvalidValue;                                                       \
+#define LTL_DISPATCH_KNX(KER, OP, K, GRID, BLOCK, S, ...)                                     \
+    switch (K) {                                                                              \
+    case 2: hipLaunchKernelGGL((KER<OP, 2, true>), GRID, BLOCK, 0, S, __VA_ARGS__); break;    \
+    case 3: hipLaunchKernelGGL((KER<OP, 3, true>), GRID, BLOCK, 0, S, __VA_ARGS__); break;    \
+    case 4: hipLaunchKernelGGL((KER<OP, 4, true>), GRID, BLOCK, 0, S, __VA_ARGS__); break;    \
+    default: return hipErrorInvalidValue;                                                     \
+    }
+#define LTL_DISPATCH_KB(KER, OP, K, GRID, BLOCK, S, ...)                                      \
+    switch (K) {                                                                              \
+    case 2: hipLaunchKernelGGL((KER<OP, 2>), GRID, BLOCK, 0, S, __VA_ARGS__); break;          \
+    case 3: hipLaunchKernelGGL((KER<OP, 3>), GRID, BLOCK, 0, S, __VA_ARGS__); break;          \
+    case 4: hipLaunchKernelGGL((KER<OP, 4>), GRID, BLOCK, 0, S, __VA_ARGS__); break;          \
+    default: return hipErrorInvalidValue;                                                     \
     }
 
 // Multi-word traces (B = 0, a.K words per trace).
@@ -2079,17 +2112,34 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
     if (K > 1 && pos_tiles <= 256) K = std::max(K, std::min({16, a.nsw, a.nw / 16}));
     const bool nx = B == 1 && !a.deferred && a.nx_fp;
     if (nx) K = std::min(K, 8);  // (one workgroup of K waves per store tile)
-    if (B == 0 && !a.deferred && a.nx_fp) {
-        const dim3 gm((unsigned)((pos_tiles + 3) / 4));
-        switch (op) {
-        case LTL_NOT: LTL_DISPATCH_KNX(k_emitm, LTL_NOT, a.K, gm, s, a); break;
-        case LTL_NEXT: LTL_DISPATCH_KNX(k_emitm, LTL_NEXT, a.K, gm, s, a); break;
-        case LTL_FIN: LTL_DISPATCH_KNX(k_emitm, LTL_FIN, a.K, gm, s, a); break;
-        case LTL_GLOB: LTL_DISPATCH_KNX(k_emitm, LTL_GLOB, a.K, gm, s, a); break;
-        case LTL_AND: LTL_DISPATCH_KNX(k_emitm, LTL_AND, a.K, gm, s, a); break;
-        case LTL_OR: LTL_DISPATCH_KNX(k_emitm, LTL_OR, a.K, gm, s, a); break;
-        case LTL_UNTIL: LTL_DISPATCH_KNX(k_emitm, LTL_UNTIL, a.K, gm, s, a); break;
-        default: return hipErrorInvalidValue;
+    if (B == 0 && !a.deferred) {
+        // one workgroup of E waves per store tile, each over whole summary
+        // groups of the traces (E <= 8 for the LDS sums of NX)
+        ChunkArgs m = a;
+        m.esplit = std::max(1, std::min(8, a.nsw));
+        const dim3 gm((unsigned)pos_tiles), bm((unsigned)(64 * m.esplit));
+        if (a.nx_fp) {
+            switch (op) {
+            case LTL_NOT: LTL_DISPATCH_KNX(k_emitm, LTL_NOT, a.K, gm, bm, s, m); break;
+            case LTL_NEXT: LTL_DISPATCH_KNX(k_emitm, LTL_NEXT, a.K, gm, bm, s, m); break;
+            case LTL_FIN: LTL_DISPATCH_KNX(k_emitm, LTL_FIN, a.K, gm, bm, s, m); break;
+            case LTL_GLOB: LTL_DISPATCH_KNX(k_emitm, LTL_GLOB, a.K, gm, bm, s, m); break;
+            case LTL_AND: LTL_DISPATCH_KNX(k_emitm, LTL_AND, a.K, gm, bm, s, m); break;
+            case LTL_OR: LTL_DISPATCH_KNX(k_emitm, LTL_OR, a.K, gm, bm, s, m); break;
+            case LTL_UNTIL: LTL_DISPATCH_KNX(k_emitm, LTL_UNTIL, a.K, gm, bm, s, m); break;
+            default: return hipErrorInvalidValue;
+            }
+        } else {
+            switch (op) {
+            case LTL_NOT: LTL_DISPATCH_KB(k_emitm, LTL_NOT, a.K, gm, bm, s, m); break;
+            case LTL_NEXT: LTL_DISPATCH_KB(k_emitm, LTL_NEXT, a.K, gm, bm, s, m); break;
+            case LTL_FIN: LTL_DISPATCH_KB(k_emitm, LTL_FIN, a.K, gm, bm, s, m); break;
+            case LTL_GLOB: LTL_DISPATCH_KB(k_emitm, LTL_GLOB, a.K, gm, bm, s, m); break;
+            case LTL_AND: LTL_DISPATCH_KB(k_emitm, LTL_AND, a.K, gm, bm, s, m); break;
+            case LTL_OR: LTL_DISPATCH_KB(k_emitm, LTL_OR, a.K, gm, bm, s, m); break;
+            case LTL_UNTIL: LTL_DISPATCH_KB(k_emitm, LTL_UNTIL, a.K, gm, bm, s, m); break;
+            default: return hipErrorInvalidValue;
+            }
         }
         return hipGetLastError();
     }
@@ -2113,19 +2163,6 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
     if (a.deferred) {
         if (op < LTL_NOT || op > LTL_UNTIL) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_emit_meta, g, b, 0, s, a, op);
-        return hipGetLastError();
-    }
-    if (B == 0) {
-        switch (op) {
-        case LTL_NOT: LTL_DISPATCH_K(k_emitm, LTL_NOT, a.K, g, s, a); break;
-        case LTL_NEXT: LTL_DISPATCH_K(k_emitm, LTL_NEXT, a.K, g, s, a); break;
-        case LTL_FIN: LTL_DISPATCH_K(k_emitm, LTL_FIN, a.K, g, s, a); break;
-        case LTL_GLOB: LTL_DISPATCH_K(k_emitm, LTL_GLOB, a.K, g, s, a); break;
-        case LTL_AND: LTL_DISPATCH_K(k_emitm, LTL_AND, a.K, g, s, a); break;
-        case LTL_OR: LTL_DISPATCH_K(k_emitm, LTL_OR, a.K, g, s, a); break;
-        case LTL_UNTIL: LTL_DISPATCH_K(k_emitm, LTL_UNTIL, a.K, g, s, a); break;
-        default: return hipErrorInvalidValue;
-        }
         return hipGetLastError();
     }
     switch (op) {
