@@ -640,19 +640,26 @@ __global__ __launch_bounds__(256) void k_gen2fm(ChunkArgs A, FusedBin F) {
     Tile t;
     if (!tile_binary(A, t)) return;
     const bool otf = chunk_otf(A);
-    const int64_t sl = A.ls + t.a, sr = A.rs + (t.valid ? t.b : A.b0);
-    const uint64_t *pl = A.store + set_base(sl, A.nw);
-    const uint64_t *pr = A.store + set_base(sr, A.nw);
-    constexpr int NC = U ? 4 : 2;
-    Acc c[NC];
-    const int T = A.nw / K;
-    for (int i = 0; i < T; ++i) {
-        uint64_t l[K], r[K], u0[K], u1[K];
+    const uint64_t *pl = A.store + set_base(A.ls + t.a, A.nw);
+    // The right operand through buffer loads bounded to the stripe's tile,
+    // one trace ahead in alternate registers (as in k_gen2f).
+    const int64_t bs = t.ti - (t.a - A.a0) * A.nbs;
+    const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(A.store + set_base(A.rs + A.b0 - A.ush + 64 * bs, A.nw)), (short)0, A.nw * 512, kBufferWord3);
+    const int lane8 = (int)(threadIdx.x & 63) * 8;
+    auto rload = [&](int i, uint64_t (&r)[K]) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            l[k] = pl[(size_t)(i * K + k) * kTileWS];
-            r[k] = pr[(size_t)(i * K + k) * kTileWS];
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(rr, lane8, (i * K + k) * 512, 0);
+            r[k] = (uint64_t)v[0] | ((uint64_t)v[1] << 32);
         }
+    };
+    constexpr int NC = U ? 4 : 2;
+    Acc c[NC];
+    auto trace = [&](int i, const uint64_t (&r)[K]) {
+        uint64_t l[K], u0[K], u1[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) l[k] = pl[(size_t)(i * K + k) * kTileWS];
         if constexpr (U) {
             trace_until<K>(l, r, u0);
             trace_until<K>(r, l, u1);
@@ -667,7 +674,18 @@ __global__ __launch_bounds__(256) void k_gen2fm(ChunkArgs A, FusedBin F) {
                 fold<false>(c[3], u1[k], aux + k);
             }
         }
+    };
+    const int T = A.nw / K;
+    uint64_t ra[K], rb[K];
+    rload(0, ra);
+    int i = 0;
+    for (; i + 2 <= T; i += 2) {
+        rload(i + 1, rb);
+        trace(i, ra);
+        rload(i + 2 < T ? i + 2 : T - 1, ra);
+        trace(i + 1, rb);
     }
+    if (i < T) trace(i, ra);
     if (!t.valid || otf) return;
     const uint64_t pair = (uint64_t)t.a * (uint64_t)A.nR + (uint64_t)t.b;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
