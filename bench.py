@@ -160,7 +160,7 @@ def kernel_kind(name):
         return "gen_until" if op == 7 else "gen_and_or"
     if k == "gen2m":
         return "gen_until"
-    if k in ("gen2f", "gen2fs"):
+    if k in ("gen2f", "gen2fs", "gen2fm"):  # the fused & | U pass (one word, with successor tests, K words)
         return "gen_binary"
     if k in ("emit", "emitm"):
         return "emit_unary" if op in unary else "emit_and_or" if op in binary else "emit_until"

@@ -45,10 +45,10 @@ __device__ __forceinline__ uint64_t until_carry64(uint64_t l, uint64_t r, uint64
 // in some half, by d with 0 < |d| < 2^32; for the other keys fixed, two
 // values k != k' of that half's key give products that differ by
 // (k - k') d, nonzero and below 2^64 in magnitude, so at most one key value
-// makes the sums equal: Pr[collision] <= 2^-32 per fingerprint, independent
-// keys for the two.  One multiply-add per half (v_mad_u64_u32), and the hash
-// is linear: since (l & r) + (l | r) = l + r half by half as integers,
-// H(l | r) = H(l) + H(r) - H(l & r) mod 2^64.
+// makes the sums equal.  The host draws odd keys (2^31 values per half):
+// Pr[collision] <= 2^-31 per fingerprint, independent keys for the two, and
+// sets that differ in one half only never collide.  One multiply-add per
+// half (v_mad_u64_u32).
 __device__ __forceinline__ void ml_fold(uint64_t &f, uint64_t c, uint64_t k) {
     f += (uint64_t)(uint32_t)c * (uint64_t)(uint32_t)k;
     f += (uint64_t)(uint32_t)(c >> 32) * (uint64_t)(uint32_t)(k >> 32);

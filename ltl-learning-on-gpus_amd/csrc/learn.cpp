@@ -70,6 +70,12 @@ uint64_t splitmix(uint64_t x) {
     return z ^ (z >> 31);
 }
 
+// Fingerprint keys (two 32-bit halves) with odd halves: two sets that differ
+// in one word half only never share a fingerprint (an odd key times a
+// nonzero difference below 2^32 is nonzero mod 2^64); any pair of different
+// sets collides with probability <= 2^-31 per fingerprint over the keys.
+uint64_t fp_key(uint64_t x) { return splitmix(x) | 0x0000000100000001ull; }
+
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
 constexpr int kCountSlots = 1 << 17;
@@ -601,8 +607,8 @@ void prepare_packed(const ltl_problem *pb, Prepared &q, uint64_t seed = kDefault
         for (size_t k = 0; k < bins[b].size(); ++k) tword.push_back(w);
         WordAux x;
         memset(&x, 0, sizeof(x));
-        x.k1 = splitmix(seed ^ (2ull * w));
-        x.k2 = splitmix(~seed ^ (2ull * w + 1));
+        x.k1 = fp_key(seed ^ (2ull * w));
+        x.k2 = fp_key(~seed ^ (2ull * w + 1));
         x.sinfo = make_sinfo((uint32_t)(t0 / 64), t0 % 64, (int)bins[b].size());
         int top = 64;
         for (int i : bins[b]) {
@@ -669,8 +675,8 @@ void prepare_multiword(const ltl_problem *pb, Prepared &q, uint64_t seed = kDefa
         WordAux &x = q.aux[w];
         memset(&x, 0, sizeof(x));
         const int i = w / K, k = w % K;
-        x.k1 = splitmix(seed ^ (2ull * w));
-        x.k2 = splitmix(~seed ^ (2ull * w + 1));
+        x.k1 = fp_key(seed ^ (2ull * w));
+        x.k2 = fp_key(~seed ^ (2ull * w + 1));
         x.lm = word_mask(pb->lens[i], k);
         x.chk = k == 0 ? 1ull : 0ull;
         x.tgt = k == 0 && i < P ? 1ull : 0ull;
