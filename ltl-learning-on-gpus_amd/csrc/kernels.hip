@@ -1417,6 +1417,9 @@ __global__ __launch_bounds__(256) void k_emit_meta(ChunkArgs A, int op) {
 // parents as the metadata records them (m_op, m_left, m_right; until with
 // the operands in formula order): the sets of a deferred level.  One lane
 // per set; packed layout.
+// SUMS: the summaries too (a deferred level); else the words only (a level
+// stored with its summaries but without words).
+template <bool SUMS>
 __global__ __launch_bounds__(256) void k_materialize(ChunkArgs A, int64_t s0, int64_t s1) {
     const int64_t s = (s0 & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (s < s0 || s >= s1) return;
@@ -1448,13 +1451,13 @@ __global__ __launch_bounds__(256) void k_materialize(ChunkArgs A, int64_t s0, in
         default: v = a; break;  // atoms are stored with their words
         }
         q[(size_t)w * kTileWS] = v;
-        summarize_pk<15>(v, x, acc, A.W, flush);
+        if constexpr (SUMS) summarize_pk<15>(v, x, acc, A.W, flush);
     }
 }
 
 // Multi-word traces (B = 0): k_materialize for K words per trace, one trace
 // per step, its summary bits from all K words.
-template <int K>
+template <int K, bool SUMS>
 __global__ __launch_bounds__(256) void k_materialize_m(ChunkArgs A, int64_t s0, int64_t s1) {
     const int64_t s = (s0 & ~63ll) + (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (s < s0 || s >= s1) return;
@@ -1487,8 +1490,10 @@ __global__ __launch_bounds__(256) void k_materialize_m(ChunkArgs A, int64_t s0, 
         }
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) q[(size_t)(i * K + kk) * kTileWS] = w[kk];
-        summarize_trace<K>(w, aux, i & 63, v);
-        flush_summary(z, i, T, A.nsw, kTileWS, v);
+        if constexpr (SUMS) {
+            summarize_trace<K>(w, aux, i & 63, v);
+            flush_summary(z, i, T, A.nsw, kTileWS, v);
+        }
     }
 }
 
@@ -2196,16 +2201,20 @@ hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_materialize(const ChunkArgs &a, int64_t s0, int64_t s1, hipStream_t s) {
+hipError_t launch_materialize(const ChunkArgs &a, int64_t s0, int64_t s1, hipStream_t s, bool sums) {
     if (a.ws != kTileWS || s0 < 0 || s1 > a.capacity) return hipErrorInvalidValue;
     if (s1 <= s0) return hipSuccess;
     const int64_t n = s1 - (s0 & ~63ll);
     const dim3 g((unsigned)((n + 255) / 256)), b(256);
-    switch (a.K) {
-    case 1: hipLaunchKernelGGL(k_materialize, g, b, 0, s, a, s0, s1); break;
-    case 2: hipLaunchKernelGGL(k_materialize_m<2>, g, b, 0, s, a, s0, s1); break;
-    case 3: hipLaunchKernelGGL(k_materialize_m<3>, g, b, 0, s, a, s0, s1); break;
-    case 4: hipLaunchKernelGGL(k_materialize_m<4>, g, b, 0, s, a, s0, s1); break;
+    switch (a.K * 2 + (sums ? 1 : 0)) {
+    case 2: hipLaunchKernelGGL(k_materialize<false>, g, b, 0, s, a, s0, s1); break;
+    case 3: hipLaunchKernelGGL(k_materialize<true>, g, b, 0, s, a, s0, s1); break;
+    case 4: hipLaunchKernelGGL((k_materialize_m<2, false>), g, b, 0, s, a, s0, s1); break;
+    case 5: hipLaunchKernelGGL((k_materialize_m<2, true>), g, b, 0, s, a, s0, s1); break;
+    case 6: hipLaunchKernelGGL((k_materialize_m<3, false>), g, b, 0, s, a, s0, s1); break;
+    case 7: hipLaunchKernelGGL((k_materialize_m<3, true>), g, b, 0, s, a, s0, s1); break;
+    case 8: hipLaunchKernelGGL((k_materialize_m<4, false>), g, b, 0, s, a, s0, s1); break;
+    case 9: hipLaunchKernelGGL((k_materialize_m<4, true>), g, b, 0, s, a, s0, s1); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -272,10 +272,10 @@ hipError_t launch_rx_tail(int op, const ChunkArgs &a, hipStream_t s);
 // one lane per output position.
 hipError_t launch_compact(int op, const ChunkArgs &a, hipStream_t s);
 hipError_t launch_emit(int op, int B, const ChunkArgs &a, hipStream_t s);
-// Words and summaries of the stored sets [s0, s1) recomputed from their
-// parents (operator and operand indices of the metadata): sets emitted with
-// deferred = 1.  Packed layout.
-hipError_t launch_materialize(const ChunkArgs &a, int64_t s0, int64_t s1, hipStream_t s);
+// Words and, with sums, summaries of the stored sets [s0, s1) recomputed
+// from their parents (operator and operand indices of the metadata): sets
+// emitted with deferred = 1 (sums), or with wskip = 1 (words only).
+hipError_t launch_materialize(const ChunkArgs &a, int64_t s0, int64_t s1, hipStream_t s, bool sums = true);
 hipError_t launch_atoms(int B, const ChunkArgs &a, int n_atoms, hipStream_t s);
 // dst[r * ws + c] = src[r * ncols + c] for r < nrows, c < ncols.
 hipError_t launch_scatter(uint64_t *dst, int64_t ws, const uint64_t *src, int ncols, int64_t nrows, hipStream_t s);

@@ -1908,7 +1908,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     if (const char *e = getenv("LTL_GEN_LZW")) lzw_off = atoi(e) == 0;
     auto materialize_lzw = [&]() -> int32_t {
         if (c->lzw_e <= c->lzw_s) return 0;
-        if (launch_materialize(c->lzw_args, c->lzw_s, c->lzw_e, s) != hipSuccess) return LTL_E_HIP;
+        if (launch_materialize(c->lzw_args, c->lzw_s, c->lzw_e, s, false) != hipSuccess) return LTL_E_HIP;
         c->lzw_s = c->lzw_e = 0;
         run.base.lzw_s = run.base.lzw_e = 0;
         return 0;
@@ -2229,7 +2229,17 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             // written first, above), and the levels after that only through
             // the summaries and the until check, which recomputes the words
             // it reads from the parents (k_chkU).
-            const bool wskip = nx && !lzw_off;
+            // (not when the level could overflow the store: check-only levels
+            // would then need the words at once)
+            bool wskip = nx && !lzw_off;
+            if (wskip) {
+                run.count_only = true;
+                run.counted = 0;
+                rc = level_blocks(cost);
+                run.count_only = false;
+                if (rc) break;
+                wskip = (uint64_t)n_stored + run.counted <= (uint64_t)c->capacity;
+            }
             if (nx) {
                 run.base.nx_fp = c->d_nxfp;
                 run.base.nx_ls = n_stored;
@@ -2764,7 +2774,7 @@ extern "C" int32_t ltl_levels(const ltl_ctx *c, int64_t *start, int64_t *end, in
 // deferred level (words and summaries).
 static hipError_t materialize_pending(ltl_ctx *c) {
     if (c->lzw_e > c->lzw_s) {
-        const hipError_t e = launch_materialize(c->lzw_args, c->lzw_s, c->lzw_e, c->stream);
+        const hipError_t e = launch_materialize(c->lzw_args, c->lzw_s, c->lzw_e, c->stream, false);
         if (e != hipSuccess) return e;
         c->lzw_s = c->lzw_e = 0;
     }
