@@ -107,3 +107,15 @@ def test_modes_long_traces(ltl, oracle, learner, gen_mode, shape):
     inst = random_instance(ltl, 3 + P, P, N, A, lo, hi)
     assert inst.K > 1
     long_parity(oracle, learner, inst, max_cost=mc)
+
+
+# Stores that overflow into check-only levels: a level whose candidates could
+# overflow the store keeps its words (the check-only levels read them), the
+# levels before it may not
+# (levels of 5, 20, 81, 521 and 2936 sets up to cost 5: the store overflows in
+# level 4, early in level 5 and late in level 5)
+@pytest.mark.parametrize("capacity", [400, 1500, 3000])
+def test_modes_overflow(ltl, oracle, learner, gen_mode, capacity):
+    inst = ltl.generate_custom(79, 64, 64, 5, 5, 40, 0.5, planted_size=9)
+    gpu, _ = check_parity(ltl, oracle, learner, inst, max_cost=5, capacity=capacity)
+    assert gpu["on_the_fly"]
