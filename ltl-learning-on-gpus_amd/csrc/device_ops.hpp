@@ -138,8 +138,18 @@ __device__ __forceinline__ uint64_t tab_val(uint64_t v, uint32_t e) { return ((u
 __device__ __forceinline__ bool tab_committed(uint64_t v, uint32_t e) { return (v & ~(kPending - 1)) == tab_val(0, e); }
 __device__ __forceinline__ uint64_t tab_payload(uint64_t v) { return v & (kPending - 1); }
 
+// A candidate insert's result with this bit set: the key was found holding a
+// stored set of the epoch (a committed value, which is final: values only
+// decrease), whose store index the low 31 bits give; the candidate is a
+// duplicate of it (the table's slot indices and the store's indices stay
+// below 2^31).
+constexpr uint32_t kSlotStored = 1u << 31;
+
 // Linear probing; key CAS over a free slot (another epoch's key), then
 // atomicMin of the value.  key and val carry the epoch e (tab_key, tab_val).
+// STORED: a key found with a committed value returns kSlotStored | its store
+// index instead of the slot.
+template <bool STORED = false>
 __device__ inline uint32_t tab_insert(uint64_t *tab, uint32_t tmask, uint64_t key, uint64_t val, uint32_t e,
                                       int *probe_fail) {
     uint32_t p = (uint32_t)key & tmask;
@@ -158,6 +168,7 @@ __device__ inline uint32_t tab_insert(uint64_t *tab, uint32_t tmask, uint64_t ke
             // set under F, ...) are committed early, so their many duplicates
             // become cached loads instead of serialized atomics on one slot.
             unsigned long long cur = __hip_atomic_load(kp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (STORED && tab_committed(cur, e)) return kSlotStored | (uint32_t)tab_payload(cur);
             if ((unsigned long long)val < cur) atomicMin(kp + 1, (unsigned long long)val);
             return p;
         }

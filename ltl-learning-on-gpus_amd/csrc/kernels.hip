@@ -216,9 +216,9 @@ __device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, con
         }
     }
     if constexpr (OP == LTL_UNTIL) {
-        A.t_slot[2 * ix] = tab_insert(A.tab, A.tmask, tab_key(c0.f1, A.epoch), tab_val(kPending | id0, A.epoch), A.epoch, &A.st->probe_fail);
+        A.t_slot[2 * ix] = tab_insert<true>(A.tab, A.tmask, tab_key(c0.f1, A.epoch), tab_val(kPending | id0, A.epoch), A.epoch, &A.st->probe_fail);
         A.t_fp2[2 * ix] = c0.f2;
-        A.t_slot[2 * ix + 1] = tab_insert(A.tab, A.tmask, tab_key(c1.f1, A.epoch), tab_val(kPending | (id0 + 1), A.epoch), A.epoch,
+        A.t_slot[2 * ix + 1] = tab_insert<true>(A.tab, A.tmask, tab_key(c1.f1, A.epoch), tab_val(kPending | (id0 + 1), A.epoch), A.epoch,
                                           &A.st->probe_fail);
         A.t_fp2[2 * ix + 1] = c1.f2;
         if constexpr (SUCC) {
@@ -226,7 +226,7 @@ __device__ __forceinline__ void gen2_tail(const ChunkArgs &A, const Tile &t, con
             A.t_succ[2 * ix + 1] = succ_flags(A, s1);
         }
     } else {
-        A.t_slot[ix] = tab_insert(A.tab, A.tmask, tab_key(c0.f1, A.epoch), tab_val(kPending | id0, A.epoch), A.epoch, &A.st->probe_fail);
+        A.t_slot[ix] = tab_insert<true>(A.tab, A.tmask, tab_key(c0.f1, A.epoch), tab_val(kPending | id0, A.epoch), A.epoch, &A.st->probe_fail);
         A.t_fp2[ix] = c0.f2;
         if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, s0);
     }
@@ -241,7 +241,7 @@ __device__ __forceinline__ void gen1_tail(const ChunkArgs &A, const Tile &t, con
     if (otf) return;
     const size_t ix = (size_t)t.ti * 64 + (threadIdx.x & 63);
     if constexpr (RX) A.t_acc[ix] = (uint8_t)acc0;
-    A.t_slot[ix] = tab_insert(A.tab, A.tmask, tab_key(c0.f1, A.epoch), tab_val(kPending | id0, A.epoch), A.epoch, &A.st->probe_fail);
+    A.t_slot[ix] = tab_insert<true>(A.tab, A.tmask, tab_key(c0.f1, A.epoch), tab_val(kPending | id0, A.epoch), A.epoch, &A.st->probe_fail);
     A.t_fp2[ix] = c0.f2;
     if constexpr (SUCC) A.t_succ[ix] = succ_flags(A, s0);
 }
@@ -466,7 +466,7 @@ __global__ __launch_bounds__(256) void k_gen_xfg(ChunkArgs A, int64_t n, int64_t
         if (CHK && c[j].mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id);
         if (otf) continue;
         const size_t k = (size_t)j * (size_t)slots + ix;
-        A.t_slot[k] = tab_insert(A.tab, A.tmask, tab_key(c[j].f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
+        A.t_slot[k] = tab_insert<true>(A.tab, A.tmask, tab_key(c[j].f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
                                 &A.st->probe_fail);
         A.t_fp2[k] = c[j].f2;
         if constexpr (SUCC) A.t_succ[k] = succ_flags(A, sc[j]);
@@ -606,7 +606,7 @@ __device__ __forceinline__ void gen2f_body(const ChunkArgs &A, const FusedBin &F
         for (int j = 0; j < NC; ++j) {
             const uint64_t id = j == 0 ? A.id_base + pair : j == 1 ? F.id_or + pair : F.id_u + 2 * pair + (uint64_t)(j - 2);
             const size_t k = j == 0 ? ix : j == 1 ? (size_t)F.off_or + ix : (size_t)F.off_u + 2 * ix + (size_t)(j - 2);
-            A.t_slot[k] = tab_insert(A.tab, A.tmask, tab_key(c[r][j].f1, A.epoch), tab_val(kPending | id, A.epoch),
+            A.t_slot[k] = tab_insert<true>(A.tab, A.tmask, tab_key(c[r][j].f1, A.epoch), tab_val(kPending | id, A.epoch),
                                      A.epoch, &A.st->probe_fail);
             A.t_fp2[k] = c[r][j].f2;
             if constexpr (SUCC) A.t_succ[k] = succ_flags(A, sc[r][j]);
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(256) void k_gen2fm(ChunkArgs A, FusedBin F) {
     for (int j = 0; j < NC; ++j) {
         const uint64_t id = j == 0 ? A.id_base + pair : j == 1 ? F.id_or + pair : F.id_u + 2 * pair + (uint64_t)(j - 2);
         const size_t k = j == 0 ? ix : j == 1 ? (size_t)F.off_or + ix : (size_t)F.off_u + 2 * ix + (size_t)(j - 2);
-        A.t_slot[k] = tab_insert(A.tab, A.tmask, tab_key(c[j].f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
+        A.t_slot[k] = tab_insert<true>(A.tab, A.tmask, tab_key(c[j].f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
                                  &A.st->probe_fail);
         A.t_fp2[k] = c[j].f2;
     }
@@ -767,7 +767,7 @@ __global__ __launch_bounds__(256) void k_gen_xfgm(ChunkArgs A, int64_t n, int64_
         if (CHK && c[j].mism <= (uint32_t)A.hamming_k) atomicMin(&A.st->found, (unsigned long long)id);
         if (otf) continue;
         const size_t k = (size_t)j * (size_t)slots + ix;
-        A.t_slot[k] = tab_insert(A.tab, A.tmask, tab_key(c[j].f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
+        A.t_slot[k] = tab_insert<true>(A.tab, A.tmask, tab_key(c[j].f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
                                 &A.st->probe_fail);
         A.t_fp2[k] = c[j].f2;
     }
@@ -893,8 +893,10 @@ __device__ __forceinline__ uint32_t decide_tile(const ChunkArgs &A, int64_t ti, 
             const size_t k = ND * ix + d;
             const uint32_t slot = A.t_slot[k];
             const uint64_t f2 = A.t_fp2[k];
-            const uint64_t v = A.tab[2 * (size_t)slot + 1];
             const uint64_t id = id0 + d;
+            // (found stored at the insert: a duplicate, no table read)
+            const uint64_t v = (slot & kSlotStored) ? tab_val(slot & ~kSlotStored, A.epoch)
+                                                    : A.tab[2 * (size_t)slot + 1];
             if (v == tab_val(kPending | id, A.epoch)) {
                 surv = true;
                 // relaxed keys: the survivor's own test (set in generation)
@@ -1369,7 +1371,7 @@ __global__ __launch_bounds__(256) void k_insert_xfg(ChunkArgs A, int64_t n, int6
         const uint64_t id = (j == 0 ? A.id_base : id_xfg + (uint64_t)(j - 1) * (uint64_t)n) + (uint64_t)a;
         const uint64_t f1 = fp[(size_t)(2 * j) * (size_t)stride + ix], f2 = fp[(size_t)(2 * j + 1) * (size_t)stride + ix];
         const size_t k = (size_t)j * (size_t)slots + ix;
-        A.t_slot[k] = tab_insert(A.tab, A.tmask, tab_key(f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
+        A.t_slot[k] = tab_insert<true>(A.tab, A.tmask, tab_key(f1, A.epoch), tab_val(kPending | id, A.epoch), A.epoch,
                                  &A.st->probe_fail);
         A.t_fp2[k] = f2;
     }

@@ -99,7 +99,7 @@ struct ChunkArgs {
     const WordAux *aux;
     uint64_t *tab;            // 2 words per slot: key, value
     DevState *st;
-    uint32_t *t_slot;         // per candidate: table slot
+    uint32_t *t_slot;         // per candidate: table slot, or kSlotStored | the stored set it duplicates
     uint64_t *t_fp2;          // per candidate: verification fingerprint
     uint64_t *t_mask;         // per tile: survivor masks (2 for U)
     uint32_t *t_cnt;          // per tile: survivors
