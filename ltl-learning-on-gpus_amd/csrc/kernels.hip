@@ -292,8 +292,8 @@ __global__ __launch_bounds__(256) void k_gen2(ChunkArgs A) {
         }
     };
     // Batches of UB words: all right-operand loads of a batch are issued
-    // before use, so that each lane keeps UB loads in flight.  Measured at
-    // config 2 (gen_and_or per call): R = 1, UB = 16: 6.4 ms; R = 2, UB = 8:
+    // before use, so that each lane keeps UB loads in flight.  Measured on
+    // 500+500 traces of 10-32 positions (gen_and_or per call): R = 1, UB = 16: 6.4 ms; R = 2, UB = 8:
     // 4.9 ms; R = 4, UB = 4 or 8: 5.5-5.6 ms.  U: R = 1, UB = 4: 5.45 ms;
     // UB = 8 or R = 2: 5.7-6.0 ms (two carry chains and four fingerprints
     // per pair keep it VALU-bound).
@@ -1908,7 +1908,7 @@ static dim3 grid_tiles(const ChunkArgs &a) {
 // tests): one while the chunk has enough tasks to fill the device; a chunk
 // of few tasks folds each task's words with 2 or 4 waves, since one wave
 // walking every word of its sets is then the whole launch (a fused
-// ~/X/F/G pass over 81 sets of config 2 took 55 us).  tasks: the waves of
+// ~/X/F/G pass over 81 sets of 341 words took 55 us).  tasks: the waves of
 // the launch with one wave per task.
 static int gen_split(int64_t tasks, int nw) {
     // (LTL_GEN_SPLIT = 1, 2 or 4 in the environment forces the choice: tests

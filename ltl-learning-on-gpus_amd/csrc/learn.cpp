@@ -518,8 +518,8 @@ static bool profiled(const ltl_ctx *c, int kind) {
 }
 
 // Launch, bracketed by HIP events on the library's stream when its kind is
-// profiled (each timed launch costs a dispatch barrier: about 5 us at
-// config 2, 1.4 ms per call with every streaming launch timed).
+// profiled (each timed launch costs a dispatch barrier: about 5 us, on
+// 500+500 traces 1.4 ms per call with every streaming launch timed).
 template <class F>
 static hipError_t timed(ltl_ctx *c, int kind, uint64_t alg, uint64_t mn, uint64_t items, F &&launch) {
     if (!profiled(c, kind)) return launch();
@@ -1564,7 +1564,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
     ba.ptrace = c->d_ptrace;
     if (B == 1) HIP_TRY(upload_init(c->d_ptrace, prep.ptrace.data(), (size_t)W * 8));
     // Packed emits: up to eight waves per store tile, each a word range of
-    // whole summary groups of 40 words or more (config 2, per call: one wave
+    // whole summary groups of 40 words or more (500+500 traces, per call: one wave
     // per tile 16.1 ms of emits, four 10.1 ms, eight 8.9 ms, sixteen 8.9 ms).
     ba.esplit = 1;
     if (B == 1) {
@@ -2215,7 +2215,7 @@ static int32_t learn_once(ltl_ctx *c, const ltl_problem *pb, ltl_result *res, bo
             // level's fused ~/X/F/G pass reads this level), for large levels
             // only: the extra arithmetic lengthens every emit wave, which a
             // level of large chunks hides better than the read it saves
-            // costs (config 2, per 1000 sets: the next level's pass 0.76 us,
+            // costs (500+500 traces of 10-32 positions, per 1000 sets: the next level's pass 0.76 us,
             // the inserts that replace it 0.22 us; the emits' extra 0.30 us
             // on level 9, generated from 0.7 M sets, 0.57 us on level 8,
             // from 0.11 M).
@@ -2423,8 +2423,8 @@ static int32_t batch_fill_result(const BatchOut &bo, const AtomNames &names, ltl
 // synchronisations, with the device idle in between.  Up to kHelpers + 1 of
 // them run at once, from host threads, each on a context of its own (stream,
 // pool, pinned staging; the calling context is one of them), so that one
-// call's launches fill another's gaps (six at once: 0.040 s for 256 config-4
-// instances, four: 0.042 s).  A helper's pool is an eighth of the calling
+// call's launches fill another's gaps (six at once: 0.040 s for 256 instances
+// of 20-400 traces, four: 0.042 s).  A helper's pool is an eighth of the calling
 // context's budget (a sixteenth of the free memory when that is 0, at most
 // 8 GiB); an instance that exhausts it (out of memory, or a store
 // that overflowed into check-only levels) is run again on the calling
